@@ -1,0 +1,454 @@
+// Golden-vector driver for the compiled reference (test infrastructure only).
+//
+// Links the reference translation units (compiled from /root/reference by
+// oracle/Makefile) and drives the hot-path classes on small synthetic inputs,
+// so that tests/golden/make_golden.py can store their outputs as fixtures.
+// Nothing here is shipped or timed; the product never links it.
+//
+//   ref_harness <command> <in.xfb> <out.xfb>
+//
+// Commands (each reads named arrays from in.xfb and writes named arrays):
+//   op        masks -> space mask, Rn numbering, activity table, A x, A x - b,
+//             (A x, x), (x, b), |x|_inf with Rn index
+//             (XSpaceModel.cpp:276-411, XOperator.cpp:837-1129, XSpace.cpp:2293-3487)
+//   divop     the same through T3dVariableDivergentedOperator (XOperator.cpp:1901-2268)
+//   cg|bicg|mr|bicgstab
+//             TSLAE solvers (XSLAE.cpp:56-414, 667-767); the iteration count is
+//             recovered by counting operator sweeps
+//   thomas    TSLAE::SolveByScalarRunning (XSLAE.cpp:416-445)
+//   lod       THydrodynamicsProblem::SolveTransferEquation__ (XProblem.cpp:2784-3475)
+//   ib        TImmersedBoundaryProblem::Interpolate / SpreadForce / Index
+//             (XProblem.cpp:8550-8791, 8931-9046)
+//
+// File format (.xfb): repeated records
+//   u32 name_len, name bytes, u8 dtype ('d' f64 | 'i' i32), u32 ndim,
+//   i64 dims[ndim], raw little-endian data.
+#include <cstdio>
+#include <cstdlib>
+#include <cstdint>
+#include <cstring>
+#include <cmath>
+#include <map>
+#include <string>
+#include <vector>
+#include <stdexcept>
+
+#include "XSLAE.h"
+#include "XOperator.h"
+#include "XSpace.h"
+#include "XSpaceModel.h"
+#include "XGrid.h"
+#include "XMask.h"
+#include "XSeparator.h"
+#include "XProblem.h"
+#include "XImmersedBoundary.h"
+
+// ----------------------------------------------------------------- xfb I/O
+struct Arr {
+  char dtype = 'd';
+  std::vector<int64_t> dims;
+  std::vector<double> d;
+  std::vector<int32_t> i;
+  int64_t size() const {
+    int64_t s = 1;
+    for (auto x : dims) s *= x;
+    return s;
+  }
+};
+typedef std::map<std::string, Arr> Bag;
+
+static Bag read_xfb(const char* path) {
+  Bag bag;
+  FILE* fp = fopen(path, "rb");
+  if (!fp) throw std::runtime_error(std::string("cannot open ") + path);
+  for (;;) {
+    uint32_t nl;
+    if (fread(&nl, 4, 1, fp) != 1) break;
+    std::string name(nl, '\0');
+    if (fread(&name[0], 1, nl, fp) != nl) throw std::runtime_error("bad name");
+    Arr a;
+    uint8_t dt;
+    uint32_t nd;
+    if (fread(&dt, 1, 1, fp) != 1 || fread(&nd, 4, 1, fp) != 1) throw std::runtime_error("bad hdr");
+    a.dtype = (char)dt;
+    a.dims.resize(nd);
+    if (nd && fread(a.dims.data(), 8, nd, fp) != nd) throw std::runtime_error("bad dims");
+    int64_t n = a.size();
+    if (a.dtype == 'd') {
+      a.d.resize(n);
+      if (n && (int64_t)fread(a.d.data(), 8, n, fp) != n) throw std::runtime_error("bad f64 data");
+    } else {
+      a.i.resize(n);
+      if (n && (int64_t)fread(a.i.data(), 4, n, fp) != n) throw std::runtime_error("bad i32 data");
+    }
+    bag[name] = a;
+  }
+  fclose(fp);
+  return bag;
+}
+
+static void write_xfb(const char* path, const Bag& bag) {
+  FILE* fp = fopen(path, "wb");
+  if (!fp) throw std::runtime_error(std::string("cannot write ") + path);
+  for (auto& kv : bag) {
+    uint32_t nl = (uint32_t)kv.first.size();
+    fwrite(&nl, 4, 1, fp);
+    fwrite(kv.first.data(), 1, nl, fp);
+    uint8_t dt = (uint8_t)kv.second.dtype;
+    fwrite(&dt, 1, 1, fp);
+    uint32_t nd = (uint32_t)kv.second.dims.size();
+    fwrite(&nd, 4, 1, fp);
+    fwrite(kv.second.dims.data(), 8, nd, fp);
+    if (dt == 'd') fwrite(kv.second.d.data(), 8, kv.second.d.size(), fp);
+    else fwrite(kv.second.i.data(), 4, kv.second.i.size(), fp);
+  }
+  fclose(fp);
+}
+
+static const Arr& need(const Bag& b, const char* name) {
+  auto it = b.find(name);
+  if (it == b.end()) throw std::runtime_error(std::string("missing input ") + name);
+  return it->second;
+}
+static double scalar(const Bag& b, const char* name) { return need(b, name).d.at(0); }
+static int iscalar(const Bag& b, const char* name) { return need(b, name).i.at(0); }
+
+static Arr mk_d(std::vector<int64_t> dims) {
+  Arr a; a.dtype = 'd'; a.dims = dims; a.d.assign(a.size(), 0.0); return a;
+}
+static Arr mk_i(std::vector<int64_t> dims) {
+  Arr a; a.dtype = 'i'; a.dims = dims; a.i.assign(a.size(), 0); return a;
+}
+
+// ----------------------------------------------------------- grid set-up
+// Inputs: x[N+1], y[L+1], z[M+1] node coordinates, codes[(N+1)(L+1)(M+1)],
+// optional normals[(N+1)(L+1)(M+1)*3] + has_normal[...].
+struct Setup {
+  int N = 0, L = 0, M = 0;            // EndIndex of the three separators
+  T3dNormalGrid* grid = nullptr;
+  T3dNumberMask* space_mask = nullptr;
+  TRnRelease3dSpaceModel* model = nullptr;
+  int model_name = 0;
+};
+
+static int g_model_counter = 700;
+
+static Setup make_setup(const Bag& in, bool all_unknown = false) {
+  Setup s;
+  const Arr& x = need(in, "x");
+  const Arr& y = need(in, "y");
+  const Arr& z = need(in, "z");
+  const Arr& codes = need(in, "codes");
+  s.N = (int)x.size() - 1; s.L = (int)y.size() - 1; s.M = (int)z.size() - 1;
+  int n = s.N + 1, l = s.L + 1, m = s.M + 1;
+  TSeparator* sx = new TRandomSeparator(n, x.d.data());
+  TSeparator* sy = new TRandomSeparator(l, y.d.data());
+  TSeparator* sz = new TRandomSeparator(m, z.d.data());
+  T3dNumberMask* mask = new T3dNumberMask(n, l, m);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < l; j++)
+      for (int k = 0; k < m; k++) (*mask)[i][j][k] = codes.i[((int64_t)i * l + j) * m + k];
+  // normals
+  T3dVector**** nrm = new T3dVector***[n];
+  auto hn = in.find("has_normal");
+  auto nv = in.find("normals");
+  for (int i = 0; i < n; i++) {
+    nrm[i] = new T3dVector**[l];
+    for (int j = 0; j < l; j++) {
+      nrm[i][j] = new T3dVector*[m];
+      for (int k = 0; k < m; k++) {
+        nrm[i][j][k] = nullptr;
+        int64_t c = ((int64_t)i * l + j) * m + k;
+        if (hn != in.end() && hn->second.i[c]) {
+          T3dVector* v = new T3dVector;
+          v->X = nv->second.d[3 * c + 0];
+          v->Y = nv->second.d[3 * c + 1];
+          v->Z = nv->second.d[3 * c + 2];
+          nrm[i][j][k] = v;
+        }
+      }
+    }
+  }
+  s.grid = new T3dNormalGrid(sx, sy, sz, *mask, nrm);
+  // Space mask: caller-supplied (the derivation rule of XProblem.cpp:4155-4182
+  // is applied by the Python side and checked separately) or all ones.
+  s.space_mask = new T3dNumberMask(n, l, m);
+  auto sm = in.find("space_mask");
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < l; j++)
+      for (int k = 0; k < m; k++) {
+        int64_t c = ((int64_t)i * l + j) * m + k;
+        (*s.space_mask)[i][j][k] = all_unknown ? 1 : (sm != in.end() ? sm->second.i[c] : 0);
+      }
+  TMask** masks = new TMask*[1];
+  masks[0] = s.space_mask;
+  s.model = new TRnRelease3dSpaceModel(1, masks);
+  s.model_name = g_model_counter++;
+  TRnSpace::AddModel(s.model_name, s.model, false);
+  return s;
+}
+
+static TRnRelease3dSpace* make_vec(const Setup& s, const Bag& in, const char* name) {
+  TRnRelease3dSpace* v = new TRnRelease3dSpace(s.model_name);
+  int n = s.N + 1, l = s.L + 1, m = s.M + 1;
+  auto it = in.find(name);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < l; j++)
+      for (int k = 0; k < m; k++)
+        (*v)[i][j][k] = (it != in.end()) ? it->second.d[((int64_t)i * l + j) * m + k] : 0.0;
+  return v;
+}
+
+static Arr dump_vec(const Setup& s, TRnSpace& vv) {
+  TRnRelease3dSpace& v = dynamic_cast<TRnRelease3dSpace&>(vv);
+  int n = s.N + 1, l = s.L + 1, m = s.M + 1;
+  Arr a = mk_d({n, l, m});
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < l; j++)
+      for (int k = 0; k < m; k++) a.d[((int64_t)i * l + j) * m + k] = v[i][j][k];
+  return a;
+}
+
+// Operator subclasses that (a) count per-cell evaluations, so the number of
+// operator sweeps (and hence solver iterations) can be recovered, and (b) make
+// operator~ the identity so BiCG can run on the symmetric (uniform-grid,
+// Actual/Fictive) masks it is tested on.  Arithmetic is the reference's own.
+static long long g_evals = 0;
+
+class CountingLaplace : public T3dLaplaceOperator {
+ public:
+  CountingLaplace(int model, int gc, const TNodesGrid* const* g)
+      : TRnLinearOperator(model), T3dLaplaceOperator(model, gc, g), model_(model), grid_(g[0]) {}
+  double operator()(int id, int i, int j, int k) const override {
+    ++g_evals;
+    return T3dLaplaceOperator::operator()(id, i, j, k);
+  }
+  void operator~() const override {}
+  TRnLinearOperator& Copy() const override {
+    const TNodesGrid* g[1] = {grid_};
+    return *(new CountingLaplace(model_, 1, g));
+  }
+  int** table() const { return fActivityTable; }
+
+ private:
+  int model_;
+  const TNodesGrid* grid_;
+};
+
+class CountingDivergented : public T3dVariableDivergentedOperator {
+ public:
+  CountingDivergented(int model, int gc, const TNodesGrid* const* g)
+      : TRnLinearOperator(model), T3dVariableDivergentedOperator(model, gc, g) {}
+  double operator()(int id, int i, int j, int k) const override {
+    ++g_evals;
+    return T3dVariableDivergentedOperator::operator()(id, i, j, k);
+  }
+};
+
+static int count_unknowns(const Setup& s) { return s.model->EndIndex; }
+
+// ----------------------------------------------------------- commands
+static void cmd_op(const Bag& in, Bag& out, bool divergented) {
+  Setup s = make_setup(in);
+  int n = s.N + 1, l = s.L + 1, m = s.M + 1;
+  const TNodesGrid* grids[1] = {s.grid};
+  TRnLinearOperator* A;
+  CountingLaplace* lap = nullptr;
+  if (divergented) {
+    CountingDivergented* d = new CountingDivergented(s.model_name, 1, grids);
+    TRnRelease3dSpace* rho = make_vec(s, in, "rho");
+    d->withDensity(*rho);
+    A = d;
+  } else {
+    lap = new CountingLaplace(s.model_name, 1, grids);
+    A = lap;
+  }
+  TRnRelease3dSpace& x = *make_vec(s, in, "xvec");
+  TRnRelease3dSpace& b = *make_vec(s, in, "bvec");
+  TRnRelease3dSpace& y = *make_vec(s, in, "yinit");
+  TRnRelease3dSpace& y2 = *make_vec(s, in, "yinit");
+  y |= (*A)(x);
+  y2 |= (*A)(x) - b;
+  double xax = ((*A)(x), x);
+  double xb = (x, b);
+  out["Ax"] = dump_vec(s, y);
+  out["Ax_minus_b"] = dump_vec(s, y2);
+  Arr sc = mk_d({2});
+  sc.d[0] = xax; sc.d[1] = xb;
+  out["dots"] = sc;
+  if (!divergented) {
+    // Rn numbering (inverse correspondence), activity table
+    int RN = count_unknowns(s);
+    Arr rn = mk_i({n, l, m});
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < l; j++)
+        for (int k = 0; k < m; k++) {
+          const TInverseCorrespondence& ic = s.model->GetInverseCorrespondence();
+          const T3dInverseCorrespondence& ic3 = dynamic_cast<const T3dInverseCorrespondence&>(ic);
+          rn.i[((int64_t)i * l + j) * m + k] = ic3[0][i][j][k];
+        }
+    out["rn"] = rn;
+    lap->ConstructActivityTables();
+    int** tab = lap->table();
+    Arr at = mk_i({RN, 8});
+    for (int r = 0; r < RN; r++) {
+      int cnt = tab[r][0];
+      at.i[r * 8 + 0] = cnt;
+      for (int q = 1; q <= cnt; q++) at.i[r * 8 + q] = tab[r][q];
+    }
+    out["act"] = at;
+    int idx = 0;
+    double amax = x.GetAbsMaxElement(idx);
+    Arr am = mk_d({1}); am.d[0] = amax; out["absmax"] = am;
+    Arr ai = mk_i({1}); ai.i[0] = idx; out["absmax_rn"] = ai;
+  }
+}
+
+static void cmd_solver(const Bag& in, Bag& out, const std::string& which) {
+  Setup s = make_setup(in);
+  const TNodesGrid* grids[1] = {s.grid};
+  CountingLaplace* A = new CountingLaplace(s.model_name, 1, grids);
+  TRnRelease3dSpace* x0 = make_vec(s, in, "x0");
+  TRnRelease3dSpace* b = make_vec(s, in, "bvec");
+  // TSLAE owns and deletes A, x, b (XSLAE.cpp:21-30); it is never destroyed here.
+  TSLAE* slae = new TSLAE(*A, *x0, *b);
+  g_evals = 0;
+  if (which == "cg") slae->SolveByConjugateGradients();
+  else if (which == "bicg") slae->SolveByBiConjugateGradients();
+  else if (which == "mr") slae->SolveByMinimalResiduals();
+  else if (which == "bicgstab") slae->SolveByBiConjugateGradientsStab();
+  else throw std::runtime_error("unknown solver");
+  long long RN = count_unknowns(s);
+  long long sweeps = RN ? g_evals / RN : 0;
+  out["x"] = dump_vec(s, const_cast<TRnSpace&>(slae->GetResult()));
+  Arr it = mk_i({2});
+  it.i[0] = (int)sweeps;
+  it.i[1] = (int)(g_evals % (RN ? RN : 1));
+  out["sweeps"] = it;
+}
+
+static void cmd_thomas(const Bag& in, Bag& out) {
+  const Arr& a = need(in, "a");
+  const Arr& b = need(in, "b");
+  const Arr& c = need(in, "c");
+  const Arr& f = need(in, "f");
+  Arr u = need(in, "u");
+  int M = (int)u.size() - 1;
+  TSLAE::SolveByScalarRunning(M, u.d.data(), a.d.data(), b.d.data(), c.d.data(), f.d.data());
+  out["u"] = u;
+}
+
+// Exposes the protected transport step.
+class LodDriver : public THydrodynamicsProblem {
+ public:
+  LodDriver(double tau) : THydrodynamicsProblem(1.0, tau, 1.0) {
+    fTimeSeparator = new TUniformSeparator(0, 1.0, tau);
+  }
+  void Solve() override {}
+  TNodesGrid* BuildGrid() override { return nullptr; }
+  void run(TRnRelease3dSpace& f, TRnRelease3dSpace& f1, TRnRelease3dSpace& u, TRnRelease3dSpace& v,
+           TRnRelease3dSpace& w, TRnRelease3dSpace& rp, const TNodesGrid& g, TRnRelease3dSpace& kap, int scheme) {
+    SolveTransferEquation__(f, f1, u, v, w, rp, g, kap, scheme);
+  }
+  double tau() const { return fTimeSeparator->SeparationValue; }
+};
+
+static void cmd_lod(const Bag& in, Bag& out) {
+  Setup s = make_setup(in);
+  LodDriver drv(scalar(in, "tau"));
+  TRnRelease3dSpace& f = *make_vec(s, in, "f");
+  TRnRelease3dSpace& f1 = *make_vec(s, in, "f_prev");
+  TRnRelease3dSpace& u = *make_vec(s, in, "u");
+  TRnRelease3dSpace& v = *make_vec(s, in, "v");
+  TRnRelease3dSpace& w = *make_vec(s, in, "w");
+  TRnRelease3dSpace& rp = *make_vec(s, in, "rp");
+  TRnRelease3dSpace& kap = *make_vec(s, in, "kappa");
+  drv.run(f, f1, u, v, w, rp, *s.grid, kap, iscalar(in, "scheme"));
+  out["f"] = dump_vec(s, f);
+  Arr t = mk_d({1}); t.d[0] = drv.tau(); out["tau_used"] = t;
+}
+
+class PlainBoundary : public TImmersedBoundary {
+ public:
+  double GetArea() override { return 0; }
+  double GetYCenter() override { return 0; }
+  double GetZCenter() override { return 0; }
+};
+
+class IbDriver : public TImmersedBoundaryProblem {
+ public:
+  IbDriver() : TImmersedBoundaryProblem(1.0, 1.0, 1.0, 1.0) {}
+  TMaskGenerator* CreateStatement() override { return nullptr; }
+  void set_boundary(TImmersedBoundary* b) { fBoundary = b; }
+  void interp(TRnRelease3dSpace& vel, T3dNormalGrid* g, int axis) { Interpolate(vel, g, axis, 0); }
+  void spread(TRnRelease3dSpace& f, T3dNormalGrid& gc, T3dNormalGrid* g, int axis) { SpreadForce(f, gc, g, axis, 0); }
+  int index(T3dNormalGrid* g, double c, int type, int axis) { return Index(g, c, type, axis, 3, 3); }
+};
+
+static void cmd_ib(const Bag& in, Bag& out) {
+  Setup s = make_setup(in, /*all_unknown=*/true);
+  int axis = iscalar(in, "axis");  // 1, 2, 3 = COORD_X, COORD_Y, COORD_Z
+  const Arr& X = need(in, "nodes");  // [n, 3]
+  const Arr& F = need(in, "force_n"); // [n]
+  int nn = (int)X.dims[0];
+  PlainBoundary* bnd = new PlainBoundary();
+  bnd->nodesCount = nn;
+  bnd->nodes = new TNode*[nn];
+  for (int q = 0; q < nn; q++) {
+    TNode* t = new TNode();
+    t->x = X.d[3 * q]; t->y = X.d[3 * q + 1]; t->z = X.d[3 * q + 2];
+    t->type = ELASTIC_NODE;
+    t->xForce = t->yForce = t->zForce = F.d[q];
+    t->xVel = t->yVel = t->zVel = 0;
+    bnd->nodes[q] = t;
+  }
+  // Allocated and never destroyed: the problem destructors expect a fully
+  // initialised problem (grids, statement) that this driver does not build.
+  IbDriver& drv = *new IbDriver();
+  drv.set_boundary(bnd);
+  TRnRelease3dSpace& vel = *make_vec(s, in, "vel");
+  TRnRelease3dSpace& force = *make_vec(s, in, "zero");
+  drv.interp(vel, s.grid, axis);
+  drv.spread(force, *s.grid, s.grid, axis);
+  Arr U = mk_d({nn});
+  Arr idx = mk_i({nn, 3});
+  for (int q = 0; q < nn; q++) {
+    TNode* t = bnd->nodes[q];
+    U.d[q] = axis == 1 ? t->xVel : axis == 2 ? t->yVel : t->zVel;
+    idx.i[3 * q + 0] = drv.index(s.grid, t->x, 1, axis);
+    idx.i[3 * q + 1] = drv.index(s.grid, t->y, 2, axis);
+    idx.i[3 * q + 2] = drv.index(s.grid, t->z, 3, axis);
+  }
+  out["U"] = U;
+  out["base"] = idx;
+  out["force"] = dump_vec(s, force);
+}
+
+int main(int argc, char** argv) {
+  if (argc != 4) {
+    fprintf(stderr, "usage: %s <command> <in.xfb> <out.xfb>\n", argv[0]);
+    return 2;
+  }
+  // The reference prints progress on stdout; keep stdout for that and
+  // report results only through the output file.
+  std::string cmd = argv[1];
+  try {
+    Bag in = read_xfb(argv[2]);
+    Bag out;
+    if (cmd == "op") cmd_op(in, out, false);
+    else if (cmd == "divop") cmd_op(in, out, true);
+    else if (cmd == "cg" || cmd == "bicg" || cmd == "mr" || cmd == "bicgstab") cmd_solver(in, out, cmd);
+    else if (cmd == "thomas") cmd_thomas(in, out);
+    else if (cmd == "lod") cmd_lod(in, out);
+    else if (cmd == "ib") cmd_ib(in, out);
+    else { fprintf(stderr, "unknown command %s\n", cmd.c_str()); return 2; }
+    write_xfb(argv[3], out);
+  } catch (const char* e) {
+    fprintf(stderr, "reference threw: %s\n", e);
+    return 3;
+  } catch (const std::exception& e) {
+    fprintf(stderr, "harness error: %s\n", e.what());
+    return 4;
+  }
+  return 0;
+}

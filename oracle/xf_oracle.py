@@ -1,0 +1,228 @@
+"""ctypes binding of liboracle.so, the CPU oracle — TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, always as the checker or the timed CPU baseline.  The product path
+(xflow_amd / libxflow_hip.so) never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+OK, EINVAL, EMAXIT, EBREAKDOWN, ENAN = 0, -1, -2, -3, -4
+
+
+class XoGrid(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("l", C.c_int), ("m", C.c_int),
+        ("hx", C.c_void_p), ("hy", C.c_void_p), ("hz", C.c_void_p),
+        ("codes", C.c_void_p), ("space_mask", C.c_void_p),
+        ("normals", C.c_void_p), ("has_normal", C.c_void_p), ("rho", C.c_void_p),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        d, i, p, i64 = C.c_double, C.c_int, C.c_void_p, C.c_int64
+        gp = C.POINTER(XoGrid)
+        sigs = {
+            "xo_set_threads": (None, [i]),
+            "xo_space_mask": (i, [i64, p, p]),
+            "xo_rn_number": (i64, [i, i, i, p, p]),
+            "xo_activity": (i, [i, i, i, p, p, p]),
+            "xo_apply": (i, [gp, p, p, d, p]),
+            "xo_apply_dot": (d, [gp, p, p]),
+            "xo_lincomb": (None, [gp, d, p, d, p, p]),
+            "xo_dot": (d, [gp, p, p]),
+            "xo_absmax": (d, [gp, p, p, p]),
+            "xo_cg3": (i, [gp, p, p, p, d, i, p, p]),
+            "xo_bicg": (i, [gp, p, p, p, d, i, p, p]),
+            "xo_mr": (i, [gp, p, p, p, d, i, p, p]),
+            "xo_bicgstab": (i, [gp, p, p, p, d, i, p, p]),
+            "xo_cg_hs": (i, [gp, p, p, p, d, i, i, p, p]),
+            "xo_thomas": (None, [i, p, p, p, p, p]),
+            "xo_lod": (i, [gp, p, p, p, p, p, p, p, d, i]),
+            "xo_ib_delta": (d, [d, d]),
+            "xo_ib_index": (i, [d, i, i, d, i]),
+            "xo_ib_interp": (None, [gp, i, i, p, p, p]),
+            "xo_ib_spread": (None, [gp, i, i, p, p, p]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Grid:
+    """Holds contiguous copies of a case's arrays and the XoGrid view of them."""
+
+    def __init__(self, hx, hy, hz, codes, space_mask=None, normals=None, has_normal=None, rho=None):
+        self.hx = np.ascontiguousarray(hx, dtype=np.float64)
+        self.hy = np.ascontiguousarray(hy, dtype=np.float64)
+        self.hz = np.ascontiguousarray(hz, dtype=np.float64)
+        self.codes = np.ascontiguousarray(codes, dtype=np.int32)
+        self.shape = self.codes.shape
+        if space_mask is None:
+            space_mask = space_mask_of(self.codes)
+        self.space_mask = np.ascontiguousarray(space_mask, dtype=np.int32)
+        self.normals = None if normals is None else np.ascontiguousarray(normals, dtype=np.float64)
+        self.has_normal = None if has_normal is None else np.ascontiguousarray(has_normal, dtype=np.int32)
+        self.rho = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
+        n, l, m = self.shape
+        self.s = XoGrid(n, l, m, _ptr(self.hx), _ptr(self.hy), _ptr(self.hz), _ptr(self.codes),
+                        _ptr(self.space_mask), _ptr(self.normals), _ptr(self.has_normal), _ptr(self.rho))
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def set_threads(n):
+    lib().xo_set_threads(int(n))
+
+
+def space_mask_of(codes):
+    codes = np.ascontiguousarray(codes, dtype=np.int32)
+    sm = np.zeros_like(codes)
+    st = lib().xo_space_mask(codes.size, codes.ctypes.data, sm.ctypes.data)
+    if st != OK:
+        raise ValueError("unknown point type in mask")
+    return sm
+
+
+def rn_number(space_mask):
+    sm = np.ascontiguousarray(space_mask, dtype=np.int32)
+    rn = np.zeros_like(sm)
+    n, l, m = sm.shape
+    cnt = lib().xo_rn_number(n, l, m, sm.ctypes.data, rn.ctypes.data)
+    if cnt < 0:
+        raise ValueError("space mask must be 0/1")
+    return rn, int(cnt)
+
+
+def activity(space_mask, rn, count):
+    sm = np.ascontiguousarray(space_mask, dtype=np.int32)
+    rn = np.ascontiguousarray(rn, dtype=np.int32)
+    act = np.zeros((count, 8), dtype=np.int32)
+    n, l, m = sm.shape
+    lib().xo_activity(n, l, m, sm.ctypes.data, rn.ctypes.data, act.ctypes.data)
+    return act
+
+
+def apply(g, x, b=None, sign_b=0.0, y=None):
+    x = _f64(x)
+    y = np.zeros_like(x) if y is None else _f64(y).copy()
+    bb = None if b is None else _f64(b)
+    st = lib().xo_apply(g.ref, x.ctypes.data, _ptr(bb), float(sign_b), y.ctypes.data)
+    if st != OK:
+        raise ValueError("operator undefined for a row of this mask")
+    return y
+
+
+def apply_dot(g, x):
+    x = _f64(x)
+    st = C.c_int(0)
+    v = lib().xo_apply_dot(g.ref, x.ctypes.data, C.byref(st))
+    if st.value != OK:
+        raise ValueError("operator undefined for a row of this mask")
+    return v
+
+
+def dot(g, a, b):
+    a, b = _f64(a), _f64(b)
+    return lib().xo_dot(g.ref, a.ctypes.data, b.ctypes.data)
+
+
+def absmax(g, x, rn):
+    x = _f64(x)
+    rn = np.ascontiguousarray(rn, dtype=np.int32)
+    idx = C.c_int32(0)
+    v = lib().xo_absmax(g.ref, x.ctypes.data, rn.ctypes.data, C.byref(idx))
+    return v, idx.value
+
+
+def _solve(fn, g, b, x0, eps, maxit, *extra):
+    b, x0 = _f64(b), _f64(x0)
+    x = np.zeros_like(x0)
+    it = C.c_int(0)
+    res = C.c_double(0)
+    st = fn(g.ref, b.ctypes.data, x0.ctypes.data, x.ctypes.data, float(eps), int(maxit), *extra,
+            C.byref(it), C.byref(res))
+    return x, it.value, res.value, st
+
+
+def cg3(g, b, x0, eps=1e-5, maxit=100000):
+    return _solve(lib().xo_cg3, g, b, x0, eps, maxit)
+
+
+def bicg(g, b, x0, eps=1e-8, maxit=100000):
+    return _solve(lib().xo_bicg, g, b, x0, eps, maxit)
+
+
+def mr(g, b, x0, eps=1e-7, maxit=1000000):
+    return _solve(lib().xo_mr, g, b, x0, eps, maxit)
+
+
+def bicgstab(g, b, x0, eps=1e-5, maxit=100000):
+    return _solve(lib().xo_bicgstab, g, b, x0, eps, maxit)
+
+
+def cg_hs(g, b, x0, rtol=1e-8, maxit=100000, project_mean=False):
+    return _solve(lib().xo_cg_hs, g, b, x0, rtol, maxit, int(bool(project_mean)))
+
+
+def thomas(u, a, b, c, f):
+    u = _f64(u).copy()
+    a, b, c, f = _f64(a), _f64(b), _f64(c), _f64(f)
+    lib().xo_thomas(len(u) - 1, u.ctypes.data, a.ctypes.data, b.ctypes.data, c.ctypes.data, f.ctypes.data)
+    return u
+
+
+def lod(g, f, f_prev, u, v, w, rp, kappa, tau, scheme):
+    f = _f64(f).copy()
+    arrs = [_f64(a) for a in (f_prev, u, v, w, rp, kappa)]
+    st = lib().xo_lod(g.ref, f.ctypes.data, *[a.ctypes.data for a in arrs], float(tau), int(scheme))
+    if st != OK:
+        raise ValueError("transport step rejected the mask (sector borders)")
+    return f
+
+
+def ib_delta(r, h):
+    return lib().xo_ib_delta(float(r), float(h))
+
+
+def ib_index(coord, typ, axis, h, end_index):
+    return lib().xo_ib_index(float(coord), int(typ), int(axis), float(h), int(end_index))
+
+
+def ib_interp(g, axis, xyz, vel):
+    xyz = _f64(xyz)
+    vel = _f64(vel)
+    U = np.zeros(xyz.shape[0])
+    lib().xo_ib_interp(g.ref, int(axis), xyz.shape[0], xyz.ctypes.data, vel.ctypes.data, U.ctypes.data)
+    return U
+
+
+def ib_spread(g, axis, xyz, F):
+    xyz, F = _f64(xyz), _f64(F)
+    out = np.zeros(g.shape)
+    lib().xo_ib_spread(g.ref, int(axis), xyz.shape[0], xyz.ctypes.data, F.ctypes.data, out.ctypes.data)
+    return out

@@ -1,0 +1,99 @@
+"""The CPU oracle against the compiled reference's golden vectors.
+
+Fixtures in tests/golden were written by tests/golden/make_golden.py from the
+reference compiled by oracle/Makefile.  Integer arrays must match exactly; the
+oracle repeats the reference's arithmetic in the same order, so floating-point
+outputs must match bit for bit too (np.array_equal).
+"""
+import numpy as np
+import pytest
+
+import xf_oracle as xo
+from conftest import golden
+
+
+def grid_from(ins, rho=False):
+    hx, hy, hz = np.diff(ins["x"]), np.diff(ins["y"]), np.diff(ins["z"])
+    return xo.Grid(hx, hy, hz, ins["codes"], ins.get("space_mask"), ins.get("normals"),
+                   ins.get("has_normal"), ins.get("rho") if rho else None)
+
+
+@pytest.mark.parametrize("name", ["op_mixed", "op_sphere16"])
+def test_masks_numbering_activity(name):
+    ins, outs = golden(name)
+    sm = xo.space_mask_of(ins["codes"])
+    assert np.array_equal(sm, ins["space_mask"])
+    rn, cnt = xo.rn_number(sm)
+    assert np.array_equal(rn, outs["rn"])
+    act = xo.activity(sm, rn, cnt)
+    assert np.array_equal(act, outs["act"])
+
+
+@pytest.mark.parametrize("name,rho", [("op_mixed", False), ("op_sphere16", False), ("divop_mixed", True)])
+def test_operator_rows_and_dots(name, rho):
+    ins, outs = golden(name)
+    g = grid_from(ins, rho)
+    y = xo.apply(g, ins["xvec"], y=ins["yinit"])
+    assert np.array_equal(y, outs["Ax"])
+    y2 = xo.apply(g, ins["xvec"], ins["bvec"], -1.0, y=ins["yinit"])
+    assert np.array_equal(y2, outs["Ax_minus_b"])
+    assert xo.apply_dot(g, ins["xvec"]) == outs["dots"][0]
+    assert xo.dot(g, ins["xvec"], ins["bvec"]) == outs["dots"][1]
+    if "absmax" in outs:
+        rn, _ = xo.rn_number(g.space_mask)
+        v, idx = xo.absmax(g, ins["xvec"], rn)
+        assert v == outs["absmax"][0] and idx == outs["absmax_rn"][0]
+
+
+# iterations from operator sweeps counted by the harness
+SWEEPS = {
+    "cg": lambda it: 3 + 2 * it,
+    "bicg": lambda it: 1 + 3 * it,
+    "mr": lambda it: 1 + 2 * it,
+    "bicgstab": lambda it: 1 + 2 * it,
+}
+
+
+@pytest.mark.parametrize("name,solver,eps", [
+    ("cg_dirichlet16", "cg", 1e-5), ("cg_neumann20", "cg", 1e-5), ("bicg_dirichlet12", "bicg", 1e-8),
+    ("mr_dirichlet8", "mr", 1e-7), ("bicgstab_dirichlet12", "bicgstab", 1e-5)])
+def test_solvers_bit_exact(name, solver, eps):
+    ins, outs = golden(name)
+    g = grid_from(ins)
+    fn = {"cg": xo.cg3, "bicg": xo.bicg, "mr": xo.mr, "bicgstab": xo.bicgstab}[solver]
+    x, it, res, st = fn(g, ins["bvec"], ins["x0"], eps)
+    assert st == xo.OK
+    assert outs["sweeps"][1] == 0
+    assert SWEEPS[solver](it) == outs["sweeps"][0]
+    assert np.array_equal(x, outs["x"])
+
+
+def test_thomas():
+    ins, outs = golden("thomas24")
+    u = xo.thomas(ins["u"], ins["a"], ins["b"], ins["c"], ins["f"])
+    assert np.array_equal(u, outs["u"])
+
+
+@pytest.mark.parametrize("name", ["lod_scheme0", "lod_scheme1"])
+def test_lod_transport(name):
+    ins, outs = golden(name)
+    g = grid_from(ins)
+    f = xo.lod(g, ins["f"], ins["f_prev"], ins["u"], ins["v"], ins["w"], ins["rp"], ins["kappa"],
+               float(ins["tau"][0]), int(ins["scheme"][0]))
+    assert outs["tau_used"][0] == ins["tau"][0]
+    assert np.array_equal(f, outs["f"])
+
+
+@pytest.mark.parametrize("axis", [1, 2, 3])
+def test_ib_interp_spread(axis):
+    ins, outs = golden("ib_axis%d" % axis)
+    g = grid_from(ins)
+    nodes = ins["nodes"]
+    h = np.diff(ins["x"])[0]
+    E = len(ins["x"]) - 1
+    base = np.array([[xo.ib_index(nodes[q, t], t + 1, axis, h, E) for t in range(3)] for q in range(len(nodes))])
+    assert np.array_equal(base, outs["base"])
+    U = xo.ib_interp(g, axis, nodes, ins["vel"])
+    assert np.array_equal(U, outs["U"])
+    F = xo.ib_spread(g, axis, nodes, ins["force_n"])
+    assert np.array_equal(F, outs["force"])
