@@ -1,0 +1,145 @@
+"""Pressure-solve benchmark: ms per solve to 1e-8 relative residual at 256^3.
+
+A step is one pressure solve of the 256^3 random-obstacle Poisson workload
+(seed 0: 1-4 solid spheres/boxes, smoothed random MAC divergence as the
+right-hand side, Neumann walls) with inputs resident in HBM.  The JSON line
+reports the plain FP64 GPU CG and, where built, the mixed-precision solver.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(case, gpu_iters, sample_iters):
+    """The CPU oracle's FP64 CG (OpenMP stencil, serial dot products), timed
+    for a bounded number of iterations and scaled to the GPU's iteration count."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import xf_oracle as xo
+    threads = min(16, os.cpu_count() or 1)
+    xo.set_threads(threads)
+    og = xo.Grid(case["hx"], case["hy"], case["hz"], case["codes"])
+    x0 = np.zeros_like(case["b"])
+    t0 = time.perf_counter()
+    _, it, _, _ = xo.cg_hs(og, case["b"], x0, 1e-30, sample_iters, project_mean=True)
+    dt = time.perf_counter() - t0
+    per_iter_ms = dt * 1e3 / max(it, 1)
+    return {"value": per_iter_ms * gpu_iters, "unit": "ms/solve", "cores": threads, "kind": "port",
+            "sample": "%d iterations of the oracle FP64 CG (xo_cg_hs, %d OpenMP threads for the stencil, serial "
+                      "dot products) at the same 256^3 case, %.1f ms/iteration, scaled by the GPU solve's %d "
+                      "iterations" % (it, threads, per_iter_ms, gpu_iters)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=12)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box in this setup has a single MI355X; "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import xflow_amd as xf
+    from xflow_amd import cases, profile
+
+    t0 = time.perf_counter()
+    case = cases.obstacle_case(args.n, seed=args.seed)
+    log("case generated in %.1f s: %d fluid cells" % (time.perf_counter() - t0, int((case["codes"] == 1).sum())))
+    ctx = xf.Context(0)
+    g = xf.Grid(ctx, case["hx"], case["hy"], case["hz"], case["codes"])
+    b = torch.from_numpy(case["b"]).cuda()
+    x0 = torch.zeros_like(b)
+    x = torch.zeros_like(b)
+    fluid = int((case["codes"] == 1).sum())
+
+    def solve(method, **kw):
+        o = xf.solve_opts(method, tol=1e-8, tol_mode=xf.XF_TOL_REL, project_mean=1, **kw)
+        return g.solve(b, x0, x, o)
+
+    for _ in range(args.warmup):
+        st = solve(xf.XF_CG)
+        log("warmup: %d iters, %.2f ms, true resid %.2e" % (st.iters, st.ms, st.resid_true))
+    # timed region: K plain-FP64 solves, per-kernel HIP events on the library's stream
+    profile.enable(ctx, True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = []
+    for _ in range(args.steps):
+        st = solve(xf.XF_CG)
+        iters.append(st.iters)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    kstats = profile.read(ctx)
+    profile.enable(ctx, False)
+    assert st.status == 0, st.status
+    log("plain FP64 CG: %.2f ms/solve wall, %s iters, true resid %.2e" % (wall_ms, iters, st.resid_true))
+
+    interior = (args.n) ** 3
+    ncell = (args.n + 2) ** 3
+    k1 = kstats.get("cg_stencil_fp64", {"ms": float("nan"), "count": 0})
+    k2 = kstats.get("cg_update_fp64", {"ms": float("nan"), "count": 0})
+    k1_bytes = 33.0 * interior
+    k2_bytes = 48.0 * ncell
+    k1_gbs = k1_bytes / (k1["ms"] * 1e-3) / 1e9
+    k2_gbs = k2_bytes / (k2["ms"] * 1e-3) / 1e9
+    it_mean = float(np.mean(iters))
+    mcups = fluid * it_mean / (wall_ms * 1e-3) / 1e6
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(case, int(it_mean), args.cpu_iters)
+
+    out = {
+        "metric": "ms per pressure solve to 1e-8 rel. residual at 256³ (plain vs NN-precond MP)",
+        "value": round(wall_ms, 3),
+        "unit": "ms/solve",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "pressure Poisson %d^3, random obstacles seed %d, FP64 CG to 1e-8 relative" %
+                               (args.n, args.seed),
+                   "solver": "plain FP64 CG (HS, fused kernels)", "iterations": it_mean,
+                   "fluid_cells": fluid, "mcell_updates_per_s": round(mcups, 1),
+                   "true_residual": st.resid_true},
+        "roofline": {"bound": "hbm", "achieved": round(k1_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(k1_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_stencil_cg<double,true> (33 B per interior cell)",
+                     "kernel_ms": k1["ms"], "update_kernel_gbs": round(k2_gbs, 1),
+                     "update_kernel_ms": k2["ms"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    g.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

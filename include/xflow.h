@@ -1,0 +1,155 @@
+/*
+ * xflow.h — C ABI of libxflow_hip.so, the MI355X (gfx950) projection-step hot path.
+ *
+ * Plain C: opaque handles, device pointers, sizes, status codes.  Every entry
+ * point returns 0 on success or a negative XF_E* code; xf_last_error() gives
+ * the message.  The C++ drop-in classes (xflow_amd/host) turn a non-zero code
+ * into `throw "<message>"`, the reference's error convention (XFlow.cpp:78-84).
+ *
+ * Data layout.  A grid has n x l x m nodes per axis (n = EndIndex_x + 1 ...,
+ * the array extents of the reference's T3dNumberMask, XMask.h:90-131).  Every
+ * field is one flat device array of n*l*m elements with k fastest,
+ * idx = (i*l + j)*m + k: the reference's [i][j][k] order.  Unknowns are the
+ * points whose space mask is 1 (XProblem.cpp:4155-4182); vector operations
+ * act on unknowns only and leave other entries untouched, as the reference's
+ * masked loops do (XSpace.cpp:2293-3487).
+ *
+ * Threading: one xf_ctx per host thread / HIP stream.  Calls on one context
+ * are stream-ordered; calls that return a host scalar synchronise the stream.
+ */
+#ifndef XFLOW_H
+#define XFLOW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XF_OK 0
+#define XF_EINVAL -1      /* bad argument, or the reference would throw     */
+#define XF_EMAXIT -2      /* iteration cap reached before the tolerance      */
+#define XF_EBREAKDOWN -3  /* Krylov breakdown (zero denominator)            */
+#define XF_ENAN -4        /* NaN/Inf residual                                */
+#define XF_EHIP -5        /* HIP runtime error                              */
+#define XF_ENOMEM -6      /* device allocation failed                        */
+#define XF_ESTALL -7      /* preconditioned solve stalled (fallback taken)   */
+
+typedef struct xf_ctx_s *xf_ctx;
+typedef struct xf_grid_s *xf_grid;
+
+/* ------------------------------------------------------------- context */
+/* No reference counterpart: the reference is single-process CPU code. */
+int xf_init(int device, xf_ctx *out);
+int xf_shutdown(xf_ctx ctx);
+const char *xf_last_error(xf_ctx ctx);
+int xf_set_stream(xf_ctx ctx, void *hip_stream); /* NULL: the context's own stream */
+int xf_synchronize(xf_ctx ctx);
+int xf_malloc(xf_ctx ctx, size_t bytes, void **dptr);
+int xf_free(xf_ctx ctx, void *dptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device; synchronous. */
+int xf_memcpy(xf_ctx ctx, void *dst, const void *src, size_t bytes, int kind);
+
+/* ---------------------------------------------------------------- grid */
+/* Replaces T3dNormalGrid(sep1, sep2, sep3, mask, normals) (XGrid.h:282,
+ * XGrid.cpp:414-441) together with TRnRelease3dSpaceModel(1, &spaceMask)
+ * (XSpaceModel.h:502, XSpaceModel.cpp:276-411) built from the space-mask
+ * rule of XProblem.cpp:4155-4182.  hx[n-1], hy[l-1], hz[m-1] are the
+ * separator steps (TSeparator::Separation, XSeparator.cpp:25-99); codes are
+ * the point types 0..8; normals (n*l*m*3, outward, axis-aligned where the
+ * operator needs them) and has_normal may be NULL.  Host pointers. */
+int xf_grid_create(xf_ctx ctx, int n, int l, int m, const double *hx, const double *hy, const double *hz,
+                   const int32_t *codes, const double *normals, const int32_t *has_normal, xf_grid *out);
+int xf_grid_destroy(xf_ctx ctx, xf_grid g);
+/* ncell = n*l*m; n_unknowns = TRnSpaceModel::EndIndex (XSpaceModel.cpp:330-358). */
+int xf_grid_info(xf_grid g, int64_t *ncell, int64_t *n_unknowns);
+/* Parity dumps computed on the device (host buffers, each may be NULL):
+ * space mask [ncell], Rn index [ncell] (1-based, 0 = not an unknown,
+ * XSpaceModel.cpp:381-406), activity table [n_unknowns*8]
+ * (T3dLaplaceOperator::ConstructActivityTables, XOperator.cpp:837-936). */
+int xf_grid_export(xf_ctx ctx, xf_grid g, int32_t *space_mask, int32_t *rn, int32_t *act);
+/* Switches the grid's operator to T3dVariableDivergentedOperator with the
+ * given density (device array [ncell]); NULL switches back to
+ * T3dLaplaceOperator.  Replaces withDensity (XOperator.cpp:2235-2268); the
+ * density is copied into a buffer the grid reuses (no per-call leak). */
+int xf_grid_set_density(xf_ctx ctx, xf_grid g, const double *rho);
+
+/* ----------------------------------------------- operator and vectors */
+/* y = A x + sign_b * b at unknowns (sign_b in {0, +1, -1}; b may be NULL
+ * when sign_b = 0).  Rows follow T3dLaplaceOperator::operator()
+ * (XOperator.cpp:938-1129) or the divergented operator
+ * (XOperator.cpp:1901-2233), in the reference's arithmetic order; rows the
+ * reference rejects give XF_EINVAL.  Replaces `y |= A(x) - b`
+ * (XSpace.cpp:3013-3217). */
+int xf_apply(xf_ctx ctx, xf_grid g, const double *x, const double *b, double sign_b, double *y);
+/* (A x, x) without storing A x (XSpace.cpp:3344-3419); *out is a host scalar. */
+int xf_apply_dot(xf_ctx ctx, xf_grid g, const double *x, double *out);
+/* res = p1*s1 + p2*s2 at unknowns (XSpace.cpp:2489-3010). */
+int xf_lincomb(xf_ctx ctx, xf_grid g, double p1, const double *s1, double p2, const double *s2, double *res);
+/* res |= s at unknowns (XSpace.cpp:2293-2363). */
+int xf_copy(xf_ctx ctx, xf_grid g, const double *s, double *res);
+/* (a, b) over unknowns (XSpace.cpp:3261-3342); host scalar. */
+int xf_dot(xf_ctx ctx, xf_grid g, const double *a, const double *b, double *out);
+/* max |x| over unknowns and the Rn index of its first occurrence
+ * (TRnSpace::GetAbsMaxElement, XSpace.cpp:961-973, 3428-3487). */
+int xf_absmax(xf_ctx ctx, xf_grid g, const double *x, double *val, int32_t *rn_index);
+
+/* --------------------------------------------------------------- solvers */
+#define XF_CG3 0       /* TSLAE::SolveByConjugateGradients, XSLAE.cpp:56-157      */
+#define XF_BICG 1      /* TSLAE::SolveByBiConjugateGradients, XSLAE.cpp:159-348   */
+#define XF_MR 2        /* TSLAE::SolveByMinimalResiduals, XSLAE.cpp:350-414      */
+#define XF_BICGSTAB 3  /* TSLAE::SolveByBiConjugateGradientsStab, 667-767        */
+#define XF_CG 4        /* Hestenes-Stiefel CG, fused kernels (throughput path)     */
+#define XF_CG_MP 5     /* FP32 inner CG + FP64 iterative refinement              */
+#define XF_FCG_NN 6    /* flexible CG with the learned preconditioner             */
+
+#define XF_TOL_ABS 0   /* stop on ||r|| <= tol (the reference's rule)            */
+#define XF_TOL_REL 1   /* stop on ||r|| <= tol * ||b - A x0||                      */
+
+typedef struct {
+  int method;        /* XF_CG3 ...                                             */
+  int tol_mode;      /* XF_TOL_ABS or XF_TOL_REL                                */
+  double tol;        /* the reference's hard-coded values: CG 1e-5, BiCG 1e-8,
+                        MR 1e-7 (2-norm), BiCGStab 1e-5 (max-norm),
+                        XSLAE.cpp:64, 166, 358, 669                              */
+  int maxit;         /* iteration cap (the reference has none)                   */
+  int check_every;   /* throughput methods: host convergence poll period        */
+  int project_mean;  /* remove the mean over unknowns (pure-Neumann systems)     */
+  double inner_tol;  /* XF_CG_MP / XF_FCG_NN: relative tolerance of inner sweeps */
+  int stall_window;  /* XF_FCG_NN: fall back to plain CG after this many
+                        iterations without progress (20)                         */
+  void *precond;     /* XF_FCG_NN: an xf_unet handle                             */
+} xf_solve_opts;
+
+typedef struct {
+  int iters;          /* outer iterations (CG3/BiCG/MR/BiCGStab: the reference's
+                         iteration counter)                                     */
+  int inner_iters;    /* XF_CG_MP: total FP32 iterations                         */
+  int status;         /* XF_OK or an XF_E* code                                  */
+  int fallback;       /* XF_FCG_NN: 1 if plain CG finished the solve             */
+  double resid;       /* final residual in the method's own stopping norm         */
+  double resid_true;  /* ||b - A x||_2 / ||b - A x0||_2 recomputed in FP64       */
+  double ms;          /* device time of the solve (HIP events)                   */
+} xf_solve_stats;
+
+void xf_solve_opts_default(int method, xf_solve_opts *o);
+/* Solves A x = b from x0 (entries of x0 outside the unknowns are kept in x,
+ * as the reference's vector copies keep them).  Device pointers. */
+int xf_solve(xf_ctx ctx, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts *opts,
+             xf_solve_stats *stats);
+
+/* ------------------------------------------------------------ profiling */
+/* No reference counterpart (the reference only prints clock() deltas,
+ * XProblem.cpp:5442-5445).  When enabled, the throughput solvers bracket
+ * each kernel launch with HIP events on the context's stream; read returns
+ * the average device time per launch of a named kernel ("cg_stencil_fp64",
+ * "cg_update_fp64", "cg_stencil_fp32", "cg_update_fp32") since the last reset. */
+int xf_profile_enable(xf_ctx ctx, int on);
+int xf_profile_reset(xf_ctx ctx);
+int xf_profile_read(xf_ctx ctx, const char *kernel, double *avg_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XFLOW_H */

@@ -1,0 +1,173 @@
+"""xflow_amd — MI355X (gfx950) projection-step hot path of xFlow.
+
+Python face of libxflow_hip.so for tests, benchmarks and drivers.  Device
+memory comes from torch (ROCm); every call goes through the C ABI in
+include/xflow.h.  The C++ drop-in classes for reference-style host code live
+in xflow_amd/host.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (XF_BICG, XF_BICGSTAB, XF_CG, XF_CG3, XF_CG_MP, XF_FCG_NN, XF_MR, XF_OK, XF_TOL_ABS,
+                   XF_TOL_REL, SolveOpts, SolveStats, XfError, lib)
+
+__all__ = ["Context", "Grid", "XfError", "XF_CG3", "XF_BICG", "XF_MR", "XF_BICGSTAB", "XF_CG", "XF_CG_MP",
+           "XF_FCG_NN", "XF_TOL_ABS", "XF_TOL_REL", "solve_opts"]
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def _host(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, a.ctypes.data
+
+
+class Context:
+    """One xf_ctx (device + stream).  Use one per host thread."""
+
+    def __init__(self, device=0):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("xflow_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+        torch.cuda.set_device(device)
+        self.device = device
+        h = C.c_void_p()
+        self._check(lib().xf_init(device, C.byref(h)), "xf_init", None)
+        self.h = h
+        self.torch_device = torch.device("cuda", device)
+        # run on torch's current stream so torch allocations and our kernels are ordered
+        self.set_stream(torch.cuda.current_stream(device).cuda_stream)
+
+    def _check(self, st, what, h=None):
+        if st != XF_OK:
+            msg = lib().xf_last_error(h if h is not None else getattr(self, "h", None))
+            raise XfError(st, "%s: %s" % (what, msg.decode() if msg else ""))
+
+    def check(self, st, what):
+        self._check(st, what, self.h)
+
+    def set_stream(self, stream_handle):
+        self.check(lib().xf_set_stream(self.h, C.c_void_p(stream_handle)), "xf_set_stream")
+
+    def synchronize(self):
+        self.check(lib().xf_synchronize(self.h), "xf_synchronize")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().xf_shutdown(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Grid:
+    """A T3dNormalGrid + space model on the device (xf_grid)."""
+
+    def __init__(self, ctx, hx, hy, hz, codes, normals=None, has_normal=None):
+        self.ctx = ctx
+        codes = np.asarray(codes)
+        self.shape = codes.shape
+        n, l, m = self.shape
+        hx, phx = _host(hx, np.float64)
+        hy, phy = _host(hy, np.float64)
+        hz, phz = _host(hz, np.float64)
+        assert hx.size == n - 1 and hy.size == l - 1 and hz.size == m - 1, "steps must have n-1, l-1, m-1 entries"
+        cd, pcd = _host(codes, np.int32)
+        pn = ph = None
+        if normals is not None:
+            nv, pn = _host(np.asarray(normals).reshape(-1, 3), np.float64)
+            assert nv.shape[0] == cd.size
+        if has_normal is not None:
+            hv, ph = _host(has_normal, np.int32)
+            assert hv.size == cd.size
+        g = C.c_void_p()
+        ctx.check(lib().xf_grid_create(ctx.h, n, l, m, phx, phy, phz, pcd, pn, ph, C.byref(g)), "xf_grid_create")
+        self.h = g
+        nc, nu = C.c_int64(), C.c_int64()
+        lib().xf_grid_info(g, C.byref(nc), C.byref(nu))
+        self.ncell, self.n_unknowns = nc.value, nu.value
+
+    def export(self, activity=True):
+        sm = np.zeros(self.shape, dtype=np.int32)
+        rn = np.zeros(self.shape, dtype=np.int32)
+        act = np.zeros((self.n_unknowns, 8), dtype=np.int32) if activity else None
+        self.ctx.check(lib().xf_grid_export(self.ctx.h, self.h, sm.ctypes.data, rn.ctypes.data,
+                                            act.ctypes.data if activity else None), "xf_grid_export")
+        return sm, rn, act
+
+    def set_density(self, rho):
+        self.ctx.check(lib().xf_grid_set_density(self.ctx.h, self.h, _ptr(rho)), "xf_grid_set_density")
+
+    def _vec(self, t, dtype=None):
+        import torch
+        assert t.is_cuda and t.is_contiguous() and t.numel() == self.ncell, "expected a contiguous device field"
+        assert t.dtype == (dtype or torch.float64)
+        return _ptr(t)
+
+    def apply(self, x, y, b=None, sign_b=0.0):
+        self.ctx.check(lib().xf_apply(self.ctx.h, self.h, self._vec(x), self._vec(b) if b is not None else None,
+                                      float(sign_b), self._vec(y)), "xf_apply")
+        return y
+
+    def apply_dot(self, x):
+        out = C.c_double()
+        self.ctx.check(lib().xf_apply_dot(self.ctx.h, self.h, self._vec(x), C.byref(out)), "xf_apply_dot")
+        return out.value
+
+    def lincomb(self, p1, s1, p2, s2, res):
+        self.ctx.check(lib().xf_lincomb(self.ctx.h, self.h, float(p1), self._vec(s1) if s1 is not None else None,
+                                        float(p2), self._vec(s2) if s2 is not None else None, self._vec(res)),
+                       "xf_lincomb")
+        return res
+
+    def copy(self, s, res):
+        self.ctx.check(lib().xf_copy(self.ctx.h, self.h, self._vec(s), self._vec(res)), "xf_copy")
+        return res
+
+    def dot(self, a, b):
+        out = C.c_double()
+        self.ctx.check(lib().xf_dot(self.ctx.h, self.h, self._vec(a), self._vec(b), C.byref(out)), "xf_dot")
+        return out.value
+
+    def absmax(self, x):
+        v, idx = C.c_double(), C.c_int32()
+        self.ctx.check(lib().xf_absmax(self.ctx.h, self.h, self._vec(x), C.byref(v), C.byref(idx)), "xf_absmax")
+        return v.value, idx.value
+
+    def solve(self, b, x0, x, opts):
+        st = SolveStats()
+        rc = lib().xf_solve(self.ctx.h, self.h, self._vec(b), self._vec(x0), self._vec(x), C.byref(opts),
+                            C.byref(st))
+        if rc not in (XF_OK,) and st.status == XF_OK:
+            self.ctx.check(rc, "xf_solve")
+        return st
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().xf_grid_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_opts(method, **kw):
+    o = SolveOpts()
+    lib().xf_solve_opts_default(method, C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o

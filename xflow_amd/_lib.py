@@ -1,0 +1,82 @@
+"""ctypes binding of libxflow_hip.so (the C ABI declared in include/xflow.h).
+
+The library is built in-tree by __graft_entry__.build() (make -C xflow_amd/csrc).
+Loading fails loudly when it is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxflow_hip.so")
+
+XF_OK, XF_EINVAL, XF_EMAXIT, XF_EBREAKDOWN, XF_ENAN, XF_EHIP, XF_ENOMEM, XF_ESTALL = 0, -1, -2, -3, -4, -5, -6, -7
+XF_CG3, XF_BICG, XF_MR, XF_BICGSTAB, XF_CG, XF_CG_MP, XF_FCG_NN = range(7)
+XF_TOL_ABS, XF_TOL_REL = 0, 1
+
+STATUS_NAMES = {0: "ok", -1: "invalid", -2: "max-iterations", -3: "breakdown", -4: "nan", -5: "hip-error",
+                -6: "out-of-memory", -7: "stalled"}
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("method", C.c_int), ("tol_mode", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
+                ("check_every", C.c_int), ("project_mean", C.c_int), ("inner_tol", C.c_double),
+                ("stall_window", C.c_int), ("precond", C.c_void_p)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("inner_iters", C.c_int), ("status", C.c_int), ("fallback", C.c_int),
+                ("resid", C.c_double), ("resid_true", C.c_double), ("ms", C.c_double)]
+
+
+# name -> (restype, argtypes); the export test checks this table against include/xflow.h.
+P, I, I64, D, SZ = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_size_t
+SIGNATURES = {
+    "xf_init": (I, [I, C.POINTER(P)]),
+    "xf_shutdown": (I, [P]),
+    "xf_last_error": (C.c_char_p, [P]),
+    "xf_set_stream": (I, [P, P]),
+    "xf_synchronize": (I, [P]),
+    "xf_malloc": (I, [P, SZ, C.POINTER(P)]),
+    "xf_free": (I, [P, P]),
+    "xf_memcpy": (I, [P, P, P, SZ, I]),
+    "xf_grid_create": (I, [P, I, I, I, P, P, P, P, P, P, C.POINTER(P)]),
+    "xf_grid_destroy": (I, [P, P]),
+    "xf_grid_info": (I, [P, C.POINTER(I64), C.POINTER(I64)]),
+    "xf_grid_export": (I, [P, P, P, P, P]),
+    "xf_grid_set_density": (I, [P, P, P]),
+    "xf_apply": (I, [P, P, P, P, D, P]),
+    "xf_apply_dot": (I, [P, P, P, C.POINTER(D)]),
+    "xf_lincomb": (I, [P, P, D, P, D, P, P]),
+    "xf_copy": (I, [P, P, P, P]),
+    "xf_dot": (I, [P, P, P, P, C.POINTER(D)]),
+    "xf_absmax": (I, [P, P, P, C.POINTER(D), C.POINTER(C.c_int32)]),
+    "xf_solve_opts_default": (None, [I, C.POINTER(SolveOpts)]),
+    "xf_solve": (I, [P, P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "xf_profile_enable": (I, [P, I]),
+    "xf_profile_reset": (I, [P]),
+    "xf_profile_read": (I, [P, C.c_char_p, C.POINTER(D), C.POINTER(I64)]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Loads libxflow_hip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libxflow_hip.so is not built: run __graft_entry__.build() "
+                               "(make -C xflow_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+class XfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%s)" % (msg, STATUS_NAMES.get(code, code)))
+        self.code = code

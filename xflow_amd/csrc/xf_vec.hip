@@ -1,0 +1,395 @@
+// Reference-row operator and masked vector kernels (parity path).
+//
+// The operator rows repeat T3dLaplaceOperator::operator() (XOperator.cpp:938-1129)
+// and T3dVariableDivergentedOperator::operator() (XOperator.cpp:1901-2233) with
+// the reference's operand order; the library is built with -ffp-contract=off
+// and f64 division on gfx950 is correctly rounded, so each row is bit-identical
+// to the reference's.  Reductions are deterministic (fixed block partition,
+// fixed-order combination of partials) but sum in a different order than the
+// reference's serial i->j->k loop.
+#include <cmath>
+
+#include "xf_internal.h"
+
+struct GridView {
+  int n, l, m;
+  const double *hx, *hy, *hz;
+  const uint8_t *codes, *space;
+  const double *nrm;
+  const uint8_t *hasn;
+  const double *rho;  // null: Laplace operator
+  const int32_t *rn;
+};
+
+static GridView view(xf_grid g) {
+  return GridView{g->n, g->l, g->m, g->d_hx, g->d_hy, g->d_hz, g->d_codes, g->d_space,
+                  g->d_normals, g->d_has_normal, g->use_rho ? g->d_rho : nullptr, g->d_rn};
+}
+
+// One operator row at an unknown cell (validity was established at grid set-up).
+__device__ inline double row_value(const GridView &g, const double *__restrict__ f, int64_t c, int i, int j,
+                                   int k) {
+  const uint8_t *mk = g.codes;
+  const double *hx = g.hx, *hy = g.hy, *hz = g.hz;
+  const int64_t sI = (int64_t)g.l * g.m, sJ = g.m;
+  const uint8_t code = mk[c];
+  if (code == PT_ACTUAL) {
+    const double hx_2X = (hx[i - 1] + hx[i]) / 2;
+    const double hx_2Y = (hy[j - 1] + hy[j]) / 2;
+    const double hx_2Z = (hz[k - 1] + hz[k]) / 2;
+    const double fc = f[c];
+    if (!g.rho) {
+      double f_i_1 = (fc - f[c - sI]) / (hx[i - 1]);
+      double f_i = (f[c + sI] - fc) / (hx[i]);
+      double f_j_1 = (fc - f[c - sJ]) / (hy[j - 1]);
+      double f_j = (f[c + sJ] - fc) / (hy[j]);
+      double f_k_1 = (fc - f[c - 1]) / (hz[k - 1]);
+      double f_k = (f[c + 1] - fc) / (hz[k]);
+      if (mk[c - sI] == PT_FICTIVE) f_i_1 = 0;
+      if (mk[c + sI] == PT_FICTIVE) f_i = 0;
+      if (mk[c - sJ] == PT_FICTIVE) f_j_1 = 0;
+      if (mk[c + sJ] == PT_FICTIVE) f_j = 0;
+      if (mk[c - 1] == PT_FICTIVE) f_k_1 = 0;
+      if (mk[c + 1] == PT_FICTIVE) f_k = 0;
+      return -((f_i - f_i_1) / hx_2X + (f_j - f_j_1) / hx_2Y + (f_k - f_k_1) / hx_2Z);
+    }
+    const double *rho = g.rho;
+    double f_i_1 = rho[c - sI] * (fc - f[c - sI]) / (hx[i - 1]);
+    double f_i = rho[c + sI] * (f[c + sI] - fc) / (hx[i]);
+    double f_j_1 = rho[c - sJ] * (fc - f[c - sJ]) / (hy[j - 1]);
+    double f_j = rho[c + sJ] * (f[c + sJ] - fc) / (hy[j]);
+    double f_k_1 = rho[c - 1] * (fc - f[c - 1]) / (hz[k - 1]);
+    double f_k = rho[c + 1] * (f[c + 1] - fc) / (hz[k]);
+    double d_i = (rho[c + sI] - rho[c]) / (hx[i - 1]);
+    double d_j = (rho[c + sJ] - rho[c]) / (hy[j - 1]);
+    double d_k = (rho[c + 1] - rho[c]) / (hz[k - 1]);
+    if (mk[c - sI] == PT_FICTIVE) f_i_1 = 0;
+    if (mk[c + sI] == PT_FICTIVE) { f_i = 0; d_i = 0; }
+    if (mk[c - sJ] == PT_FICTIVE) f_j_1 = 0;
+    if (mk[c + sJ] == PT_FICTIVE) { f_j = 0; d_j = 0; }
+    if (mk[c - 1] == PT_FICTIVE) f_k_1 = 0;
+    if (mk[c + 1] == PT_FICTIVE) { f_k = 0; d_k = 0; }
+    double lap = (f_i - f_i_1) / hx_2X + (f_j - f_j_1) / hx_2Y + (f_k - f_k_1) / hx_2Z;
+    double divd = d_i + d_j + d_k;
+    return -(rho[c] * lap - divd * fc);  // XOperator.cpp:2025
+  }
+  const double nx = g.nrm[3 * c], ny = g.nrm[3 * c + 1], nz = g.nrm[3 * c + 2];
+  if (code == PT_PREDEFINED) {  // XOperator.cpp:1057-1068
+    double s = 0;
+    if (nx > 0) s = (f[c] + f[c - sI]) / 2;
+    else if (nx < 0) s = (f[c + sI] + f[c]) / 2;
+    if (ny > 0) s = (f[c] + f[c - sJ]) / 2;
+    else if (ny < 0) s = (f[c + sJ] + f[c]) / 2;
+    if (nz > 0) s = (f[c] + f[c - 1]) / 2;
+    else if (nz < 0) s = (f[c + 1] + f[c]) / 2;
+    return s;
+  }
+  // PreNormal (XOperator.cpp:1083-1118, divergented 2113-2224)
+  int sI_ = 0, sI_1 = 0, sJ_ = 0, sJ_1 = 0, sK_ = 0, sK_1 = 0;
+  double hxl = 1, hyl = 1, hzl = 1;
+  if (nx > 0) { sI_ = 0; sI_1 = -1; hxl = hx[i - 1]; }
+  else if (nx < 0) { sI_ = 1; sI_1 = 0; hxl = hx[i]; }
+  if (ny > 0) { sJ_ = 0; sJ_1 = -1; hyl = hy[j - 1]; }
+  else if (ny < 0) { sJ_ = 1; sJ_1 = 0; hyl = hy[j]; }
+  if (nz > 0) { sK_ = 0; sK_1 = -1; hzl = hz[k - 1]; }
+  else if (nz < 0) { sK_ = 1; sK_1 = 0; hzl = hz[k]; }
+#define FV(a, b, e) f[c + (int64_t)(a) * sI + (int64_t)(b) * sJ + (e)]
+  double fl = (FV(sI_, 0, 0) + FV(sI_, sJ_, 0) + FV(sI_, 0, sK_) + FV(sI_, sJ_, sK_)) / 4;
+  double fl_1 = (FV(sI_1, 0, 0) + FV(sI_1, sJ_, 0) + FV(sI_1, 0, sK_) + FV(sI_1, sJ_, sK_)) / 4;
+  double dfx = (fl - fl_1) / hxl;
+  fl = (FV(0, sJ_, 0) + FV(sI_, sJ_, 0) + FV(0, sJ_, sK_) + FV(sI_, sJ_, sK_)) / 4;
+  fl_1 = (FV(0, sJ_1, 0) + FV(sI_, sJ_1, 0) + FV(0, sJ_1, sK_) + FV(sI_, sJ_1, sK_)) / 4;
+  double dfy = (fl - fl_1) / hyl;
+  fl = (FV(0, 0, sK_) + FV(sI_, 0, sK_) + FV(0, sJ_, sK_) + FV(sI_, sJ_, sK_)) / 4;
+  fl_1 = (FV(0, 0, sK_1) + FV(sI_, 0, sK_1) + FV(0, sJ_, sK_1) + FV(sI_, sJ_, sK_1)) / 4;
+  double dfz = (fl - fl_1) / hzl;
+#undef FV
+  if (!g.rho) return dfx * nx + dfy * ny + dfz * nz;
+  const double *rho = g.rho;
+  // the reference's precedence: rho[i+1] - rho[i]/h (XOperator.cpp:2211-2213)
+  double d_x = rho[c + sI] - rho[c] / (hx[i - 1]);
+  double d_y = rho[c + sJ] - rho[c] / (hy[j - 1]);
+  double d_z = rho[c + 1] - rho[c] / (hz[k - 1]);
+  double lap = dfx * nx + dfy * ny + dfz * nz;
+  double divd = d_x + d_y + d_z;
+  return rho[c] * lap - divd * f[c];
+}
+
+__device__ inline void decompose(int64_t c, int l, int m, int &i, int &j, int &k) {
+  k = (int)(c % m);
+  int64_t t = c / m;
+  j = (int)(t % l);
+  i = (int)(t / l);
+}
+
+__global__ void k_apply_exact(GridView g, const double *__restrict__ x, const double *__restrict__ b,
+                              double sign_b, double *__restrict__ y) {
+  int64_t nc = (int64_t)g.n * g.l * g.m;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    if (!g.space[c]) continue;
+    int i, j, k;
+    decompose(c, g.l, g.m, i, j, k);
+    double v = row_value(g, x, c, i, j, k);
+    if (sign_b > 0) v = v + b[c];
+    else if (sign_b < 0) v = v - b[c];
+    y[c] = v;
+  }
+}
+
+// sum over unknowns of (A x)_c * w_c; result into *out (device).
+__global__ void k_apply_dot_exact(GridView g, const double *__restrict__ x, const double *__restrict__ w,
+                                  double *partials, unsigned *counter, double *out) {
+  __shared__ double lds[16];
+  int64_t nc = (int64_t)g.n * g.l * g.m;
+  double acc = 0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    if (!g.space[c]) continue;
+    int i, j, k;
+    decompose(c, g.l, g.m, i, j, k);
+    acc += row_value(g, x, c, i, j, k) * w[c];
+  }
+  double bs = block_sum(acc, lds);
+  double tot;
+  if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) *out = tot;
+}
+
+// res may alias s1 or s2 (in-place updates such as result |= result - t*r).
+__global__ void k_lincomb(int64_t nc, const uint8_t *__restrict__ space, double p1, const double *s1,
+                          double p2, const double *s2, double *res) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    if (!space[c]) continue;
+    double v;
+    if (p1 != 0 && p2 != 0) v = p1 * s1[c] + p2 * s2[c];
+    else if (p1 != 0) v = p1 * s1[c];
+    else if (p2 != 0) v = p2 * s2[c];
+    else v = 0;
+    res[c] = v;
+  }
+}
+
+__global__ void k_copy_masked(int64_t nc, const uint8_t *__restrict__ space, const double *s, double *res) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
+    if (space[c]) res[c] = s[c];
+}
+
+__global__ void k_dot(int64_t nc, const uint8_t *__restrict__ space, const double *__restrict__ a,
+                      const double *__restrict__ b, double *partials, unsigned *counter, double *out) {
+  __shared__ double lds[16];
+  double acc = 0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
+    if (space[c]) acc += a[c] * b[c];
+  double bs = block_sum(acc, lds);
+  double tot;
+  if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) *out = tot;
+}
+
+// max |x| with the smallest Rn index among the maxima (the reference keeps the
+// first strict maximum in i->j->k order, which is the smallest Rn index).
+__global__ void k_absmax(int64_t nc, const uint8_t *__restrict__ space, const int32_t *__restrict__ rn,
+                         const double *__restrict__ x, double *pv, int32_t *pi, unsigned *counter, double *out,
+                         int32_t *out_idx) {
+  __shared__ double lv[kBlock];
+  __shared__ int32_t li[kBlock];
+  __shared__ bool am_last;
+  double best = 0;
+  int32_t bi = 0x7fffffff;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    if (!space[c]) continue;
+    double v = fabs(x[c]);
+    if (v > best || (v == best && v > 0 && rn[c] < bi)) { best = v; bi = rn[c]; }
+  }
+  lv[threadIdx.x] = best;
+  li[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      double v2 = lv[threadIdx.x + s];
+      int32_t i2 = li[threadIdx.x + s];
+      if (v2 > lv[threadIdx.x] || (v2 == lv[threadIdx.x] && i2 < li[threadIdx.x])) {
+        lv[threadIdx.x] = v2;
+        li[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pv[blockIdx.x] = lv[0];
+    pi[blockIdx.x] = li[0];
+    __threadfence();
+    unsigned prev = atomicAdd(counter, 1u);
+    am_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!am_last || threadIdx.x != 0) return;
+  __threadfence();
+  double b = 0;
+  int32_t idx = 0x7fffffff;
+  for (unsigned q = 0; q < gridDim.x; q++) {
+    double v = __hip_atomic_load(&pv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int32_t iq = __hip_atomic_load(&pi[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v > b || (v == b && iq < idx)) { b = v; idx = iq; }
+  }
+  *out = b;
+  *out_idx = (b > 0) ? idx : 1;
+  *counter = 0;
+}
+
+static int reduce_blocks(int64_t nc) {
+  int nb = grid_blocks(nc);
+  return nb > 2048 ? 2048 : nb;
+}
+static int stream_blocks(int64_t nc) {
+  int nb = grid_blocks(nc);
+  return nb > 65536 ? 65536 : nb;
+}
+
+void launch_apply_exact(xf_ctx ctx, xf_grid g, const double *x, const double *b, double sign_b, double *y) {
+  XF_REQUIRE((g->use_rho ? g->bad_rows_div : g->bad_rows_lap) == 0, XF_EINVAL,
+             "operator(): not implemented for a row of this mask (the reference throws here)");
+  hipLaunchKernelGGL(k_apply_exact, dim3(stream_blocks(g->ncell)), dim3(kBlock), 0, ctx->stream, view(g), x, b,
+                     sign_b, y);
+  check_kernel("k_apply_exact");
+}
+
+double run_apply_dot_exact(xf_ctx ctx, xf_grid g, const double *x, const double *w, int slot, double *dev_out) {
+  XF_REQUIRE((g->use_rho ? g->bad_rows_div : g->bad_rows_lap) == 0, XF_EINVAL,
+             "operator(): not implemented for a row of this mask (the reference throws here)");
+  double *out = dev_out ? dev_out : ctx->d_scalar + slot;
+  hipLaunchKernelGGL(k_apply_dot_exact, dim3(reduce_blocks(g->ncell)), dim3(kBlock), 0, ctx->stream, view(g), x, w,
+                     ctx->partials + (size_t)slot * kMaxPartials, ctx->counters + slot, out);
+  check_kernel("k_apply_dot_exact");
+  if (dev_out) return 0;
+  double h;
+  XF_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  XF_HIP(hipStreamSynchronize(ctx->stream));
+  return h;
+}
+
+void launch_lincomb(xf_ctx ctx, xf_grid g, double p1, const double *s1, double p2, const double *s2, double *res) {
+  hipLaunchKernelGGL(k_lincomb, dim3(stream_blocks(g->ncell)), dim3(kBlock), 0, ctx->stream, g->ncell, g->d_space,
+                     p1, s1, p2, s2, res);
+  check_kernel("k_lincomb");
+}
+
+double run_dot(xf_ctx ctx, xf_grid g, const double *a, const double *b, int slot, double *dev_out) {
+  double *out = dev_out ? dev_out : ctx->d_scalar + slot;
+  hipLaunchKernelGGL(k_dot, dim3(reduce_blocks(g->ncell)), dim3(kBlock), 0, ctx->stream, g->ncell, g->d_space, a,
+                     b, ctx->partials + (size_t)slot * kMaxPartials, ctx->counters + slot, out);
+  check_kernel("k_dot");
+  if (dev_out) return 0;
+  double h;
+  XF_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  XF_HIP(hipStreamSynchronize(ctx->stream));
+  return h;
+}
+
+// --------------------------------------------------------------- C ABI
+extern "C" int xf_apply(xf_ctx c, xf_grid g, const double *x, const double *b, double sign_b, double *y) {
+  return guarded(c, [&] {
+    XF_REQUIRE(g && x && y, XF_EINVAL, "xf_apply: null argument");
+    XF_REQUIRE(sign_b == 0 || b, XF_EINVAL, "xf_apply: b is null");
+    launch_apply_exact(c, g, x, b, sign_b, y);
+  });
+}
+
+extern "C" int xf_apply_dot(xf_ctx c, xf_grid g, const double *x, double *out) {
+  return guarded(c, [&] {
+    XF_REQUIRE(g && x && out, XF_EINVAL, "xf_apply_dot: null argument");
+    *out = run_apply_dot_exact(c, g, x, x, 0, nullptr);
+  });
+}
+
+extern "C" int xf_lincomb(xf_ctx c, xf_grid g, double p1, const double *s1, double p2, const double *s2,
+                          double *res) {
+  return guarded(c, [&] {
+    XF_REQUIRE(g && res, XF_EINVAL, "xf_lincomb: null argument");
+    XF_REQUIRE((p1 == 0 || s1) && (p2 == 0 || s2), XF_EINVAL,
+               "Incorrect combination: p<>0 but s=NULL.");  // XSpace.cpp:2522-2525
+    launch_lincomb(c, g, p1, s1, p2, s2, res);
+  });
+}
+
+extern "C" int xf_copy(xf_ctx c, xf_grid g, const double *s, double *res) {
+  return guarded(c, [&] {
+    XF_REQUIRE(g && s && res, XF_EINVAL, "xf_copy: null argument");
+    hipLaunchKernelGGL(k_copy_masked, dim3(stream_blocks(g->ncell)), dim3(kBlock), 0, c->stream, g->ncell,
+                       g->d_space, s, res);
+    check_kernel("k_copy_masked");
+  });
+}
+
+extern "C" int xf_dot(xf_ctx c, xf_grid g, const double *a, const double *b, double *out) {
+  return guarded(c, [&] {
+    XF_REQUIRE(g && a && b && out, XF_EINVAL, "xf_dot: null argument");
+    *out = run_dot(c, g, a, b, 0, nullptr);
+  });
+}
+
+extern "C" int xf_absmax(xf_ctx c, xf_grid g, const double *x, double *val, int32_t *rn_index) {
+  return guarded(c, [&] {
+    XF_REQUIRE(g && x && val, XF_EINVAL, "xf_absmax: null argument");
+    int nb = reduce_blocks(g->ncell);
+    if (nb > kBlock * 8) nb = kBlock * 8;
+    double *pv = c->partials;
+    int32_t *pi = (int32_t *)(c->partials + kMaxPartials);
+    hipLaunchKernelGGL(k_absmax, dim3(nb), dim3(kBlock), 0, c->stream, g->ncell, g->d_space, g->d_rn, x, pv, pi,
+                       c->counters, c->d_scalar, c->d_iscalar);
+    check_kernel("k_absmax");
+    double v;
+    int32_t idx;
+    XF_HIP(hipMemcpyAsync(&v, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipMemcpyAsync(&idx, c->d_iscalar, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipStreamSynchronize(c->stream));
+    *val = v;
+    if (rn_index) *rn_index = idx;
+  });
+}
+
+// ------------------------------------------------ done-gated solver variants
+__global__ void k_apply_gated(GridView g, const double *__restrict__ x, const double *__restrict__ b, double sign_b,
+                              double *__restrict__ y, const DevState *st) {
+  if (st->done) return;
+  int64_t nc = (int64_t)g.n * g.l * g.m;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    if (!g.space[c]) continue;
+    int i, j, k;
+    decompose(c, g.l, g.m, i, j, k);
+    double v = row_value(g, x, c, i, j, k);
+    if (sign_b > 0) v = v + b[c];
+    else if (sign_b < 0) v = v - b[c];
+    y[c] = v;
+  }
+}
+
+__global__ void k_apply_dot_gated(GridView g, const double *__restrict__ x, const double *__restrict__ w,
+                                  double *partials, unsigned *counter, double *out, const DevState *st) {
+  __shared__ double lds[16];
+  if (st->done) return;
+  int64_t nc = (int64_t)g.n * g.l * g.m;
+  double acc = 0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    if (!g.space[c]) continue;
+    int i, j, k;
+    decompose(c, g.l, g.m, i, j, k);
+    acc += row_value(g, x, c, i, j, k) * w[c];
+  }
+  double bs = block_sum(acc, lds);
+  double tot;
+  if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) *out = tot;
+}
+
+void launch_apply_gated(xf_ctx ctx, xf_grid g, const double *x, const double *b, double sign_b, double *y) {
+  XF_REQUIRE((g->use_rho ? g->bad_rows_div : g->bad_rows_lap) == 0, XF_EINVAL,
+             "operator(): not implemented for a row of this mask (the reference throws here)");
+  hipLaunchKernelGGL(k_apply_gated, dim3(stream_blocks(g->ncell)), dim3(kBlock), 0, ctx->stream, view(g), x, b,
+                     sign_b, y, ctx->state);
+  check_kernel("k_apply_gated");
+}
+
+void launch_apply_dot_gated(xf_ctx ctx, xf_grid g, const double *x, const double *w, double *out, int rslot) {
+  XF_REQUIRE((g->use_rho ? g->bad_rows_div : g->bad_rows_lap) == 0, XF_EINVAL,
+             "operator(): not implemented for a row of this mask (the reference throws here)");
+  hipLaunchKernelGGL(k_apply_dot_gated, dim3(reduce_blocks(g->ncell)), dim3(kBlock), 0, ctx->stream, view(g), x, w,
+                     ctx->partials + (size_t)rslot * kMaxPartials, ctx->counters + rslot, out, ctx->state);
+  check_kernel("k_apply_dot_gated");
+}

@@ -1,0 +1,22 @@
+"""Per-kernel HIP-event timing of the throughput solvers (xf_profile_* in the C ABI)."""
+import ctypes as C
+
+from ._lib import lib
+
+KERNELS = ("cg_stencil_fp64", "cg_update_fp64", "cg_stencil_fp32", "cg_update_fp32",
+           "mp_residual_fp64", "mp_correct")
+
+
+def enable(ctx, on=True):
+    lib().xf_profile_reset(ctx.h)
+    ctx.check(lib().xf_profile_enable(ctx.h, 1 if on else 0), "xf_profile_enable")
+
+
+def read(ctx):
+    out = {}
+    for k in KERNELS:
+        ms, n = C.c_double(), C.c_int64()
+        lib().xf_profile_read(ctx.h, k.encode(), C.byref(ms), C.byref(n))
+        if n.value:
+            out[k] = {"ms": ms.value, "count": n.value}
+    return out
