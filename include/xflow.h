@@ -139,6 +139,55 @@ void xf_solve_opts_default(int method, xf_solve_opts *o);
 int xf_solve(xf_ctx ctx, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts *opts,
              xf_solve_stats *stats);
 
+/* ------------------------------------------------------------ transport */
+/* One LOD transport step, THydrodynamicsProblem::SolveTransferEquation__
+ * (XProblem.cpp:2784-3475; direction order 1, all terms on): f is the field
+ * at the new level (updated in place, entries outside the sectors kept),
+ * f_prev the previous level, u/v/w the transport velocities, rp the source /
+ * boundary values, kappa the diffusion coefficient, scheme 0 central, 1
+ * upwind.  Point types come from the grid; sector borders the reference
+ * rejects give XF_EINVAL.  Device pointers, all [ncell]. */
+int xf_lod_transport(xf_ctx ctx, xf_grid g, double *f, const double *f_prev, const double *u, const double *v,
+                     const double *w, const double *rp, const double *kappa, double tau, int scheme);
+/* nsys independent tridiagonal systems a u[m-1] + b u[m] + c u[m+1] = f,
+ * m = 1..M-1, u[0] and u[M] given (TSLAE::SolveByScalarRunning,
+ * XSLAE.cpp:416-445); system s at [s*(M+1), (s+1)*(M+1)).  Device pointers. */
+int xf_thomas_batched(xf_ctx ctx, int nsys, int M, double *u, const double *a, const double *b, const double *c,
+                      const double *f);
+
+/* ---------------------------------------------------- immersed boundary */
+/* axis 1, 2, 3 = COORD_X, COORD_Y, COORD_Z (XProblem.h:1105-1107); the grid
+ * is the velocity component's grid (uniform steps, Separation[0] per axis);
+ * xyz[nn*3] node positions, device pointers. */
+/* U[q] = h_axis^3 sum_{7^3 window} vel * phi phi phi (Interpolate, XProblem.cpp:8620-8791). */
+int xf_ib_interp(xf_ctx ctx, xf_grid g, int axis, int nn, const double *xyz, const double *vel, double *U);
+/* force = 0, then force += F[q] phi phi phi over each node's window, nodes in
+ * order (SpreadForce, XProblem.cpp:8550-8617, without its mask side effect). */
+int xf_ib_spread(xf_ctx ctx, xf_grid g, int axis, int nn, const double *xyz, const double *F, double *force);
+/* xyz_prev[.,axis] = xyz[.,axis]; xyz[.,axis] += U dt (UpdateBoundaryPosition,
+ * XProblem.cpp:8858-8894); xyz_prev may be NULL. */
+int xf_ib_update_position(xf_ctx ctx, int axis, int nn, double *xyz, double *xyz_prev, const double *U, double dt);
+
+/* ------------------------------------------------------- projection glue */
+/* Staggered fields live on the pressure array shape: u[i,j,k] is the x-face
+ * between pressure cells (i,j,k) and (i+1,j,k) (v, w likewise).  hxU[n-1],
+ * hyV[l-1], hzW[m-1] are the velocity grids' separator steps (host arrays).
+ * rho, pbc and bc may be NULL (density 1, boundary values 0).  Device fields. */
+/* rhs = -(rho^2 du/dx + dv/dy + dw/dz)/tau at Actual cells, 0 at Fictive,
+ * pbc at PreDefined (TVPProblem::SolveBySplitting, XProblem.cpp:5474-5513). */
+int xf_pressure_rhs(xf_ctx ctx, xf_grid gp, const double *u, const double *v, const double *w, const double *rho,
+                    const double *hxU, const double *hyV, const double *hzW, const double *pbc, double tau,
+                    double *rhs);
+/* u = u* - tau (p[+1]-p)/hP / rho along axis (1..3) on the velocity grid gv,
+ * Defined points checked against bc, PreDefined points mirrored about their
+ * Actual neighbour (XProblem.cpp:5598-5778); hP = pressure steps (host). */
+int xf_velocity_correct(xf_ctx ctx, xf_grid gv, int axis, const double *u_star, const double *p, const double *rho,
+                        const double *hP, const double *bc, double tau, double *u_out);
+/* max |div u| over Actual pressure cells and its flat cell index (-1 if 0)
+ * (control balance, XProblem.cpp:5782-5830). */
+int xf_divergence_max(xf_ctx ctx, xf_grid gp, const double *u, const double *v, const double *w, const double *hxU,
+                      const double *hyV, const double *hzW, double *max_abs, int64_t *cell);
+
 /* ------------------------------------------------------------ profiling */
 /* No reference counterpart (the reference only prints clock() deltas,
  * XProblem.cpp:5442-5445).  When enabled, the throughput solvers bracket

@@ -808,3 +808,92 @@ void xo_ib_spread(const xo_grid *g, int axis, int nn, const double *xyz, const d
         }
   }
 }
+
+/* ------------------------------------------------------- projection glue */
+
+/* Pressure right-hand side, TVPProblem::SolveBySplitting XProblem.cpp:5474-5513
+ * (staggered fields on the pressure array shape). */
+int xo_pressure_rhs(int n, int l, int m, const int32_t *codes, const double *u, const double *v, const double *w,
+                    const double *rho, const double *hxU, const double *hyV, const double *hzW, const double *pbc,
+                    double tau, double *rhs) {
+  int64_t sI = (int64_t)l * m, sJ = m;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < l; j++)
+      for (int k = 0; k < m; k++) {
+        int64_t c = ((int64_t)i * l + j) * m + k;
+        int code = codes[c];
+        if (code == 0) rhs[c] = 0;
+        else if (code == 1) {
+          if (i < 1 || j < 1 || k < 1) return XO_EINVAL;
+          double d = rho ? rho[c] : 1.0;
+          double div_ = d * d * ((u[c] - u[c - sI]) / hxU[i - 1]) + ((v[c] - v[c - sJ]) / hyV[j - 1]) +
+                        ((w[c] - w[c - 1]) / hzW[k - 1]);
+          rhs[c] = -(div_ / tau);
+        } else if (code == 4) rhs[c] = pbc ? pbc[c] : 0.0;
+        else return XO_EINVAL;
+      }
+  return XO_OK;
+}
+
+/* Velocity correction, XProblem.cpp:5598-5778 (axis 1..3). */
+int xo_velocity_correct(int n, int l, int m, int axis, const int32_t *codes, const double *us, const double *p,
+                        const double *rho, const double *hP, const double *bc, double tau, double *out) {
+  int64_t S[3] = {(int64_t)l * m, m, 1};
+  int E[3] = {n, l, m};
+  int a = axis - 1;
+  for (int pass = 1; pass <= 2; pass++)
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < l; j++)
+        for (int k = 0; k < m; k++) {
+          int64_t c = ((int64_t)i * l + j) * m + k;
+          int code = codes[c];
+          if ((pass == 1) == (code == 4)) continue;
+          if (code == 0) out[c] = 0;
+          else if (code == 1 || code == 5 || code == 7) {
+            int idx[3] = {i, j, k};
+            if (idx[a] + 1 >= E[a]) return XO_EINVAL;
+            double gradP = (p[c + S[a]] - p[c]) / hP[idx[a]];
+            out[c] = us[c] - tau * gradP / (rho ? rho[c] : 1.0);
+          } else if (code == 3) {
+            double dv = bc ? bc[c] : 0.0;
+            if (us[c] != dv) return XO_EINVAL;
+            out[c] = dv;
+          } else if (code == 4) {
+            int64_t nb = -1;
+            int cnt = 0;
+            for (int dir = 0; dir < 3; dir++)
+              for (int sign = 0; sign < 2; sign++) {
+                int sh = -1 + sign * 2;
+                int aa = i + (dir == 0 ? sh : 0), bb = j + (dir == 1 ? sh : 0), ee = k + (dir == 2 ? sh : 0);
+                if (aa >= 0 && aa < n && bb >= 0 && bb < l && ee >= 0 && ee < m) {
+                  int64_t cc = ((int64_t)aa * l + bb) * m + ee;
+                  if (codes[cc] == 1) { nb = cc; cnt++; }
+                }
+              }
+            if (cnt != 1) return XO_EINVAL;
+            double dv = bc ? bc[c] : 0.0;
+            if (fabs(us[c] - (2 * dv - us[nb])) > 1e-10) return XO_EINVAL;
+            out[c] = 2 * dv - out[nb];
+          } else return XO_EINVAL;
+        }
+  return XO_OK;
+}
+
+/* Control balance, XProblem.cpp:5782-5830: max |div u| over Actual cells and
+ * the first cell (i->j->k order) attaining it (-1 if all are zero). */
+double xo_divergence_max(int n, int l, int m, const int32_t *codes, const double *u, const double *v, const double *w,
+                         const double *hxU, const double *hyV, const double *hzW, int64_t *cell) {
+  int64_t sI = (int64_t)l * m, sJ = m, best_c = -1;
+  double best = 0;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < l; j++)
+      for (int k = 0; k < m; k++) {
+        int64_t c = ((int64_t)i * l + j) * m + k;
+        if (codes[c] != 1 || i < 1 || j < 1 || k < 1) continue;
+        double div = ((u[c] - u[c - sI]) / hxU[i - 1]) + ((v[c] - v[c - sJ]) / hyV[j - 1]) +
+                     ((w[c] - w[c - 1]) / hzW[k - 1]);
+        if (fabs(div) > best) { best = fabs(div); best_c = c; }
+      }
+  if (cell) *cell = best_c;
+  return best;
+}

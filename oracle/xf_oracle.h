@@ -73,6 +73,14 @@ int  xo_ib_index(double coord, int type, int axis, double h, int end_index);
 void xo_ib_interp(const xo_grid *g, int axis, int nn, const double *xyz, const double *vel, double *U);
 void xo_ib_spread(const xo_grid *g, int axis, int nn, const double *xyz, const double *F, double *force);
 
+int xo_pressure_rhs(int n, int l, int m, const int32_t *codes, const double *u, const double *v, const double *w,
+                    const double *rho, const double *hxU, const double *hyV, const double *hzW, const double *pbc,
+                    double tau, double *rhs);
+int xo_velocity_correct(int n, int l, int m, int axis, const int32_t *codes, const double *us, const double *p,
+                        const double *rho, const double *hP, const double *bc, double tau, double *out);
+double xo_divergence_max(int n, int l, int m, const int32_t *codes, const double *u, const double *v, const double *w,
+                         const double *hxU, const double *hyV, const double *hzW, int64_t *cell);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,9 @@ def lib():
             "xo_ib_index": (i, [d, i, i, d, i]),
             "xo_ib_interp": (None, [gp, i, i, p, p, p]),
             "xo_ib_spread": (None, [gp, i, i, p, p, p]),
+            "xo_pressure_rhs": (i, [i, i, i, p, p, p, p, p, p, p, p, p, d, p]),
+            "xo_velocity_correct": (i, [i, i, i, i, p, p, p, p, p, p, d, p]),
+            "xo_divergence_max": (d, [i, i, i, p, p, p, p, p, p, p, p]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -226,3 +229,41 @@ def ib_spread(g, axis, xyz, F):
     out = np.zeros(g.shape)
     lib().xo_ib_spread(g.ref, int(axis), xyz.shape[0], xyz.ctypes.data, F.ctypes.data, out.ctypes.data)
     return out
+
+
+def pressure_rhs(codes, u, v, w, hxU, hyV, hzW, tau, rho=None, pbc=None):
+    codes = np.ascontiguousarray(codes, dtype=np.int32)
+    n, l, m = codes.shape
+    u, v, w, hxU, hyV, hzW = [_f64(a) for a in (u, v, w, hxU, hyV, hzW)]
+    rho = None if rho is None else _f64(rho)
+    pbc = None if pbc is None else _f64(pbc)
+    rhs = np.zeros(codes.shape)
+    st = lib().xo_pressure_rhs(n, l, m, codes.ctypes.data, u.ctypes.data, v.ctypes.data, w.ctypes.data, _ptr(rho),
+                               hxU.ctypes.data, hyV.ctypes.data, hzW.ctypes.data, _ptr(pbc), float(tau),
+                               rhs.ctypes.data)
+    if st != OK:
+        raise ValueError("pressure rhs: unsupported point type")
+    return rhs
+
+
+def velocity_correct(codes, axis, us, p, hP, tau, rho=None, bc=None, out=None):
+    codes = np.ascontiguousarray(codes, dtype=np.int32)
+    n, l, m = codes.shape
+    us, p, hP = _f64(us), _f64(p), _f64(hP)
+    rho = None if rho is None else _f64(rho)
+    bc = None if bc is None else _f64(bc)
+    out = np.zeros(codes.shape) if out is None else _f64(out).copy()
+    st = lib().xo_velocity_correct(n, l, m, int(axis), codes.ctypes.data, us.ctypes.data, p.ctypes.data, _ptr(rho),
+                                   hP.ctypes.data, _ptr(bc), float(tau), out.ctypes.data)
+    if st != OK:
+        raise ValueError("velocity correction: border check failed")
+    return out
+
+
+def divergence_max(codes, u, v, w, hxU, hyV, hzW):
+    codes = np.ascontiguousarray(codes, dtype=np.int32)
+    n, l, m = codes.shape
+    arrs = [_f64(a) for a in (u, v, w, hxU, hyV, hzW)]
+    cell = C.c_int64(0)
+    val = lib().xo_divergence_max(n, l, m, codes.ctypes.data, *[a.ctypes.data for a in arrs], C.byref(cell))
+    return val, cell.value

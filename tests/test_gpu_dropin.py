@@ -1,0 +1,22 @@
+"""The drop-in C++ classes (xflow_amd/host) driven by reference-style host
+code (tests/host/dropin_poisson.cpp, built by __graft_entry__.build())."""
+import json
+import os
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+
+def test_dropin_manufactured_poisson():
+    exe = os.path.join(ROOT, "tests", "host", "dropin_poisson")
+    assert os.path.exists(exe), "build first: __graft_entry__.build()"
+    p = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=240)
+    print(p.stdout, p.stderr)
+    res = json.loads(p.stdout.strip().splitlines()[-1])
+    assert p.returncode == 0, res
+    assert res["trunc24"] < res["trunc12"] / 3
+    assert res["err_cg24"] < 5e-3

@@ -151,6 +151,49 @@ class Grid:
             self.ctx.check(rc, "xf_solve")
         return st
 
+    def lod_transport(self, f, f_prev, u, v, w, rp, kappa, tau, scheme):
+        self.ctx.check(lib().xf_lod_transport(self.ctx.h, self.h, self._vec(f), self._vec(f_prev), self._vec(u),
+                                              self._vec(v), self._vec(w), self._vec(rp), self._vec(kappa),
+                                              float(tau), int(scheme)), "xf_lod_transport")
+        return f
+
+    def ib_interp(self, axis, xyz, vel, U):
+        nn = xyz.shape[0]
+        self.ctx.check(lib().xf_ib_interp(self.ctx.h, self.h, int(axis), nn, _ptr(xyz), self._vec(vel), _ptr(U)),
+                       "xf_ib_interp")
+        return U
+
+    def ib_spread(self, axis, xyz, F, force):
+        nn = xyz.shape[0]
+        self.ctx.check(lib().xf_ib_spread(self.ctx.h, self.h, int(axis), nn, _ptr(xyz), _ptr(F), self._vec(force)),
+                       "xf_ib_spread")
+        return force
+
+    def pressure_rhs(self, u, v, w, hxU, hyV, hzW, tau, rhs, rho=None, pbc=None):
+        hx, phx = _host(hxU, np.float64)
+        hy, phy = _host(hyV, np.float64)
+        hz, phz = _host(hzW, np.float64)
+        self.ctx.check(lib().xf_pressure_rhs(self.ctx.h, self.h, self._vec(u), self._vec(v), self._vec(w),
+                                             _ptr(rho), phx, phy, phz, _ptr(pbc), float(tau), self._vec(rhs)),
+                       "xf_pressure_rhs")
+        return rhs
+
+    def velocity_correct(self, axis, u_star, p, hP, tau, u_out, rho=None, bc=None):
+        hp, php = _host(hP, np.float64)
+        self.ctx.check(lib().xf_velocity_correct(self.ctx.h, self.h, int(axis), self._vec(u_star), self._vec(p),
+                                                 _ptr(rho), php, _ptr(bc), float(tau), self._vec(u_out)),
+                       "xf_velocity_correct")
+        return u_out
+
+    def divergence_max(self, u, v, w, hxU, hyV, hzW):
+        hx, phx = _host(hxU, np.float64)
+        hy, phy = _host(hyV, np.float64)
+        hz, phz = _host(hzW, np.float64)
+        val, idx = C.c_double(), C.c_int64()
+        self.ctx.check(lib().xf_divergence_max(self.ctx.h, self.h, self._vec(u), self._vec(v), self._vec(w), phx,
+                                               phy, phz, C.byref(val), C.byref(idx)), "xf_divergence_max")
+        return val.value, idx.value
+
     def close(self):
         if getattr(self, "h", None) and self.ctx.h:
             lib().xf_grid_destroy(self.ctx.h, self.h)
@@ -161,6 +204,20 @@ class Grid:
             self.close()
         except Exception:
             pass
+
+
+def thomas_batched(ctx, u, a, b, c, f):
+    """Solves u.shape[0] tridiagonal systems of size u.shape[1] in place."""
+    nsys, M1 = u.shape
+    ctx.check(lib().xf_thomas_batched(ctx.h, nsys, M1 - 1, _ptr(u), _ptr(a), _ptr(b), _ptr(c), _ptr(f)),
+              "xf_thomas_batched")
+    return u
+
+
+def ib_update_position(ctx, axis, xyz, U, dt, xyz_prev=None):
+    ctx.check(lib().xf_ib_update_position(ctx.h, int(axis), xyz.shape[0], _ptr(xyz), _ptr(xyz_prev), _ptr(U),
+                                          float(dt)), "xf_ib_update_position")
+    return xyz
 
 
 def solve_opts(method, **kw):

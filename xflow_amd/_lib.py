@@ -52,6 +52,14 @@ SIGNATURES = {
     "xf_absmax": (I, [P, P, P, C.POINTER(D), C.POINTER(C.c_int32)]),
     "xf_solve_opts_default": (None, [I, C.POINTER(SolveOpts)]),
     "xf_solve": (I, [P, P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "xf_lod_transport": (I, [P, P, P, P, P, P, P, P, P, D, I]),
+    "xf_thomas_batched": (I, [P, I, I, P, P, P, P, P]),
+    "xf_ib_interp": (I, [P, P, I, I, P, P, P]),
+    "xf_ib_spread": (I, [P, P, I, I, P, P, P]),
+    "xf_ib_update_position": (I, [P, I, I, P, P, P, D]),
+    "xf_pressure_rhs": (I, [P, P, P, P, P, P, P, P, P, P, D, P]),
+    "xf_velocity_correct": (I, [P, P, I, P, P, P, P, P, D, P]),
+    "xf_divergence_max": (I, [P, P, P, P, P, P, P, P, C.POINTER(D), C.POINTER(I64)]),
     "xf_profile_enable": (I, [P, I]),
     "xf_profile_reset": (I, [P]),
     "xf_profile_read": (I, [P, C.c_char_p, C.POINTER(D), C.POINTER(I64)]),

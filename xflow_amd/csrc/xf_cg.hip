@@ -394,10 +394,85 @@ void solve_hs_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
   s.resid = c->h_state->s[HS_RR0] > 0 ? sqrt(c->h_state->s[HS_RR] / c->h_state->s[HS_RR0]) : 0;
 }
 
+template <typename T>
+__global__ void k_sq_unknowns(int64_t nc, const uint8_t *__restrict__ flags, const T *__restrict__ v, DevState *st,
+                              double *partials, unsigned *counter, int slot) {
+  __shared__ double lds[16];
+  double acc = 0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
+    if (flags[c] & FL_UNKNOWN) acc += (double)v[c] * (double)v[c];
+  double bs = block_sum(acc, lds);
+  double tot;
+  if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) st->s[slot] = tot;
+}
+
+// Mixed precision: FP32 Hestenes-Stiefel CG sweeps on the correction
+// equation A e = r, with the residual r = b - A x recomputed in FP64 from the
+// FP64 iterate after every sweep (iterative refinement), FP64 dot-product
+// accumulation everywhere and the correction added to x in FP64.
 void solve_mp_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
                  xf_solve_stats &s) {
-  (void)c; (void)g; (void)b; (void)x0; (void)x; (void)o; (void)s;
-  XF_REQUIRE(false, XF_EINVAL, "mixed precision: not built yet");
+  require_fast(g);
+  CgWork *w = work(c, g, true);
+  const int64_t nc = g->ncell;
+  Tiles tl = tiles(g);
+  const int every = o.check_every > 0 ? o.check_every : 16;
+  const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
+  upload_state_zero(c);
+  XF_HIP(hipMemcpyAsync(x, x0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
+  std::vector<ProfMark> marks;
+  auto residual = [&]() {
+    int e = c->prof ? prof_begin(c) : -1;
+    hipLaunchKernelGGL((k_residual_fast<float>), dim3(tl.total), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
+                       g->d_flags, g->d_cx, g->d_cy, g->d_cz, b, x, w->r32, c->state, c->partials, c->counters,
+                       (int)HS_AUX);
+    if (c->prof) {
+      prof_end(c, e);
+      marks.push_back({"mp_residual_fp64", e});
+    }
+    XF_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipStreamSynchronize(c->stream));
+    if (c->prof) prof_collect(c, marks, {{"mp_residual_fp64", 1}, {"mp_correct", 1}});
+    return c->h_state->s[HS_AUX];
+  };
+  double rr = residual();
+  const double rr0 = rr;
+  const double stop2 = o.tol_mode == XF_TOL_REL ? o.tol * o.tol * rr0 : o.tol * o.tol;
+  int outer = 0, inner_total = 0, status = XF_OK;
+  const int max_outer = 60;
+  while (rr > stop2) {
+    if (rr != rr) { status = XF_ENAN; break; }
+    if (outer >= max_outer || inner_total >= o.maxit) { status = XF_EMAXIT; break; }
+    outer++;
+    if (o.project_mean && g->nunk > 0) {
+      hipLaunchKernelGGL((k_sum_unknowns<float>), dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags,
+                         w->r32, c->state, c->partials, c->counters, (int)HS_SUM);
+      hipLaunchKernelGGL((k_sub_mean<float>), dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags, w->r32,
+                         c->state, (int)HS_SUM, (double)g->nunk);
+    }
+    hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags, w->r32,
+                       c->state, c->partials, c->counters, (int)HS_AUX);
+    XF_HIP(hipMemsetAsync(w->e32, 0, sizeof(float) * nc, c->stream));
+    XF_HIP(hipMemsetAsync(w->p32[0], 0, sizeof(float) * nc, c->stream));
+    hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, inner_tol, 1, (int)HS_AUX);
+    check_kernel("mp inner start");
+    c->h_state->iter = 0;
+    hs_iterations<float, float>(c, g, w->e32, w->r32, w->p32[0], w->p32[1], w->q32, g->d_cxf, g->d_cyf, g->d_czf,
+                                o.maxit - inner_total, every);
+    inner_total += c->h_state->iter;
+    if (c->h_state->status == XF_ENAN) { status = XF_ENAN; break; }
+    int e = c->prof ? prof_begin(c) : -1;
+    hipLaunchKernelGGL(k_add_correction, dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, x, w->e32);
+    if (c->prof) {
+      prof_end(c, e);
+      marks.push_back({"mp_correct", e});
+    }
+    rr = residual();
+  }
+  s.iters = outer;
+  s.inner_iters = inner_total;
+  s.status = status;
+  s.resid = rr0 > 0 ? sqrt(rr / rr0) : 0;
 }
 
 // ||b - A x|| / ||b - A x0|| with the reference rows.

@@ -1,0 +1,248 @@
+// Projection-step glue around the pressure solve (TVPProblem::SolveBySplitting):
+// the pressure right-hand side -div(u*)/tau (XProblem.cpp:5474-5513), the
+// velocity correction u = u* - tau grad p / rho (XProblem.cpp:5598-5778) and
+// the divergence monitor max |div u| (XProblem.cpp:5782-5830).
+//
+// Staggered fields are stored on the pressure array shape: u[i,j,k] is the
+// x-face between pressure cells (i,j,k) and (i+1,j,k), v and w likewise; the
+// face steps hxU[i-1], hyV[j-1], hzW[k-1] of cell (i,j,k) are the velocity
+// grids' separator steps (arrays of n-1, l-1, m-1 entries).  The arithmetic
+// keeps the reference's operand order, including its rho^2 precedence on the
+// x term of the divergence (XProblem.cpp:5495).
+#include <cmath>
+
+#include "xf_internal.h"
+
+__global__ void k_pressure_rhs(int n, int l, int m, const uint8_t *__restrict__ codes, const double *__restrict__ u,
+                               const double *__restrict__ v, const double *__restrict__ w,
+                               const double *__restrict__ rho, const double *__restrict__ hxU,
+                               const double *__restrict__ hyV, const double *__restrict__ hzW,
+                               const double *__restrict__ pbc, double tau, double *__restrict__ rhs, int *err) {
+  const int64_t nc = (int64_t)n * l * m, sI = (int64_t)l * m, sJ = m;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    const uint8_t code = codes[c];
+    if (code == PT_FICTIVE) {
+      rhs[c] = 0;
+    } else if (code == PT_ACTUAL) {
+      const int k = (int)(c % m), j = (int)((c / m) % l), i = (int)(c / sI);
+      if (i < 1 || j < 1 || k < 1) { atomicOr(err, 2); continue; }
+      const double d = rho ? rho[c] : 1.0;
+      const double div_ = d * d * ((u[c] - u[c - sI]) / hxU[i - 1]) + ((v[c] - v[c - sJ]) / hyV[j - 1]) +
+                          ((w[c] - w[c - 1]) / hzW[k - 1]);
+      rhs[c] = -(div_ / tau);
+    } else if (code == PT_PREDEFINED) {
+      rhs[c] = pbc ? pbc[c] : 0.0;
+    } else {
+      atomicOr(err, 1);  // XProblem.cpp:5502-5505
+    }
+  }
+}
+
+// pass 1: every point but PreDefined (XProblem.cpp:5636-5698)
+__global__ void k_vel_correct(int n, int l, int m, int axis, const uint8_t *__restrict__ codes,
+                              const double *__restrict__ us, const double *__restrict__ p,
+                              const double *__restrict__ rho, const double *__restrict__ hP,
+                              const double *__restrict__ bc, double tau, double *__restrict__ out, int *err) {
+  const int64_t nc = (int64_t)n * l * m;
+  const int64_t S[3] = {(int64_t)l * m, (int64_t)m, 1};
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    const uint8_t code = codes[c];
+    if (code == PT_PREDEFINED) continue;
+    if (code == PT_FICTIVE) {
+      out[c] = 0;
+    } else if (code == PT_ACTUAL || code == PT_NORMAL || code == PT_EQUATION) {
+      const int idx[3] = {(int)(c / S[0]), (int)((c / m) % l), (int)(c % m)};
+      const int E[3] = {n, l, m};
+      if (idx[axis] + 1 >= E[axis]) { atomicOr(err, 2); continue; }
+      const double gradP = (p[c + S[axis]] - p[c]) / hP[idx[axis]];
+      out[c] = us[c] - tau * gradP / (rho ? rho[c] : 1.0);
+    } else if (code == PT_DEFINED) {
+      const double dv = bc ? bc[c] : 0.0;
+      if (us[c] != dv) atomicOr(err, 4);  // XProblem.cpp:5684-5688
+      out[c] = dv;
+    } else {
+      atomicOr(err, 1);
+    }
+  }
+}
+
+// pass 2: PreDefined points mirror their single Actual neighbour (XProblem.cpp:5698-5758)
+__global__ void k_vel_correct_predef(int n, int l, int m, const uint8_t *__restrict__ codes,
+                                     const double *__restrict__ us, const double *__restrict__ bc,
+                                     double *__restrict__ out, int *err) {
+  const int64_t nc = (int64_t)n * l * m;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    if (codes[c] != PT_PREDEFINED) continue;
+    const int k = (int)(c % m), j = (int)((c / m) % l), i = (int)(c / ((int64_t)l * m));
+    int64_t nb = -1;
+    int count = 0;
+    for (int dir = 0; dir < 3; dir++)
+      for (int sign = 0; sign < 2; sign++) {
+        const int sh = -1 + sign * 2;
+        const int a = i + (dir == 0 ? sh : 0), b = j + (dir == 1 ? sh : 0), e = k + (dir == 2 ? sh : 0);
+        if (a >= 0 && a < n && b >= 0 && b < l && e >= 0 && e < m) {
+          const int64_t cc = ((int64_t)a * l + b) * m + e;
+          if (codes[cc] == PT_ACTUAL) {
+            nb = cc;
+            count++;
+          }
+        }
+      }
+    if (count != 1) { atomicOr(err, 8); continue; }
+    const double dv = bc ? bc[c] : 0.0;
+    if (fabs(us[c] - (2 * dv - us[nb])) > 1e-10) { atomicOr(err, 16); continue; }
+    out[c] = 2 * dv - out[nb];
+  }
+}
+
+// max |div u| over Actual pressure cells with the first (i->j->k) maximising cell
+__global__ void k_divmax(int n, int l, int m, const uint8_t *__restrict__ codes, const double *__restrict__ u,
+                         const double *__restrict__ v, const double *__restrict__ w, const double *__restrict__ hxU,
+                         const double *__restrict__ hyV, const double *__restrict__ hzW, double *pv, int64_t *pi,
+                         unsigned *counter, double *out, int64_t *out_idx) {
+  __shared__ double lv[kBlock];
+  __shared__ int64_t li[kBlock];
+  __shared__ bool am_last;
+  const int64_t nc = (int64_t)n * l * m, sI = (int64_t)l * m, sJ = m;
+  double best = 0;
+  int64_t bi = -1;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    if (codes[c] != PT_ACTUAL) continue;
+    const int k = (int)(c % m), j = (int)((c / m) % l), i = (int)(c / sI);
+    if (i < 1 || j < 1 || k < 1) continue;
+    const double div = ((u[c] - u[c - sI]) / hxU[i - 1]) + ((v[c] - v[c - sJ]) / hyV[j - 1]) +
+                       ((w[c] - w[c - 1]) / hzW[k - 1]);
+    const double a = fabs(div);
+    if (a > best) { best = a; bi = c; }
+  }
+  lv[threadIdx.x] = best;
+  li[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double v2 = lv[threadIdx.x + s];
+      const int64_t i2 = li[threadIdx.x + s];
+      if (v2 > lv[threadIdx.x] || (v2 == lv[threadIdx.x] && i2 >= 0 && (li[threadIdx.x] < 0 || i2 < li[threadIdx.x]))) {
+        lv[threadIdx.x] = v2;
+        li[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pv[blockIdx.x] = lv[0];
+    pi[blockIdx.x] = li[0];
+    __threadfence();
+    am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!am_last || threadIdx.x != 0) return;
+  __threadfence();
+  double b = 0;
+  int64_t idx = -1;
+  for (unsigned q = 0; q < gridDim.x; q++) {
+    const double v2 = __hip_atomic_load(&pv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int64_t i2 = __hip_atomic_load(&pi[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v2 > b || (v2 == b && i2 >= 0 && (idx < 0 || i2 < idx))) {
+      b = v2;
+      idx = i2;
+    }
+  }
+  *out = b;
+  *out_idx = b > 0 ? idx : -1;
+  *counter = 0;
+}
+
+namespace {
+struct Steps {
+  double *d = nullptr;
+  Steps(const double *h, int cnt, hipStream_t s) {
+    XF_HIP(hipMalloc(&d, sizeof(double) * (cnt > 0 ? cnt : 1)));
+    XF_HIP(hipMemcpyAsync(d, h, sizeof(double) * cnt, hipMemcpyHostToDevice, s));
+  }
+  ~Steps() { hipFree(d); }
+};
+struct ErrFlag {
+  int *d = nullptr;
+  hipStream_t s;
+  explicit ErrFlag(hipStream_t st) : s(st) {
+    XF_HIP(hipMalloc(&d, sizeof(int)));
+    XF_HIP(hipMemsetAsync(d, 0, sizeof(int), s));
+  }
+  int read() {
+    int h = 0;
+    XF_HIP(hipMemcpyAsync(&h, d, sizeof(int), hipMemcpyDeviceToHost, s));
+    XF_HIP(hipStreamSynchronize(s));
+    return h;
+  }
+  ~ErrFlag() { hipFree(d); }
+};
+int nblocks(int64_t nc) {
+  int64_t b = (nc + kBlock - 1) / kBlock;
+  return (int)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
+}
+}  // namespace
+
+extern "C" int xf_pressure_rhs(xf_ctx c, xf_grid gp, const double *u, const double *v, const double *w,
+                               const double *rho, const double *hxU, const double *hyV, const double *hzW,
+                               const double *pbc, double tau, double *rhs) {
+  return guarded(c, [&] {
+    XF_REQUIRE(gp && u && v && w && hxU && hyV && hzW && rhs, XF_EINVAL, "xf_pressure_rhs: null argument");
+    XF_REQUIRE(tau > 0, XF_EINVAL, "xf_pressure_rhs: tau must be positive");
+    Steps sx(hxU, gp->n - 1, c->stream), sy(hyV, gp->l - 1, c->stream), sz(hzW, gp->m - 1, c->stream);
+    ErrFlag err(c->stream);
+    hipLaunchKernelGGL(k_pressure_rhs, dim3(nblocks(gp->ncell)), dim3(kBlock), 0, c->stream, gp->n, gp->l, gp->m,
+                       gp->d_codes, u, v, w, rho, sx.d, sy.d, sz.d, pbc, tau, rhs, err.d);
+    check_kernel("k_pressure_rhs");
+    const int e = err.read();
+    XF_REQUIRE(e == 0, XF_EINVAL,
+               "TVPProblem::SolveBySplitting: Method is not implemented for this border point type of pressure.");
+  });
+}
+
+extern "C" int xf_velocity_correct(xf_ctx c, xf_grid gv, int axis, const double *u_star, const double *p,
+                                   const double *rho, const double *hP, const double *bc, double tau, double *u_out) {
+  return guarded(c, [&] {
+    XF_REQUIRE(gv && u_star && p && hP && u_out, XF_EINVAL, "xf_velocity_correct: null argument");
+    XF_REQUIRE(axis >= 1 && axis <= 3, XF_EINVAL, "Unknown velocity number.");
+    XF_REQUIRE(u_out != u_star, XF_EINVAL, "xf_velocity_correct: u_out must not alias u_star");
+    const int E[3] = {gv->n, gv->l, gv->m};
+    Steps sp(hP, E[axis - 1] - 1, c->stream);
+    ErrFlag err(c->stream);
+    hipLaunchKernelGGL(k_vel_correct, dim3(nblocks(gv->ncell)), dim3(kBlock), 0, c->stream, gv->n, gv->l, gv->m,
+                       axis - 1, gv->d_codes, u_star, p, rho, sp.d, bc, tau, u_out, err.d);
+    hipLaunchKernelGGL(k_vel_correct_predef, dim3(nblocks(gv->ncell)), dim3(kBlock), 0, c->stream, gv->n, gv->l,
+                       gv->m, gv->d_codes, u_star, bc, u_out, err.d);
+    check_kernel("k_vel_correct");
+    const int e = err.read();
+    XF_REQUIRE(!(e & 4), XF_EINVAL, "TVPProblem::SolveBySplitting: Defined border condition checking error.");
+    XF_REQUIRE(!(e & 8), XF_EINVAL, "TVPProblem::SolveBySplitting: PreDefined border condition checking error - "
+                                    "actual values count must be 1.");
+    XF_REQUIRE(!(e & 16), XF_EINVAL,
+               "TVPProblem::SolveBySplitting: PreDefined border condition checking error - value incorrect.");
+    XF_REQUIRE(e == 0, XF_EINVAL,
+               "TVPProblem::SolveBySplitting: Method is not implemented for this point border type of velocity.");
+  });
+}
+
+extern "C" int xf_divergence_max(xf_ctx c, xf_grid gp, const double *u, const double *v, const double *w,
+                                 const double *hxU, const double *hyV, const double *hzW, double *max_abs,
+                                 int64_t *cell) {
+  return guarded(c, [&] {
+    XF_REQUIRE(gp && u && v && w && hxU && hyV && hzW && max_abs, XF_EINVAL, "xf_divergence_max: null argument");
+    Steps sx(hxU, gp->n - 1, c->stream), sy(hyV, gp->l - 1, c->stream), sz(hzW, gp->m - 1, c->stream);
+    int nb = nblocks(gp->ncell);
+    if (nb > 2048) nb = 2048;
+    double *pv = c->partials;
+    int64_t *pi = (int64_t *)(c->partials + kMaxPartials);
+    int64_t *d_idx = (int64_t *)(c->partials + 2 * kMaxPartials);
+    hipLaunchKernelGGL(k_divmax, dim3(nb), dim3(kBlock), 0, c->stream, gp->n, gp->l, gp->m, gp->d_codes, u, v, w,
+                       sx.d, sy.d, sz.d, pv, pi, c->counters, c->d_scalar, d_idx);
+    check_kernel("k_divmax");
+    int64_t idx;
+    XF_HIP(hipMemcpyAsync(max_abs, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipMemcpyAsync(&idx, d_idx, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipStreamSynchronize(c->stream));
+    if (cell) *cell = idx;
+  });
+}

@@ -1,0 +1,310 @@
+// Drop-in C++ host classes with the reference's call signatures, backed by
+// libxflow_hip.so through the C ABI in include/xflow.h.
+//
+// Reference-style host code (masks, separators, a T3dNormalGrid, an Rn space
+// model, TRnRelease3dSpace vectors, T3dLaplaceOperator, TSLAE) compiles
+// against this header unchanged at the call sites listed below and runs its
+// pressure solve on the GPU:
+//   T3dNumberMask(N,L,M), operator[]                    XMask.h:90-131
+//   TUniformSeparator / TRandomSeparator                XSeparator.cpp:25-99
+//   T3dNormalGrid(sep1, sep2, sep3, mask, normals)      XGrid.h:279-299
+//   TRnRelease3dSpaceModel(dataCount, masks)            XSpaceModel.h:499-533
+//   TRnSpace::AddModel / RemoveModel / RemoveAllModels  XSpace.h:272-274
+//   TRnRelease3dSpace(modelName), operator[]            XSpace.h:595-633
+//   x |= a*y + b*z, x |= A(y) - b, (x, y), (A(x), x),
+//   GetAbsMaxElement, EndIndex                          XSpace.h:17-221
+//   T3dLaplaceOperator(model, gridsCount, grids),
+//   operator()(idData,i,j,k), ConstructActivityTables   XOperator.h:292-319
+//   T3dVariableDivergentedOperator::withDensity         XOperator.h:410-423
+//   TSLAE(A, x, b), Result / GetResult, UpdateP/UpdateRP,
+//   SolveByConjugateGradients, SolveByBiConjugateGradients,
+//   SolveByMinimalResiduals, SolveByBiConjugateGradientsStab,
+//   static SolveByScalarRunning                         XSLAE.h:9-48
+// Differences, by design: TSLAE does not own (delete) its arguments (the
+// reference's destructor deletes them, XSLAE.cpp:21-30, which double-frees at
+// XProblem.cpp:767-768); the space mask of a model used with an operator must
+// be the one XProblem.cpp:4155-4182 derives from the grid's point types;
+// solvers stop with an error instead of looping forever on NaN or stagnation.
+// Errors are thrown as `const char*`, the reference's convention.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/xflow.h"
+
+const int TFictivePoint = 0;
+const int TActualPoint = 1;
+const int TBorderPoint = 2;
+const int TDefinedBorderPoint = 3;
+const int TPreDefinedBorderPoint = 4;
+const int TNormalBorderPoint = 5;
+const int TPreNormalBorderPoint = 6;
+const int TEquationBorderPoint = 7;
+const int TPreEquationBorderPoint = 8;
+
+namespace xflow_dropin {
+xf_ctx context();                                // one per host thread, device 0
+[[noreturn]] void fail(const std::string &msg);  // throws a stable const char*
+void check(int status, const char *what);
+}  // namespace xflow_dropin
+
+// ------------------------------------------------------------------ masks
+class TMask {
+ public:
+  virtual ~TMask() {}
+};
+
+class T3dNumberMask : public TMask {
+ public:
+  T3dNumberMask(int N, int L, int M);
+  T3dNumberMask &Copy() const;
+  int *const *operator[](int i) const { return rows_[i].data(); }
+  int GetN() const { return n_; }
+  int GetL() const { return l_; }
+  int GetM() const { return m_; }
+  const int *data() const { return data_.data(); }
+
+ private:
+  int n_, l_, m_;
+  std::vector<int> data_;
+  std::vector<std::vector<int *>> rows_;
+};
+
+// ------------------------------------------------------------- separators
+class TSeparator {
+ public:
+  virtual ~TSeparator() {}
+  int GetEndIndex() const { return end_index_; }
+  const double *GetSeparation() const { return sep_.data(); }
+  const double *GetDimension() const { return dim_.data(); }
+
+ protected:
+  int end_index_ = 0;
+  std::vector<double> sep_, dim_;
+};
+
+// XSeparator.cpp:25-64: node count rounded up with a 1e-11 slack, step
+// recomputed, coordinates by cumulative addition.
+class TUniformSeparator : public TSeparator {
+ public:
+  TUniformSeparator(double start, double end, double step);
+};
+
+// XSeparator.cpp:81-99
+class TRandomSeparator : public TSeparator {
+ public:
+  TRandomSeparator(int count, const double *dims);
+};
+
+struct T2dVector {
+  double X, Y;
+};
+struct T3dVector : public T2dVector {
+  double Z;
+};
+
+class TNodesGrid {
+ public:
+  virtual ~TNodesGrid() {}
+};
+
+class T3dNormalGrid : public TNodesGrid {
+ public:
+  T3dNormalGrid(TSeparator *s1, TSeparator *s2, TSeparator *s3, const TMask &mask,
+                const T3dVector *const *const *const *normals, bool deleteSeparators = true);
+  ~T3dNormalGrid();
+  const TSeparator &GetSeparator1() const { return *s_[0]; }
+  const TSeparator &GetSeparator2() const { return *s_[1]; }
+  const TSeparator &GetSeparator3() const { return *s_[2]; }
+  const T3dNumberMask &GetMask() const { return *mask_; }
+  // device grid for a given space mask (built on first use)
+  xf_grid device_grid() const;
+
+ private:
+  TSeparator *s_[3];
+  bool own_;
+  std::unique_ptr<T3dNumberMask> mask_;
+  std::vector<double> normals_;
+  std::vector<int32_t> has_normal_;
+  mutable xf_grid dev_ = nullptr;
+};
+
+// ------------------------------------------------------------ space model
+class TRnSpaceModel {
+ public:
+  virtual ~TRnSpaceModel() {}
+  int GetEndIndex() const { return end_index_; }
+
+ protected:
+  int end_index_ = 0;
+};
+
+class TRnRelease3dSpaceModel : public TRnSpaceModel {
+ public:
+  TRnRelease3dSpaceModel(int dataCount, const TMask *const *masks);
+  const T3dNumberMask &GetMask(int id) const;
+  int64_t ncell() const { return (int64_t)mask_->GetN() * mask_->GetL() * mask_->GetM(); }
+
+ private:
+  const T3dNumberMask *mask_;
+};
+
+// ------------------------------------------------------------- vectors
+class T3dLaplaceOperator;
+class TRnSpace;
+class TRnRelease3dSpace;
+
+struct TLinTerm {  // p * s
+  double p;
+  const TRnSpace *s;
+};
+struct TLinComb {  // p1*s1 + p2*s2
+  TLinTerm a, b;
+};
+struct TOpApply {  // A(x)
+  const T3dLaplaceOperator *A;
+  const TRnSpace *x;
+};
+struct TOpComb {  // A(x) + sign*b
+  TOpApply ax;
+  double sign;
+  const TRnSpace *b;
+};
+
+class TRnSpace {
+ public:
+  virtual ~TRnSpace();
+  static void AddModel(int name, TRnSpaceModel *model, bool deleteModel = false);
+  static void RemoveModel(int name);
+  static void RemoveAllModels();
+  static TRnRelease3dSpaceModel *model(int name);
+
+  int GetEndIndex() const { return model_->GetEndIndex(); }
+  TRnSpace &operator|=(const TRnSpace &s);   // masked copy
+  TRnSpace &operator|=(double zero);         // only 0 is supported (XSpace.h:92-102)
+  TRnSpace &operator|=(const TLinTerm &t);
+  TRnSpace &operator|=(const TLinComb &t);
+  TRnSpace &operator|=(const TOpApply &t);
+  TRnSpace &operator|=(const TOpComb &t);
+  double GetAbsMaxElement();
+  double GetAbsMaxElement(int &elementIndex);
+  TRnSpace &Copy() const;            // full copy, caller owns
+  TRnSpace &CreateInstance() const;  // zero vector of the same model, caller owns
+
+  // device access
+  double *device() const;            // uploads pending host writes
+  const TRnRelease3dSpaceModel &GetModel() const { return *model_; }
+  int model_name() const { return model_name_; }
+  void mark_device_modified() const;
+
+ protected:
+  explicit TRnSpace(int modelName);
+  int model_name_;
+  TRnRelease3dSpaceModel *model_;
+  mutable double *dev_ = nullptr;
+  mutable std::vector<double> host_;
+  mutable bool host_valid_ = true;   // host copy equals the device copy
+  mutable bool host_dirty_ = false;  // host copy was handed out for writing
+  void sync_to_host() const;
+  friend class TRnRelease3dSpace;
+};
+
+class TRnRelease3dSpace : public TRnSpace {
+ public:
+  explicit TRnRelease3dSpace(int modelName);
+  double **operator[](int i);  // host view; device copy refreshed on next device use
+
+ private:
+  std::vector<std::vector<double *>> rows_;
+};
+
+TLinTerm operator*(double p, const TRnSpace &s);
+TLinComb operator+(const TLinTerm &a, const TLinTerm &b);
+TLinComb operator-(const TLinTerm &a, const TLinTerm &b);
+TLinComb operator+(const TRnSpace &a, const TLinTerm &b);
+TLinComb operator-(const TRnSpace &a, const TLinTerm &b);
+TLinComb operator+(const TLinTerm &a, const TRnSpace &b);
+TLinComb operator-(const TLinTerm &a, const TRnSpace &b);
+TLinComb operator+(const TRnSpace &a, const TRnSpace &b);
+TLinComb operator-(const TRnSpace &a, const TRnSpace &b);
+TOpComb operator-(const TOpApply &a, const TRnSpace &b);
+TOpComb operator+(const TOpApply &a, const TRnSpace &b);
+double operator,(const TRnSpace &a, const TRnSpace &b);
+double operator,(const TOpApply &a, const TRnSpace &b);
+
+// ------------------------------------------------------------- operators
+class TRnLinearOperator {
+ public:
+  virtual ~TRnLinearOperator() {}
+};
+
+class T3dLaplaceOperator : public TRnLinearOperator {
+ public:
+  T3dLaplaceOperator(int modelName, int gridsCount, const TNodesGrid *const *grids);
+  TOpApply operator()(const TRnSpace &x) const {
+    last_arg_ = &x;
+    return TOpApply{this, &x};
+  }
+  // one row of A at (i, j, k) for the vector last passed to operator()(x)
+  // (the reference's per-cell virtual call, XOperator.cpp:938-1129)
+  double operator()(int idData, int i, int j, int k) const;
+  void ConstructActivityTables();
+  const std::vector<int32_t> &ActivityTable() const { return act_; }
+  xf_grid device_grid() const { return grid_->device_grid(); }
+  int model_name() const { return model_name_; }
+
+ protected:
+  int model_name_;
+  const T3dNormalGrid *grid_;
+  std::vector<int32_t> act_;
+  mutable const TRnSpace *last_arg_ = nullptr;
+  mutable std::vector<double> row_cache_;
+  mutable const double *row_cache_src_ = nullptr;
+};
+
+class T3dVariableDivergentedOperator : public T3dLaplaceOperator {
+ public:
+  T3dVariableDivergentedOperator(int modelName, int gridsCount, const TNodesGrid *const *grids)
+      : T3dLaplaceOperator(modelName, gridsCount, grids) {}
+  void withDensity(TRnRelease3dSpace &density);
+};
+
+// ------------------------------------------------------------------ TSLAE
+class TSLAE {
+ public:
+  TSLAE(const TRnLinearOperator &A, TRnSpace &x, TRnSpace &b);
+  virtual ~TSLAE();
+  const TRnSpace &GetResult() const { return *result_; }
+  // `slae.Result` reads like the reference's Property (XSLAE.h:20)
+  struct ResultProperty {
+    const TSLAE *owner;
+    operator const TRnSpace &() const { return owner->GetResult(); }
+  } Result{this};
+  static void SolveByScalarRunning(int M, double *u, const double *a, const double *b, const double *c,
+                                   const double *f);
+  void UpdateP(TRnSpace &p) { *x_ |= p; }
+  void UpdateRP(TRnSpace &rp) { *b_ |= rp; }
+  void SolveByConjugateGradients() { solve(XF_CG3); }
+  void SolveByBiConjugateGradients() { solve(XF_BICG); }
+  void SolveByMinimalResiduals() { solve(XF_MR); }
+  void SolveByBiConjugateGradientsStab() { solve(XF_BICGSTAB); }
+  // extensions: the GPU throughput, mixed-precision and learned-preconditioner paths
+  void SolveByFastConjugateGradients(double rel_tol = 1e-8, bool project_mean = true) {
+    solve(XF_CG, rel_tol, project_mean);
+  }
+  void SolveByMixedPrecision(double rel_tol = 1e-8, bool project_mean = true) {
+    solve(XF_CG_MP, rel_tol, project_mean);
+  }
+  const xf_solve_stats &LastStats() const { return stats_; }
+
+ private:
+  void solve(int method, double rel_tol = 0, bool project_mean = false);
+  const T3dLaplaceOperator *A_;
+  TRnSpace *x_, *b_;
+  std::unique_ptr<TRnSpace> result_;
+  xf_solve_stats stats_{};
+};
