@@ -188,6 +188,33 @@ int xf_velocity_correct(xf_ctx ctx, xf_grid gv, int axis, const double *u_star, 
 int xf_divergence_max(xf_ctx ctx, xf_grid gp, const double *u, const double *v, const double *w, const double *hxU,
                       const double *hyV, const double *hzW, double *max_abs, int64_t *cell);
 
+/* ------------------------------------------------- learned preconditioner */
+/* No reference counterpart: the learned preconditioner of XF_FCG_NN.  A
+ * 3-level 3-D U-Net (channels 16/32/64, 3x3x3 convolutions, BF16 weights and
+ * activations, FP32 accumulation).  Parameters are one FP32 vector: for each
+ * layer c0..c9, W[Cout][27][Cin] (tap = 9 dz + 3 dy + dx, offsets -1..1)
+ * then b[Cout].  Crop tensors are device [B][D][H][W] FP32 with D, H
+ * multiples of 16 and W a multiple of 64. */
+typedef struct xf_unet_s *xf_unet;
+int xf_unet_create(xf_ctx ctx, int seed, xf_unet *out);
+int xf_unet_destroy(xf_ctx ctx, xf_unet u);
+int xf_unet_param_count(xf_unet u, int64_t *count);
+int xf_unet_get_params(xf_ctx ctx, xf_unet u, float *host);
+int xf_unet_set_params(xf_ctx ctx, xf_unet u, const float *host);
+int xf_unet_get_grads(xf_ctx ctx, xf_unet u, float *host);
+/* y = net(r, mask) on crops (no scaling) */
+int xf_unet_forward(xf_ctx ctx, xf_unet u, int B, int D, int H, int W, const float *r, const float *mask, float *y);
+/* loss = mean over the mask of (net(r, mask) - target)^2 and its parameter
+ * gradient (read with xf_unet_get_grads) */
+int xf_unet_loss_grad(xf_ctx ctx, xf_unet u, int B, int D, int H, int W, const float *r, const float *mask,
+                      const float *target, double *loss);
+/* loss_grad followed by one Adam step (beta 0.9 / 0.999, eps 1e-8) */
+int xf_unet_train_step(xf_ctx ctx, xf_unet u, int B, int D, int H, int W, const float *r, const float *mask,
+                       const float *target, float lr, double *loss);
+/* the preconditioner on a pressure grid: z = s net(r/s) on the unknowns, 0
+ * elsewhere, s = rms of r over the unknowns (FP32 fields [ncell]) */
+int xf_unet_apply(xf_ctx ctx, xf_unet u, xf_grid g, const float *r, float *z);
+
 /* ------------------------------------------------------------ profiling */
 /* No reference counterpart (the reference only prints clock() deltas,
  * XProblem.cpp:5442-5445).  When enabled, the throughput solvers bracket

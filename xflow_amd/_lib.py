@@ -60,6 +60,16 @@ SIGNATURES = {
     "xf_pressure_rhs": (I, [P, P, P, P, P, P, P, P, P, P, D, P]),
     "xf_velocity_correct": (I, [P, P, I, P, P, P, P, P, D, P]),
     "xf_divergence_max": (I, [P, P, P, P, P, P, P, P, C.POINTER(D), C.POINTER(I64)]),
+    "xf_unet_create": (I, [P, I, C.POINTER(P)]),
+    "xf_unet_destroy": (I, [P, P]),
+    "xf_unet_param_count": (I, [P, C.POINTER(I64)]),
+    "xf_unet_get_params": (I, [P, P, P]),
+    "xf_unet_set_params": (I, [P, P, P]),
+    "xf_unet_get_grads": (I, [P, P, P]),
+    "xf_unet_forward": (I, [P, P, I, I, I, I, P, P, P]),
+    "xf_unet_loss_grad": (I, [P, P, I, I, I, I, P, P, P, C.POINTER(D)]),
+    "xf_unet_train_step": (I, [P, P, I, I, I, I, P, P, P, C.c_float, C.POINTER(D)]),
+    "xf_unet_apply": (I, [P, P, P, P, P]),
     "xf_profile_enable": (I, [P, I]),
     "xf_profile_reset": (I, [P]),
     "xf_profile_read": (I, [P, C.c_char_p, C.POINTER(D), C.POINTER(I64)]),

@@ -279,6 +279,13 @@ __global__ void k_hs_start(DevState *st, double tol, int rel, int slot_rr) {
   }
 }
 
+// restart plain CG inside FCG: rr = (r, r), beta = 0, keep the stopping target
+__global__ void k_restart_beta(DevState *st) {
+  st->s[HS_RR] = st->s[20 + 2];  // FC_RR2
+  st->s[HS_BETA] = 0;
+  st->done = 0;
+}
+
 // x += (X) e at unknowns
 __global__ void k_add_correction(int64_t nc, double *__restrict__ x, const float *__restrict__ e) {
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
@@ -472,6 +479,154 @@ void solve_mp_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
   s.iters = outer;
   s.inner_iters = inner_total;
   s.status = status;
+  s.resid = rr0 > 0 ? sqrt(rr / rr0) : 0;
+}
+
+// --------------------------------------------------- flexible CG + network
+enum { FC_PR = 20, FC_ZQ, FC_RR2, FC_SUMSQ };
+
+template <typename T>
+__global__ void k_dot_t(int64_t nc, const T *__restrict__ a, const T *__restrict__ b, DevState *st, double *partials,
+                        unsigned *counter, int slot) {
+  __shared__ double lds[16];
+  double acc = 0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
+    acc += (double)a[c] * (double)b[c];
+  double bs = block_sum(acc, lds);
+  double tot;
+  if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) st->s[slot] = tot;
+}
+
+// p = z + beta p, beta = -(z, q_old)/(p_old, q_old) (first=1: p = z)
+__global__ void k_fcg_dir(int64_t nc, const float *__restrict__ z, float *__restrict__ p, const DevState *st,
+                          int first) {
+  const double pq = st->s[HS_PQ];
+  const float beta = first || pq == 0 ? 0.0f : (float)(-st->s[FC_ZQ] / pq);
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
+    p[c] = first ? z[c] : z[c] + beta * p[c];
+}
+
+// alpha = (p, r)/(p, q)
+__global__ void k_fcg_alpha(DevState *st) {
+  const double pq = st->s[HS_PQ];
+  st->s[HS_ALPHA] = pq != 0 ? st->s[FC_PR] / pq : 0.0;
+}
+
+// Flexible CG (A-orthogonalisation against the previous direction, Notay's
+// FCG(1)) in FP32 with the network as preconditioner, inside the same FP64
+// iterative refinement as the mixed-precision solver.  If the inner residual
+// does not improve for stall_window iterations the inner solve continues with
+// plain FP32 CG (fallback).
+void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
+                  xf_solve_stats &s) {
+  require_fast(g);
+  XF_REQUIRE(o.precond, XF_EINVAL, "XF_FCG_NN needs opts.precond (an xf_unet)");
+  struct xf_unet_s *net = (struct xf_unet_s *)o.precond;
+  CgWork *w = work(c, g, true);
+  const int64_t nc = g->ncell;
+  Tiles tl = tiles(g);
+  const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
+  const int stall = o.stall_window > 0 ? o.stall_window : 20;
+  float *z = nullptr;
+  {
+    hipError_t e = hipMalloc(&z, sizeof(float) * nc);
+    XF_REQUIRE(e == hipSuccess, XF_ENOMEM, "FCG: out of device memory");
+    XF_HIP(hipMemsetAsync(z, 0, sizeof(float) * nc, c->stream));
+  }
+  struct Free {
+    float *p;
+    ~Free() { hipFree(p); }
+  } fz{z};
+  upload_state_zero(c);
+  XF_HIP(hipMemcpyAsync(x, x0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
+  auto poll = [&]() {
+    XF_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipStreamSynchronize(c->stream));
+  };
+  auto residual = [&]() {
+    hipLaunchKernelGGL((k_residual_fast<float>), dim3(tl.total), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
+                       g->d_flags, g->d_cx, g->d_cy, g->d_cz, b, x, w->r32, c->state, c->partials, c->counters,
+                       (int)HS_AUX);
+    poll();
+    return c->h_state->s[HS_AUX];
+  };
+  double rr = residual();
+  const double rr0 = rr;
+  const double stop2 = o.tol_mode == XF_TOL_REL ? o.tol * o.tol * rr0 : o.tol * o.tol;
+  int outer = 0, inner_total = 0, status = XF_OK, fallback = 0;
+  float *p = w->p32[0];
+  while (rr > stop2) {
+    if (rr != rr) { status = XF_ENAN; break; }
+    if (outer >= 60 || inner_total >= o.maxit) { status = XF_EMAXIT; break; }
+    outer++;
+    if (o.project_mean && g->nunk > 0) {
+      hipLaunchKernelGGL((k_sum_unknowns<float>), dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags,
+                         w->r32, c->state, c->partials, c->counters, (int)HS_SUM);
+      hipLaunchKernelGGL((k_sub_mean<float>), dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags, w->r32,
+                         c->state, (int)HS_SUM, (double)g->nunk);
+    }
+    hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags, w->r32,
+                       c->state, c->partials, c->counters, (int)HS_AUX);
+    XF_HIP(hipMemsetAsync(w->e32, 0, sizeof(float) * nc, c->stream));
+    XF_HIP(hipMemsetAsync(p, 0, sizeof(float) * nc, c->stream));
+    hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, inner_tol, 1, (int)HS_AUX);
+    poll();
+    double best = c->h_state->s[HS_RR];
+    int since_best = 0, it = 0;
+    bool use_net = !fallback;
+    while (!c->h_state->done) {
+      if (!use_net) {
+        // plain CG from the current (e, r): restart with p = r
+        XF_HIP(hipMemsetAsync(w->p32[0], 0, sizeof(float) * nc, c->stream));
+        XF_HIP(hipMemsetAsync(w->p32[1], 0, sizeof(float) * nc, c->stream));
+        hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags,
+                           w->r32, c->state, c->partials, c->counters, (int)FC_RR2);
+        hipLaunchKernelGGL(k_restart_beta, dim3(1), dim3(1), 0, c->stream, c->state);
+        const int before = c->h_state->iter;
+        hs_iterations<float, float>(c, g, w->e32, w->r32, w->p32[0], w->p32[1], w->q32, g->d_cxf, g->d_cyf,
+                                    g->d_czf, o.maxit - inner_total, 16);
+        it += c->h_state->iter - before;
+        break;
+      }
+      // z = M(r)
+      hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags,
+                         w->r32, c->state, c->partials, c->counters, (int)FC_SUMSQ);
+      unet_apply_grid(c, net, g, w->r32, z, c->state->s + FC_SUMSQ);
+      if (it > 0)
+        hipLaunchKernelGGL((k_dot_t<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, z, w->q32, c->state,
+                           c->partials, c->counters, (int)FC_ZQ);
+      hipLaunchKernelGGL(k_fcg_dir, dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, z, p, c->state, it == 0 ? 1 : 0);
+      // q = A p, (p, q); (p, r); alpha
+      hipLaunchKernelGGL((k_stencil_cg<float, false>), dim3(tl.total), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
+                         g->d_flags, g->d_cxf, g->d_cyf, g->d_czf, w->r32, p, p, w->q32, c->state, c->partials,
+                         c->counters);
+      hipLaunchKernelGGL((k_dot_t<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, p, w->r32, c->state,
+                         c->partials + kMaxPartials, c->counters + 1, (int)FC_PR);
+      hipLaunchKernelGGL(k_fcg_alpha, dim3(1), dim3(1), 0, c->stream, c->state);
+      hipLaunchKernelGGL((k_cg_update<float, float>), dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, w->e32, p,
+                         w->r32, w->q32, c->state, c->partials + kMaxPartials, c->counters + 1,
+                         o.maxit - inner_total);
+      check_kernel("fcg iteration");
+      poll();
+      it++;
+      const double rrn = c->h_state->s[HS_RR];
+      if (rrn < 0.99 * best) {
+        best = rrn;
+        since_best = 0;
+      } else if (++since_best >= stall) {
+        use_net = false;
+        fallback = 1;
+      }
+    }
+    inner_total += it;
+    if (c->h_state->status == XF_ENAN) { status = XF_ENAN; break; }
+    hipLaunchKernelGGL(k_add_correction, dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, x, w->e32);
+    rr = residual();
+  }
+  s.iters = outer;
+  s.inner_iters = inner_total;
+  s.status = status;
+  s.fallback = fallback;
   s.resid = rr0 > 0 ? sqrt(rr / rr0) : 0;
 }
 

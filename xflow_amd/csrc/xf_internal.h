@@ -189,3 +189,7 @@ void solve_hs_cg(xf_ctx ctx, xf_grid g, const double *b, const double *x0, doubl
 void solve_mp_cg(xf_ctx ctx, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
                  xf_solve_stats &s);
 double true_residual(xf_ctx ctx, xf_grid g, const double *b, const double *x, const double *x0);
+void solve_fcg_nn(xf_ctx ctx, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
+                  xf_solve_stats &s);
+// the learned preconditioner z = M(r) on a grid (xf_unet.hip); d_sumsq = sum of r^2 over the unknowns
+void unet_apply_grid(xf_ctx c, struct xf_unet_s *u, xf_grid g, const float *r32, float *z32, const double *d_sumsq);

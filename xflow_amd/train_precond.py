@@ -1,0 +1,115 @@
+"""Training of the learned preconditioner on residual/error pairs harvested
+from the solver itself.
+
+Harvest: for each Poisson problem (random solid spheres/boxes, RHS from the
+smoothed-random-velocity generator, seeds 100+), the FP64 GPU CG solves to a
+1e-12 relative residual for x*, then reruns with iteration caps k to collect
+the CG iterates x_k; each pair is (r_k = b - A x_k, e_k = x* - x_k) with the
+mean over the fluid removed from e_k, both divided by s = rms(r_k) over the
+fluid (the scaling the preconditioner applies).  Training draws random crops
+of the pairs and minimises the masked mean-squared error with Adam.
+
+    python -m xflow_amd.train_precond --out weights.npy [--problems 40 --steps 1500]
+"""
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+import xflow_amd as xf
+from xflow_amd import cases
+from xflow_amd.unet import UNet
+
+ITERATES = (0, 2, 5, 10, 20, 40, 80, 160, 320)
+
+
+def harvest(ctx, ncell, problems, seed0=100, log=print):
+    """Returns device tensors R, E, M of shape [P*len(ITERATES), ncell, ncell, ncell] (interior only)."""
+    Rs, Es, Ms = [], [], []
+    for pidx in range(problems):
+        seed = seed0 + pidx
+        case = cases.poisson_case(ncell, cases.random_obstacles(seed), seed)
+        g = xf.Grid(ctx, case["hx"], case["hy"], case["hz"], case["codes"])
+        b = torch.from_numpy(case["b"]).cuda()
+        x0 = torch.zeros_like(b)
+        xs = torch.zeros_like(b)
+        st = g.solve(b, x0, xs, xf.solve_opts(xf.XF_CG, tol=1e-12, tol_mode=xf.XF_TOL_REL, project_mean=1))
+        fluid = torch.from_numpy((case["codes"] == 1).astype(np.float32)).cuda()
+        nf = fluid.sum()
+        for k in ITERATES:
+            xk = torch.zeros_like(b)
+            if k > 0:
+                g.solve(b, x0, xk, xf.solve_opts(xf.XF_CG, tol=1e-30, tol_mode=xf.XF_TOL_REL, project_mean=1,
+                                                 maxit=k))
+            rk = torch.zeros_like(b)
+            g.apply(xk, rk, b, -1.0)  # A x - b
+            rk = -rk * fluid.double()
+            ek = (xs - xk) * fluid.double()
+            ek = (ek - (ek.sum() / nf) * fluid.double())
+            s = torch.sqrt((rk * rk).sum() / nf)
+            if s <= 0:
+                continue
+            sl = (slice(1, -1),) * 3
+            Rs.append((rk / s).float()[sl].contiguous())
+            Es.append((ek / s).float()[sl].contiguous())
+            Ms.append(fluid[sl].contiguous())
+        g.close()
+        log("harvest problem %d: CG %d iterations" % (pidx, st.iters))
+    return torch.stack(Rs), torch.stack(Es), torch.stack(Ms)
+
+
+def crops(R, E, M, B, S, gen):
+    n, D, H, W = R.shape
+    idx = torch.randint(0, n, (B,), generator=gen)
+    zs = torch.randint(0, D - S + 1, (B,), generator=gen)
+    ys = torch.randint(0, H - S + 1, (B,), generator=gen)
+    xs = torch.randint(0, W - S + 1, (B,), generator=gen)
+    rr, ee, mm = [], [], []
+    for q in range(B):
+        sl = (idx[q], slice(zs[q], zs[q] + S), slice(ys[q], ys[q] + S), slice(xs[q], xs[q] + S))
+        rr.append(R[sl])
+        ee.append(E[sl])
+        mm.append(M[sl])
+    return torch.stack(rr).contiguous(), torch.stack(mm).contiguous(), torch.stack(ee).contiguous()
+
+
+def train(ctx, R, E, M, steps=1500, batch=4, crop=64, lr=1e-3, seed=0, log=print):
+    net = UNet(ctx, seed=seed)
+    gen = torch.Generator().manual_seed(seed)
+    t0 = time.time()
+    hist = []
+    for it in range(steps):
+        lr_t = lr * (0.1 ** (it / max(steps, 1)))  # exponential decay to lr/10
+        r, m, e = crops(R, E, M, batch, crop, gen)
+        loss = net.train_step(r, m, e, lr_t)
+        hist.append(loss)
+        if it % 100 == 0 or it == steps - 1:
+            log("step %d loss %.4e (%.1f s)" % (it, loss, time.time() - t0))
+    return net, hist
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--ncell", type=int, default=128)
+    ap.add_argument("--problems", type=int, default=24)
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--crop", type=int, default=64)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    args = ap.parse_args()
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    ctx = xf.Context(0)
+    t0 = time.time()
+    R, E, M = harvest(ctx, args.ncell, args.problems, log=log)
+    log("harvested %d pairs in %.1f s" % (R.shape[0], time.time() - t0))
+    net, hist = train(ctx, R, E, M, args.steps, args.batch, args.crop, args.lr, log=log)
+    np.save(args.out, net.get_params())
+    log("saved %s; loss %.4e -> %.4e" % (args.out, hist[0], np.mean(hist[-50:])))
+
+
+if __name__ == "__main__":
+    main()
