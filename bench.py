@@ -45,6 +45,17 @@ def cpu_baseline(case, gpu_iters, sample_iters):
                       "iterations" % (it, threads, per_iter_ms, gpu_iters)}
 
 
+def load_network(ctx, args):
+    """The trained preconditioner: saved parameters if present, else None."""
+    from xflow_amd.unet import UNet
+    if not os.path.exists(args.weights):
+        log("no trained network at %s: skipping the learned-preconditioner solve" % args.weights)
+        return None
+    net = UNet(ctx, seed=0)
+    net.set_params(np.load(args.weights))
+    return net
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +65,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--plain-only", action="store_true")
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--nn-inner-tol", type=float, default=1e-4)
+    ap.add_argument("--weights", default=os.path.join(ROOT, "xflow_amd", "weights", "unet_poisson.npy"))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has a single MI355X; "
@@ -94,7 +109,42 @@ def main():
     kstats = profile.read(ctx)
     profile.enable(ctx, False)
     assert st.status == 0, st.status
+    plain = {"ms": round(wall_ms, 3), "iterations": float(np.mean(iters)), "true_residual": st.resid_true}
     log("plain FP64 CG: %.2f ms/solve wall, %s iters, true resid %.2e" % (wall_ms, iters, st.resid_true))
+
+    solvers = {"plain_fp64_cg": plain}
+    # mixed precision: FP32 CG sweeps + FP64 refinement
+    if not args.plain_only:
+        st = solve(xf.XF_CG_MP, inner_tol=args.inner_tol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = solve(xf.XF_CG_MP, inner_tol=args.inner_tol)
+        torch.cuda.synchronize()
+        mp_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        solvers["mixed_precision_cg"] = {"ms": round(mp_ms, 3), "outer": st.iters, "inner_iterations": st.inner_iters,
+                                         "true_residual": st.resid_true, "status": st.status}
+        log("mixed precision: %.2f ms/solve, %d outer / %d inner, true resid %.2e" %
+            (mp_ms, st.iters, st.inner_iters, st.resid_true))
+        net = load_network(ctx, args)
+        if net is not None:
+            def nn():
+                o = xf.solve_opts(xf.XF_FCG_NN, tol=1e-8, tol_mode=xf.XF_TOL_REL, project_mean=1,
+                                  inner_tol=args.nn_inner_tol)
+                o.precond = net.h
+                return g.solve(b, x0, x, o)
+            st = nn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                st = nn()
+            torch.cuda.synchronize()
+            nn_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            solvers["nn_preconditioned_mp"] = {"ms": round(nn_ms, 3), "outer": st.iters,
+                                               "inner_iterations": st.inner_iters, "fallback": st.fallback,
+                                               "true_residual": st.resid_true, "status": st.status}
+            log("NN-preconditioned MP: %.2f ms/solve, %d outer / %d inner, fallback %d, true resid %.2e" %
+                (nn_ms, st.iters, st.inner_iters, st.fallback, st.resid_true))
 
     interior = (args.n) ** 3
     ncell = (args.n + 2) ** 3
@@ -104,7 +154,7 @@ def main():
     k2_bytes = 48.0 * ncell
     k1_gbs = k1_bytes / (k1["ms"] * 1e-3) / 1e9
     k2_gbs = k2_bytes / (k2["ms"] * 1e-3) / 1e9
-    it_mean = float(np.mean(iters))
+    it_mean = plain["iterations"]
     mcups = fluid * it_mean / (wall_ms * 1e-3) / 1e6
 
     cpu = None
@@ -124,11 +174,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "pressure Poisson %d^3, random obstacles seed %d, FP64 CG to 1e-8 relative" %
+        "config": {"workload": "pressure Poisson %d^3, random obstacles seed %d, solve to 1e-8 relative residual" %
                                (args.n, args.seed),
-                   "solver": "plain FP64 CG (HS, fused kernels)", "iterations": it_mean,
+                   "value_solver": "plain FP64 CG (HS, fused kernels)", "iterations": it_mean,
                    "fluid_cells": fluid, "mcell_updates_per_s": round(mcups, 1),
-                   "true_residual": st.resid_true},
+                   "solvers": solvers},
         "roofline": {"bound": "hbm", "achieved": round(k1_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(k1_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_stencil_cg<double,true> (33 B per interior cell)",

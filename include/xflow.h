@@ -215,6 +215,15 @@ int xf_unet_train_step(xf_ctx ctx, xf_unet u, int B, int D, int H, int W, const 
  * elsewhere, s = rms of r over the unknowns (FP32 fields [ncell]) */
 int xf_unet_apply(xf_ctx ctx, xf_unet u, xf_grid g, const float *r, float *z);
 
+/* Transport velocities (u, v, w) at the nodes of velocity component `axis`
+ * (1..3): the component itself and the other two averaged over the four
+ * surrounding faces; axis 0: at the cell centres (each component averaged
+ * over its two faces).  No reference counterpart as a routine: the build's
+ * interpolation for its time-step drivers (the reference's own is inline in
+ * XProblem.cpp:5091-5305). */
+int xf_transport_velocity(xf_ctx ctx, xf_grid g, int axis, const double *u, const double *v, const double *w,
+                          double *tu, double *tv, double *tw);
+
 /* ------------------------------------------------------------ profiling */
 /* No reference counterpart (the reference only prints clock() deltas,
  * XProblem.cpp:5442-5445).  When enabled, the throughput solvers bracket

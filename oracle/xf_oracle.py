@@ -267,3 +267,40 @@ def divergence_max(codes, u, v, w, hxU, hyV, hzW):
     cell = C.c_int64(0)
     val = lib().xo_divergence_max(n, l, m, codes.ctypes.data, *[a.ctypes.data for a in arrs], C.byref(cell))
     return val, cell.value
+
+
+def transport_velocity(axis, u, v, w):
+    """numpy restatement of k_transport_vel (xflow_amd/csrc/xf_projection.hip)."""
+    f = [np.asarray(u, dtype=np.float64), np.asarray(v, dtype=np.float64), np.asarray(w, dtype=np.float64)]
+    a = axis - 1
+    N = f[0].shape
+    if axis == 0:
+        out = []
+        for e in range(3):
+            o = np.zeros(N)
+            s1 = [slice(None)] * 3
+            s0 = [slice(None)] * 3
+            s1[e] = slice(1, N[e])
+            s0[e] = slice(0, N[e] - 1)
+            o[tuple(s1)] = 0.5 * (f[e][tuple(s0)] + f[e][tuple(s1)])
+            out.append(o)
+        return out
+    out = []
+    for e in range(3):
+        if e == a:
+            out.append(f[e].copy())
+            continue
+        o = np.zeros(N)
+        sl_c = [slice(None)] * 3
+        sl_c[a] = slice(0, N[a] - 1)
+        sl_c[e] = slice(1, N[e])
+
+        def sh(da, de):
+            s = [slice(None)] * 3
+            s[a] = slice(0 + da, N[a] - 1 + da)
+            s[e] = slice(1 + de, N[e] + de)
+            return tuple(s)
+        fe = f[e]
+        o[tuple(sl_c)] = 0.25 * (((fe[sh(0, 0)] + fe[sh(0, -1)]) + fe[sh(1, 0)]) + fe[sh(1, -1)])
+        out.append(o)
+    return out

@@ -70,6 +70,7 @@ SIGNATURES = {
     "xf_unet_loss_grad": (I, [P, P, I, I, I, I, P, P, P, C.POINTER(D)]),
     "xf_unet_train_step": (I, [P, P, I, I, I, I, P, P, P, C.c_float, C.POINTER(D)]),
     "xf_unet_apply": (I, [P, P, P, P, P]),
+    "xf_transport_velocity": (I, [P, P, I, P, P, P, P, P, P]),
     "xf_profile_enable": (I, [P, I]),
     "xf_profile_reset": (I, [P]),
     "xf_profile_read": (I, [P, C.c_char_p, C.POINTER(D), C.POINTER(I64)]),

@@ -155,3 +155,43 @@ def sphere_case(ncell=128, seed=0, norm=None):
 def obstacle_case(ncell=256, seed=0, norm=None):
     """1-4 random spheres/boxes (the 256^3 mixed-precision / learned workloads)."""
     return poisson_case(ncell, random_obstacles(seed), seed, norm)
+
+
+def icosphere(subdivisions=4, radius=1.0, centre=(0.0, 0.0, 0.0)):
+    """Vertices and triangles of an icosphere: the icosahedron with each
+    triangle split into four `subdivisions` times, midpoints pushed to the
+    sphere (12 -> 42 -> 162 -> 642 -> 2562 vertices)."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdivisions):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                v = verts[a] + verts[b]
+                verts.append(v / np.linalg.norm(v))
+                cache[key] = len(verts) - 1
+            return cache[key]
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(verts) * radius + np.asarray(centre, dtype=np.float64)
+    return V, np.array(faces, dtype=np.int64)
+
+
+def vertex_areas(V, F):
+    """Lumped (one third of each adjacent triangle) surface area per vertex."""
+    a = np.zeros(len(V))
+    tri = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    area = 0.5 * np.linalg.norm(tri, axis=1)
+    for q in range(3):
+        np.add.at(a, F[:, q], area / 3.0)
+    return a

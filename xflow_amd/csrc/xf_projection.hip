@@ -246,3 +246,51 @@ extern "C" int xf_divergence_max(xf_ctx c, xf_grid gp, const double *u, const do
     if (cell) *cell = idx;
   });
 }
+
+// Transport velocities at the nodes of velocity component `axis` (1..3) on
+// the staggered pressure-shape layout: the component itself, and each other
+// component averaged over the four faces around the node (the build's
+// interpolation; the reference interpolates to the component's faces in
+// XProblem.cpp:5091-5305).  Ring entries copy the component's value.
+__global__ void k_transport_vel(int n, int l, int m, int axis, const double *__restrict__ u,
+                                const double *__restrict__ v, const double *__restrict__ w, double *__restrict__ tu,
+                                double *__restrict__ tv, double *__restrict__ tw) {
+  const int64_t nc = (int64_t)n * l * m;
+  const int64_t S[3] = {(int64_t)l * m, (int64_t)m, 1};
+  const int E[3] = {n, l, m};
+  const double *f[3] = {u, v, w};
+  double *o[3] = {tu, tv, tw};
+  const int a = axis - 1;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    const int idx[3] = {(int)(c / S[0]), (int)((c / m) % l), (int)(c % m)};
+    if (a < 0) {  // cell centres: each component averaged over its two faces
+      for (int e = 0; e < 3; e++) o[e][c] = idx[e] >= 1 ? 0.5 * (f[e][c - S[e]] + f[e][c]) : 0.0;
+      continue;
+    }
+    for (int e = 0; e < 3; e++) {
+      if (e == a) {
+        o[e][c] = f[e][c];
+        continue;
+      }
+      // node of component a sits at +1/2 along a; component e at +1/2 along e
+      const bool ok = idx[a] + 1 < E[a] && idx[e] >= 1;
+      if (!ok) {
+        o[e][c] = 0;
+        continue;
+      }
+      const double s = f[e][c] + f[e][c - S[e]] + f[e][c + S[a]] + f[e][c + S[a] - S[e]];
+      o[e][c] = 0.25 * s;
+    }
+  }
+}
+
+extern "C" int xf_transport_velocity(xf_ctx c, xf_grid g, int axis, const double *u, const double *v,
+                                     const double *w, double *tu, double *tv, double *tw) {
+  return guarded(c, [&] {
+    XF_REQUIRE(g && u && v && w && tu && tv && tw, XF_EINVAL, "xf_transport_velocity: null argument");
+    XF_REQUIRE(axis >= 0 && axis <= 3, XF_EINVAL, "Unknown velocity number.");
+    hipLaunchKernelGGL(k_transport_vel, dim3(nblocks(g->ncell)), dim3(kBlock), 0, c->stream, g->n, g->l, g->m, axis, u,
+                       v, w, tu, tv, tw);
+    check_kernel("k_transport_vel");
+  });
+}
