@@ -1,6 +1,6 @@
 // Reference-style host code on the drop-in classes (xflow_amd/host): the
 // manufactured-solution Poisson check of T3dPoissonProblem
-// (XProblem.cpp:554-817: u = sin x sin y sin z, -Lap u = 3u) written against
+// (XProblem.cpp:554-817, here with u = (1 + x) sin x sin y sin z) written against
 // the reference's call signatures, solved on the GPU.  Prints one JSON line;
 // exits non-zero when a check fails.  Run by tests/test_gpu_dropin.py.
 #include <cmath>
@@ -10,7 +10,12 @@
 
 #include "xflow_dropin.h"
 
-static double sol(double x, double y, double z) { return sin(x) * sin(y) * sin(z); }
+// u = (1 + x) sin x sin y sin z, -Lap u = 3 (1 + x) S - 2 cos x sin y sin z: zero on the
+// boundary of [0, pi]^3 and not a discrete eigenvector, so the solvers iterate.
+static double sol(double x, double y, double z) { return (1 + x) * sin(x) * sin(y) * sin(z); }
+static double rhs_of(double x, double y, double z) {
+  return 3 * (1 + x) * sin(x) * sin(y) * sin(z) - 2 * cos(x) * sin(y) * sin(z);
+}
 
 struct Result {
   int n;
@@ -53,7 +58,7 @@ static Result run(int cells) {
     for (int j = 0; j <= Lc; j++)
       for (int k = 0; k <= M; k++) {
         solution[i][j][k] = sol(x[i], y[j], z[k]);
-        rightPart[i][j][k] = 3 * sol(x[i], y[j], z[k]);
+        rightPart[i][j][k] = rhs_of(x[i], y[j], z[k]);
         // boundary values ride in the initial guess, as in the reference
         approximation[i][j][k] = mask[i][j][k] == TActualPoint ? 0.0 : sol(x[i], y[j], z[k]);
       }

@@ -131,9 +131,13 @@ def sphere_case(ncell, seed=0):
     return cases.sphere_case(ncell, seed)
 
 
-@pytest.mark.parametrize("ncell", [24, 40])
-def test_throughput_cg_matches_oracle(ctx, ncell):
-    case = sphere_case(ncell)
+@pytest.mark.parametrize("shape", [(24, 24, 24), (40, 40, 40), (30, 45, 37)])
+def test_throughput_cg_matches_oracle(ctx, shape):
+    from xflow_amd import cases
+    codes = cases.neumann_codes(None, [cases.sphere_obstacle((0.5, 0.5, 0.5), 0.2)], shape=shape)
+    b = cases.divergence_rhs(codes, seed=0)
+    case = {"codes": codes, "hx": cases.axis_steps(shape[0]), "hy": cases.axis_steps(shape[1]),
+            "hz": cases.axis_steps(shape[2]), "b": b}
     g = xf.Grid(ctx, case["hx"], case["hy"], case["hz"], case["codes"])
     b = dev(case["b"])
     x0 = torch.zeros_like(b)

@@ -84,13 +84,13 @@ def test_grid_apply_is_scaled_crop_forward(ctx):
     m = (case["codes"] == 1).astype(np.float32)
     rr = case["b"].astype(np.float32)
     s = np.sqrt((rr[m > 0].astype(np.float64) ** 2).mean())
-    ri = (rr / s * m)[1:-1, 1:-1, 1:-1][None]
-    mi = m[1:-1, 1:-1, 1:-1][None]
+    ri = (rr / np.float32(s) * m)[1:-1, 1:-1, 1:-1][None].astype(np.float32)
+    mi = m[1:-1, 1:-1, 1:-1][None].astype(np.float32)
     y = torch.zeros(ri.shape, dtype=torch.float32, device="cuda")
     net.forward(torch.from_numpy(np.ascontiguousarray(ri)).cuda(), torch.from_numpy(np.ascontiguousarray(mi)).cuda(), y)
     torch.cuda.synchronize()
     want = np.zeros_like(rr)
-    want[1:-1, 1:-1, 1:-1] = y.cpu().numpy()[0] * s * mi[0]
+    want[1:-1, 1:-1, 1:-1] = y.cpu().numpy()[0] * np.float32(s) * mi[0]
     assert rel(z.cpu().numpy(), want) < 1e-5
 
 

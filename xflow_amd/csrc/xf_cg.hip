@@ -70,6 +70,7 @@ enum { HS_RR = 0, HS_PQ, HS_ALPHA, HS_BETA, HS_STOP2, HS_RR0, HS_SUM, HS_CNT, HS
 
 struct Tiles {
   int kt, jt, it, total;
+  int blocks;  // 8 * ceil(total / 8): the XCD-aware mapping is a bijection on [0, blocks)
 };
 static Tiles tiles(xf_grid g) {
   Tiles t;
@@ -77,6 +78,7 @@ static Tiles tiles(xf_grid g) {
   t.jt = (g->l - 2 + TJ - 1) / TJ;
   t.it = (g->n - 2 + IC - 1) / IC;
   t.total = t.kt * t.jt * t.it;
+  t.blocks = 8 * ((t.total + 7) / 8);
   return t;
 }
 
@@ -337,7 +339,7 @@ static void hs_iterations(xf_ctx c, xf_grid g, X *x, T *r, T *p0, T *p1, T *q, c
     for (int s = 0; s < every; s++, it++) {
       T *pold = pb[it & 1], *pnew = pb[(it + 1) & 1];
       int e1 = c->prof ? prof_begin(c) : -1;
-      hipLaunchKernelGGL((k_stencil_cg<T, true>), dim3(tl.total), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
+      hipLaunchKernelGGL((k_stencil_cg<T, true>), dim3(tl.blocks), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
                          g->d_flags, cx, cy, cz, r, pold, pnew, q, c->state, c->partials, c->counters);
       if (c->prof) {
         prof_end(c, e1);
@@ -379,7 +381,7 @@ void solve_hs_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
   XF_HIP(hipMemcpyAsync(x, x0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
   XF_HIP(hipMemsetAsync(w->p[0], 0, sizeof(double) * nc, c->stream));
   XF_HIP(hipMemsetAsync(w->p[1], 0, sizeof(double) * nc, c->stream));
-  hipLaunchKernelGGL((k_residual_fast<double>), dim3(tl.total), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
+  hipLaunchKernelGGL((k_residual_fast<double>), dim3(tl.blocks), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
                      g->d_flags, g->d_cx, g->d_cy, g->d_cz, b, x, w->r, c->state, c->partials, c->counters,
                      (int)HS_AUX);
   if (o.project_mean && g->nunk > 0) {
@@ -430,7 +432,7 @@ void solve_mp_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
   std::vector<ProfMark> marks;
   auto residual = [&]() {
     int e = c->prof ? prof_begin(c) : -1;
-    hipLaunchKernelGGL((k_residual_fast<float>), dim3(tl.total), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
+    hipLaunchKernelGGL((k_residual_fast<float>), dim3(tl.blocks), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
                        g->d_flags, g->d_cx, g->d_cy, g->d_cz, b, x, w->r32, c->state, c->partials, c->counters,
                        (int)HS_AUX);
     if (c->prof) {
@@ -544,7 +546,7 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
     XF_HIP(hipStreamSynchronize(c->stream));
   };
   auto residual = [&]() {
-    hipLaunchKernelGGL((k_residual_fast<float>), dim3(tl.total), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
+    hipLaunchKernelGGL((k_residual_fast<float>), dim3(tl.blocks), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
                        g->d_flags, g->d_cx, g->d_cy, g->d_cz, b, x, w->r32, c->state, c->partials, c->counters,
                        (int)HS_AUX);
     poll();
@@ -597,7 +599,7 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
                            c->partials, c->counters, (int)FC_ZQ);
       hipLaunchKernelGGL(k_fcg_dir, dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, z, p, c->state, it == 0 ? 1 : 0);
       // q = A p, (p, q); (p, r); alpha
-      hipLaunchKernelGGL((k_stencil_cg<float, false>), dim3(tl.total), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
+      hipLaunchKernelGGL((k_stencil_cg<float, false>), dim3(tl.blocks), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
                          g->d_flags, g->d_cxf, g->d_cyf, g->d_czf, w->r32, p, p, w->q32, c->state, c->partials,
                          c->counters);
       hipLaunchKernelGGL((k_dot_t<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, p, w->r32, c->state,
