@@ -145,7 +145,7 @@ def case_solvers():
         ("cg_neumann20", "cg", 20, 1e3, True),
         ("bicg_dirichlet12", "bicg", 12, 1.0, False),
         ("mr_dirichlet8", "mr", 8, 10.0, False),
-        ("bicgstab_dirichlet12", "bicgstab", 12, 1e5, False),
+        ("bicgstab_dirichlet12", "bicgstab", 12, 1e8, False),
     ]:
         if neumann:
             codes = cases.neumann_codes(nc, [cases.sphere_obstacle((0.5, 0.5, 0.5), 0.2)]).astype(np.int32)

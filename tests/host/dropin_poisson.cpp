@@ -128,7 +128,7 @@ int main() {
     // second-order operator and solution error (halving h divides by ~4)
     bool ok = b.trunc < a.trunc / 3 && b.err_cg < a.err_cg / 3 && b.err_cg < 5e-3 &&
               fabs(b.err_bicgstab - b.err_cg) < 1e-4 && fabs(b.err_fast - b.err_cg) < 1e-4 &&
-              fabs(a.err_mr - a.err_cg) < 1e-4 && thomas_err < 1e-14 && a.it_cg > 5;
+              fabs(a.err_mr - a.err_cg) < 1e-4 && thomas_err < 1e-14 && a.it_cg >= 3 && b.it_cg > a.it_cg;
     return ok ? 0 : 1;
   } catch (const char *e) {
     fprintf(stderr, "thrown: %s\n", e);
