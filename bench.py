@@ -36,11 +36,11 @@ def cpu_baseline(case, gpu_iters, sample_iters):
     og = xo.Grid(case["hx"], case["hy"], case["hz"], case["codes"])
     x0 = np.zeros_like(case["b"])
     t0 = time.perf_counter()
-    _, it, _, _ = xo.cg_hs(og, case["b"], x0, 1e-30, sample_iters, project_mean=True)
+    _, it, _, _ = xo.cg_cg(og, case["b"], x0, 1e-30, sample_iters, project_mean=True)
     dt = time.perf_counter() - t0
     per_iter_ms = dt * 1e3 / max(it, 1)
     return {"value": per_iter_ms * gpu_iters, "unit": "ms/solve", "cores": threads, "kind": "port",
-            "sample": "%d iterations of the oracle FP64 CG (xo_cg_hs, %d OpenMP threads for the stencil, serial "
+            "sample": "%d iterations of the oracle FP64 single-reduction CG (xo_cg_cg, %d OpenMP threads for the stencil, serial "
                       "dot products) at the same 256^3 case, %.1f ms/iteration, scaled by the GPU solve's %d "
                       "iterations" % (it, threads, per_iter_ms, gpu_iters)}
 
@@ -148,12 +148,15 @@ def main():
 
     interior = (args.n) ** 3
     ncell = (args.n + 2) ** 3
-    k1 = kstats.get("cg_stencil_fp64", {"ms": float("nan"), "count": 0})
-    k2 = kstats.get("cg_update_fp64", {"ms": float("nan"), "count": 0})
-    k1_bytes = 33.0 * interior
-    k2_bytes = 48.0 * ncell
-    k1_gbs = k1_bytes / (k1["ms"] * 1e-3) / 1e9
-    k2_gbs = k2_bytes / (k2["ms"] * 1e-3) / 1e9
+    ka = kstats.get("cg_stencil_fp64", {"ms": float("nan"), "count": 0})
+    kb = kstats.get("cg_update_fp64", {"ms": float("nan"), "count": 0})
+    # algorithmic bytes: KA reads r and the flag byte and writes w (17 B per
+    # interior cell); KB reads r, w, p, s, x and writes p, s, x, r (72 B per
+    # array cell)
+    ka_bytes = 17.0 * interior
+    kb_bytes = 72.0 * ncell
+    ka_gbs = ka_bytes / (ka["ms"] * 1e-3) / 1e9
+    kb_gbs = kb_bytes / (kb["ms"] * 1e-3) / 1e9
     it_mean = plain["iterations"]
     mcups = fluid * it_mean / (wall_ms * 1e-3) / 1e6
 
@@ -176,14 +179,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": "pressure Poisson %d^3, random obstacles seed %d, solve to 1e-8 relative residual" %
                                (args.n, args.seed),
-                   "value_solver": "plain FP64 CG (HS, fused kernels)", "iterations": it_mean,
+                   "value_solver": "plain FP64 CG (single-reduction, two kernels per iteration)", "iterations": it_mean,
                    "fluid_cells": fluid, "mcell_updates_per_s": round(mcups, 1),
                    "solvers": solvers},
-        "roofline": {"bound": "hbm", "achieved": round(k1_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(k1_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_stencil_cg<double,true> (33 B per interior cell)",
-                     "kernel_ms": k1["ms"], "update_kernel_gbs": round(k2_gbs, 1),
-                     "update_kernel_ms": k2["ms"]},
+        "roofline": {"bound": "hbm", "achieved": round(kb_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(kb_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_cgcg_update<double,double> (72 B per array cell)",
+                     "kernel_us": round(kb["ms"] * 1e3, 2), "launches": kb["count"],
+                     "stencil_kernel": "k_cgcg_stencil<double> (17 B per interior cell)",
+                     "stencil_gbs": round(ka_gbs, 1), "stencil_us": round(ka["ms"] * 1e3, 2)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

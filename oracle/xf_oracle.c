@@ -16,6 +16,10 @@
  *       (tests/golden), plus closed forms in tests/test_oracle_math.py;
  *   xo_cg_hs: textbook Hestenes-Stiefel CG (not in the reference), pinned by
  *       a dense numpy solve on small systems and by agreement with xo_cg3.
+ *   xo_cg_cg: single-reduction CG of Chronopoulos and Gear (J. Comput. Appl.
+ *       Math. 25, 1989; not in the reference), the throughput solver's
+ *       recurrence; pinned by a dense numpy solve and by agreement with
+ *       xo_cg_hs (the two are the same method in exact arithmetic).
  *
  * Dot products are serial sums in i->j->k order, as in the reference
  * (XSpace.cpp:3219-3342).  Element-wise loops may run on several OpenMP
@@ -550,6 +554,68 @@ int xo_cg_hs(const xo_grid *g, const double *b, const double *x0, double *x, dou
   if (iters) *iters = it;
   if (resid) *resid = r0 > 0 ? sqrt(rr) / r0 : 0;
   free(r); free(p); free(q); free(e);
+  return st;
+}
+
+/* Single-reduction CG: per iteration w = A r, gamma = (r, r), delta = (w, r);
+ * stop when gamma <= rtol^2 gamma_0; beta = gamma/gamma_old,
+ * alpha = gamma/(delta - beta gamma/alpha_old) (alpha = gamma/delta first);
+ * p = r + beta p, s = w + beta s, e += alpha p, r -= alpha s. */
+int xo_cg_cg(const xo_grid *g, const double *b, const double *x0, double *x, double rtol, int maxit,
+             int project_mean, int *iters, double *resid) {
+  int64_t nc = (int64_t)g->n * g->l * g->m;
+  double *r = zeros(nc), *p = zeros(nc), *s = zeros(nc), *w = zeros(nc), *e = zeros(nc);
+  int st = XO_OK, it = 0;
+  memcpy(x, x0, sizeof(double) * nc);
+  xo_apply(g, x, b, -1, r);
+  int64_t nu = 0;
+  for (int64_t c = 0; c < nc; c++) {
+    if (g->space_mask[c] > 0) { r[c] = -r[c]; nu++; } else r[c] = 0;
+  }
+  if (project_mean && nu) {
+    double sm = 0;
+    for (int64_t c = 0; c < nc; c++) if (g->space_mask[c] > 0) sm += r[c];
+    sm /= (double)nu;
+    for (int64_t c = 0; c < nc; c++) if (g->space_mask[c] > 0) r[c] -= sm;
+  }
+  const double rr0 = xo_dot(g, r, r), stop2 = rtol * rtol * rr0;
+  double gam = rr0, gold = 0, aold = 0;
+  if (!(rr0 <= stop2)) {
+    for (;;) {
+      xo_apply(g, r, NULL, 0, w);
+      gam = xo_dot(g, r, r);
+      double del = xo_dot(g, w, r);
+      if (gam != gam || del != del) { st = XO_ENAN; break; }
+      if (gam <= stop2) break;
+      if (it >= maxit) { st = XO_EMAXIT; break; }
+      double beta = 0, alpha;
+      if (it == 0) alpha = gam / del;
+      else {
+        beta = gam / gold;
+        alpha = gam / (del - beta * gam / aold);
+      }
+      for (int64_t c = 0; c < nc; c++)
+        if (g->space_mask[c] > 0) {
+          p[c] = r[c] + beta * p[c];
+          s[c] = w[c] + beta * s[c];
+          e[c] += alpha * p[c];
+          r[c] -= alpha * s[c];
+        }
+      gold = gam;
+      aold = alpha;
+      it++;
+    }
+  }
+  if (project_mean && nu) {
+    double sm = 0;
+    for (int64_t c = 0; c < nc; c++) if (g->space_mask[c] > 0) sm += e[c];
+    sm /= (double)nu;
+    for (int64_t c = 0; c < nc; c++) if (g->space_mask[c] > 0) e[c] -= sm;
+  }
+  for (int64_t c = 0; c < nc; c++) if (g->space_mask[c] > 0) x[c] += e[c];
+  if (iters) *iters = it;
+  if (resid) *resid = rr0 > 0 ? sqrt(gam / rr0) : 0;
+  free(r); free(p); free(s); free(w); free(e);
   return st;
 }
 

@@ -63,6 +63,8 @@ int xo_bicgstab(const xo_grid *g, const double *b, const double *x0, double *x, 
                 int *iters, double *resid);
 int xo_cg_hs(const xo_grid *g, const double *b, const double *x0, double *x, double rtol, int maxit,
              int project_mean, int *iters, double *resid);
+int xo_cg_cg(const xo_grid *g, const double *b, const double *x0, double *x, double rtol, int maxit,
+             int project_mean, int *iters, double *resid);
 
 void xo_thomas(int M, double *u, const double *a, const double *b, const double *c, const double *f);
 int  xo_lod(const xo_grid *g, double *f, const double *f_prev, const double *u, const double *v,

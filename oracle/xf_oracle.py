@@ -48,6 +48,7 @@ def lib():
             "xo_mr": (i, [gp, p, p, p, d, i, p, p]),
             "xo_bicgstab": (i, [gp, p, p, p, d, i, p, p]),
             "xo_cg_hs": (i, [gp, p, p, p, d, i, i, p, p]),
+            "xo_cg_cg": (i, [gp, p, p, p, d, i, i, p, p]),
             "xo_thomas": (None, [i, p, p, p, p, p]),
             "xo_lod": (i, [gp, p, p, p, p, p, p, p, d, i]),
             "xo_ib_delta": (d, [d, d]),
@@ -190,6 +191,11 @@ def bicgstab(g, b, x0, eps=1e-5, maxit=100000):
 
 def cg_hs(g, b, x0, rtol=1e-8, maxit=100000, project_mean=False):
     return _solve(lib().xo_cg_hs, g, b, x0, rtol, maxit, int(bool(project_mean)))
+
+
+def cg_cg(g, b, x0, rtol=1e-8, maxit=100000, project_mean=False):
+    """Single-reduction (Chronopoulos-Gear) CG: the throughput solver's recurrence."""
+    return _solve(lib().xo_cg_cg, g, b, x0, rtol, maxit, int(bool(project_mean)))
 
 
 def thomas(u, a, b, c, f):

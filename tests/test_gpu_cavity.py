@@ -33,7 +33,7 @@ def oracle_cavity(n, steps, dt=1e-3, re=100.0):
             tr = xo.transport_velocity(a + 1, *vel)
             star.append(xo.lod(vgs[a], vel[a], vel[a], tr[0], tr[1], tr[2], bc[a], kappa, dt, 0))
         rhs = xo.pressure_rhs(pcodes, star[0], star[1], star[2], st, st, st, dt)
-        p, it, _, s = xo.cg_hs(pg, rhs, p, 1e-13, 100000, project_mean=True)
+        p, it, _, s = xo.cg_cg(pg, rhs, p, 1e-13, 100000, project_mean=True)
         vel = [xo.velocity_correct(vc[a], a + 1, star[a], p, st, dt, bc=bc[a], out=vel[a]) for a in range(3)]
     return vel, p
 

@@ -56,7 +56,7 @@ def oracle_channel(ny, steps, subdivisions):
             star[a] = star[a].copy()
             star[a][m] = 1.0 * star[a][m] + dt * force[m]
         rhs = xo.pressure_rhs(pcodes, *star, *st, dt)
-        p, it, _, s = xo.cg_hs(pg, rhs, p, 1e-13, 100000, project_mean=True)
+        p, it, _, s = xo.cg_cg(pg, rhs, p, 1e-13, 100000, project_mean=True)
         vel = [xo.velocity_correct(vc[a], a + 1, star[a], p, st[a], dt, bc=bc[a], out=vel[a]) for a in range(3)]
         tr = xo.transport_velocity(0, *vel)
         c = xo.lod(sg, c, c, tr[0], tr[1], tr[2], np.zeros(shape), kf, dt, 1)

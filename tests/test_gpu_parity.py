@@ -145,7 +145,7 @@ def test_throughput_cg_matches_oracle(ctx, shape):
     st = g.solve(b, x0, x, xf.solve_opts(xf.XF_CG, tol=1e-12, tol_mode=xf.XF_TOL_REL, project_mean=1))
     assert st.status == 0
     og = xo.Grid(case["hx"], case["hy"], case["hz"], case["codes"])
-    xr, it, res, s2 = xo.cg_hs(og, case["b"], np.zeros_like(case["b"]), 1e-12, 100000, project_mean=True)
+    xr, it, res, s2 = xo.cg_cg(og, case["b"], np.zeros_like(case["b"]), 1e-12, 100000, project_mean=True)
     assert s2 == 0
     assert abs(st.iters - it) <= 2, (st.iters, it)
     m = case["codes"] == 1

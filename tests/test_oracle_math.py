@@ -78,7 +78,7 @@ def spd_case(nc=6):
 
 
 @pytest.mark.parametrize("solver,eps", [("cg3", 1e-11), ("bicg", 1e-11), ("mr", 1e-11), ("bicgstab", 1e-12),
-                                        ("cg_hs", 1e-13)])
+                                        ("cg_hs", 1e-13), ("cg_cg", 1e-13)])
 def test_solvers_match_dense_solve(solver, eps):
     codes, g = spd_case()
     A, idx = dense_matrix(g)
@@ -89,7 +89,7 @@ def test_solvers_match_dense_solve(solver, eps):
     x, it, res, st = getattr(xo, solver)(g, b, np.zeros_like(b), eps)
     assert st == xo.OK
     assert np.allclose(x.ravel()[idx], want, rtol=1e-8, atol=1e-9 * np.abs(want).max())
-    if solver in ("cg3", "bicg", "cg_hs"):
+    if solver in ("cg3", "bicg", "cg_hs", "cg_cg"):
         assert it <= idx.size + 5  # CG-type: at most n steps in exact arithmetic (+ rounding slack)
 
 
@@ -103,6 +103,31 @@ def test_cg3_and_hs_cg_iterates_agree():
     xh, ith, _, _ = xo.cg_hs(g, b, np.zeros_like(b), 1e-8)
     assert abs(it3 - ith) <= 2
     assert np.linalg.norm(x3 - xh) <= 1e-6 * np.linalg.norm(xh)
+
+
+@pytest.mark.parametrize("neumann", [False, True])
+def test_single_reduction_cg_agrees_with_hs_cg(neumann):
+    """Chronopoulos-Gear and Hestenes-Stiefel CG produce the same iterates in
+    exact arithmetic: same iteration count (+-2 for rounding) and the same
+    solution; a wrong alpha or beta recurrence changes the count or diverges."""
+    from xflow_amd import cases as cs
+    if neumann:
+        codes = cs.neumann_codes(14, [cs.sphere_obstacle((0.5, 0.5, 0.5), 0.25)]).astype(np.int32)
+        b = cs.divergence_rhs(codes, seed=4)
+    else:
+        codes, _ = spd_case(12)
+        b = cs.divergence_rhs(codes, seed=4)
+        b[codes == 1] += 0.2
+    g = grid_from_codes(codes, 1.0 / (codes.shape[0] - 2))
+    xh, ith, rh, sh = xo.cg_hs(g, b, np.zeros_like(b), 1e-11, project_mean=neumann)
+    xc, itc, rc, sc = xo.cg_cg(g, b, np.zeros_like(b), 1e-11, project_mean=neumann)
+    assert sh == xo.OK and sc == xo.OK
+    assert abs(ith - itc) <= 2, (ith, itc)
+    m = g.space_mask > 0
+    a, r = xh[m], xc[m]
+    if neumann:
+        a, r = a - a.mean(), r - r.mean()
+    assert np.linalg.norm(a - r) <= 1e-9 * np.linalg.norm(a)
 
 
 def test_mr_residual_monotone():

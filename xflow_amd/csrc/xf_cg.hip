@@ -1,14 +1,24 @@
 // Throughput conjugate gradients for the masked 7-point pressure operator.
 //
-// Hestenes-Stiefel CG with two fused kernels per iteration on zero-padded
-// vectors (entries off the unknowns are 0 in r, p, q):
-//   K1  p' = r + beta p (on the fly, double-buffered), q = A p', (p', q)
-//       -> alpha = (r, r) / (p', q)                  [2.5-D marching stencil]
-//   K2  x += alpha p', r -= alpha q, (r, r)
-//       -> beta, convergence flag                    [streaming]
-// Algorithmic traffic per cell and iteration: K1 reads r, p, the flag byte and
-// writes p', q; K2 reads x, p', r, q and writes x, r: 33 + 48 = 81 B in FP64,
-// 41 B in FP32 (x stays FP64 in the mixed-precision outer loop).
+// Plain solves use the single-reduction CG of Chronopoulos and Gear ("s-step
+// iterative methods for symmetric linear systems", J. Comput. Appl. Math. 25,
+// 1989): both inner products of an iteration come from one pass, so one
+// iteration is two kernels with one grid-wide reduction between them:
+//   KA  w = A r, per-tile partials of gamma = (r, r) and delta = (w, r)
+//                                                   [2.5-D marching stencil]
+//   KB  every block reduces the partials itself (same order in every block,
+//       so all blocks hold bit-identical gamma, delta), then
+//       beta = gamma/gamma_old, alpha = gamma/(delta - beta gamma/alpha_old),
+//       p = r + beta p, s = w + beta s, x += alpha p, r -= alpha s
+//                                                   [streaming]
+// No kernel needs a device-scope fence or atomic: a fence per block (the
+// "last block finishes the sum" pattern) writes back the XCD's L2 and cost
+// more than the data movement itself on this chip.  Scalars that carry over
+// between iterations (gamma, alpha) live in two parity slots of the device
+// state, written by block 0 of KB into the slot the next launch reads.
+// Algorithmic traffic per cell and iteration: KA reads r and the flag byte
+// and writes w (17 B in FP64, 9 B in FP32, interior cells); KB reads r, w, p,
+// s, x and writes p, s, x, r (72 B in FP64, 36 B in FP32, all array cells).
 //
 // The stencil is the reference's Actual-row formula (XOperator.cpp:971-1019)
 // with the step ratios folded into per-node coefficients
@@ -20,8 +30,9 @@
 // (Neumann-by-mask and Dirichlet-by-Defined-point masks); other masks use the
 // reference-semantics solvers.
 //
-// Reductions are deterministic: a fixed block partition, block sums in wave
-// order, and the last block adds the partials in block order.
+// Reductions are deterministic: fixed tile partition, block sums in wave
+// order, partials added in block order.
+#include <algorithm>
 #include <cmath>
 
 #include "xf_internal.h"
@@ -30,17 +41,21 @@ constexpr int TK = 64;   // k extent of a tile (one wavefront)
 constexpr int TJ = 4;    // j extent of a tile (waves per block)
 constexpr int IC = 32;   // i planes marched per block
 constexpr int kStreamBlocks = 4096;
+constexpr int kCgTileBlocks = 2048;   // KA blocks (persistent over tiles): bounds the partials KB reads
+constexpr int kCgUpdateBlocks = 1024; // KB blocks (grid-stride)
 
 struct CgWork {
   int64_t n = 0;
   double *r = nullptr, *p[2] = {nullptr, nullptr}, *q = nullptr, *tmp = nullptr;
   float *r32 = nullptr, *p32[2] = {nullptr, nullptr}, *q32 = nullptr, *e32 = nullptr;
+  double *part = nullptr;  // KA partials: [kCgTileBlocks][2]
 };
 
 void free_cg_work(CgWork *w) {
   if (!w) return;
   hipFree(w->r); hipFree(w->p[0]); hipFree(w->p[1]); hipFree(w->q); hipFree(w->tmp);
   hipFree(w->r32); hipFree(w->p32[0]); hipFree(w->p32[1]); hipFree(w->q32); hipFree(w->e32);
+  hipFree(w->part);
   delete w;
 }
 
@@ -59,6 +74,7 @@ static CgWork *work(xf_ctx c, xf_grid g, bool fp32) {
     }
   };
   get(&w->r, n * 8); get(&w->p[0], n * 8); get(&w->p[1], n * 8); get(&w->q, n * 8);
+  get(&w->part, sizeof(double) * 2 * kCgTileBlocks);
   if (fp32) {
     get(&w->r32, n * 4); get(&w->p32[0], n * 4); get(&w->p32[1], n * 4); get(&w->q32, n * 4); get(&w->e32, n * 4);
   }
@@ -66,7 +82,8 @@ static CgWork *work(xf_ctx c, xf_grid g, bool fp32) {
 }
 
 // State slots of the throughput solvers.
-enum { HS_RR = 0, HS_PQ, HS_ALPHA, HS_BETA, HS_STOP2, HS_RR0, HS_SUM, HS_CNT, HS_AUX };
+enum { HS_RR = 0, HS_PQ, HS_ALPHA, HS_BETA, HS_STOP2, HS_RR0, HS_SUM, HS_CNT, HS_AUX,
+       CG_GAMMA = 10, CG_ALPHA = 12 };  // CG_GAMMA, CG_ALPHA: two parity slots each
 
 struct Tiles {
   int kt, jt, it, total;
@@ -82,20 +99,20 @@ static Tiles tiles(xf_grid g) {
   return t;
 }
 
-// K1: optional p' = r + beta p_old, q = A p', deterministic (p', q).
+// q = A p over the interior and (p, q) -> alpha = rr / (p, q) (the
+// flexible-CG step; plain solves use the KA/KB pair below).
 // One wavefront per (j) row of 64 k-cells; each block marches IC planes in i,
-// keeping p'(i-1), p'(i), p'(i+1) of its own column in registers; the four
+// keeping p(i-1), p(i), p(i+1) of its own column in registers; the four
 // in-plane neighbours come from the L1/L2-resident rows of the same plane.
 // Block index -> tile mapping is XCD-aware: the hardware deals workgroups to
 // the 8 XCDs round-robin, so consecutive tiles (which share halo rows) are
 // given to workgroups on the same XCD.
-template <typename T, bool FUSE>
-__global__ __launch_bounds__(256) void k_stencil_cg(int n, int l, int m, Tiles tl, const uint8_t *__restrict__ flags,
-                                                     const T *__restrict__ cx, const T *__restrict__ cy,
-                                                     const T *__restrict__ cz, const T *__restrict__ r,
-                                                     const T *__restrict__ pold, T *__restrict__ pnew,
-                                                     T *__restrict__ q, DevState *st, double *partials,
-                                                     unsigned *counter) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_pq(int n, int l, int m, Tiles tl, const uint8_t *__restrict__ flags,
+                                                   const T *__restrict__ cx, const T *__restrict__ cy,
+                                                   const T *__restrict__ cz, const T *__restrict__ p,
+                                                   T *__restrict__ q, DevState *st, double *partials,
+                                                   unsigned *counter) {
   __shared__ double lds[16];
   if (st->done) return;
   const int per = (tl.total + 7) / 8;
@@ -112,22 +129,17 @@ __global__ __launch_bounds__(256) void k_stencil_cg(int n, int l, int m, Tiles t
     const int i1 = min(i0 + IC, n - 1);
     if (k <= m - 2 && j <= l - 2) {
       const int64_t sI = (int64_t)l * m, sJ = m;
-      const T beta = FUSE ? (T)st->s[HS_BETA] : T(0);
-      auto P = [&](int64_t c) -> T {
-        if (FUSE) return r[c] + beta * pold[c];
-        return pold[c];
-      };
       const T czm = cz[2 * k], czp = cz[2 * k + 1];
       const T cym = cy[2 * j], cyp = cy[2 * j + 1];
       int64_t c = ((int64_t)i0 * l + j) * m + k;
-      T pm = P(c - sI), pc = P(c);
+      T pm = p[c - sI], pc = p[c];
       for (int i = i0; i < i1; i++, c += sI) {
-        const T pp = P(c + sI);
+        const T pp = p[c + sI];
         const uint8_t fl = flags[c];
         T v = 0;
         if (fl & FL_ACTUAL) {
           const T cxm = cx[2 * i], cxp = cx[2 * i + 1];
-          const T pjm = P(c - sJ), pjp = P(c + sJ), pkm = P(c - 1), pkp = P(c + 1);
+          const T pjm = p[c - sJ], pjp = p[c + sJ], pkm = p[c - 1], pkp = p[c + 1];
           T tx = ((fl & FL_IP) ? cxp * (pp - pc) : T(0)) - ((fl & FL_IM) ? cxm * (pc - pm) : T(0));
           T ty = ((fl & FL_JP) ? cyp * (pjp - pc) : T(0)) - ((fl & FL_JM) ? cym * (pc - pjm) : T(0));
           T tz = ((fl & FL_KP) ? czp * (pkp - pc) : T(0)) - ((fl & FL_KM) ? czm * (pc - pkm) : T(0));
@@ -135,7 +147,6 @@ __global__ __launch_bounds__(256) void k_stencil_cg(int n, int l, int m, Tiles t
           acc += (double)pc * (double)v;
         }
         q[c] = v;
-        if (FUSE) pnew[c] = pc;
         pm = pc;
         pc = pp;
       }
@@ -150,6 +161,153 @@ __global__ __launch_bounds__(256) void k_stencil_cg(int n, int l, int m, Tiles t
       st->status = (tot != tot) ? XF_ENAN : st->status;
     } else {
       st->s[HS_ALPHA] = st->s[HS_RR] / tot;
+    }
+  }
+}
+
+// Block sum valid in every thread.  Butterfly shuffles leave the same value in
+// all lanes (a + b == b + a exactly), and the wave sums are added in wave
+// order, so every block that sums the same data gets the same bits.
+__device__ inline double block_sum_all(double v, double *lds) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double r = 0;
+  for (int q = 0; q < nw; q++) r += lds[q];
+  return r;
+}
+
+// KA: w = A r over the interior, per-block partials (gamma, delta) =
+// ((r, r), (w, r)) over Actual cells.  Blocks are persistent over tiles
+// (bounded partial count); tile order within an XCD as in k_apply_pq.
+template <typename T>
+__global__ __launch_bounds__(256) void k_cgcg_stencil(int n, int l, int m, Tiles tl,
+                                                       const uint8_t *__restrict__ flags, const T *__restrict__ cx,
+                                                       const T *__restrict__ cy, const T *__restrict__ cz,
+                                                       const T *__restrict__ r, T *__restrict__ w,
+                                                       const DevState *st, double *__restrict__ part) {
+  __shared__ double lds[8];
+  if (st->done) return;
+  const int per = (tl.total + 7) / 8;      // tiles per XCD
+  const int bx = gridDim.x >> 3;            // blocks per XCD
+  const int xcd = blockIdx.x & 7;
+  double gam = 0, del = 0;
+  const int64_t sI = (int64_t)l * m, sJ = m;
+  for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
+    const int t = xcd * per + tq;
+    if (t >= tl.total) break;
+    const int ktile = t % tl.kt;
+    const int jtile = (t / tl.kt) % tl.jt;
+    const int itile = t / (tl.kt * tl.jt);
+    const int k = 1 + ktile * TK + (threadIdx.x & 63);
+    const int j = 1 + jtile * TJ + (threadIdx.x >> 6);
+    const int i0 = 1 + itile * IC;
+    const int i1 = min(i0 + IC, n - 1);
+    if (k > m - 2 || j > l - 2) continue;
+    const T czm = cz[2 * k], czp = cz[2 * k + 1];
+    const T cym = cy[2 * j], cyp = cy[2 * j + 1];
+    int64_t c = ((int64_t)i0 * l + j) * m + k;
+    T rm = r[c - sI], rc = r[c];
+    for (int i = i0; i < i1; i++, c += sI) {
+      const T rp = r[c + sI];
+      const uint8_t fl = flags[c];
+      T v = 0;
+      if (fl & FL_ACTUAL) {
+        const T cxm = cx[2 * i], cxp = cx[2 * i + 1];
+        const T rjm = r[c - sJ], rjp = r[c + sJ], rkm = r[c - 1], rkp = r[c + 1];
+        T tx = ((fl & FL_IP) ? cxp * (rp - rc) : T(0)) - ((fl & FL_IM) ? cxm * (rc - rm) : T(0));
+        T ty = ((fl & FL_JP) ? cyp * (rjp - rc) : T(0)) - ((fl & FL_JM) ? cym * (rc - rjm) : T(0));
+        T tz = ((fl & FL_KP) ? czp * (rkp - rc) : T(0)) - ((fl & FL_KM) ? czm * (rc - rkm) : T(0));
+        v = -(tx + ty + tz);
+        gam += (double)rc * (double)rc;
+        del += (double)v * (double)rc;
+      }
+      w[c] = v;
+      rm = rc;
+      rc = rp;
+    }
+  }
+  gam = block_sum_all(gam, lds);
+  del = block_sum_all(del, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = gam;
+    part[2 * blockIdx.x + 1] = del;
+  }
+}
+
+// KB: reduce KA's partials, decide (converged / NaN / iteration cap /
+// breakdown: the same decision in every block), then the fused update.
+// `it` is the number of updates applied before this launch; the first launch
+// (it == 0) takes p = r, s = w without reading p and s.
+template <typename T, typename X>
+__global__ __launch_bounds__(256) void k_cgcg_update(int64_t nc, int npart, const double *__restrict__ part,
+                                                      X *__restrict__ x, T *__restrict__ r, const T *__restrict__ w,
+                                                      T *__restrict__ p, T *__restrict__ s, DevState *st, int it,
+                                                      int maxit) {
+  __shared__ double lds[8];
+  if (st->done) return;
+  double gam = 0, del = 0;
+  for (int q = threadIdx.x; q < npart; q += blockDim.x) {
+    gam += part[2 * q];
+    del += part[2 * q + 1];
+  }
+  gam = block_sum_all(gam, lds);
+  del = block_sum_all(del, lds);
+  const int par = it & 1;
+  double beta = 0, alpha;
+  if (it == 0) {
+    alpha = gam / del;
+  } else {
+    beta = gam / st->s[CG_GAMMA + par];
+    alpha = gam / (del - beta * gam / st->s[CG_ALPHA + par]);
+  }
+  int stop = 0, status = XF_OK;
+  if (gam != gam || del != del) {
+    stop = 1;
+    status = XF_ENAN;
+  } else if (gam <= st->s[HS_STOP2]) {
+    stop = 1;
+  } else if (it >= maxit) {
+    stop = 1;
+    status = XF_EMAXIT;
+  } else if (!(fabs(alpha) < INFINITY) || alpha == 0) {
+    stop = 1;
+    status = XF_EBREAKDOWN;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->s[HS_RR] = gam;
+    if (stop) {
+      st->iter = it;
+      st->status = status;
+      st->done = 1;
+    } else {
+      st->s[CG_GAMMA + (par ^ 1)] = gam;
+      st->s[CG_ALPHA + (par ^ 1)] = alpha;
+      st->iter = it + 1;
+    }
+  }
+  if (stop) return;
+  const T a = (T)alpha, bt = (T)beta;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  if (it == 0) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += stride) {
+      const T pn = r[c], sn = w[c];
+      p[c] = pn;
+      s[c] = sn;
+      x[c] = x[c] + (X)a * (X)pn;
+      r[c] = pn - a * sn;
+    }
+  } else {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += stride) {
+      const T rc = r[c];
+      const T pn = rc + bt * p[c];
+      const T sn = w[c] + bt * s[c];
+      p[c] = pn;
+      s[c] = sn;
+      x[c] = x[c] + (X)a * (X)pn;
+      r[c] = rc - a * sn;
     }
   }
 }
@@ -322,45 +480,53 @@ static int reduce_nb(int64_t nc) {
   return nb > 2048 ? 2048 : nb;
 }
 
-// Runs HS-CG iterations on A e = r (r, p zero-padded; x is the FP64 or FP32
-// accumulator).  State must hold HS_RR/HS_STOP2 (k_hs_start).
+// KA tiling for the persistent stencil: at most kCgTileBlocks blocks, a multiple of 8.
+static int cgcg_blocks(const Tiles &tl) {
+  int nb = 8 * ((tl.total + 7) / 8);
+  return nb > kCgTileBlocks ? kCgTileBlocks : nb;
+}
+
+// Runs single-reduction CG iterations on A e = r from p = r (r zero-padded,
+// x the FP64 or FP32 accumulator, w and s scratch).  State must hold
+// HS_STOP2 and done = 0 (k_hs_start).  Polls the device state every `every`
+// iterations; h_state->iter counts the updates applied.
 template <typename T, typename X>
-static void hs_iterations(xf_ctx c, xf_grid g, X *x, T *r, T *p0, T *p1, T *q, const T *cx, const T *cy,
-                          const T *cz, int maxit, int every) {
+static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r, T *w, T *p, T *s, const T *cx, const T *cy,
+                            const T *cz, int maxit, int every) {
+  CgWork *wk = g->work;
   Tiles tl = tiles(g);
-  T *pb[2] = {p0, p1};
+  const int nbA = cgcg_blocks(tl);
+  const int nbB = std::max(1, std::min(kCgUpdateBlocks, (int)((g->ncell + 255) / 256)));
   const bool f64 = sizeof(T) == 8;
   const char *n1 = f64 ? "cg_stencil_fp64" : "cg_stencil_fp32";
   const char *n2 = f64 ? "cg_update_fp64" : "cg_update_fp32";
   std::vector<ProfMark> marks;
   int it = 0;
   for (;;) {
-    const int before = c->h_state->iter;
-    for (int s = 0; s < every; s++, it++) {
-      T *pold = pb[it & 1], *pnew = pb[(it + 1) & 1];
+    for (int q = 0; q < every; q++, it++) {
       int e1 = c->prof ? prof_begin(c) : -1;
-      hipLaunchKernelGGL((k_stencil_cg<T, true>), dim3(tl.blocks), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
-                         g->d_flags, cx, cy, cz, r, pold, pnew, q, c->state, c->partials, c->counters);
+      hipLaunchKernelGGL((k_cgcg_stencil<T>), dim3(nbA), dim3(256), 0, c->stream, g->n, g->l, g->m, tl, g->d_flags,
+                         cx, cy, cz, r, w, c->state, wk->part);
       if (c->prof) {
         prof_end(c, e1);
         marks.push_back({n1, e1});
       }
       int e2 = c->prof ? prof_begin(c) : -1;
-      hipLaunchKernelGGL((k_cg_update<T, X>), dim3(stream_nb(g->ncell)), dim3(256), 0, c->stream, g->ncell, x,
-                         pnew, r, q, c->state, c->partials + kMaxPartials, c->counters + 1, maxit);
+      hipLaunchKernelGGL((k_cgcg_update<T, X>), dim3(nbB), dim3(256), 0, c->stream, g->ncell, nbA, wk->part, x, r,
+                         w, p, s, c->state, it, maxit);
       if (c->prof) {
         prof_end(c, e2);
         marks.push_back({n2, e2});
       }
     }
-    check_kernel("hs_iterations");
+    check_kernel("cg iterations");
     XF_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
     XF_HIP(hipStreamSynchronize(c->stream));
     if (c->prof) {
-      long long ran = c->h_state->iter - before;
-      // the stencil of the converging iteration ran too; a stencil that hit a
-      // zero (p, Ap) stopped the loop before its update
-      prof_collect(c, marks, {{n1, ran}, {n2, ran}});
+      // every stencil up to the deciding one ran in full; the deciding update
+      // launch only reduced, so only the `ran` updates are counted
+      const long long ran = c->h_state->iter;
+      prof_collect(c, marks, {{n1, ran + (c->h_state->done ? 1 : 0)}, {n2, ran}});
     }
     if (c->h_state->done) break;
   }
@@ -370,7 +536,7 @@ static void upload_state_zero(xf_ctx c) {
   XF_HIP(hipMemsetAsync(c->state, 0, sizeof(DevState), c->stream));
 }
 
-// Plain FP64 HS-CG from x0 on b - A x0.
+// Plain FP64 CG from x0 on b - A x0.
 void solve_hs_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
                  xf_solve_stats &s) {
   require_fast(g);
@@ -379,8 +545,6 @@ void solve_hs_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
   Tiles tl = tiles(g);
   upload_state_zero(c);
   XF_HIP(hipMemcpyAsync(x, x0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
-  XF_HIP(hipMemsetAsync(w->p[0], 0, sizeof(double) * nc, c->stream));
-  XF_HIP(hipMemsetAsync(w->p[1], 0, sizeof(double) * nc, c->stream));
   hipLaunchKernelGGL((k_residual_fast<double>), dim3(tl.blocks), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
                      g->d_flags, g->d_cx, g->d_cy, g->d_cz, b, x, w->r, c->state, c->partials, c->counters,
                      (int)HS_AUX);
@@ -392,12 +556,12 @@ void solve_hs_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
     hipLaunchKernelGGL(k_resid_sq, dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_space, w->r, c->partials,
                        c->counters, c->state->s + HS_AUX);
   }
-  check_kernel("hs start");
+  check_kernel("cg start");
   hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, o.tol, o.tol_mode == XF_TOL_REL ? 1 : 0,
                      (int)HS_AUX);
   c->h_state->iter = 0;
-  hs_iterations<double, double>(c, g, x, w->r, w->p[0], w->p[1], w->q, g->d_cx, g->d_cy, g->d_cz, o.maxit,
-                                o.check_every > 0 ? o.check_every : 16);
+  cgcg_iterations<double, double>(c, g, x, w->r, w->q, w->p[0], w->p[1], g->d_cx, g->d_cy, g->d_cz, o.maxit,
+                                  o.check_every > 0 ? o.check_every : 16);
   s.iters = c->h_state->iter;
   s.status = c->h_state->status;
   s.resid = c->h_state->s[HS_RR0] > 0 ? sqrt(c->h_state->s[HS_RR] / c->h_state->s[HS_RR0]) : 0;
@@ -462,12 +626,13 @@ void solve_mp_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
     hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags, w->r32,
                        c->state, c->partials, c->counters, (int)HS_AUX);
     XF_HIP(hipMemsetAsync(w->e32, 0, sizeof(float) * nc, c->stream));
-    XF_HIP(hipMemsetAsync(w->p32[0], 0, sizeof(float) * nc, c->stream));
-    hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, inner_tol, 1, (int)HS_AUX);
+    // the last sweep only needs the factor that is still missing (with margin)
+    const double itol = std::min(0.5, std::max(inner_tol, 0.3 * sqrt(stop2 / rr)));
+    hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, itol, 1, (int)HS_AUX);
     check_kernel("mp inner start");
     c->h_state->iter = 0;
-    hs_iterations<float, float>(c, g, w->e32, w->r32, w->p32[0], w->p32[1], w->q32, g->d_cxf, g->d_cyf, g->d_czf,
-                                o.maxit - inner_total, every);
+    cgcg_iterations<float, float>(c, g, w->e32, w->r32, w->q32, w->p32[0], w->p32[1], g->d_cxf, g->d_cyf, g->d_czf,
+                                  o.maxit - inner_total, every);
     inner_total += c->h_state->iter;
     if (c->h_state->status == XF_ENAN) { status = XF_ENAN; break; }
     int e = c->prof ? prof_begin(c) : -1;
@@ -579,15 +744,12 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
     while (!c->h_state->done) {
       if (!use_net) {
         // plain CG from the current (e, r): restart with p = r
-        XF_HIP(hipMemsetAsync(w->p32[0], 0, sizeof(float) * nc, c->stream));
-        XF_HIP(hipMemsetAsync(w->p32[1], 0, sizeof(float) * nc, c->stream));
         hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags,
                            w->r32, c->state, c->partials, c->counters, (int)FC_RR2);
         hipLaunchKernelGGL(k_restart_beta, dim3(1), dim3(1), 0, c->stream, c->state);
-        const int before = c->h_state->iter;
-        hs_iterations<float, float>(c, g, w->e32, w->r32, w->p32[0], w->p32[1], w->q32, g->d_cxf, g->d_cyf,
-                                    g->d_czf, o.maxit - inner_total, 16);
-        it += c->h_state->iter - before;
+        cgcg_iterations<float, float>(c, g, w->e32, w->r32, w->q32, w->p32[0], w->p32[1], g->d_cxf, g->d_cyf,
+                                      g->d_czf, std::max(1, o.maxit - inner_total - it), 16);
+        it += c->h_state->iter;
         break;
       }
       // z = M(r)
@@ -599,9 +761,8 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
                            c->partials, c->counters, (int)FC_ZQ);
       hipLaunchKernelGGL(k_fcg_dir, dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, z, p, c->state, it == 0 ? 1 : 0);
       // q = A p, (p, q); (p, r); alpha
-      hipLaunchKernelGGL((k_stencil_cg<float, false>), dim3(tl.blocks), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
-                         g->d_flags, g->d_cxf, g->d_cyf, g->d_czf, w->r32, p, p, w->q32, c->state, c->partials,
-                         c->counters);
+      hipLaunchKernelGGL((k_apply_pq<float>), dim3(tl.blocks), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
+                         g->d_flags, g->d_cxf, g->d_cyf, g->d_czf, p, w->q32, c->state, c->partials, c->counters);
       hipLaunchKernelGGL((k_dot_t<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, p, w->r32, c->state,
                          c->partials + kMaxPartials, c->counters + 1, (int)FC_PR);
       hipLaunchKernelGGL(k_fcg_alpha, dim3(1), dim3(1), 0, c->stream, c->state);
