@@ -45,14 +45,14 @@ def cpu_baseline(case, gpu_iters, sample_iters):
                       "iterations" % (it, threads, per_iter_ms, gpu_iters)}
 
 
-def load_network(ctx, args):
-    """The trained preconditioner: saved parameters if present, else None."""
+def load_network(ctx, path):
+    """A trained network: saved parameters if present, else None."""
     from xflow_amd.unet import UNet
-    if not os.path.exists(args.weights):
-        log("no trained network at %s: skipping the learned-preconditioner solve" % args.weights)
+    if not os.path.exists(path):
+        log("no trained network at %s: skipping its solve" % path)
         return None
     net = UNet(ctx, seed=0)
-    net.set_params(np.load(args.weights))
+    net.set_params(np.load(path))
     return net
 
 
@@ -66,9 +66,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--plain-only", action="store_true")
+    ap.add_argument("--skip-fullres", action="store_true")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--nn-inner-tol", type=float, default=1e-4)
     ap.add_argument("--weights", default=os.path.join(ROOT, "xflow_amd", "weights", "unet_poisson.npy"))
+    ap.add_argument("--coarse-weights", default=os.path.join(ROOT, "xflow_amd", "weights", "unet_coarse64.npy"))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has a single MI355X; "
@@ -126,25 +128,36 @@ def main():
                                          "true_residual": st.resid_true, "status": st.status}
         log("mixed precision: %.2f ms/solve, %d outer / %d inner, true resid %.2e" %
             (mp_ms, st.iters, st.inner_iters, st.resid_true))
-        net = load_network(ctx, args)
-        if net is not None:
-            def nn():
-                o = xf.solve_opts(xf.XF_FCG_NN, tol=1e-8, tol_mode=xf.XF_TOL_REL, project_mean=1,
-                                  inner_tol=args.nn_inner_tol)
-                o.precond = net.h
-                return g.solve(b, x0, x, o)
-            st = nn()
+
+        def timed(o, label):
+            st = g.solve(b, x0, x, o)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(args.steps):
-                st = nn()
+                st = g.solve(b, x0, x, o)
             torch.cuda.synchronize()
-            nn_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-            solvers["nn_preconditioned_mp"] = {"ms": round(nn_ms, 3), "outer": st.iters,
-                                               "inner_iterations": st.inner_iters, "fallback": st.fallback,
-                                               "true_residual": st.resid_true, "status": st.status}
-            log("NN-preconditioned MP: %.2f ms/solve, %d outer / %d inner, fallback %d, true resid %.2e" %
-                (nn_ms, st.iters, st.inner_iters, st.fallback, st.resid_true))
+            ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            rec = {"ms": round(ms, 3), "outer": st.iters, "inner_iterations": st.inner_iters,
+                   "fallback": st.fallback, "true_residual": st.resid_true, "status": st.status,
+                   "speedup_vs_plain_fp64": round(wall_ms / ms, 2)}
+            log("%s: %.2f ms/solve, %d outer / %d inner, fallback %d, true resid %.2e" %
+                (label, ms, st.iters, st.inner_iters, st.fallback, st.resid_true))
+            return rec
+
+        def opts(method, net):
+            o = xf.solve_opts(method, tol=1e-8, tol_mode=xf.XF_TOL_REL, project_mean=1, inner_tol=args.nn_inner_tol)
+            o.precond = net.h if net is not None else None
+            return o
+
+        coarse = load_network(ctx, args.coarse_weights)
+        solvers["mp_fcg_cycle_no_network"] = timed(opts(xf.XF_FCG_MGNN, None),
+                                                   "MP flexible CG, V-cycle, Jacobi coarsest solve")
+        if coarse is not None:
+            solvers["nn_preconditioned_mp"] = timed(opts(xf.XF_FCG_MGNN, coarse),
+                                                    "MP flexible CG, V-cycle, network coarsest solve")
+        full = load_network(ctx, args.weights)
+        if full is not None and not args.skip_fullres:
+            solvers["nn_fullres_mp"] = timed(opts(xf.XF_FCG_NN, full), "MP flexible CG, full-resolution network")
 
     interior = (args.n) ** 3
     ncell = (args.n + 2) ** 3

@@ -103,6 +103,9 @@ int xf_absmax(xf_ctx ctx, xf_grid g, const double *x, double *val, int32_t *rn_i
 #define XF_CG 4        /* Hestenes-Stiefel CG, fused kernels (throughput path)     */
 #define XF_CG_MP 5     /* FP32 inner CG + FP64 iterative refinement              */
 #define XF_FCG_NN 6    /* flexible CG with the learned preconditioner             */
+#define XF_FCG_MGNN 7  /* flexible CG, V-cycle preconditioner with the learned
+                          network as its coarsest-grid solve (precond may be
+                          NULL: damped-Jacobi sweeps there instead)              */
 
 #define XF_TOL_ABS 0   /* stop on ||r|| <= tol (the reference's rule)            */
 #define XF_TOL_REL 1   /* stop on ||r|| <= tol * ||b - A x0||                      */
@@ -119,7 +122,7 @@ typedef struct {
   double inner_tol;  /* XF_CG_MP / XF_FCG_NN: relative tolerance of inner sweeps */
   int stall_window;  /* XF_FCG_NN: fall back to plain CG after this many
                         iterations without progress (20)                         */
-  void *precond;     /* XF_FCG_NN: an xf_unet handle                             */
+  void *precond;     /* XF_FCG_NN, XF_FCG_MGNN: an xf_unet handle               */
 } xf_solve_opts;
 
 typedef struct {
@@ -204,16 +207,33 @@ int xf_unet_set_params(xf_ctx ctx, xf_unet u, const float *host);
 int xf_unet_get_grads(xf_ctx ctx, xf_unet u, float *host);
 /* y = net(r, mask) on crops (no scaling) */
 int xf_unet_forward(xf_ctx ctx, xf_unet u, int B, int D, int H, int W, const float *r, const float *mask, float *y);
-/* loss = mean over the mask of (net(r, mask) - target)^2 and its parameter
- * gradient (read with xf_unet_get_grads) */
+/* the training loss of net(r, mask) against target (xf_unet_set_loss) and its
+ * parameter gradient (read with xf_unet_get_grads) */
 int xf_unet_loss_grad(xf_ctx ctx, xf_unet u, int B, int D, int H, int W, const float *r, const float *mask,
                       const float *target, double *loss);
+/* training loss: XF_LOSS_MSE (default) = mean over the mask of (y - t)^2;
+ * XF_LOSS_COSINE = mean over samples of 1 - (y,t)^2/((y,y)(t,t)) over the
+ * mask, invariant to the scale of y per sample (what a flexible-CG step sees) */
+#define XF_LOSS_MSE 0
+#define XF_LOSS_COSINE 1
+int xf_unet_set_loss(xf_ctx ctx, xf_unet u, int mode);
 /* loss_grad followed by one Adam step (beta 0.9 / 0.999, eps 1e-8) */
 int xf_unet_train_step(xf_ctx ctx, xf_unet u, int B, int D, int H, int W, const float *r, const float *mask,
                        const float *target, float lr, double *loss);
 /* the preconditioner on a pressure grid: z = s net(r/s) on the unknowns, 0
  * elsewhere, s = rms of r over the unknowns (FP32 fields [ncell]) */
 int xf_unet_apply(xf_ctx ctx, xf_unet u, xf_grid g, const float *r, float *z);
+
+/* No reference counterpart: the V-cycle of XF_FCG_MGNN on its own.  Grids are
+ * halved per axis while the interior extents stay even and the x extent stays
+ * at least 64; coarse cells are unknowns if any child is (uniform steps
+ * required).  xf_mg_levels builds the hierarchy and returns its depth;
+ * xf_mg_apply computes z = M(r) for FP32 fields [ncell] of g (net may be
+ * NULL). */
+int xf_mg_levels(xf_ctx ctx, xf_grid g, int *levels);
+/* the grid of one level (0 = g), owned by g: do not destroy */
+int xf_mg_level(xf_ctx ctx, xf_grid g, int level, xf_grid *out);
+int xf_mg_apply(xf_ctx ctx, xf_grid g, xf_unet net, const float *r, float *z);
 
 /* Transport velocities (u, v, w) at the nodes of velocity component `axis`
  * (1..3): the component itself and the other two averaged over the four

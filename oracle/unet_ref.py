@@ -67,10 +67,19 @@ def forward(layers, r, mask):
     return y[:, 0]
 
 
-def loss_and_grads(params, r, mask, target):
+def loss_and_grads(params, r, mask, target, cosine=False):
+    """Loss and parameter gradients by autograd: the masked mean squared error,
+    or (cosine=True) the mean over samples of 1 - (y,t)^2/((y,y)(t,t)) over the
+    mask."""
     layers = unflatten(params)
     r, mask, target = (torch.as_tensor(np.asarray(a, dtype=np.float32)) for a in (r, mask, target))
     y = forward(layers, r, mask)
-    loss = (((y - target) * mask) ** 2).sum() / mask.sum()
+    if cosine:
+        ym = (y * mask).double().flatten(1)
+        tm = (target * mask).double().flatten(1)
+        yt, yy, tt = (ym * tm).sum(1), (ym * ym).sum(1), (tm * tm).sum(1)
+        loss = (1 - yt * yt / (yy * tt)).mean()
+    else:
+        loss = (((y - target) * mask) ** 2).sum() / mask.sum()
     loss.backward()
     return float(loss.detach()), flatten_grads(layers), y.detach().numpy()

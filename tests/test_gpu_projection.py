@@ -104,3 +104,25 @@ def test_mixed_precision_solver(ctx, ncell):
     a, r = xg[m] - xg[m].mean(), xr[m] - xr[m].mean()
     assert np.linalg.norm(a - r) / np.linalg.norm(r) <= 1e-6
     assert st.iters >= 2 and st.inner_iters > it * 0.8
+
+
+@pytest.mark.parametrize("shape", [(14, 12, 300), (12, 10, 257)])
+def test_mixed_precision_long_rows(ctx, shape):
+    """Rows of at least 256 cells take the 4-floats-per-lane stencil (the last
+    k-tile of a row is shifted back and overlaps the one before it)."""
+    codes = cases.neumann_codes(None, [cases.box_obstacle((0.5, 0.5, 0.4), 0.2)], shape=shape)
+    bb = cases.divergence_rhs(codes, seed=2, sigma=1.5)
+    h = [cases.axis_steps(s_) for s_ in shape]
+    g = xf.Grid(ctx, h[0], h[1], h[2], codes)
+    b = dev(bb)
+    x = torch.zeros_like(b)
+    st = g.solve(b, torch.zeros_like(b), x, xf.solve_opts(xf.XF_CG_MP, tol=1e-10, tol_mode=xf.XF_TOL_REL,
+                                                            project_mean=1, inner_tol=1e-4))
+    assert st.status == 0, st.status
+    assert st.resid_true <= 2e-10
+    og = xo.Grid(h[0], h[1], h[2], codes)
+    xr, it, _, s2 = xo.cg_cg(og, bb, np.zeros_like(bb), 1e-12, 100000, project_mean=True)
+    m = codes == 1
+    xg = host(x)
+    a, r = xg[m] - xg[m].mean(), xr[m] - xr[m].mean()
+    assert np.linalg.norm(a - r) / np.linalg.norm(r) <= 1e-6

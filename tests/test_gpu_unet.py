@@ -13,7 +13,7 @@ import torch
 import unet_ref
 import xflow_amd as xf
 from xflow_amd import cases
-from xflow_amd.unet import UNet, layer_slices
+from xflow_amd.unet import LOSS_COSINE, UNet, layer_slices
 
 pytestmark = pytest.mark.gpu
 
@@ -59,6 +59,21 @@ def test_gradients_match_reference(ctx):
     for l, (ws, bs, _) in enumerate(layer_slices()):
         assert rel(grads[ws], wg[ws]) < 5e-2, ("weights", l, rel(grads[ws], wg[ws]))
         assert rel(grads[bs], wg[bs]) < 5e-2, ("bias", l, rel(grads[bs], wg[bs]))
+
+
+def test_cosine_loss_gradients_match_reference(ctx):
+    net = UNet(ctx, seed=8)
+    net.set_loss(LOSS_COSINE)
+    r, mask, target = crop_batch(9, B=2)
+    loss = net.loss_grad(torch.from_numpy(r).cuda(), torch.from_numpy(mask).cuda(), torch.from_numpy(target).cuda())
+    grads = net.get_grads()
+    wl, wg, _ = unet_ref.loss_and_grads(net.get_params(), r, mask, target, cosine=True)
+    # the cosine gradient g (t - c y) cancels between its two terms, so BF16
+    # roundings weigh more than in the squared-error test: 8e-2 relative L2
+    assert loss == pytest.approx(wl, rel=2e-2, abs=2e-3)
+    for l, (ws, bs, _) in enumerate(layer_slices()):
+        assert rel(grads[ws], wg[ws]) < 8e-2, ("weights", l, rel(grads[ws], wg[ws]))
+        assert rel(grads[bs], wg[bs]) < 8e-2, ("bias", l, rel(grads[bs], wg[bs]))
 
 
 def test_training_reduces_loss(ctx):

@@ -9,11 +9,11 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (XF_BICG, XF_BICGSTAB, XF_CG, XF_CG3, XF_CG_MP, XF_FCG_NN, XF_MR, XF_OK, XF_TOL_ABS,
+from ._lib import (XF_BICG, XF_BICGSTAB, XF_CG, XF_CG3, XF_CG_MP, XF_FCG_MGNN, XF_FCG_NN, XF_MR, XF_OK, XF_TOL_ABS,
                    XF_TOL_REL, SolveOpts, SolveStats, XfError, lib)
 
 __all__ = ["Context", "Grid", "XfError", "XF_CG3", "XF_BICG", "XF_MR", "XF_BICGSTAB", "XF_CG", "XF_CG_MP",
-           "XF_FCG_NN", "XF_TOL_ABS", "XF_TOL_REL", "solve_opts"]
+           "XF_FCG_NN", "XF_FCG_MGNN", "XF_TOL_ABS", "XF_TOL_REL", "solve_opts"]
 
 
 def _ptr(t):
@@ -194,7 +194,35 @@ class Grid:
                                                phy, phz, C.byref(val), C.byref(idx)), "xf_divergence_max")
         return val.value, idx.value
 
+    def mg_levels(self):
+        """Depth of the multigrid hierarchy of XF_FCG_MGNN (built on first use)."""
+        n = C.c_int()
+        self.ctx.check(lib().xf_mg_levels(self.ctx.h, self.h, C.byref(n)), "xf_mg_levels")
+        return n.value
+
+    def mg_level(self, k):
+        """The rediscretised grid of hierarchy level k (0 = this grid), owned by this grid."""
+        h = C.c_void_p()
+        self.ctx.check(lib().xf_mg_level(self.ctx.h, self.h, int(k), C.byref(h)), "xf_mg_level")
+        g = Grid.__new__(Grid)
+        g.ctx, g.h, g._owner, g._borrowed = self.ctx, h, self, True
+        g.shape = tuple(((s - 2) >> k) + 2 for s in self.shape)
+        nc, nu = C.c_int64(), C.c_int64()
+        lib().xf_grid_info(h, C.byref(nc), C.byref(nu))
+        g.ncell, g.n_unknowns = nc.value, nu.value
+        return g
+
+    def mg_apply(self, net, r, z):
+        """z = M(r): one V-cycle (FP32 fields), the network as the coarsest solve (net may be None)."""
+        import torch
+        self.ctx.check(lib().xf_mg_apply(self.ctx.h, self.h, net.h if net is not None else None,
+                                         self._vec(r, torch.float32), self._vec(z, torch.float32)), "xf_mg_apply")
+        return z
+
     def close(self):
+        if getattr(self, "_borrowed", False):
+            self.h = None
+            return
         if getattr(self, "h", None) and self.ctx.h:
             lib().xf_grid_destroy(self.ctx.h, self.h)
             self.h = None

@@ -90,6 +90,33 @@ def neumann_codes(ncell, obstacles, shape=None, length=(1.0, 1.0, 1.0)):
     return codes
 
 
+def _any_pairs(a, axis):
+    """Halve one axis of a ringed boolean array: ring entries stay, interior
+    entries 2I-1 and 2I (1-based) combine with 'or'."""
+    a = np.moveaxis(a, axis, 0)
+    out = np.concatenate([a[:1], a[1:-1:2] | a[2:-1:2], a[-1:]], axis=0)
+    return np.moveaxis(out, 0, axis)
+
+
+def coarsen_codes(codes):
+    """Point types of the grid halved per axis, as the multigrid hierarchy of
+    the flexible-CG preconditioner forms them: a coarse cell is Actual if any
+    of its children (8 inside, fewer on the ring) is, Defined if any child is
+    another non-Fictive type, Fictive otherwise."""
+    assert all((s_ - 2) % 2 == 0 for s_ in codes.shape)
+
+    def reduce(x):
+        for ax in range(3):
+            x = _any_pairs(x, ax)
+        return x
+    ca = reduce(codes == ACTUAL)
+    cb = reduce((codes != ACTUAL) & (codes != FICTIVE))
+    out = np.zeros(ca.shape, dtype=np.uint8)
+    out[cb] = DEFINED
+    out[ca] = ACTUAL
+    return out
+
+
 def _gauss_kernel(sigma, truncate=4.0):
     r = int(truncate * sigma + 0.5)
     x = np.arange(-r, r + 1, dtype=np.float64)

@@ -237,6 +237,100 @@ __global__ __launch_bounds__(256) void k_cgcg_stencil(int n, int l, int m, Tiles
   }
 }
 
+// KA with CPL consecutive k-cells per lane (one 16-byte load per lane and
+// row: 2 doubles or 4 floats), k-neighbours across lanes by shuffles.  A wave
+// covers 64*CPL cells of a row; the last k-tile of a row is shifted back to end
+// at m-2, so every vector lies inside the row; cells it shares with the tile
+// before are written twice with the same value and counted once (only from
+// the tile's nominal start).  Requires m - 2 >= 64*CPL.
+template <typename T, int CPL>
+struct VecT {
+  T v[CPL];
+};
+
+template <typename T, int CPL>
+__global__ __launch_bounds__(256) void k_cgcg_stencil_vec(int n, int l, int m, Tiles tl,
+                                                           const uint8_t *__restrict__ flags,
+                                                           const T *__restrict__ cx, const T *__restrict__ cy,
+                                                           const T *__restrict__ cz, const T *__restrict__ r,
+                                                           T *__restrict__ w, const DevState *st,
+                                                           double *__restrict__ part) {
+  __shared__ double lds[8];
+  if (st->done) return;
+  using V = VecT<T, CPL>;
+  const int per = (tl.total + 7) / 8;
+  const int bx = gridDim.x >> 3;
+  const int xcd = blockIdx.x & 7;
+  const int lane = threadIdx.x & 63;
+  const int kspan = 64 * CPL;
+  double gam = 0, del = 0;
+  const int64_t sI = (int64_t)l * m, sJ = m;
+  for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
+    const int t = xcd * per + tq;
+    if (t >= tl.total) break;
+    const int ktile = t % tl.kt;
+    const int jtile = (t / tl.kt) % tl.jt;
+    const int itile = t / (tl.kt * tl.jt);
+    const int kcount = 1 + ktile * kspan;  // cells from here on are this tile's own
+    const int kstart = min(kcount, m - 1 - kspan);
+    const int k0 = kstart + lane * CPL;
+    const int j = 1 + jtile * TJ + (threadIdx.x >> 6);
+    const int i0 = 1 + itile * IC;
+    const int i1 = min(i0 + IC, n - 1);
+    if (j > l - 2) continue;
+    T czm[CPL], czp[CPL];
+    bool own[CPL];
+#pragma unroll
+    for (int e = 0; e < CPL; e++) {
+      czm[e] = cz[2 * (k0 + e)];
+      czp[e] = cz[2 * (k0 + e) + 1];
+      own[e] = k0 + e >= kcount;
+    }
+    const T cym = cy[2 * j], cyp = cy[2 * j + 1];
+    int64_t c = ((int64_t)i0 * l + j) * m + k0;
+    V rm, rc;
+    __builtin_memcpy(&rm, r + c - sI, sizeof(V));
+    __builtin_memcpy(&rc, r + c, sizeof(V));
+    for (int i = i0; i < i1; i++, c += sI) {
+      V rp, rjm, rjp, out;
+      __builtin_memcpy(&rp, r + c + sI, sizeof(V));
+      __builtin_memcpy(&rjm, r + c - sJ, sizeof(V));
+      __builtin_memcpy(&rjp, r + c + sJ, sizeof(V));
+      uint8_t fl[CPL];
+      __builtin_memcpy(fl, flags + c, CPL);
+      T left = __shfl_up(rc.v[CPL - 1], 1, 64), right = __shfl_down(rc.v[0], 1, 64);
+      if (lane == 0) left = r[c - 1];
+      if (lane == 63) right = r[c + CPL];
+      const T cxm = cx[2 * i], cxp = cx[2 * i + 1];
+#pragma unroll
+      for (int e = 0; e < CPL; e++) {
+        const uint8_t f = fl[e];
+        const T x0 = rc.v[e];
+        const T km = e == 0 ? left : rc.v[e - 1];
+        const T kp = e == CPL - 1 ? right : rc.v[e + 1];
+        T tx = ((f & FL_IP) ? cxp * (rp.v[e] - x0) : T(0)) - ((f & FL_IM) ? cxm * (x0 - rm.v[e]) : T(0));
+        T ty = ((f & FL_JP) ? cyp * (rjp.v[e] - x0) : T(0)) - ((f & FL_JM) ? cym * (x0 - rjm.v[e]) : T(0));
+        T tz = ((f & FL_KP) ? czp[e] * (kp - x0) : T(0)) - ((f & FL_KM) ? czm[e] * (x0 - km) : T(0));
+        const T v = (f & FL_ACTUAL) ? -(tx + ty + tz) : T(0);
+        out.v[e] = v;
+        if ((f & FL_ACTUAL) && own[e]) {
+          gam += (double)x0 * (double)x0;
+          del += (double)v * (double)x0;
+        }
+      }
+      __builtin_memcpy(w + c, &out, sizeof(V));
+      rm = rc;
+      rc = rp;
+    }
+  }
+  gam = block_sum_all(gam, lds);
+  del = block_sum_all(del, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = gam;
+    part[2 * blockIdx.x + 1] = del;
+  }
+}
+
 // KB: reduce KA's partials, decide (converged / NaN / iteration cap /
 // breakdown: the same decision in every block), then the fused update.
 // `it` is the number of updates applied before this launch; the first launch
@@ -494,7 +588,14 @@ template <typename T, typename X>
 static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r, T *w, T *p, T *s, const T *cx, const T *cy,
                             const T *cz, int maxit, int every) {
   CgWork *wk = g->work;
+  constexpr int CPL = 16 / sizeof(T);  // one 16-byte vector per lane and row
+  const bool vec = g->m - 2 >= 64 * CPL;
   Tiles tl = tiles(g);
+  if (vec) {
+    tl.kt = (g->m - 2 + 64 * CPL - 1) / (64 * CPL);
+    tl.total = tl.kt * tl.jt * tl.it;
+    tl.blocks = 8 * ((tl.total + 7) / 8);
+  }
   const int nbA = cgcg_blocks(tl);
   const int nbB = std::max(1, std::min(kCgUpdateBlocks, (int)((g->ncell + 255) / 256)));
   const bool f64 = sizeof(T) == 8;
@@ -505,8 +606,12 @@ static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r, T *w, T *p, T *s, c
   for (;;) {
     for (int q = 0; q < every; q++, it++) {
       int e1 = c->prof ? prof_begin(c) : -1;
-      hipLaunchKernelGGL((k_cgcg_stencil<T>), dim3(nbA), dim3(256), 0, c->stream, g->n, g->l, g->m, tl, g->d_flags,
-                         cx, cy, cz, r, w, c->state, wk->part);
+      if (vec)
+        hipLaunchKernelGGL((k_cgcg_stencil_vec<T, CPL>), dim3(nbA), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
+                           g->d_flags, cx, cy, cz, r, w, c->state, wk->part);
+      else
+        hipLaunchKernelGGL((k_cgcg_stencil<T>), dim3(nbA), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
+                           g->d_flags, cx, cy, cz, r, w, c->state, wk->part);
       if (c->prof) {
         prof_end(c, e1);
         marks.push_back({n1, e1});
@@ -687,7 +792,8 @@ __global__ void k_fcg_alpha(DevState *st) {
 void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
                   xf_solve_stats &s) {
   require_fast(g);
-  XF_REQUIRE(o.precond, XF_EINVAL, "XF_FCG_NN needs opts.precond (an xf_unet)");
+  const bool mgc = o.method == XF_FCG_MGNN;
+  XF_REQUIRE(mgc || o.precond, XF_EINVAL, "XF_FCG_NN needs opts.precond (an xf_unet)");
   struct xf_unet_s *net = (struct xf_unet_s *)o.precond;
   CgWork *w = work(c, g, true);
   const int64_t nc = g->ncell;
@@ -753,9 +859,13 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
         break;
       }
       // z = M(r)
-      hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags,
-                         w->r32, c->state, c->partials, c->counters, (int)FC_SUMSQ);
-      unet_apply_grid(c, net, g, w->r32, z, c->state->s + FC_SUMSQ);
+      if (mgc) {
+        mg_apply(c, g, net, w->r32, z);
+      } else {
+        hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags,
+                           w->r32, c->state, c->partials, c->counters, (int)FC_SUMSQ);
+        unet_apply_grid(c, net, g, w->r32, z, c->state->s + FC_SUMSQ);
+      }
       if (it > 0)
         hipLaunchKernelGGL((k_dot_t<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, z, w->q32, c->state,
                            c->partials, c->counters, (int)FC_ZQ);

@@ -195,8 +195,9 @@ static T *upload(const std::vector<T> &v) {
   return d;
 }
 
-static void free_grid(xf_grid g) {
+void free_grid(xf_grid g) {
   if (!g) return;
+  free_mg(g->mg);
   hipFree(g->d_hx); hipFree(g->d_hy); hipFree(g->d_hz);
   hipFree(g->d_cx); hipFree(g->d_cy); hipFree(g->d_cz);
   hipFree(g->d_cxf); hipFree(g->d_cyf); hipFree(g->d_czf);

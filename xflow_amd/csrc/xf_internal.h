@@ -106,8 +106,11 @@ struct xf_grid_s {
   int64_t bad_rows_div = 0;           // rows the divergented operator rejects
   int64_t special_rows = 0;           // unknown rows that are not Actual (codes 4, 5, 6, 8)
   struct CgWork *work = nullptr;      // throughput-solver vectors, allocated on first use
+  struct MgHier *mg = nullptr;        // multigrid hierarchy (XF_FCG_MGNN), built on first use
 };
 void free_cg_work(struct CgWork *w);
+void free_mg(struct MgHier *h);
+void free_grid(xf_grid g);
 
 // ---------------------------------------------------------------- helpers
 inline int grid_blocks(int64_t n, int bs = kBlock) {
@@ -193,3 +196,4 @@ void solve_fcg_nn(xf_ctx ctx, xf_grid g, const double *b, const double *x0, doub
                   xf_solve_stats &s);
 // the learned preconditioner z = M(r) on a grid (xf_unet.hip); d_sumsq = sum of r^2 over the unknowns
 void unet_apply_grid(xf_ctx c, struct xf_unet_s *u, xf_grid g, const float *r32, float *z32, const double *d_sumsq);
+void mg_apply(xf_ctx c, xf_grid g, struct xf_unet_s *net, const float *r, float *z);

@@ -1,0 +1,402 @@
+// Geometric V-cycle preconditioner with the learned network as its
+// coarsest-grid solve (XF_FCG_MGNN).
+//
+// The pressure grid is halved per axis while every interior extent stays
+// even and the next x extent is at least 64 (the network's minimum), down to
+// a coarsest grid (64^3 for a 256^3 problem).  Coarse cells are unknowns if
+// any of their 8 children is an unknown, "boundary value" cells (a zero
+// correction, open face) if any child is a Defined/boundary point, Fictive
+// otherwise; coarse steps are the doubled fine steps, so each coarse grid
+// carries the same masked 7-point operator rediscretised (its own xf_grid).
+// One cycle, all in FP32 with FP64 dot products:
+//   level k < coarsest: nu damped-Jacobi sweeps from z = 0 (omega = 0.8),
+//     t = r - A z, r_{k+1} = mean of t over the unknown children,
+//     V-cycle on level k+1, z += z_{k+1}(parent) on the unknowns,
+//     nu damped-Jacobi sweeps;
+//   coarsest level: y = net(r) (the network applied as on any grid: input
+//     r / rms(r) and the unknown mask, output times rms(r)), scaled by the
+//     energy-optimal factor (r, y) / (y, A y); without a network nu_c
+//     Jacobi sweeps instead (the classical baseline of the same cycle).
+// The rediscretised coarse equation A_H e_H = mean(t) is consistent for
+// smooth errors (both sides approximate -lap e); the flexible CG around the
+// cycle absorbs any remaining scale.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "xf_internal.h"
+
+constexpr float kOmega = 0.8f;
+constexpr int kSweeps = 2;        // pre- and post-smoothing sweeps per level
+constexpr int kCoarseSweeps = 8;  // coarsest-level sweeps without a network
+constexpr int kMgBlocks = 512;    // blocks of the reduction kernels
+
+struct MgLevel {
+  xf_grid g = nullptr;  // the rediscretised grid of this level (owned, except level 0)
+  float *r = nullptr, *z = nullptr, *z2 = nullptr, *t = nullptr;
+};
+
+struct MgHier {
+  int nlev = 0;
+  MgLevel lev[8];
+  double *part = nullptr;  // [kMgBlocks][2] dot partials
+  double *scal = nullptr;  // [4]: sum r^2, (r, y), (y, A y), alpha
+};
+
+// ----------------------------------------------------------------- kernels
+// One thread per interior cell, k fastest: (64, 4) threads per block over
+// (k, j), one i-plane per blockIdx.z.
+struct Dims {
+  int n, l, m;
+};
+
+__device__ inline float diag_of(uint8_t f, float cxm, float cxp, float cym, float cyp, float czm, float czp) {
+  float d = 0;
+  if (f & FL_IM) d += cxm;
+  if (f & FL_IP) d += cxp;
+  if (f & FL_JM) d += cym;
+  if (f & FL_JP) d += cyp;
+  if (f & FL_KM) d += czm;
+  if (f & FL_KP) d += czp;
+  return d;
+}
+
+// MODE 0: out = omega D^-1 r;  MODE 1: out = z + omega D^-1 (r - A z);
+// MODE 2: out = r - A z;  MODE 3: out = A z.  0 at cells that are not Actual.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_mg_sweep(Dims d, const uint8_t *__restrict__ flags,
+                                                   const float *__restrict__ cx, const float *__restrict__ cy,
+                                                   const float *__restrict__ cz, const float *__restrict__ r,
+                                                   const float *__restrict__ z, float *__restrict__ out) {
+  const int k = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int i = 1 + blockIdx.z;
+  if (k > d.m - 2 || j > d.l - 2) return;
+  const int64_t sI = (int64_t)d.l * d.m, sJ = d.m;
+  const int64_t c = ((int64_t)i * d.l + j) * d.m + k;
+  const uint8_t f = flags[c];
+  float v = 0;
+  if (f & FL_ACTUAL) {
+    const float cxm = cx[2 * i], cxp = cx[2 * i + 1], cym = cy[2 * j], cyp = cy[2 * j + 1];
+    const float czm = cz[2 * k], czp = cz[2 * k + 1];
+    const float dg = diag_of(f, cxm, cxp, cym, cyp, czm, czp);
+    const float wd = dg > 0 ? kOmega / dg : 0.f;  // a cell with no open face is left alone
+    if (MODE == 0) {
+      v = wd * r[c];
+    } else {
+      const float zc = z[c];
+      float tx = ((f & FL_IP) ? cxp * (z[c + sI] - zc) : 0.f) - ((f & FL_IM) ? cxm * (zc - z[c - sI]) : 0.f);
+      float ty = ((f & FL_JP) ? cyp * (z[c + sJ] - zc) : 0.f) - ((f & FL_JM) ? cym * (zc - z[c - sJ]) : 0.f);
+      float tz = ((f & FL_KP) ? czp * (z[c + 1] - zc) : 0.f) - ((f & FL_KM) ? czm * (zc - z[c - 1]) : 0.f);
+      const float az = -(tx + ty + tz);
+      if (MODE == 1) v = zc + wd * (r[c] - az);
+      else if (MODE == 2) v = r[c] - az;
+      else v = az;
+    }
+  }
+  out[c] = v;
+}
+
+// coarse r = mean of the fine t over the unknown children (one thread per coarse interior cell)
+__global__ __launch_bounds__(256) void k_mg_restrict(Dims fd, Dims cd, const uint8_t *__restrict__ fflags,
+                                                      const float *__restrict__ t, float *__restrict__ rc) {
+  const int K = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int J = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int I = 1 + blockIdx.z;
+  if (K > cd.m - 2 || J > cd.l - 2) return;
+  float s = 0;
+  int cnt = 0;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int e = 0; e < 2; e++) {
+        const int64_t q = ((int64_t)(2 * I - 1 + a) * fd.l + (2 * J - 1 + b)) * fd.m + (2 * K - 1 + e);
+        if (fflags[q] & FL_UNKNOWN) {
+          s += t[q];
+          cnt++;
+        }
+      }
+  rc[((int64_t)I * cd.l + J) * cd.m + K] = cnt ? s / (float)cnt : 0.f;
+}
+
+// fine z += alpha * coarse z(parent) on the fine unknowns (alpha from *scale, or 1)
+__global__ __launch_bounds__(256) void k_mg_prolong(Dims fd, Dims cd, const uint8_t *__restrict__ fflags,
+                                                     const float *__restrict__ zc, float *__restrict__ z,
+                                                     const double *__restrict__ scale) {
+  const int k = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int i = 1 + blockIdx.z;
+  if (k > fd.m - 2 || j > fd.l - 2) return;
+  const int64_t c = ((int64_t)i * fd.l + j) * fd.m + k;
+  if (!(fflags[c] & FL_UNKNOWN)) return;
+  const float a = scale ? (float)*scale : 1.0f;
+  z[c] += a * zc[((int64_t)((i + 1) >> 1) * cd.l + ((j + 1) >> 1)) * cd.m + ((k + 1) >> 1)];
+}
+
+// per-block partials of (a, b) and (b, c) over the unknowns (c may be null: only (a, b))
+__global__ __launch_bounds__(256) void k_mg_dots(int64_t nc, const uint8_t *__restrict__ flags,
+                                                  const float *__restrict__ a, const float *__restrict__ b,
+                                                  const float *__restrict__ cc, double *__restrict__ part) {
+  __shared__ double lds[16];
+  double s0 = 0, s1 = 0;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nc; q += (int64_t)gridDim.x * blockDim.x) {
+    if (!(flags[q] & FL_UNKNOWN)) continue;
+    s0 += (double)a[q] * (double)b[q];
+    if (cc) s1 += (double)b[q] * (double)cc[q];
+  }
+  s0 = block_sum(s0, lds);
+  s1 = block_sum(s1, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s0;
+    part[2 * blockIdx.x + 1] = s1;
+  }
+}
+
+// MODE 0: scal[0] = sum of the first partials; MODE 1: scal[1], scal[2] and
+// alpha = scal[1] / scal[2] (0 if not finite and positive)
+template <int MODE>
+__global__ void k_mg_finish(const double *__restrict__ part, int np, double *__restrict__ scal) {
+  __shared__ double lds[16];
+  double s0 = 0, s1 = 0;
+  for (int q = threadIdx.x; q < np; q += blockDim.x) {
+    s0 += part[2 * q];
+    s1 += part[2 * q + 1];
+  }
+  s0 = block_sum(s0, lds);
+  s1 = block_sum(s1, lds);
+  if (threadIdx.x == 0) {
+    if (MODE == 0) {
+      scal[0] = s0;
+    } else {
+      scal[1] = s0;
+      scal[2] = s1;
+      const double a = s0 / s1;
+      scal[3] = (s1 > 0 && a == a && fabs(a) < 1e30) ? a : 0.0;
+    }
+  }
+}
+
+// z *= scal[3] on the whole array
+__global__ void k_mg_scale(int64_t nc, float *__restrict__ z, const double *__restrict__ scal) {
+  const float a = (float)scal[3];
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nc; q += (int64_t)gridDim.x * blockDim.x)
+    z[q] *= a;
+}
+
+// ------------------------------------------------------------------ host
+static dim3 cell_grid(const xf_grid g) { return dim3((g->m - 2 + 63) / 64, (g->l - 2 + 3) / 4, g->n - 2); }
+static Dims dims_of(const xf_grid g) { return Dims{g->n, g->l, g->m}; }
+
+static float *zalloc(xf_ctx c, int64_t n) {
+  float *p = nullptr;
+  hipError_t e = hipMalloc(&p, sizeof(float) * n);
+  XF_REQUIRE(e == hipSuccess, XF_ENOMEM, "multigrid: out of device memory");
+  XF_HIP(hipMemsetAsync(p, 0, sizeof(float) * n, c->stream));
+  return p;
+}
+
+static bool uniform(const std::vector<double> &h) {
+  for (double v : h)
+    if (std::fabs(v - h[0]) > 1e-12 * std::fabs(h[0])) return false;
+  return !h.empty();
+}
+
+void free_mg(MgHier *h) {
+  if (!h) return;
+  for (int k = 0; k < h->nlev; k++) {
+    MgLevel &L = h->lev[k];
+    if (k > 0) {
+      hipFree(L.r);
+      free_grid(L.g);
+    }
+    hipFree(L.z); hipFree(L.z2); hipFree(L.t);
+  }
+  hipFree(h->part);
+  hipFree(h->scal);
+  delete h;
+}
+
+// Builds the hierarchy of g on first use (level 0 is g itself; its r is the
+// caller's residual).
+static MgHier *hierarchy(xf_ctx c, xf_grid g) {
+  if (g->mg) return g->mg;
+  XF_REQUIRE(uniform(g->hx) && uniform(g->hy) && uniform(g->hz), XF_EINVAL,
+             "XF_FCG_MGNN: the multigrid cycle needs uniform steps along each axis");
+  MgHier *h = new MgHier();
+  struct Guard {
+    MgHier *&h;
+    ~Guard() { if (h) free_mg(h); }
+  } guard{h};
+  h->lev[0].g = g;
+  std::vector<uint8_t> fcodes((size_t)g->ncell);
+  XF_HIP(hipMemcpy(fcodes.data(), g->d_codes, g->ncell, hipMemcpyDeviceToHost));
+  int nlev = 1;
+  while (nlev < 8) {
+    const xf_grid f = h->lev[nlev - 1].g;
+    const int N = f->n - 2, L = f->l - 2, M = f->m - 2;
+    if (N % 2 || L % 2 || M % 2 || M / 2 < 64 || N / 2 < 2 || L / 2 < 2) break;
+    const int cn = N / 2 + 2, cl = L / 2 + 2, cm = M / 2 + 2;
+    std::vector<int32_t> cc((size_t)cn * cl * cm, 0);
+    auto kids = [](int I, int Nc, int *out) {  // fine indices of coarse index I (with ring)
+      if (I == 0) { out[0] = 0; return 1; }
+      if (I == Nc + 1) { out[0] = 2 * Nc + 1; return 1; }
+      out[0] = 2 * I - 1; out[1] = 2 * I; return 2;
+    };
+    for (int I = 0; I < cn; I++) {
+      int ki[2];
+      const int ni = kids(I, cn - 2, ki);
+      for (int J = 0; J < cl; J++) {
+        int kj[2];
+        const int nj = kids(J, cl - 2, kj);
+        for (int K = 0; K < cm; K++) {
+          int kk[2];
+          const int nk = kids(K, cm - 2, kk);
+          bool unk = false, bnd = false;
+          for (int a = 0; a < ni; a++)
+            for (int b = 0; b < nj; b++)
+              for (int e = 0; e < nk; e++) {
+                const uint8_t v = fcodes[((size_t)ki[a] * f->l + kj[b]) * f->m + kk[e]];
+                if (v == PT_ACTUAL) unk = true;
+                else if (v != PT_FICTIVE) bnd = true;
+              }
+          cc[((size_t)I * cl + J) * cm + K] = unk ? PT_ACTUAL : (bnd ? PT_DEFINED : PT_FICTIVE);
+        }
+      }
+    }
+    std::vector<double> hx(cn - 1, 2 * f->hx[0]), hy(cl - 1, 2 * f->hy[0]), hz(cm - 1, 2 * f->hz[0]);
+    xf_grid cg = nullptr;
+    const int st = xf_grid_create(c, cn, cl, cm, hx.data(), hy.data(), hz.data(), cc.data(), nullptr, nullptr, &cg);
+    XF_REQUIRE(st == XF_OK, st, "multigrid: coarse grid: " + c->err);
+    h->lev[nlev].g = cg;
+    h->nlev = ++nlev;  // owned from here on
+    fcodes.assign((size_t)cg->ncell, 0);
+    for (size_t q = 0; q < fcodes.size(); q++) fcodes[q] = (uint8_t)cc[q];
+  }
+  h->nlev = nlev;
+  for (int k = 0; k < nlev; k++) {
+    MgLevel &L = h->lev[k];
+    const int64_t nc = L.g->ncell;
+    if (k > 0) L.r = zalloc(c, nc);
+    L.z = zalloc(c, nc);
+    L.z2 = zalloc(c, nc);
+    L.t = zalloc(c, nc);
+  }
+  XF_HIP(hipMalloc(&h->part, sizeof(double) * 2 * kMgBlocks));
+  XF_HIP(hipMalloc(&h->scal, sizeof(double) * 4));
+  XF_HIP(hipMemsetAsync(h->scal, 0, sizeof(double) * 4, c->stream));
+  g->mg = h;
+  h = nullptr;  // the grid owns it now
+  return g->mg;
+}
+
+int mg_levels(xf_ctx c, xf_grid g) { return hierarchy(c, g)->nlev; }
+
+static int red_blocks(int64_t nc) { return (int)std::max<int64_t>(1, std::min<int64_t>(kMgBlocks, (nc + 255) / 256)); }
+
+// nu sweeps of damped Jacobi on A z = r starting from z = 0; result in L.z
+static void smooth_from_zero(xf_ctx c, MgLevel &L, const float *r, int nu) {
+  const xf_grid g = L.g;
+  const Dims d = dims_of(g);
+  hipLaunchKernelGGL(k_mg_sweep<0>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
+                     g->d_czf, r, (const float *)nullptr, L.z);
+  for (int s = 1; s < nu; s++) {
+    hipLaunchKernelGGL(k_mg_sweep<1>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
+                       g->d_czf, r, L.z, L.z2);
+    std::swap(L.z, L.z2);
+  }
+}
+
+// nu sweeps of damped Jacobi continuing from L.z
+static void smooth(xf_ctx c, MgLevel &L, const float *r, int nu) {
+  const xf_grid g = L.g;
+  const Dims d = dims_of(g);
+  for (int s = 0; s < nu; s++) {
+    hipLaunchKernelGGL(k_mg_sweep<1>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
+                       g->d_czf, r, L.z, L.z2);
+    std::swap(L.z, L.z2);
+  }
+}
+
+// z of the coarsest level from its r
+static void coarse_solve(xf_ctx c, MgHier *h, struct xf_unet_s *net) {
+  MgLevel &L = h->lev[h->nlev - 1];
+  const xf_grid g = L.g;
+  const float *rin = L.r;
+  if (!net) {
+    smooth_from_zero(c, L, rin, kCoarseSweeps);
+    return;
+  }
+  XF_REQUIRE((g->n - 2) % 16 == 0 && (g->l - 2) % 16 == 0 && (g->m - 2) % 64 == 0, XF_EINVAL,
+             "XF_FCG_MGNN: the coarsest grid does not fit the network (interior extents must be multiples of "
+             "16, 16 and 64)");
+  const int nb = red_blocks(g->ncell);
+  // rms of r over the unknowns -> the network's input scaling
+  hipLaunchKernelGGL(k_mg_dots, dim3(nb), dim3(256), 0, c->stream, g->ncell, g->d_flags, rin, rin,
+                     (const float *)nullptr, h->part);
+  hipLaunchKernelGGL(k_mg_finish<0>, dim3(1), dim3(256), 0, c->stream, h->part, nb, h->scal);
+  unet_apply_grid(c, net, g, rin, L.z, h->scal);
+  // energy-optimal scale: alpha = (r, y) / (y, A y)
+  const Dims d = dims_of(g);
+  hipLaunchKernelGGL(k_mg_sweep<3>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
+                     g->d_czf, (const float *)nullptr, L.z, L.t);
+  hipLaunchKernelGGL(k_mg_dots, dim3(nb), dim3(256), 0, c->stream, g->ncell, g->d_flags, rin, L.z, L.t, h->part);
+  hipLaunchKernelGGL(k_mg_finish<1>, dim3(1), dim3(256), 0, c->stream, h->part, nb, h->scal);
+  hipLaunchKernelGGL(k_mg_scale, dim3(red_blocks(g->ncell)), dim3(256), 0, c->stream, g->ncell, L.z, h->scal);
+}
+
+static void vcycle(xf_ctx c, MgHier *h, int k, const float *r, struct xf_unet_s *net) {
+  if (k == h->nlev - 1) {
+    if (k == 0) {
+      // a single level: the coarse solve reads lev[0].r, which is the caller's r
+      MgLevel &L = h->lev[0];
+      float *keep = L.r;
+      L.r = const_cast<float *>(r);
+      coarse_solve(c, h, net);
+      L.r = keep;
+    } else {
+      coarse_solve(c, h, net);
+    }
+    return;
+  }
+  MgLevel &L = h->lev[k], &C = h->lev[k + 1];
+  const xf_grid g = L.g;
+  const Dims d = dims_of(g);
+  smooth_from_zero(c, L, r, kSweeps);
+  hipLaunchKernelGGL(k_mg_sweep<2>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
+                     g->d_czf, r, L.z, L.t);
+  hipLaunchKernelGGL(k_mg_restrict, cell_grid(C.g), dim3(256), 0, c->stream, d, dims_of(C.g), g->d_flags, L.t, C.r);
+  vcycle(c, h, k + 1, C.r, net);
+  hipLaunchKernelGGL(k_mg_prolong, cell_grid(g), dim3(256), 0, c->stream, d, dims_of(C.g), g->d_flags, C.z, L.z,
+                     (const double *)nullptr);
+  smooth(c, L, r, kSweeps);
+}
+
+// z = M(r): one V-cycle on the grid's hierarchy (r, z FP32 fields of the fine grid)
+void mg_apply(xf_ctx c, xf_grid g, struct xf_unet_s *net, const float *r, float *z) {
+  MgHier *h = hierarchy(c, g);
+  vcycle(c, h, 0, r, net);
+  XF_HIP(hipMemcpyAsync(z, h->lev[0].z, sizeof(float) * g->ncell, hipMemcpyDeviceToDevice, c->stream));
+  check_kernel("multigrid cycle");
+}
+
+extern "C" int xf_mg_levels(xf_ctx c, xf_grid g, int *levels) {
+  return guarded(c, [&] {
+    XF_REQUIRE(g && levels, XF_EINVAL, "xf_mg_levels: null argument");
+    *levels = mg_levels(c, g);
+  });
+}
+
+extern "C" int xf_mg_level(xf_ctx c, xf_grid g, int level, xf_grid *out) {
+  return guarded(c, [&] {
+    XF_REQUIRE(g && out, XF_EINVAL, "xf_mg_level: null argument");
+    MgHier *h = hierarchy(c, g);
+    XF_REQUIRE(level >= 0 && level < h->nlev, XF_EINVAL, "xf_mg_level: no such level");
+    *out = h->lev[level].g;
+  });
+}
+
+extern "C" int xf_mg_apply(xf_ctx c, xf_grid g, xf_unet net, const float *r, float *z) {
+  return guarded(c, [&] {
+    XF_REQUIRE(g && r && z && r != z, XF_EINVAL, "xf_mg_apply: bad argument");
+    mg_apply(c, g, net, r, z);
+  });
+}

@@ -491,7 +491,8 @@ extern "C" int xf_solve(xf_ctx c, xf_grid g, const double *b, const double *x0, 
       }
       case XF_CG: solve_hs_cg(c, g, b, x0, x, o, s); break;
       case XF_CG_MP: solve_mp_cg(c, g, b, x0, x, o, s); break;
-      case XF_FCG_NN: solve_fcg_nn(c, g, b, x0, x, o, s); break;
+      case XF_FCG_NN:
+      case XF_FCG_MGNN: solve_fcg_nn(c, g, b, x0, x, o, s); break;
       default: XF_REQUIRE(false, XF_EINVAL, "xf_solve: unknown method");
     }
     XF_HIP(hipEventRecord(c->ev1, c->stream));

@@ -421,6 +421,69 @@ __global__ void k_mse(const float *__restrict__ y, const float *__restrict__ t, 
   if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) *out = tot * inv_count;
 }
 
+// Scale-invariant loss per sample b: L_b = 1 - (y,t)^2 / ((y,y)(t,t)) over
+// the masked cells, the loss that matters to a flexible-CG step (which
+// rescales the preconditioned residual itself); the batch loss is the mean.
+// Three deterministic passes: per-(sample, chunk) partial dots, per-sample
+// coefficients, then dL/dy = -(2 (y,t) / ((y,y)(t,t) B)) (t - ((y,t)/(y,y)) y).
+constexpr int kCosChunks = 64;
+
+__global__ void k_cos_dots(const float *__restrict__ y, const float *__restrict__ t, const float *__restrict__ mask,
+                           int64_t per_sample, double *__restrict__ part) {
+  __shared__ double lds[16];
+  const int b = blockIdx.y, ch = blockIdx.x;
+  const int64_t chunk = (per_sample + kCosChunks - 1) / kCosChunks;
+  const int64_t beg = ch * chunk, end = min(per_sample, beg + chunk);
+  const float *yb = y + b * per_sample, *tb = t + b * per_sample, *mb = mask + b * per_sample;
+  double yt = 0, yy = 0, tt = 0;
+  for (int64_t q = beg + threadIdx.x; q < end; q += blockDim.x) {
+    const double m = mb[q], yv = yb[q] * m, tv = tb[q] * m;
+    yt += yv * tv;
+    yy += yv * yv;
+    tt += tv * tv;
+  }
+  double v[3] = {yt, yy, tt};
+  for (int e = 0; e < 3; e++) {
+    const double sum = block_sum(v[e], lds);
+    if (threadIdx.x == 0) part[((int64_t)b * kCosChunks + ch) * 3 + e] = sum;
+  }
+}
+
+// one thread per sample: coefficients (g_b, c_b) and the mean loss
+__global__ void k_cos_finish(const double *__restrict__ part, int B, double *__restrict__ coef, double *loss) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double tot = 0;
+  for (int b = 0; b < B; b++) {
+    double yt = 0, yy = 0, tt = 0;
+    for (int ch = 0; ch < kCosChunks; ch++) {
+      const double *q = part + ((int64_t)b * kCosChunks + ch) * 3;
+      yt += q[0];
+      yy += q[1];
+      tt += q[2];
+    }
+    double g = 0, cb = 0, lb = 1;
+    if (yy > 0 && tt > 0) {
+      lb = 1 - yt * yt / (yy * tt);
+      g = -2 * yt / (yy * tt * B);
+      cb = yt / yy;
+    }
+    coef[2 * b] = g;
+    coef[2 * b + 1] = cb;
+    tot += lb;
+  }
+  *loss = tot / B;
+}
+
+__global__ void k_cos_grad(const float *__restrict__ y, const float *__restrict__ t, const float *__restrict__ mask,
+                           int64_t per_sample, int B, const double *__restrict__ coef, float *__restrict__ dy) {
+  const int64_t n = per_sample * B;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(q / per_sample);
+    const double g = coef[2 * b], cb = coef[2 * b + 1];
+    dy[q] = (float)(g * ((double)t[q] - cb * (double)y[q]) * (double)mask[q]);
+  }
+}
+
 __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
                        int64_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
@@ -465,6 +528,9 @@ struct xf_unet_s {
   bf16 *wtT[NL] = {};  // input-gradient weights [Cin][Kpad']
   int kpad[NL], kpadT[NL];
   int step = 0;
+  int loss_mode = XF_LOSS_MSE;
+  double *cos_part = nullptr;  // [B][kCosChunks][3] partial sums of the cosine loss
+  int cos_B = 0;
   bool packed = false;
   UnetWork work;
   bool train_buffers = false;
@@ -622,7 +688,7 @@ extern "C" int xf_unet_destroy(xf_ctx c, xf_unet u) {
     if (!u) return;
     XF_HIP(hipStreamSynchronize(c->stream));
     u->work.release();
-    hipFree(u->p); hipFree(u->g); hipFree(u->m); hipFree(u->v);
+    hipFree(u->p); hipFree(u->g); hipFree(u->m); hipFree(u->v); hipFree(u->cos_part);
     for (int l = 0; l < NL; l++) { hipFree(u->wt[l]); hipFree(u->wtT[l]); }
     delete u;
   });
@@ -734,11 +800,26 @@ static double backward(xf_ctx c, xf_unet u, int B, int D, int H, int W, const fl
   Dst9 d{w.y, nullptr, nullptr, 0, 0};
   hipLaunchKernelGGL(k_conv_last, dim3(eblocks(n1)), dim3(256), 0, c->stream, w.a[8], u->p + u->off_w[9],
                      u->p + u->off_b[9], d, B, D, H, W);
-  // mean over the masked cells: the mask sum first, then the loss
-  hipLaunchKernelGGL(k_sum_f32, dim3(std::min(eblocks(n1), 2048)), dim3(256), 0, c->stream, mask, n1, c->partials,
-                     c->counters, c->d_scalar + 1);
-  hipLaunchKernelGGL(k_mse, dim3(std::min(eblocks(n1), 2048)), dim3(256), 0, c->stream, w.y, target, mask, n1, w.dy,
-                     c->partials + kMaxPartials, c->counters + 1, c->d_scalar, c->d_scalar + 1);
+  if (u->loss_mode == XF_LOSS_COSINE) {
+    if (u->cos_B < B) {
+      hipFree(u->cos_part);
+      u->cos_part = nullptr;
+      u->cos_part = dalloc<double>((size_t)B * kCosChunks * 3 + 2 * (size_t)B);
+      u->cos_B = B;
+    }
+    const int64_t per = (int64_t)D * H * W;
+    double *coef = u->cos_part + (size_t)B * kCosChunks * 3;
+    hipLaunchKernelGGL(k_cos_dots, dim3(kCosChunks, B), dim3(256), 0, c->stream, w.y, target, mask, per, u->cos_part);
+    hipLaunchKernelGGL(k_cos_finish, dim3(1), dim3(64), 0, c->stream, u->cos_part, B, coef, c->d_scalar);
+    hipLaunchKernelGGL(k_cos_grad, dim3(std::min(eblocks(n1), 4096)), dim3(256), 0, c->stream, w.y, target, mask, per,
+                       B, coef, w.dy);
+  } else {
+    // mean over the masked cells: the mask sum first, then the loss
+    hipLaunchKernelGGL(k_sum_f32, dim3(std::min(eblocks(n1), 2048)), dim3(256), 0, c->stream, mask, n1, c->partials,
+                       c->counters, c->d_scalar + 1);
+    hipLaunchKernelGGL(k_mse, dim3(std::min(eblocks(n1), 2048)), dim3(256), 0, c->stream, w.y, target, mask, n1, w.dy,
+                       c->partials + kMaxPartials, c->counters + 1, c->d_scalar, c->d_scalar + 1);
+  }
   XF_HIP(hipMemsetAsync(u->g, 0, sizeof(float) * u->nparam, c->stream));
   const int D2 = D / 2, H2 = H / 2, W2 = W / 2, D4 = D / 4, H4 = H / 4, W4 = W / 4;
   const int64_t n2 = n1 / 8, n3 = n1 / 64;
@@ -799,6 +880,13 @@ static double backward(xf_ctx c, xf_unet u, int B, int D, int H, int W, const fl
   XF_HIP(hipMemcpyAsync(&loss, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   XF_HIP(hipStreamSynchronize(c->stream));
   return loss;
+}
+
+extern "C" int xf_unet_set_loss(xf_ctx c, xf_unet u, int mode) {
+  return guarded(c, [&] {
+    XF_REQUIRE(u && (mode == XF_LOSS_MSE || mode == XF_LOSS_COSINE), XF_EINVAL, "xf_unet_set_loss: bad mode");
+    u->loss_mode = mode;
+  });
 }
 
 extern "C" int xf_unet_loss_grad(xf_ctx c, xf_unet u, int B, int D, int H, int W, const float *r, const float *mask,

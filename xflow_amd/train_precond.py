@@ -21,17 +21,32 @@ import torch
 
 import xflow_amd as xf
 from xflow_amd import cases
-from xflow_amd.unet import UNet
+from xflow_amd.unet import LOSS_COSINE, LOSS_MSE, UNet
 
 ITERATES = (0, 2, 5, 10, 20, 40, 80, 160, 320)
 
 
-def harvest(ctx, ncell, problems, seed0=100, log=print):
+def coarse_case(ncell, fine, seed):
+    """A problem on the grid the multigrid cycle reaches from a fine^3 problem:
+    the fine random-obstacle mask coarsened (cases.coarsen_codes) down to
+    ncell^3, the right-hand side generated on that grid."""
+    codes = cases.neumann_codes(fine, cases.random_obstacles(seed))
+    while codes.shape[0] - 2 > ncell:
+        codes = cases.coarsen_codes(codes)
+    b = cases.divergence_rhs(codes, seed)
+    h = cases.axis_steps(ncell)
+    return {"codes": codes, "hx": h, "hy": h.copy(), "hz": h.copy(), "b": b}
+
+
+def harvest(ctx, ncell, problems, seed0=100, coarsen_from=0, log=print):
     """Returns device tensors R, E, M of shape [P*len(ITERATES), ncell, ncell, ncell] (interior only)."""
     Rs, Es, Ms = [], [], []
     for pidx in range(problems):
         seed = seed0 + pidx
-        case = cases.poisson_case(ncell, cases.random_obstacles(seed), seed)
+        if coarsen_from:
+            case = coarse_case(ncell, coarsen_from, seed)
+        else:
+            case = cases.poisson_case(ncell, cases.random_obstacles(seed), seed)
         g = xf.Grid(ctx, case["hx"], case["hy"], case["hz"], case["codes"])
         b = torch.from_numpy(case["b"]).cuda()
         x0 = torch.zeros_like(b)
@@ -76,8 +91,9 @@ def crops(R, E, M, B, S, gen):
     return torch.stack(rr).contiguous(), torch.stack(mm).contiguous(), torch.stack(ee).contiguous()
 
 
-def train(ctx, R, E, M, steps=1500, batch=4, crop=64, lr=1e-3, seed=0, log=print):
+def train(ctx, R, E, M, steps=1500, batch=4, crop=64, lr=1e-3, seed=0, loss="cosine", log=print):
     net = UNet(ctx, seed=seed)
+    net.set_loss(LOSS_COSINE if loss == "cosine" else LOSS_MSE)
     gen = torch.Generator().manual_seed(seed)
     t0 = time.time()
     hist = []
@@ -91,6 +107,45 @@ def train(ctx, R, E, M, steps=1500, batch=4, crop=64, lr=1e-3, seed=0, log=print
     return net, hist
 
 
+def evaluate_mg(ctx, net, ncell, seed, log=print):
+    """Plain CG vs the V-cycle-preconditioned flexible CG without and with the
+    network on one held-out problem (iterations to 1e-8)."""
+    case = cases.poisson_case(ncell, cases.random_obstacles(seed), seed)
+    g = xf.Grid(ctx, case["hx"], case["hy"], case["hz"], case["codes"])
+    b = torch.from_numpy(case["b"]).cuda()
+    x = torch.zeros_like(b)
+    st0 = g.solve(b, torch.zeros_like(b), x, xf.solve_opts(xf.XF_CG, tol=1e-8, tol_mode=xf.XF_TOL_REL, project_mean=1))
+    res = {"plain": st0.iters}
+    for name, p in (("cycle", None), ("cycle+net", net)):
+        o = xf.solve_opts(xf.XF_FCG_MGNN, tol=1e-8, tol_mode=xf.XF_TOL_REL, project_mean=1, inner_tol=1e-4)
+        o.precond = p.h if p is not None else None
+        x.zero_()
+        st = g.solve(b, torch.zeros_like(b), x, o)
+        res[name] = (st.inner_iters, st.fallback, st.ms, st.resid_true)
+    g.close()
+    log("evaluate %d^3 seed %d: plain CG %d; cycle without network %s; cycle with network %s "
+        "(inner iterations, fallback, ms, true residual)" % (ncell, seed, res["plain"], res["cycle"], res["cycle+net"]))
+    return res
+
+
+def evaluate(ctx, net, ncell, seed, log=print):
+    """Plain FP32-refined CG vs the network-preconditioned flexible CG on one
+    held-out problem: (plain iterations, network iterations, fallback)."""
+    case = cases.poisson_case(ncell, cases.random_obstacles(seed), seed)
+    g = xf.Grid(ctx, case["hx"], case["hy"], case["hz"], case["codes"])
+    b = torch.from_numpy(case["b"]).cuda()
+    x = torch.zeros_like(b)
+    st0 = g.solve(b, torch.zeros_like(b), x, xf.solve_opts(xf.XF_CG, tol=1e-8, tol_mode=xf.XF_TOL_REL, project_mean=1))
+    o = xf.solve_opts(xf.XF_FCG_NN, tol=1e-8, tol_mode=xf.XF_TOL_REL, project_mean=1, inner_tol=1e-4)
+    o.precond = net.h
+    x.zero_()
+    st1 = g.solve(b, torch.zeros_like(b), x, o)
+    g.close()
+    log("evaluate %d^3 seed %d: plain CG %d iterations; network FCG %d outer / %d inner, fallback %d, "
+        "true residual %.2e" % (ncell, seed, st0.iters, st1.iters, st1.inner_iters, st1.fallback, st1.resid_true))
+    return st0.iters, st1.inner_iters, st1.fallback
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
@@ -100,15 +155,23 @@ def main():
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--crop", type=int, default=64)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--loss", choices=("cosine", "mse"), default="cosine")
+    ap.add_argument("--eval-ncell", type=int, default=128)
+    ap.add_argument("--coarsen-from", type=int, default=0,
+                    help="train the coarsest-grid network of the V-cycle: fine^3 masks coarsened to ncell^3")
     args = ap.parse_args()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     ctx = xf.Context(0)
     t0 = time.time()
-    R, E, M = harvest(ctx, args.ncell, args.problems, log=log)
+    R, E, M = harvest(ctx, args.ncell, args.problems, coarsen_from=args.coarsen_from, log=log)
     log("harvested %d pairs in %.1f s" % (R.shape[0], time.time() - t0))
-    net, hist = train(ctx, R, E, M, args.steps, args.batch, args.crop, args.lr, log=log)
+    net, hist = train(ctx, R, E, M, args.steps, args.batch, args.crop, args.lr, loss=args.loss, log=log)
     np.save(args.out, net.get_params())
     log("saved %s; loss %.4e -> %.4e" % (args.out, hist[0], np.mean(hist[-50:])))
+    if args.eval_ncell and args.coarsen_from:
+        evaluate_mg(ctx, net, args.eval_ncell, 999, log=log)
+    elif args.eval_ncell:
+        evaluate(ctx, net, args.eval_ncell, 999, log=log)
 
 
 if __name__ == "__main__":

@@ -6,6 +6,7 @@ import numpy as np
 
 from ._lib import lib
 
+LOSS_MSE, LOSS_COSINE = 0, 1
 LCIN = (2, 16, 16, 32, 32, 64, 96, 32, 48, 16)
 LCOUT = (16, 16, 32, 32, 64, 64, 32, 32, 16, 1)
 
@@ -43,6 +44,11 @@ class UNet:
         a = np.ascontiguousarray(a, dtype=np.float32)
         assert a.size == self.nparam
         self.ctx.check(lib().xf_unet_set_params(self.ctx.h, self.h, a.ctypes.data), "xf_unet_set_params")
+
+    def set_loss(self, mode):
+        """LOSS_MSE (mean masked squared error) or LOSS_COSINE (per sample
+        1 - cos^2 between output and target over the mask, scale-free)."""
+        self.ctx.check(lib().xf_unet_set_loss(self.ctx.h, self.h, int(mode)), "xf_unet_set_loss")
 
     def get_grads(self):
         a = np.zeros(self.nparam, dtype=np.float32)
