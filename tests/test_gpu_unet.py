@@ -13,7 +13,7 @@ import torch
 import unet_ref
 import xflow_amd as xf
 from xflow_amd import cases
-from xflow_amd.unet import LOSS_COSINE, UNet, layer_slices
+from xflow_amd.unet import LOSS_COSINE, LOSS_ENERGY, UNet, layer_slices
 
 pytestmark = pytest.mark.gpu
 
@@ -74,6 +74,28 @@ def test_cosine_loss_gradients_match_reference(ctx):
     for l, (ws, bs, _) in enumerate(layer_slices()):
         assert rel(grads[ws], wg[ws]) < 8e-2, ("weights", l, rel(grads[ws], wg[ws]))
         assert rel(grads[bs], wg[bs]) < 8e-2, ("bias", l, rel(grads[bs], wg[bs]))
+
+
+def test_energy_loss_gradients_match_reference(ctx):
+    net = UNet(ctx, seed=10)
+    net.set_loss(LOSS_ENERGY)
+    r, mask, _ = crop_batch(11, B=2)
+    # a target with (t, r) > 0: t solves a smoothed problem, r is its residual
+    t = torch.from_numpy(r).unsqueeze(1)
+    target = torch.nn.functional.avg_pool3d(t, 5, stride=1, padding=2)[:, 0].numpy() * mask
+    rin = unet_ref.crop_laplacian(torch.from_numpy(target), torch.from_numpy(mask)).numpy().astype(np.float32)
+    loss = net.loss_grad(torch.from_numpy(rin).cuda(), torch.from_numpy(mask).cuda(),
+                         torch.from_numpy(np.ascontiguousarray(target, dtype=np.float32)).cuda())
+    grads = net.get_grads()
+    wl, wg, _ = unet_ref.loss_and_grads(net.get_params(), rin, mask, target, energy=True)
+    assert 0 <= loss <= 1
+    assert loss == pytest.approx(wl, rel=2e-2, abs=2e-3)
+    for l, (ws, bs, _) in enumerate(layer_slices()):
+        assert rel(grads[ws], wg[ws]) < 8e-2, ("weights", l, rel(grads[ws], wg[ws]))
+        # the last bias moves every output alike, along the null space of the
+        # Neumann crop operator: its gradient is ~0 and compared absolutely
+        tol = 8e-2 * max(np.linalg.norm(wg[bs]), 1e-2 * np.linalg.norm(wg[ws]))
+        assert np.linalg.norm(grads[bs] - wg[bs]) <= tol, ("bias", l, rel(grads[bs], wg[bs]))
 
 
 def test_training_reduces_loss(ctx):

@@ -15,8 +15,9 @@
 //     nu damped-Jacobi sweeps;
 //   coarsest level: y = net(r) (the network applied as on any grid: input
 //     r / rms(r) and the unknown mask, output times rms(r)), scaled by the
-//     energy-optimal factor (r, y) / (y, A y); without a network nu_c
-//     Jacobi sweeps instead (the classical baseline of the same cycle).
+//     energy-optimal factor (r, y) / (y, A y), then nu Jacobi sweeps; without
+//     a network nu_c Jacobi sweeps instead (the classical baseline of the
+//     same cycle).
 // The rediscretised coarse equation A_H e_H = mean(t) is consistent for
 // smooth errors (both sides approximate -lap e); the flexible CG around the
 // cycle absorbs any remaining scale.
@@ -341,6 +342,8 @@ static void coarse_solve(xf_ctx c, MgHier *h, struct xf_unet_s *net) {
   hipLaunchKernelGGL(k_mg_dots, dim3(nb), dim3(256), 0, c->stream, g->ncell, g->d_flags, rin, L.z, L.t, h->part);
   hipLaunchKernelGGL(k_mg_finish<1>, dim3(1), dim3(256), 0, c->stream, h->part, nb, h->scal);
   hipLaunchKernelGGL(k_mg_scale, dim3(red_blocks(g->ncell)), dim3(256), 0, c->stream, g->ncell, L.z, h->scal);
+  // a few sweeps smooth what the network leaves at the grid scale
+  smooth(c, L, rin, kSweeps);
 }
 
 static void vcycle(xf_ctx c, MgHier *h, int k, const float *r, struct xf_unet_s *net) {

@@ -484,6 +484,93 @@ __global__ void k_cos_grad(const float *__restrict__ y, const float *__restrict_
   }
 }
 
+// Energy loss per sample b over the crop operator L (the masked 7-point
+// Laplacian in cell units, faces closed at the crop edge and next to masked-out
+// cells): L_b = 1 - (y, L e)^2 / ((y, L y) (e, L e)), the fraction of the
+// error's energy norm that a correction along y, optimally scaled, leaves
+// behind (e the target); in [0, 1] by Cauchy-Schwarz.
+// dL/dy = -(2 (y,Le) / ((y,Ly)(e,Le) B)) (L e - ((y,Le)/(y,Ly)) L y).
+__global__ void k_en_apply(const float *__restrict__ y, const float *__restrict__ mask, int B, int D, int H, int W,
+                           float *__restrict__ q) {
+  const int64_t n = (int64_t)B * D * H * W;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    if (mask[c] <= 0) {
+      q[c] = 0;
+      continue;
+    }
+    const int x = (int)(c % W), yy = (int)((c / W) % H), z = (int)((c / ((int64_t)W * H)) % D);
+    const float yc = y[c];
+    float v = 0;
+    if (x > 0 && mask[c - 1] > 0) v += yc - y[c - 1];
+    if (x < W - 1 && mask[c + 1] > 0) v += yc - y[c + 1];
+    if (yy > 0 && mask[c - W] > 0) v += yc - y[c - W];
+    if (yy < H - 1 && mask[c + W] > 0) v += yc - y[c + W];
+    const int64_t pz = (int64_t)W * H;
+    if (z > 0 && mask[c - pz] > 0) v += yc - y[c - pz];
+    if (z < D - 1 && mask[c + pz] > 0) v += yc - y[c + pz];
+    q[c] = v;
+  }
+}
+
+// per-(sample, chunk) partials of (y, L e), (y, L y), (e, L e) over the mask
+__global__ void k_en_dots(const float *__restrict__ y, const float *__restrict__ qe, const float *__restrict__ q,
+                          const float *__restrict__ e, const float *__restrict__ mask, int64_t per_sample,
+                          double *__restrict__ part) {
+  __shared__ double lds[16];
+  const int b = blockIdx.y, ch = blockIdx.x;
+  const int64_t chunk = (per_sample + kCosChunks - 1) / kCosChunks;
+  const int64_t beg = ch * chunk, end = min(per_sample, beg + chunk);
+  const int64_t o = b * per_sample;
+  double a = 0, bb = 0, cc = 0;
+  for (int64_t t = beg + threadIdx.x; t < end; t += blockDim.x) {
+    const int64_t k = o + t;
+    if (mask[k] <= 0) continue;
+    a += (double)y[k] * qe[k];
+    bb += (double)y[k] * q[k];
+    cc += (double)e[k] * qe[k];
+  }
+  double v[3] = {a, bb, cc};
+  for (int w = 0; w < 3; w++) {
+    const double sum = block_sum(v[w], lds);
+    if (threadIdx.x == 0) part[((int64_t)b * kCosChunks + ch) * 3 + w] = sum;
+  }
+}
+
+__global__ void k_en_finish(const double *__restrict__ part, int B, double *__restrict__ coef, double *loss) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double tot = 0;
+  for (int b = 0; b < B; b++) {
+    double a = 0, bb = 0, cc = 0;
+    for (int ch = 0; ch < kCosChunks; ch++) {
+      const double *q = part + ((int64_t)b * kCosChunks + ch) * 3;
+      a += q[0];
+      bb += q[1];
+      cc += q[2];
+    }
+    double g = 0, cb = 0, lb = 1;
+    if (bb > 0 && cc > 0) {
+      lb = 1 - a * a / (bb * cc);
+      g = -2 * a / (bb * cc * B);
+      cb = a / bb;
+    }
+    coef[2 * b] = g;
+    coef[2 * b + 1] = cb;
+    tot += lb;
+  }
+  *loss = tot / B;
+}
+
+// qe may alias dy: each thread reads qe[k] before it writes dy[k]
+__global__ void k_en_grad(const float *qe, const float *__restrict__ q, const float *__restrict__ mask,
+                          int64_t per_sample, int B, const double *__restrict__ coef, float *dy) {
+  const int64_t n = per_sample * B;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(k / per_sample);
+    const double v = qe[k];
+    dy[k] = mask[k] > 0 ? (float)(coef[2 * b] * (v - coef[2 * b + 1] * (double)q[k])) : 0.f;
+  }
+}
+
 __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
                        int64_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
@@ -505,7 +592,7 @@ struct UnetWork {
   int B = 0, D = 0, H = 0, W = 0;
   bf16 *a[9] = {};
   bf16 *p1 = nullptr, *p2 = nullptr;
-  float *y = nullptr, *dy = nullptr;
+  float *y = nullptr, *dy = nullptr, *q = nullptr;  // q: L y of the energy loss
   // gradients
   bf16 *g8 = nullptr, *gcat8 = nullptr, *g7 = nullptr, *g6 = nullptr, *gcat6 = nullptr, *g5 = nullptr, *g4 = nullptr,
        *gp2 = nullptr, *g3 = nullptr, *g2 = nullptr, *gp1 = nullptr, *g1 = nullptr, *g0 = nullptr;
@@ -514,8 +601,8 @@ struct UnetWork {
     bf16 *gs[] = {p1, p2, g8, gcat8, g7, g6, gcat6, g5, g4, gp2, g3, g2, gp1, g1, g0};
     for (auto p : gs) hipFree(p);
     p1 = p2 = g8 = gcat8 = g7 = g6 = gcat6 = g5 = g4 = gp2 = g3 = g2 = gp1 = g1 = g0 = nullptr;
-    hipFree(y); hipFree(dy);
-    y = dy = nullptr;
+    hipFree(y); hipFree(dy); hipFree(q);
+    y = dy = q = nullptr;
     B = D = H = W = 0;
   }
 };
@@ -611,6 +698,7 @@ static void ensure_work(xf_unet u, int B, int D, int H, int W, bool train) {
   u->train_buffers = train;
   if (train) {
     w.dy = dalloc<float>(n1);
+    w.q = dalloc<float>(n1);
     w.g8 = dalloc<bf16>(n1 * 16); w.gcat8 = dalloc<bf16>(n1 * 48);
     w.g7 = dalloc<bf16>(n2 * 32); w.g6 = dalloc<bf16>(n2 * 32); w.gcat6 = dalloc<bf16>(n2 * 96);
     w.g5 = dalloc<bf16>(n3 * 64); w.g4 = dalloc<bf16>(n3 * 64); w.gp2 = dalloc<bf16>(n3 * 32);
@@ -800,7 +888,7 @@ static double backward(xf_ctx c, xf_unet u, int B, int D, int H, int W, const fl
   Dst9 d{w.y, nullptr, nullptr, 0, 0};
   hipLaunchKernelGGL(k_conv_last, dim3(eblocks(n1)), dim3(256), 0, c->stream, w.a[8], u->p + u->off_w[9],
                      u->p + u->off_b[9], d, B, D, H, W);
-  if (u->loss_mode == XF_LOSS_COSINE) {
+  if (u->loss_mode == XF_LOSS_COSINE || u->loss_mode == XF_LOSS_ENERGY) {
     if (u->cos_B < B) {
       hipFree(u->cos_part);
       u->cos_part = nullptr;
@@ -809,10 +897,23 @@ static double backward(xf_ctx c, xf_unet u, int B, int D, int H, int W, const fl
     }
     const int64_t per = (int64_t)D * H * W;
     double *coef = u->cos_part + (size_t)B * kCosChunks * 3;
+    if (u->loss_mode == XF_LOSS_ENERGY) {
+      // q = L y into w.q, L e into w.dy (overwritten in place by the gradient)
+      hipLaunchKernelGGL(k_en_apply, dim3(std::min(eblocks(n1), 4096)), dim3(256), 0, c->stream, w.y, mask, B, D, H, W,
+                         w.q);
+      hipLaunchKernelGGL(k_en_apply, dim3(std::min(eblocks(n1), 4096)), dim3(256), 0, c->stream, target, mask, B, D,
+                         H, W, w.dy);
+      hipLaunchKernelGGL(k_en_dots, dim3(kCosChunks, B), dim3(256), 0, c->stream, w.y, w.dy, w.q, target, mask, per,
+                         u->cos_part);
+      hipLaunchKernelGGL(k_en_finish, dim3(1), dim3(64), 0, c->stream, u->cos_part, B, coef, c->d_scalar);
+      hipLaunchKernelGGL(k_en_grad, dim3(std::min(eblocks(n1), 4096)), dim3(256), 0, c->stream, w.dy, w.q, mask, per,
+                         B, coef, w.dy);
+    } else {
     hipLaunchKernelGGL(k_cos_dots, dim3(kCosChunks, B), dim3(256), 0, c->stream, w.y, target, mask, per, u->cos_part);
     hipLaunchKernelGGL(k_cos_finish, dim3(1), dim3(64), 0, c->stream, u->cos_part, B, coef, c->d_scalar);
     hipLaunchKernelGGL(k_cos_grad, dim3(std::min(eblocks(n1), 4096)), dim3(256), 0, c->stream, w.y, target, mask, per,
                        B, coef, w.dy);
+    }
   } else {
     // mean over the masked cells: the mask sum first, then the loss
     hipLaunchKernelGGL(k_sum_f32, dim3(std::min(eblocks(n1), 2048)), dim3(256), 0, c->stream, mask, n1, c->partials,
@@ -884,7 +985,8 @@ static double backward(xf_ctx c, xf_unet u, int B, int D, int H, int W, const fl
 
 extern "C" int xf_unet_set_loss(xf_ctx c, xf_unet u, int mode) {
   return guarded(c, [&] {
-    XF_REQUIRE(u && (mode == XF_LOSS_MSE || mode == XF_LOSS_COSINE), XF_EINVAL, "xf_unet_set_loss: bad mode");
+    XF_REQUIRE(u && (mode == XF_LOSS_MSE || mode == XF_LOSS_COSINE || mode == XF_LOSS_ENERGY), XF_EINVAL,
+               "xf_unet_set_loss: bad mode");
     u->loss_mode = mode;
   });
 }

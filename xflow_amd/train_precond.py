@@ -21,7 +21,7 @@ import torch
 
 import xflow_amd as xf
 from xflow_amd import cases
-from xflow_amd.unet import LOSS_COSINE, LOSS_MSE, UNet
+from xflow_amd.unet import LOSS_COSINE, LOSS_ENERGY, LOSS_MSE, UNet
 
 ITERATES = (0, 2, 5, 10, 20, 40, 80, 160, 320)
 
@@ -93,7 +93,7 @@ def crops(R, E, M, B, S, gen):
 
 def train(ctx, R, E, M, steps=1500, batch=4, crop=64, lr=1e-3, seed=0, loss="cosine", log=print):
     net = UNet(ctx, seed=seed)
-    net.set_loss(LOSS_COSINE if loss == "cosine" else LOSS_MSE)
+    net.set_loss({"energy": LOSS_ENERGY, "cosine": LOSS_COSINE, "mse": LOSS_MSE}[loss])
     gen = torch.Generator().manual_seed(seed)
     t0 = time.time()
     hist = []
@@ -155,7 +155,7 @@ def main():
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--crop", type=int, default=64)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--loss", choices=("cosine", "mse"), default="cosine")
+    ap.add_argument("--loss", choices=("energy", "cosine", "mse"), default="energy")
     ap.add_argument("--eval-ncell", type=int, default=128)
     ap.add_argument("--coarsen-from", type=int, default=0,
                     help="train the coarsest-grid network of the V-cycle: fine^3 masks coarsened to ncell^3")

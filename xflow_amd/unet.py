@@ -6,7 +6,7 @@ import numpy as np
 
 from ._lib import lib
 
-LOSS_MSE, LOSS_COSINE = 0, 1
+LOSS_MSE, LOSS_COSINE, LOSS_ENERGY = 0, 1, 2
 LCIN = (2, 16, 16, 32, 32, 64, 96, 32, 48, 16)
 LCOUT = (16, 16, 32, 32, 64, 64, 32, 32, 16, 1)
 
