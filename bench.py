@@ -203,6 +203,12 @@ def main():
                      "stencil_gbs": round(ka_gbs, 1), "stencil_us": round(ka["ms"] * 1e3, 2)},
         "cpu_baseline": cpu,
     }
+    nn = solvers.get("nn_preconditioned_mp")
+    if nn is not None:
+        out["nn_precond_mp"] = {"ms_per_solve": nn["ms"], "speedup_vs_plain_fp64": nn["speedup_vs_plain_fp64"],
+                                "solver": "FP32 flexible CG in FP64 refinement, V-cycle preconditioner with the "
+                                          "trained network as its coarsest-grid (64^3) solve",
+                                "true_residual": nn["true_residual"]}
     print(json.dumps(out), flush=True)
     g.close()
     ctx.close()
