@@ -159,6 +159,20 @@ def main():
         if full is not None and not args.skip_fullres:
             solvers["nn_fullres_mp"] = timed(opts(xf.XF_FCG_NN, full), "MP flexible CG, full-resolution network")
 
+    # host-buffer boundary: the drop-in classes upload b and download x once per
+    # solve; time that PCIe round trip (pinned host memory) to quote the
+    # PCIe-inclusive rate beside the resident-input value
+    hb = torch.empty(b.shape, dtype=torch.float64, pin_memory=True)
+    hb.copy_(b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        b.copy_(hb, non_blocking=True)
+        hb.copy_(x, non_blocking=True)
+    torch.cuda.synchronize()
+    pcie_ms = (time.perf_counter() - t0) * 1e3 / 3
+    log("PCIe round trip of b and x (%.0f MB each): %.2f ms" % (b.numel() * 8 / 1e6, pcie_ms))
+
     interior = (args.n) ** 3
     ncell = (args.n + 2) ** 3
     ka = kstats.get("cg_stencil_fp64", {"ms": float("nan"), "count": 0})
@@ -203,12 +217,15 @@ def main():
                      "stencil_gbs": round(ka_gbs, 1), "stencil_us": round(ka["ms"] * 1e3, 2)},
         "cpu_baseline": cpu,
     }
+    out["host_boundary"] = {"pcie_round_trip_ms": round(pcie_ms, 3),
+                            "plain_fp64_cg_ms_with_pcie": round(wall_ms + pcie_ms, 3)}
     nn = solvers.get("nn_preconditioned_mp")
     if nn is not None:
         out["nn_precond_mp"] = {"ms_per_solve": nn["ms"], "speedup_vs_plain_fp64": nn["speedup_vs_plain_fp64"],
                                 "solver": "FP32 flexible CG in FP64 refinement, V-cycle preconditioner with the "
                                           "trained network as its coarsest-grid (64^3) solve",
-                                "true_residual": nn["true_residual"]}
+                                "true_residual": nn["true_residual"],
+                                "ms_with_pcie": round(nn["ms"] + pcie_ms, 3)}
     print(json.dumps(out), flush=True)
     g.close()
     ctx.close()

@@ -99,71 +99,13 @@ static Tiles tiles(xf_grid g) {
   return t;
 }
 
-// q = A p over the interior and (p, q) -> alpha = rr / (p, q) (the
-// flexible-CG step; plain solves use the KA/KB pair below).
-// One wavefront per (j) row of 64 k-cells; each block marches IC planes in i,
-// keeping p(i-1), p(i), p(i+1) of its own column in registers; the four
-// in-plane neighbours come from the L1/L2-resident rows of the same plane.
-// Block index -> tile mapping is XCD-aware: the hardware deals workgroups to
-// the 8 XCDs round-robin, so consecutive tiles (which share halo rows) are
-// given to workgroups on the same XCD.
-template <typename T>
-__global__ __launch_bounds__(256) void k_apply_pq(int n, int l, int m, Tiles tl, const uint8_t *__restrict__ flags,
-                                                   const T *__restrict__ cx, const T *__restrict__ cy,
-                                                   const T *__restrict__ cz, const T *__restrict__ p,
-                                                   T *__restrict__ q, DevState *st, double *partials,
-                                                   unsigned *counter) {
-  __shared__ double lds[16];
-  if (st->done) return;
-  const int per = (tl.total + 7) / 8;
-  const int w = blockIdx.x;
-  const int t = (w & 7) * per + (w >> 3);
-  double acc = 0;
-  if (t < tl.total) {
-    const int ktile = t % tl.kt;
-    const int jtile = (t / tl.kt) % tl.jt;
-    const int itile = t / (tl.kt * tl.jt);
-    const int k = 1 + ktile * TK + (threadIdx.x & 63);
-    const int j = 1 + jtile * TJ + (threadIdx.x >> 6);
-    const int i0 = 1 + itile * IC;
-    const int i1 = min(i0 + IC, n - 1);
-    if (k <= m - 2 && j <= l - 2) {
-      const int64_t sI = (int64_t)l * m, sJ = m;
-      const T czm = cz[2 * k], czp = cz[2 * k + 1];
-      const T cym = cy[2 * j], cyp = cy[2 * j + 1];
-      int64_t c = ((int64_t)i0 * l + j) * m + k;
-      T pm = p[c - sI], pc = p[c];
-      for (int i = i0; i < i1; i++, c += sI) {
-        const T pp = p[c + sI];
-        const uint8_t fl = flags[c];
-        T v = 0;
-        if (fl & FL_ACTUAL) {
-          const T cxm = cx[2 * i], cxp = cx[2 * i + 1];
-          const T pjm = p[c - sJ], pjp = p[c + sJ], pkm = p[c - 1], pkp = p[c + 1];
-          T tx = ((fl & FL_IP) ? cxp * (pp - pc) : T(0)) - ((fl & FL_IM) ? cxm * (pc - pm) : T(0));
-          T ty = ((fl & FL_JP) ? cyp * (pjp - pc) : T(0)) - ((fl & FL_JM) ? cym * (pc - pjm) : T(0));
-          T tz = ((fl & FL_KP) ? czp * (pkp - pc) : T(0)) - ((fl & FL_KM) ? czm * (pc - pkm) : T(0));
-          v = -(tx + ty + tz);
-          acc += (double)pc * (double)v;
-        }
-        q[c] = v;
-        pm = pc;
-        pc = pp;
-      }
-    }
-  }
-  double bs = block_sum(acc, lds);
-  double tot;
-  if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) {
-    st->s[HS_PQ] = tot;
-    if (tot == 0 || tot != tot) {
-      st->done = 1;
-      st->status = (tot != tot) ? XF_ENAN : st->status;
-    } else {
-      st->s[HS_ALPHA] = st->s[HS_RR] / tot;
-    }
-  }
-}
+// The marching stencils below: one wavefront per (j) row of 64 k-cells (or
+// 64*CPL with vectors); each block marches IC planes in i, keeping the three
+// planes i-1, i, i+1 of its own column in registers; the four in-plane
+// neighbours come from the L1/L2-resident rows of the same plane.  Block index
+// -> tile mapping is XCD-aware: the hardware deals workgroups to the 8 XCDs
+// round-robin, so consecutive tiles (which share halo rows) are given to
+// workgroups on the same XCD.
 
 // Block sum valid in every thread.  Butterfly shuffles leave the same value in
 // all lanes (a + b == b + a exactly), and the wave sums are added in wave
@@ -181,7 +123,7 @@ __device__ inline double block_sum_all(double v, double *lds) {
 
 // KA: w = A r over the interior, per-block partials (gamma, delta) =
 // ((r, r), (w, r)) over Actual cells.  Blocks are persistent over tiles
-// (bounded partial count); tile order within an XCD as in k_apply_pq.
+// (bounded partial count), consecutive tiles on one XCD.
 template <typename T>
 __global__ __launch_bounds__(256) void k_cgcg_stencil(int n, int l, int m, Tiles tl,
                                                        const uint8_t *__restrict__ flags, const T *__restrict__ cx,
@@ -402,42 +344,6 @@ __global__ __launch_bounds__(256) void k_cgcg_update(int64_t nc, int npart, cons
       s[c] = sn;
       x[c] = x[c] + (X)a * (X)pn;
       r[c] = rc - a * sn;
-    }
-  }
-}
-
-// K2: x += alpha p, r -= alpha q, (r, r) -> beta and the stopping test.
-template <typename T, typename X>
-__global__ __launch_bounds__(256) void k_cg_update(int64_t nc, X *__restrict__ x, const T *__restrict__ p,
-                                                    T *__restrict__ r, const T *__restrict__ q, DevState *st,
-                                                    double *partials, unsigned *counter, int maxit) {
-  __shared__ double lds[16];
-  if (st->done) return;
-  const double alpha = st->s[HS_ALPHA];
-  const T a = (T)alpha;
-  double acc = 0;
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
-    const T pc = p[c];
-    const T rn = r[c] - a * q[c];
-    r[c] = rn;
-    x[c] = x[c] + (X)a * (X)pc;
-    acc += (double)rn * (double)rn;
-  }
-  double bs = block_sum(acc, lds);
-  double tot;
-  if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) {
-    const double rr = st->s[HS_RR];
-    st->s[HS_BETA] = tot / rr;
-    st->s[HS_RR] = tot;
-    st->iter += 1;
-    if (tot != tot) {
-      st->done = 1;
-      st->status = XF_ENAN;
-    } else if (tot <= st->s[HS_STOP2]) {
-      st->done = 1;
-    } else if (st->iter >= maxit) {
-      st->done = 1;
-      st->status = XF_EMAXIT;
     }
   }
 }
@@ -757,31 +663,130 @@ void solve_mp_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
 // --------------------------------------------------- flexible CG + network
 enum { FC_PR = 20, FC_ZQ, FC_RR2, FC_SUMSQ };
 
-template <typename T>
-__global__ void k_dot_t(int64_t nc, const T *__restrict__ a, const T *__restrict__ b, DevState *st, double *partials,
-                        unsigned *counter, int slot) {
+// The flexible-CG step in fence-free kernels: each reduction kernel writes
+// one partial (pair) per block, a one-block finisher adds them in block order
+// and applies the scalar step (no device-scope fences in the streaming
+// passes).
+constexpr int kFcgBlocks = 2048;
+
+// partials of (a, b) (second of the pair 0)
+__global__ __launch_bounds__(256) void k_dot_part(int64_t nc, const float *__restrict__ a, const float *__restrict__ b,
+                                                  const DevState *st, double *__restrict__ part) {
   __shared__ double lds[16];
+  if (st->done) return;
   double acc = 0;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
     acc += (double)a[c] * (double)b[c];
-  double bs = block_sum(acc, lds);
-  double tot;
-  if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) st->s[slot] = tot;
+  const double bs = block_sum(acc, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = bs;
+    part[2 * blockIdx.x + 1] = 0;
+  }
 }
 
-// p = z + beta p, beta = -(z, q_old)/(p_old, q_old) (first=1: p = z)
-__global__ void k_fcg_dir(int64_t nc, const float *__restrict__ z, float *__restrict__ p, const DevState *st,
-                          int first) {
+// p = z + beta p, beta = -(z, q_old)/(p_old, q_old) (first: p = z), and
+// partials of (p, r)
+__global__ __launch_bounds__(256) void k_fcg_dir(int64_t nc, const float *__restrict__ z, float *__restrict__ p,
+                                                 const float *__restrict__ r, const DevState *st, int first,
+                                                 double *__restrict__ part) {
+  __shared__ double lds[16];
+  if (st->done) return;
   const double pq = st->s[HS_PQ];
   const float beta = first || pq == 0 ? 0.0f : (float)(-st->s[FC_ZQ] / pq);
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
-    p[c] = first ? z[c] : z[c] + beta * p[c];
+  double acc = 0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    const float pn = first ? z[c] : z[c] + beta * p[c];
+    p[c] = pn;
+    acc += (double)pn * (double)r[c];
+  }
+  const double bs = block_sum(acc, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = bs;
+    part[2 * blockIdx.x + 1] = 0;
+  }
 }
 
-// alpha = (p, r)/(p, q)
-__global__ void k_fcg_alpha(DevState *st) {
-  const double pq = st->s[HS_PQ];
-  st->s[HS_ALPHA] = pq != 0 ? st->s[FC_PR] / pq : 0.0;
+// e += alpha p, r -= alpha q and partials of (r, r)
+__global__ __launch_bounds__(256) void k_fcg_update(int64_t nc, float *__restrict__ e, const float *__restrict__ p,
+                                                    float *__restrict__ r, const float *__restrict__ q,
+                                                    const DevState *st, double *__restrict__ part) {
+  __shared__ double lds[16];
+  if (st->done) return;
+  const float a = (float)st->s[HS_ALPHA];
+  double acc = 0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    const float rn = r[c] - a * q[c];
+    r[c] = rn;
+    e[c] = e[c] + a * p[c];
+    acc += (double)rn * (double)rn;
+  }
+  const double bs = block_sum(acc, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = bs;
+    part[2 * blockIdx.x + 1] = 0;
+  }
+}
+
+// Finishers (one block): ZQ: s[FC_ZQ] = sum; PR: s[FC_PR] = sum; PQ: s[HS_PQ]
+// = sum of the second partials (the stencil's (A p, p)), alpha = (p, r)/(p, q)
+// or stop on 0 / NaN; RR: s[HS_RR] = sum, iteration count and the stopping
+// test.
+enum { FIN_ZQ, FIN_PR, FIN_PQ, FIN_RR };
+template <int MODE>
+__global__ void k_fcg_finish(const double *__restrict__ part, int np, DevState *st, int maxit) {
+  __shared__ double lds[16];
+  if (st->done) return;
+  double s0 = 0;
+  for (int q = threadIdx.x; q < np; q += blockDim.x) s0 += part[2 * q + (MODE == FIN_PQ ? 1 : 0)];
+  const double tot = block_sum(s0, lds);
+  if (threadIdx.x != 0) return;
+  if (MODE == FIN_ZQ) {
+    st->s[FC_ZQ] = tot;
+  } else if (MODE == FIN_PR) {
+    st->s[FC_PR] = tot;
+  } else if (MODE == FIN_PQ) {
+    st->s[HS_PQ] = tot;
+    if (tot == 0 || tot != tot) {
+      st->done = 1;
+      if (tot != tot) st->status = XF_ENAN;
+    } else {
+      st->s[HS_ALPHA] = st->s[FC_PR] / tot;
+    }
+  } else {
+    st->s[HS_BETA] = tot / st->s[HS_RR];
+    st->s[HS_RR] = tot;
+    st->iter += 1;
+    if (tot != tot) {
+      st->done = 1;
+      st->status = XF_ENAN;
+    } else if (tot <= st->s[HS_STOP2]) {
+      st->done = 1;
+    } else if (st->iter >= maxit) {
+      st->done = 1;
+      st->status = XF_EMAXIT;
+    }
+  }
+}
+
+// q = A p over the interior with per-block partials ((p, p), (A p, p)) in
+// part (the plain solver's stencil, p in the role of r); returns the partial count.
+static int stencil_pq(xf_ctx c, xf_grid g, const float *p, float *q, double *part) {
+  constexpr int CPL = 4;
+  const bool vec = g->m - 2 >= 64 * CPL;
+  Tiles tl = tiles(g);
+  if (vec) {
+    tl.kt = (g->m - 2 + 64 * CPL - 1) / (64 * CPL);
+    tl.total = tl.kt * tl.jt * tl.it;
+    tl.blocks = 8 * ((tl.total + 7) / 8);
+  }
+  const int nb = cgcg_blocks(tl);
+  if (vec)
+    hipLaunchKernelGGL((k_cgcg_stencil_vec<float, CPL>), dim3(nb), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
+                       g->d_flags, g->d_cxf, g->d_cyf, g->d_czf, p, q, c->state, part);
+  else
+    hipLaunchKernelGGL((k_cgcg_stencil<float>), dim3(nb), dim3(256), 0, c->stream, g->n, g->l, g->m, tl, g->d_flags,
+                       g->d_cxf, g->d_cyf, g->d_czf, p, q, c->state, part);
+  return nb;
 }
 
 // Flexible CG (A-orthogonalisation against the previous direction, Notay's
@@ -798,6 +803,7 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
   CgWork *w = work(c, g, true);
   const int64_t nc = g->ncell;
   Tiles tl = tiles(g);
+  const int nbS = std::max(1, std::min(kFcgBlocks, (int)((nc + 255) / 256)));
   const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
   const int stall = o.stall_window > 0 ? o.stall_window : 20;
   float *z = nullptr;
@@ -866,18 +872,19 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
                            w->r32, c->state, c->partials, c->counters, (int)FC_SUMSQ);
         unet_apply_grid(c, net, g, w->r32, z, c->state->s + FC_SUMSQ);
       }
-      if (it > 0)
-        hipLaunchKernelGGL((k_dot_t<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, z, w->q32, c->state,
-                           c->partials, c->counters, (int)FC_ZQ);
-      hipLaunchKernelGGL(k_fcg_dir, dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, z, p, c->state, it == 0 ? 1 : 0);
-      // q = A p, (p, q); (p, r); alpha
-      hipLaunchKernelGGL((k_apply_pq<float>), dim3(tl.blocks), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
-                         g->d_flags, g->d_cxf, g->d_cyf, g->d_czf, p, w->q32, c->state, c->partials, c->counters);
-      hipLaunchKernelGGL((k_dot_t<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, p, w->r32, c->state,
-                         c->partials + kMaxPartials, c->counters + 1, (int)FC_PR);
-      hipLaunchKernelGGL(k_fcg_alpha, dim3(1), dim3(1), 0, c->stream, c->state);
-      hipLaunchKernelGGL((k_cg_update<float, float>), dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, w->e32, p,
-                         w->r32, w->q32, c->state, c->partials + kMaxPartials, c->counters + 1,
+      if (it > 0) {
+        hipLaunchKernelGGL(k_dot_part, dim3(nbS), dim3(256), 0, c->stream, nc, z, w->q32, c->state, w->part);
+        hipLaunchKernelGGL(k_fcg_finish<FIN_ZQ>, dim3(1), dim3(256), 0, c->stream, w->part, nbS, c->state, 0);
+      }
+      hipLaunchKernelGGL(k_fcg_dir, dim3(nbS), dim3(256), 0, c->stream, nc, z, p, w->r32, c->state, it == 0 ? 1 : 0,
+                         w->part);
+      hipLaunchKernelGGL(k_fcg_finish<FIN_PR>, dim3(1), dim3(256), 0, c->stream, w->part, nbS, c->state, 0);
+      // q = A p with the partials ((p, p), (A p, p)); alpha = (p, r)/(p, q)
+      const int npq = stencil_pq(c, g, p, w->q32, w->part);
+      hipLaunchKernelGGL(k_fcg_finish<FIN_PQ>, dim3(1), dim3(256), 0, c->stream, w->part, npq, c->state, 0);
+      hipLaunchKernelGGL(k_fcg_update, dim3(nbS), dim3(256), 0, c->stream, nc, w->e32, p, w->r32, w->q32, c->state,
+                         w->part);
+      hipLaunchKernelGGL(k_fcg_finish<FIN_RR>, dim3(1), dim3(256), 0, c->stream, w->part, nbS, c->state,
                          o.maxit - inner_total);
       check_kernel("fcg iteration");
       poll();

@@ -98,6 +98,97 @@ __global__ __launch_bounds__(256) void k_mg_sweep(Dims d, const uint8_t *__restr
   out[c] = v;
 }
 
+// The same sweeps on rows of at least 256 cells: 4 consecutive k-cells per
+// lane (16-byte loads), k-neighbours by lane shuffles, IC i-planes marched
+// per block with the planes i-1, i, i+1 in registers; blocks persistent over
+// the (k, j, i) tiles, consecutive tiles on one XCD.  The last k-tile of a row
+// is shifted back to end at m-2 (cells it shares with the tile before get the
+// same value twice).
+struct SweepTiles {
+  int kt, jt, it, total;
+};
+constexpr int kSweepIC = 16;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, const uint8_t *__restrict__ flags,
+                                                       const float *__restrict__ cx, const float *__restrict__ cy,
+                                                       const float *__restrict__ cz, const float *__restrict__ r,
+                                                       const float *__restrict__ z, float *__restrict__ out) {
+  constexpr int CPL = 4;
+  struct V {
+    float v[CPL];
+  };
+  const int per = (tl.total + 7) / 8, bx = gridDim.x >> 3, xcd = blockIdx.x & 7;
+  const int lane = threadIdx.x & 63;
+  const int64_t sI = (int64_t)d.l * d.m, sJ = d.m;
+  for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
+    const int t = xcd * per + tq;
+    if (t >= tl.total) break;
+    const int ktile = t % tl.kt, jtile = (t / tl.kt) % tl.jt, itile = t / (tl.kt * tl.jt);
+    const int k0 = min(1 + ktile * 64 * CPL, d.m - 1 - 64 * CPL) + lane * CPL;
+    const int j = 1 + jtile * 4 + (threadIdx.x >> 6);
+    const int i0 = 1 + itile * kSweepIC, i1 = min(i0 + kSweepIC, d.n - 1);
+    if (j > d.l - 2) continue;
+    float czm[CPL], czp[CPL];
+#pragma unroll
+    for (int e = 0; e < CPL; e++) {
+      czm[e] = cz[2 * (k0 + e)];
+      czp[e] = cz[2 * (k0 + e) + 1];
+    }
+    const float cym = cy[2 * j], cyp = cy[2 * j + 1];
+    int64_t c = ((int64_t)i0 * d.l + j) * d.m + k0;
+    V zm{}, zc{};
+    if (MODE != 0) {
+      __builtin_memcpy(&zm, z + c - sI, sizeof(V));
+      __builtin_memcpy(&zc, z + c, sizeof(V));
+    }
+    for (int i = i0; i < i1; i++, c += sI) {
+      uint8_t fl[CPL];
+      __builtin_memcpy(fl, flags + c, CPL);
+      V rr{}, zp{}, zjm{}, zjp{}, o;
+      if (MODE != 3) __builtin_memcpy(&rr, r + c, sizeof(V));
+      float left = 0, right = 0;
+      if (MODE != 0) {
+        __builtin_memcpy(&zp, z + c + sI, sizeof(V));
+        __builtin_memcpy(&zjm, z + c - sJ, sizeof(V));
+        __builtin_memcpy(&zjp, z + c + sJ, sizeof(V));
+        left = __shfl_up(zc.v[CPL - 1], 1, 64);
+        right = __shfl_down(zc.v[0], 1, 64);
+        if (lane == 0) left = z[c - 1];
+        if (lane == 63) right = z[c + CPL];
+      }
+      const float cxm = cx[2 * i], cxp = cx[2 * i + 1];
+#pragma unroll
+      for (int e = 0; e < CPL; e++) {
+        const uint8_t f = fl[e];
+        const float dg = diag_of(f, cxm, cxp, cym, cyp, czm[e], czp[e]);
+        const float wd = dg > 0 ? kOmega / dg : 0.f;
+        float v = 0;
+        if (MODE == 0) {
+          v = wd * rr.v[e];
+        } else {
+          const float x0 = zc.v[e];
+          const float km = e == 0 ? left : zc.v[e - 1];
+          const float kp = e == CPL - 1 ? right : zc.v[e + 1];
+          float tx = ((f & FL_IP) ? cxp * (zp.v[e] - x0) : 0.f) - ((f & FL_IM) ? cxm * (x0 - zm.v[e]) : 0.f);
+          float ty = ((f & FL_JP) ? cyp * (zjp.v[e] - x0) : 0.f) - ((f & FL_JM) ? cym * (x0 - zjm.v[e]) : 0.f);
+          float tz = ((f & FL_KP) ? czp[e] * (kp - x0) : 0.f) - ((f & FL_KM) ? czm[e] * (x0 - km) : 0.f);
+          const float az = -(tx + ty + tz);
+          if (MODE == 1) v = x0 + wd * (rr.v[e] - az);
+          else if (MODE == 2) v = rr.v[e] - az;
+          else v = az;
+        }
+        o.v[e] = (f & FL_ACTUAL) ? v : 0.f;
+      }
+      __builtin_memcpy(out + c, &o, sizeof(V));
+      if (MODE != 0) {
+        zm = zc;
+        zc = zp;
+      }
+    }
+  }
+}
+
 // coarse r = mean of the fine t over the unknown children (one thread per coarse interior cell)
 __global__ __launch_bounds__(256) void k_mg_restrict(Dims fd, Dims cd, const uint8_t *__restrict__ fflags,
                                                       const float *__restrict__ t, float *__restrict__ rc) {
@@ -186,6 +277,25 @@ __global__ void k_mg_scale(int64_t nc, float *__restrict__ z, const double *__re
 // ------------------------------------------------------------------ host
 static dim3 cell_grid(const xf_grid g) { return dim3((g->m - 2 + 63) / 64, (g->l - 2 + 3) / 4, g->n - 2); }
 static Dims dims_of(const xf_grid g) { return Dims{g->n, g->l, g->m}; }
+
+// out = sweep MODE on grid g (the vector kernel where rows allow it)
+template <int MODE>
+static void sweep(xf_ctx c, const xf_grid g, const float *r, const float *z, float *out) {
+  const Dims d = dims_of(g);
+  if (g->m - 2 >= 256) {
+    SweepTiles tl;
+    tl.kt = (g->m - 2 + 255) / 256;
+    tl.jt = (g->l - 2 + 3) / 4;
+    tl.it = (g->n - 2 + kSweepIC - 1) / kSweepIC;
+    tl.total = tl.kt * tl.jt * tl.it;
+    const int nb = std::min(2048, 8 * ((tl.total + 7) / 8));
+    hipLaunchKernelGGL(k_mg_sweep_vec<MODE>, dim3(nb), dim3(256), 0, c->stream, d, tl, g->d_flags, g->d_cxf,
+                       g->d_cyf, g->d_czf, r, z, out);
+  } else {
+    hipLaunchKernelGGL(k_mg_sweep<MODE>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
+                       g->d_czf, r, z, out);
+  }
+}
 
 static float *zalloc(xf_ctx c, int64_t n) {
   float *p = nullptr;
@@ -296,12 +406,9 @@ static int red_blocks(int64_t nc) { return (int)std::max<int64_t>(1, std::min<in
 // nu sweeps of damped Jacobi on A z = r starting from z = 0; result in L.z
 static void smooth_from_zero(xf_ctx c, MgLevel &L, const float *r, int nu) {
   const xf_grid g = L.g;
-  const Dims d = dims_of(g);
-  hipLaunchKernelGGL(k_mg_sweep<0>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
-                     g->d_czf, r, (const float *)nullptr, L.z);
+  sweep<0>(c, g, r, nullptr, L.z);
   for (int s = 1; s < nu; s++) {
-    hipLaunchKernelGGL(k_mg_sweep<1>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
-                       g->d_czf, r, L.z, L.z2);
+    sweep<1>(c, g, r, L.z, L.z2);
     std::swap(L.z, L.z2);
   }
 }
@@ -309,10 +416,8 @@ static void smooth_from_zero(xf_ctx c, MgLevel &L, const float *r, int nu) {
 // nu sweeps of damped Jacobi continuing from L.z
 static void smooth(xf_ctx c, MgLevel &L, const float *r, int nu) {
   const xf_grid g = L.g;
-  const Dims d = dims_of(g);
   for (int s = 0; s < nu; s++) {
-    hipLaunchKernelGGL(k_mg_sweep<1>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
-                       g->d_czf, r, L.z, L.z2);
+    sweep<1>(c, g, r, L.z, L.z2);
     std::swap(L.z, L.z2);
   }
 }
@@ -336,9 +441,7 @@ static void coarse_solve(xf_ctx c, MgHier *h, struct xf_unet_s *net) {
   hipLaunchKernelGGL(k_mg_finish<0>, dim3(1), dim3(256), 0, c->stream, h->part, nb, h->scal);
   unet_apply_grid(c, net, g, rin, L.z, h->scal);
   // energy-optimal scale: alpha = (r, y) / (y, A y)
-  const Dims d = dims_of(g);
-  hipLaunchKernelGGL(k_mg_sweep<3>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
-                     g->d_czf, (const float *)nullptr, L.z, L.t);
+  sweep<3>(c, g, nullptr, L.z, L.t);
   hipLaunchKernelGGL(k_mg_dots, dim3(nb), dim3(256), 0, c->stream, g->ncell, g->d_flags, rin, L.z, L.t, h->part);
   hipLaunchKernelGGL(k_mg_finish<1>, dim3(1), dim3(256), 0, c->stream, h->part, nb, h->scal);
   hipLaunchKernelGGL(k_mg_scale, dim3(red_blocks(g->ncell)), dim3(256), 0, c->stream, g->ncell, L.z, h->scal);
@@ -364,8 +467,7 @@ static void vcycle(xf_ctx c, MgHier *h, int k, const float *r, struct xf_unet_s 
   const xf_grid g = L.g;
   const Dims d = dims_of(g);
   smooth_from_zero(c, L, r, kSweeps);
-  hipLaunchKernelGGL(k_mg_sweep<2>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
-                     g->d_czf, r, L.z, L.t);
+  sweep<2>(c, g, r, L.z, L.t);
   hipLaunchKernelGGL(k_mg_restrict, cell_grid(C.g), dim3(256), 0, c->stream, d, dims_of(C.g), g->d_flags, L.t, C.r);
   vcycle(c, h, k + 1, C.r, net);
   hipLaunchKernelGGL(k_mg_prolong, cell_grid(g), dim3(256), 0, c->stream, d, dims_of(C.g), g->d_flags, C.z, L.z,
