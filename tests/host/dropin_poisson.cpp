@@ -19,8 +19,8 @@ static double rhs_of(double x, double y, double z) {
 
 struct Result {
   int n;
-  double trunc, err_cg, err_bicgstab, err_mr, err_fast;
-  int it_cg, it_bicgstab, it_mr, it_fast;
+  double trunc, err_cg, err_bicgstab, err_mr, err_fast, err_cycle;
+  int it_cg, it_bicgstab, it_mr, it_fast, it_cycle;
 };
 
 static Result run(int cells) {
@@ -94,6 +94,11 @@ static Result run(int cells) {
   R.err_fast = field_err(slae4.Result);
   R.it_fast = slae4.LastStats().iters;
 
+  TSLAE slae5(A, approximation, rightPart);
+  slae5.SolveByLearnedPreconditioner(nullptr, 1e-10, false);  // the cycle without a network
+  R.err_cycle = field_err(slae5.Result);
+  R.it_cycle = slae5.LastStats().inner_iters;
+
   if (cells <= 12) {
     TSLAE slae3(A, approximation, rightPart);
     slae3.SolveByMinimalResiduals();
@@ -122,12 +127,14 @@ int main() {
     for (int q = 0; q <= 5; q++) thomas_err = fmax(thomas_err, fabs(u[q] - (1 + q / 5.0)));
     printf("{\"trunc12\": %.3e, \"trunc24\": %.3e, \"err_cg12\": %.3e, \"err_cg24\": %.3e, \"it_cg12\": %d, "
            "\"it_cg24\": %d, \"err_bicgstab24\": %.3e, \"it_bicgstab24\": %d, \"err_fast24\": %.3e, "
-           "\"it_fast24\": %d, \"err_mr12\": %.3e, \"it_mr12\": %d, \"thomas_err\": %.3e}\n",
+           "\"it_fast24\": %d, \"err_cycle24\": %.3e, \"it_cycle24\": %d, \"err_mr12\": %.3e, \"it_mr12\": %d, "
+           "\"thomas_err\": %.3e}\n",
            a.trunc, b.trunc, a.err_cg, b.err_cg, a.it_cg, b.it_cg, b.err_bicgstab, b.it_bicgstab, b.err_fast,
-           b.it_fast, a.err_mr, a.it_mr, thomas_err);
+           b.it_fast, b.err_cycle, b.it_cycle, a.err_mr, a.it_mr, thomas_err);
     // second-order operator and solution error (halving h divides by ~4)
     bool ok = b.trunc < a.trunc / 3 && b.err_cg < a.err_cg / 3 && b.err_cg < 5e-3 &&
               fabs(b.err_bicgstab - b.err_cg) < 1e-4 && fabs(b.err_fast - b.err_cg) < 1e-4 &&
+              fabs(b.err_cycle - b.err_cg) < 1e-4 && b.it_cycle < b.it_fast &&
               fabs(a.err_mr - a.err_cg) < 1e-4 && thomas_err < 1e-14 && a.it_cg >= 3 && b.it_cg > a.it_cg;
     return ok ? 0 : 1;
   } catch (const char *e) {

@@ -507,27 +507,33 @@ static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r, T *w, T *p, T *s, c
   const bool f64 = sizeof(T) == 8;
   const char *n1 = f64 ? "cg_stencil_fp64" : "cg_stencil_fp32";
   const char *n2 = f64 ? "cg_update_fp64" : "cg_update_fp32";
+  // per-kernel events on every kProfEvery-th iteration only (markers between
+  // kernels cost a few microseconds each)
+  constexpr int kProfEvery = 8;
   std::vector<ProfMark> marks;
+  std::vector<int> sampled;  // iteration index of each sampled (KA, KB) pair of this batch
   int it = 0;
   for (;;) {
     for (int q = 0; q < every; q++, it++) {
-      int e1 = c->prof ? prof_begin(c) : -1;
+      const bool pm = c->prof && it % kProfEvery == 0;
+      int e1 = pm ? prof_begin(c) : -1;
       if (vec)
         hipLaunchKernelGGL((k_cgcg_stencil_vec<T, CPL>), dim3(nbA), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
                            g->d_flags, cx, cy, cz, r, w, c->state, wk->part);
       else
         hipLaunchKernelGGL((k_cgcg_stencil<T>), dim3(nbA), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
                            g->d_flags, cx, cy, cz, r, w, c->state, wk->part);
-      if (c->prof) {
+      if (pm) {
         prof_end(c, e1);
         marks.push_back({n1, e1});
       }
-      int e2 = c->prof ? prof_begin(c) : -1;
+      int e2 = pm ? prof_begin(c) : -1;
       hipLaunchKernelGGL((k_cgcg_update<T, X>), dim3(nbB), dim3(256), 0, c->stream, g->ncell, nbA, wk->part, x, r,
                          w, p, s, c->state, it, maxit);
-      if (c->prof) {
+      if (pm) {
         prof_end(c, e2);
         marks.push_back({n2, e2});
+        sampled.push_back(it);
       }
     }
     check_kernel("cg iterations");
@@ -535,9 +541,15 @@ static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r, T *w, T *p, T *s, c
     XF_HIP(hipStreamSynchronize(c->stream));
     if (c->prof) {
       // every stencil up to the deciding one ran in full; the deciding update
-      // launch only reduced, so only the `ran` updates are counted
-      const long long ran = c->h_state->iter;
-      prof_collect(c, marks, {{n1, ran + (c->h_state->done ? 1 : 0)}, {n2, ran}});
+      // launch only reduced, so only the updates before it are counted
+      const long long ran = c->h_state->iter, done = c->h_state->done ? 1 : 0;
+      long long na = 0, nb = 0;
+      for (int v : sampled) {
+        if (v < ran + done) na++;
+        if (v < ran) nb++;
+      }
+      prof_collect(c, marks, {{n1, na}, {n2, nb}});
+      sampled.clear();
     }
     if (c->h_state->done) break;
   }

@@ -419,11 +419,12 @@ TSLAE::TSLAE(const TRnLinearOperator &A, TRnSpace &x, TRnSpace &b) : x_(&x), b_(
 
 TSLAE::~TSLAE() {}
 
-void TSLAE::solve(int method, double rel_tol, bool project_mean) {
+void TSLAE::solve(int method, double rel_tol, bool project_mean, void *precond) {
   xf_solve_opts o;
   xf_solve_opts_default(method, &o);
   if (rel_tol > 0) o.tol = rel_tol;
   o.project_mean = project_mean ? 1 : 0;
+  o.precond = precond;
   int st = xf_solve(context(), A_->device_grid(), b_->device(), x_->device(), result_->device(), &o, &stats_);
   result_->mark_device_modified();
   check(st, "TSLAE::Solve");

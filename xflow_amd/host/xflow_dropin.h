@@ -299,10 +299,16 @@ class TSLAE {
   void SolveByMixedPrecision(double rel_tol = 1e-8, bool project_mean = true) {
     solve(XF_CG_MP, rel_tol, project_mean);
   }
+  // flexible CG in FP64 refinement with the V-cycle preconditioner; `net` (an
+  // xf_unet, e.g. with saved parameters loaded) is its coarsest-grid solve, or
+  // nullptr for damped-Jacobi sweeps there
+  void SolveByLearnedPreconditioner(xf_unet net, double rel_tol = 1e-8, bool project_mean = true) {
+    solve(XF_FCG_MGNN, rel_tol, project_mean, net);
+  }
   const xf_solve_stats &LastStats() const { return stats_; }
 
  private:
-  void solve(int method, double rel_tol = 0, bool project_mean = false);
+  void solve(int method, double rel_tol = 0, bool project_mean = false, void *precond = nullptr);
   const T3dLaplaceOperator *A_;
   TRnSpace *x_, *b_;
   std::unique_ptr<TRnSpace> result_;
