@@ -38,12 +38,31 @@ def coarse_case(ncell, fine, seed):
     return {"codes": codes, "hx": h, "hy": h.copy(), "hz": h.copy(), "b": b}
 
 
-def harvest(ctx, ncell, problems, seed0=100, coarsen_from=0, log=print):
-    """Returns device tensors R, E, M of shape [P*len(ITERATES), ncell, ncell, ncell] (interior only)."""
+def box_case(fine_shape, seed):
+    """The coarsest grid of a uniform-step box (fine_shape cells, e.g. the IB
+    channel's 256x128x128 pressure grid): all fluid (the immersed body is not
+    in the pressure mask) for even seeds, random obstacles for odd ones; the
+    right-hand side generated on the coarsest grid."""
+    obs = cases.random_obstacles(seed) if seed % 2 else []
+    length = tuple(s / fine_shape[1] for s in fine_shape)
+    codes = cases.neumann_codes(None, obs, shape=fine_shape, length=length)
+    while (codes.shape[2] - 2) // 2 >= 64 and all((s - 2) % 2 == 0 for s in codes.shape):
+        codes = cases.coarsen_codes(codes)
+    shape = tuple(s - 2 for s in codes.shape)
+    hc = length[0] / shape[0]
+    h = [np.full(n + 1, hc) for n in shape]
+    b = cases.divergence_rhs(codes, seed, h=[hc, hc, hc])
+    return {"codes": codes, "hx": h[0], "hy": h[1], "hz": h[2], "b": b}
+
+
+def harvest(ctx, ncell, problems, seed0=100, coarsen_from=0, box=None, log=print):
+    """Returns device tensors R, E, M of shape [P*len(ITERATES), D, H, W] (interior only)."""
     Rs, Es, Ms = [], [], []
     for pidx in range(problems):
         seed = seed0 + pidx
-        if coarsen_from:
+        if box:
+            case = box_case(box, seed)
+        elif coarsen_from:
             case = coarse_case(ncell, coarsen_from, seed)
         else:
             case = cases.poisson_case(ncell, cases.random_obstacles(seed), seed)
@@ -78,6 +97,9 @@ def harvest(ctx, ncell, problems, seed0=100, coarsen_from=0, log=print):
 
 def crops(R, E, M, B, S, gen):
     n, D, H, W = R.shape
+    if S is None or S <= 0:  # whole fields
+        idx = torch.randint(0, n, (B,), generator=gen)
+        return R[idx].contiguous(), M[idx].contiguous(), E[idx].contiguous()
     idx = torch.randint(0, n, (B,), generator=gen)
     zs = torch.randint(0, D - S + 1, (B,), generator=gen)
     ys = torch.randint(0, H - S + 1, (B,), generator=gen)
@@ -159,16 +181,23 @@ def main():
     ap.add_argument("--eval-ncell", type=int, default=128)
     ap.add_argument("--coarsen-from", type=int, default=0,
                     help="train the coarsest-grid network of the V-cycle: fine^3 masks coarsened to ncell^3")
+    ap.add_argument("--box", default="",
+                    help="train on the coarsest grid of a DxHxW-cell box (e.g. 256x128x128), whole fields")
     args = ap.parse_args()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     ctx = xf.Context(0)
     t0 = time.time()
-    R, E, M = harvest(ctx, args.ncell, args.problems, coarsen_from=args.coarsen_from, log=log)
+    box = tuple(int(v) for v in args.box.split("x")) if args.box else None
+    R, E, M = harvest(ctx, args.ncell, args.problems, coarsen_from=args.coarsen_from, box=box, log=log)
+    if box:
+        args.crop = 0
     log("harvested %d pairs in %.1f s" % (R.shape[0], time.time() - t0))
     net, hist = train(ctx, R, E, M, args.steps, args.batch, args.crop, args.lr, loss=args.loss, log=log)
     np.save(args.out, net.get_params())
     log("saved %s; loss %.4e -> %.4e" % (args.out, hist[0], np.mean(hist[-50:])))
-    if args.eval_ncell and args.coarsen_from:
+    if box:
+        pass
+    elif args.eval_ncell and args.coarsen_from:
         evaluate_mg(ctx, net, args.eval_ncell, 999, log=log)
     elif args.eval_ncell:
         evaluate(ctx, net, args.eval_ncell, 999, log=log)
