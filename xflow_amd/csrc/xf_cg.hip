@@ -18,7 +18,9 @@
 // state, written by block 0 of KB into the slot the next launch reads.
 // Algorithmic traffic per cell and iteration: KA reads r and the flag byte
 // and writes w (17 B in FP64, 9 B in FP32, interior cells); KB reads r, w, p,
-// s, x and writes p, s, x, r (72 B in FP64, 36 B in FP32, all array cells).
+// s and writes p, s, r, and on odd iterations also reads and writes x with
+// two steps at once (56 / 72 B in FP64, mean 64 B; 28 / 36 B in FP32, all
+// array cells).
 //
 // The stencil is the reference's Actual-row formula (XOperator.cpp:971-1019)
 // with the step ratios folded into per-node coefficients
@@ -326,26 +328,72 @@ __global__ __launch_bounds__(256) void k_cgcg_update(int64_t nc, int npart, cons
   }
   if (stop) return;
   const T a = (T)alpha, bt = (T)beta;
+  // x is updated on odd iterations only, with both steps in the order the
+  // eager update would add them (x + a_old p_old) + a p: same bits, 16 B per
+  // cell less every other iteration; k_cgcg_flush adds a pending even step
+  const bool xup = it & 1;
+  const X aold = xup ? (X)st->s[CG_ALPHA + par] : X(0);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  if (it == 0) {
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += stride) {
-      const T pn = r[c], sn = w[c];
-      p[c] = pn;
-      s[c] = sn;
-      x[c] = x[c] + (X)a * (X)pn;
-      r[c] = pn - a * sn;
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t c0 = 0;  // first element of the scalar tail
+  if (sizeof(X) == sizeof(T) && ((uintptr_t)x & 15) == 0) {
+    // 16-byte vectors (2 doubles / 4 floats) over the aligned bulk (the work
+    // vectors are allocation-aligned; x is the caller's); the same
+    // per-element arithmetic as the scalar tail below
+    constexpr int V = 16 / sizeof(T);
+    struct alignas(16) Vt {
+      T v[V];
+    };
+    struct alignas(16) Vx {
+      X v[V];
+    };
+    const int64_t nv = nc / V;
+    for (int64_t q = tid; q < nv; q += stride) {
+      Vt rc = ((const Vt *)r)[q], ww = ((const Vt *)w)[q], pp, ss;
+      Vx xx;
+      if (xup) xx = ((const Vx *)x)[q];
+      if (it != 0) {
+        pp = ((const Vt *)p)[q];
+        ss = ((const Vt *)s)[q];
+      }
+#pragma unroll
+      for (int e = 0; e < V; e++) {
+        const T pn = it == 0 ? rc.v[e] : rc.v[e] + bt * pp.v[e];
+        const T sn = it == 0 ? ww.v[e] : ww.v[e] + bt * ss.v[e];
+        if (xup) xx.v[e] = (xx.v[e] + aold * (X)pp.v[e]) + (X)a * (X)pn;
+        pp.v[e] = pn;
+        ss.v[e] = sn;
+        rc.v[e] = rc.v[e] - a * sn;
+      }
+      ((Vt *)p)[q] = pp;
+      ((Vt *)s)[q] = ss;
+      if (xup) ((Vx *)x)[q] = xx;
+      ((Vt *)r)[q] = rc;
     }
-  } else {
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += stride) {
-      const T rc = r[c];
-      const T pn = rc + bt * p[c];
-      const T sn = w[c] + bt * s[c];
-      p[c] = pn;
-      s[c] = sn;
-      x[c] = x[c] + (X)a * (X)pn;
-      r[c] = rc - a * sn;
-    }
+    c0 = nv * V;
   }
+  for (int64_t c = c0 + tid; c < nc; c += stride) {
+    const T rc = r[c];
+    const T po = it == 0 ? T(0) : p[c];
+    const T pn = it == 0 ? rc : rc + bt * po;
+    const T sn = it == 0 ? w[c] : w[c] + bt * s[c];
+    if (xup) x[c] = (x[c] + aold * (X)po) + (X)a * (X)pn;
+    p[c] = pn;
+    s[c] = sn;
+    r[c] = rc - a * sn;
+  }
+}
+
+// After the last KB: an even number of updates leaves the last one (even
+// index, so not yet in x) pending: x += alpha_last p.
+template <typename T, typename X>
+__global__ __launch_bounds__(256) void k_cgcg_flush(int64_t nc, X *__restrict__ x, const T *__restrict__ p,
+                                                     const DevState *st) {
+  const int applied = st->iter;
+  if (!(applied & 1)) return;
+  const X a = (X)st->s[CG_ALPHA + (applied & 1)];
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
+    x[c] = x[c] + a * (X)p[c];
 }
 
 // r = b - A x over the interior with the fast rows (x read everywhere, so
@@ -507,15 +555,15 @@ static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r, T *w, T *p, T *s, c
   const bool f64 = sizeof(T) == 8;
   const char *n1 = f64 ? "cg_stencil_fp64" : "cg_stencil_fp32";
   const char *n2 = f64 ? "cg_update_fp64" : "cg_update_fp32";
-  // per-kernel events on every kProfEvery-th iteration only (markers between
-  // kernels cost a few microseconds each)
+  // per-kernel events on two of every kProfEvery iterations only (markers
+  // between kernels cost a few microseconds each)
   constexpr int kProfEvery = 8;
   std::vector<ProfMark> marks;
   std::vector<int> sampled;  // iteration index of each sampled (KA, KB) pair of this batch
   int it = 0;
   for (;;) {
     for (int q = 0; q < every; q++, it++) {
-      const bool pm = c->prof && it % kProfEvery == 0;
+      const bool pm = c->prof && it % kProfEvery < 2;  // an even and an odd update
       int e1 = pm ? prof_begin(c) : -1;
       if (vec)
         hipLaunchKernelGGL((k_cgcg_stencil_vec<T, CPL>), dim3(nbA), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
@@ -553,6 +601,9 @@ static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r, T *w, T *p, T *s, c
     }
     if (c->h_state->done) break;
   }
+  if (c->h_state->iter & 1)
+    hipLaunchKernelGGL((k_cgcg_flush<T, X>), dim3(nbB), dim3(256), 0, c->stream, g->ncell, x, p, c->state);
+  check_kernel("cg flush");
 }
 
 static void upload_state_zero(xf_ctx c) {

@@ -175,9 +175,12 @@ __global__ void k_conv_first(Src0 s, const float *__restrict__ w, const float *_
   // w: [16][27][2] fp32 master weights
   const int64_t n = (int64_t)B * s.D * s.H * s.W;
   const float inv = src0_inv(s);
+  // 32-bit index arithmetic (volumes stay below 2^31 cells; 64-bit division
+  // dominated this kernel)
+  const unsigned W = s.W, H = s.H, D = s.D;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-    const int x = (int)(q % s.W), y = (int)((q / s.W) % s.H), z = (int)((q / ((int64_t)s.W * s.H)) % s.D);
-    const int b = (int)(q / ((int64_t)s.W * s.H * s.D));
+    const unsigned qq = (unsigned)q, qw = qq / W, qh = qw / H;
+    const int x = (int)(qq - qw * W), y = (int)(qw - qh * H), z = (int)(qh % D), b = (int)(qh / D);
     float in[27][2];
     for (int t = 0; t < 27; t++)
       src0_read(s, b, z + t / 9 - 1, y + (t / 3) % 3 - 1, x + t % 3 - 1, inv, in[t][0], in[t][1]);
@@ -210,8 +213,8 @@ __global__ void k_conv_last(const bf16 *__restrict__ a8, const float *__restrict
     s = (float)sqrt(ms > 0 ? ms : 0.0);
   }
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-    const int x = (int)(q % W), y = (int)((q / W) % H), z = (int)((q / ((int64_t)W * H)) % D);
-    const int b = (int)(q / ((int64_t)W * H * D));
+    const unsigned qq = (unsigned)q, qw = qq / (unsigned)W, qh = qw / (unsigned)H;
+    const int x = (int)(qq - qw * W), y = (int)(qw - qh * H), z = (int)(qh % (unsigned)D), b = (int)(qh / D);
     float acc = bias[0];
     for (int t = 0; t < 27; t++) {
       const int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
@@ -358,8 +361,8 @@ __global__ void k_last_dx(const float *__restrict__ dy, const float *__restrict_
                           int H, int W) {
   const int64_t n = (int64_t)B * D * H * W;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-    const int x = (int)(q % W), y = (int)((q / W) % H), z = (int)((q / ((int64_t)W * H)) % D);
-    const int b = (int)(q / ((int64_t)W * H * D));
+    const unsigned qq = (unsigned)q, qw = qq / (unsigned)W, qh = qw / (unsigned)H;
+    const int x = (int)(qq - qw * W), y = (int)(qw - qh * H), z = (int)(qh % (unsigned)D), b = (int)(qh / D);
     float acc[16] = {};
     for (int t = 0; t < 27; t++) {
       // output cell o = q - offset(t) contributes w[t] * dy[o] to this input cell
@@ -683,6 +686,7 @@ static void ensure_work(xf_unet u, int B, int D, int H, int W, bool train) {
   UnetWork &w = u->work;
   if (w.B == B && w.D == D && w.H == H && w.W == W && (!train || u->train_buffers)) return;
   w.release();
+  XF_REQUIRE((int64_t)B * D * H * W < ((int64_t)1 << 31), XF_EINVAL, "unet: more than 2^31 cells in one call");
   XF_REQUIRE(W % 64 == 0 && H % 16 == 0 && D % 16 == 0, XF_EINVAL,
              "unet: the volume must be a multiple of 16 in z, y and of 64 in x (three levels of 16-cell rows)");
   const size_t n1 = (size_t)B * D * H * W, n2 = n1 / 8, n3 = n1 / 64;

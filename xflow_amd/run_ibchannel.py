@@ -28,17 +28,14 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--every", type=int, default=20)
-    ap.add_argument("--weights", default=os.path.join(ROOT, "xflow_amd", "weights", "unet_coarse_channel.npy"),
-                    help="coarsest-grid network (falls back to unet_coarse64.npy if absent)")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "xflow_amd", "weights", "unet_coarse64.npy"),
+                    help="coarsest-grid network")
     args = ap.parse_args()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     ctx = xf.Context(0)
     net = UNet(ctx, seed=0)
-    path = args.weights
-    if not os.path.exists(path):
-        path = os.path.join(ROOT, "xflow_amd", "weights", "unet_coarse64.npy")
-    log("coarsest-grid network: %s" % path)
-    net.set_params(np.load(path))
+    log("coarsest-grid network: %s" % args.weights)
+    net.set_params(np.load(args.weights))
     ref = IBChannel(ctx, ny=args.ny, subdivisions=4, tol=args.tol)
     nn = IBChannel(ctx, ny=args.ny, subdivisions=4, tol=args.tol, solver=xf.XF_FCG_MGNN, precond=net.h)
     times = {"fp64": 0.0, "nn": 0.0}
