@@ -191,6 +191,24 @@ int xf_velocity_correct(xf_ctx ctx, xf_grid gv, int axis, const double *u_star, 
 int xf_divergence_max(xf_ctx ctx, xf_grid gp, const double *u, const double *v, const double *w, const double *hxU,
                       const double *hyV, const double *hzW, double *max_abs, int64_t *cell);
 
+/* ------------------------------------------------------- packed operator */
+/* T1dPackOperator (XOperator.cpp:1135-1571; XOperator.h:323-383): the grid's
+ * operator as packed rows over the Rn unknowns, for A and for A^T (operator~),
+ * with the rows' entries in activity-table order and zero coefficients left
+ * out, as the reference builds them; the product with Rn-ordered vectors sums
+ * each row in stored order (T1dPackOperator::operator(), XOperator.cpp:
+ * 1551-1568).  Device vectors of length n_unknowns; export copies to host
+ * arrays (counts[n_unknowns], vals/cols[nnz], 0-based columns). */
+typedef struct xf_pack_s *xf_pack;
+int xf_pack_create(xf_ctx ctx, xf_grid g, xf_pack *out);
+int xf_pack_destroy(xf_ctx ctx, xf_pack p);
+int xf_pack_info(xf_pack p, int64_t *n_unknowns, int64_t *nnz, int64_t *nnz_transpose);
+int xf_pack_export(xf_ctx ctx, xf_pack p, int transpose, int32_t *counts, double *vals, int32_t *cols);
+int xf_pack_apply(xf_ctx ctx, xf_pack p, int transpose, const double *x, double *y);
+/* field [ncell] <-> Rn vector [n_unknowns] (TRnSpace::operator[] order, XSpace.cpp:911-920) */
+int xf_pack_gather(xf_ctx ctx, xf_pack p, const double *field, double *vec);
+int xf_pack_scatter(xf_ctx ctx, xf_pack p, const double *vec, double *field);
+
 /* ------------------------------------------------- learned preconditioner */
 /* No reference counterpart: the learned preconditioner of XF_FCG_NN.  A
  * 3-level 3-D U-Net (channels 16/32/64, 3x3x3 convolutions, BF16 weights and

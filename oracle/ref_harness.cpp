@@ -19,6 +19,9 @@
 //   lod       THydrodynamicsProblem::SolveTransferEquation__ (XProblem.cpp:2784-3475)
 //   ib        TImmersedBoundaryProblem::Interpolate / SpreadForce / Index
 //             (XProblem.cpp:8550-8791, 8931-9046)
+//   pack      T1dPackOperator built from the Laplace operator: packed rows of
+//             A and of its transpose (counts, values, 0-based columns) and
+//             A x, A^T x on the Rn-ordered x (XOperator.cpp:1135-1571)
 //
 // File format (.xfb): repeated records
 //   u32 name_len, name bytes, u8 dtype ('d' f64 | 'i' i32), u32 ndim,
@@ -247,6 +250,34 @@ class CountingDivergented : public T3dVariableDivergentedOperator {
 
 static int count_unknowns(const Setup& s) { return s.model->EndIndex; }
 
+// Exposes the packed rows the reference builds (arithmetic and order are its own).
+class PackAccess : public T1dPackOperator {
+ public:
+  PackAccess(int model, const TRnLinearOperator& A, double** v)
+      : TRnLinearOperator(model), T1dPackOperator(model, A, v) {}
+  void dump(bool transpose, Arr& counts, Arr& vals, Arr& cols) const {
+    const int R = transpose ? fRowsCountT : fRowsCount;
+    const int* cnt = transpose ? fRowsElementsCountT : fRowsElementsCount;
+    double* const* mv = transpose ? fMatrixPackT : fMatrixPack;
+    int* const* mc = transpose ? fColumnsT : fColumns;
+    int64_t nnz = 0;
+    for (int r = 0; r < R; r++) nnz += cnt[r];
+    counts = mk_i({R});
+    vals = mk_d({nnz});
+    cols = mk_i({nnz});
+    int64_t q = 0;
+    for (int r = 0; r < R; r++) {
+      counts.i[r] = cnt[r];
+      for (int e = 0; e < cnt[r]; e++, q++) {
+        vals.d[q] = mv[r][e];
+        cols.i[q] = mc[r][e];
+      }
+    }
+  }
+};
+
+
+
 // ----------------------------------------------------------- commands
 static void cmd_op(const Bag& in, Bag& out, bool divergented) {
   Setup s = make_setup(in);
@@ -325,6 +356,40 @@ static void cmd_solver(const Bag& in, Bag& out, const std::string& which) {
   it.i[0] = (int)sweeps;
   it.i[1] = (int)(g_evals % (RN ? RN : 1));
   out["sweeps"] = it;
+}
+
+static void cmd_pack(const Bag& in, Bag& out) {
+  Setup s = make_setup(in);
+  const TNodesGrid* grids[1] = {s.grid};
+  CountingLaplace* lap = new CountingLaplace(s.model_name, 1, grids);
+  const int RN = count_unknowns(s);
+  T1dNumberMask* m1 = new T1dNumberMask(RN);
+  for (int q = 0; q < RN; q++) (*m1)[q] = TActualPoint;
+  TMask** ms = new TMask*[1];
+  ms[0] = m1;
+  TRnRelease1dSpaceModel* model1 = new TRnRelease1dSpaceModel(1, ms);
+  const int name1 = g_model_counter++;
+  TRnSpace::AddModel(name1, model1, false);
+  double* nullv = nullptr;
+  PackAccess* P = new PackAccess(name1, *lap, &nullv);
+  P->dump(false, out["counts"], out["vals"], out["cols"]);
+  P->dump(true, out["countsT"], out["valsT"], out["colsT"]);
+  TRnRelease3dSpace& x3 = *make_vec(s, in, "xvec");
+  TRnRelease1dSpace& x1 = *new TRnRelease1dSpace(name1);
+  for (int r = 1; r <= RN; r++) x1[r - 1] = ((TRnSpace&)x3)[r];
+  TRnRelease1dSpace& y1 = *new TRnRelease1dSpace(name1);
+  TRnRelease1dSpace& y2 = *new TRnRelease1dSpace(name1);
+  TRnLinearOperator& PA = *P;  // the packed operator through its linear-operator face
+  y1 |= PA(x1);
+  ~(*P);
+  y2 |= PA(x1);
+  Arr a = mk_d({RN}), b = mk_d({RN});
+  for (int r = 0; r < RN; r++) {
+    a.d[r] = y1[r];
+    b.d[r] = y2[r];
+  }
+  out["Ax"] = a;
+  out["ATx"] = b;
 }
 
 static void cmd_thomas(const Bag& in, Bag& out) {
@@ -441,6 +506,7 @@ int main(int argc, char** argv) {
     else if (cmd == "thomas") cmd_thomas(in, out);
     else if (cmd == "lod") cmd_lod(in, out);
     else if (cmd == "ib") cmd_ib(in, out);
+    else if (cmd == "pack") cmd_pack(in, out);
     else { fprintf(stderr, "unknown command %s\n", cmd.c_str()); return 2; }
     write_xfb(argv[3], out);
   } catch (const char* e) {

@@ -16,6 +16,8 @@
  *       (tests/golden), plus closed forms in tests/test_oracle_math.py;
  *   xo_cg_hs: textbook Hestenes-Stiefel CG (not in the reference), pinned by
  *       a dense numpy solve on small systems and by agreement with xo_cg3.
+ *   xo_pack / xo_pack_apply: T1dPackOperator's packed rows and product,
+ *       checked bit-exactly against the compiled reference's rows.
  *   xo_cg_cg: single-reduction CG of Chronopoulos and Gear (J. Comput. Appl.
  *       Math. 25, 1989; not in the reference), the throughput solver's
  *       recurrence; pinned by a dense numpy solve and by agreement with
@@ -617,6 +619,73 @@ int xo_cg_cg(const xo_grid *g, const double *b, const double *x0, double *x, dou
   if (resid) *resid = rr0 > 0 ? sqrt(gam / rr0) : 0;
   free(r); free(p); free(s); free(w); free(e);
   return st;
+}
+
+/* ------------------------------------------------------ packed operator */
+
+/* T1dPackOperator (XOperator.cpp:1135-1345): the operator's matrix in packed
+ * rows over the Rn unknowns.  Row r of A (transpose = 0) lists, in the order
+ * of the activity-table row r (self, then the i-1, i+1, j-1, j+1, k-1, k+1
+ * neighbours that are unknowns), the non-zero coefficients A[r][q]; row r of
+ * A^T lists, in the same order over q, the non-zero A[q][r].  A coefficient is
+ * the row evaluated on a unit vector, as the reference's column probing does
+ * (XOperator.cpp:1207-1224).  counts[nunk], vals/cols sized 7*nunk; returns
+ * the number of entries, or XO_EINVAL where a row throws or is empty. */
+int64_t xo_pack(const xo_grid *g, int transpose, int32_t *counts, double *vals, int32_t *cols) {
+  const int64_t nc = (int64_t)g->n * g->l * g->m;
+  int32_t *rn = (int32_t *)malloc(sizeof(int32_t) * nc);
+  int64_t nunk = xo_rn_number(g->n, g->l, g->m, g->space_mask, rn);
+  int64_t *cell = (int64_t *)malloc(sizeof(int64_t) * (nunk > 0 ? nunk : 1));
+  for (int64_t c = 0; c < nc; c++)
+    if (rn[c] > 0) cell[rn[c] - 1] = c;
+  double *f = zeros(nc);
+  int64_t q = 0, st = 0;
+  static const int d[7][3] = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  for (int64_t r = 0; r < nunk && st == 0; r++) {
+    const int64_t c = cell[r];
+    const int i = (int)(c / ((int64_t)g->l * g->m)), j = (int)((c / g->m) % g->l), k = (int)(c % g->m);
+    int cnt = 0;
+    for (int e = 0; e < 7; e++) {
+      const int a = i + d[e][0], b = j + d[e][1], w = k + d[e][2];
+      if (a < 0 || a >= g->n || b < 0 || b >= g->l || w < 0 || w >= g->m) continue;
+      const int64_t cc = IDX(g, a, b, w);
+      if (g->space_mask[cc] <= 0) continue;
+      double v = 0;
+      int s2;
+      if (!transpose) {  /* A[r][rn(cc)]: row at c, unit vector at cc */
+        f[cc] = 1;
+        s2 = row_value(g, f, i, j, k, &v);
+        f[cc] = 0;
+      } else {           /* A[rn(cc)][r]: row at cc, unit vector at c */
+        f[c] = 1;
+        s2 = row_value(g, f, a, b, w, &v);
+        f[c] = 0;
+      }
+      if (s2) { st = XO_EINVAL; break; }
+      if (v != 0) {
+        vals[q] = v;
+        cols[q] = rn[cc] - 1;
+        q++;
+        cnt++;
+      }
+    }
+    if (!cnt) st = XO_EINVAL;
+    counts[r] = cnt;
+  }
+  free(rn); free(cell); free(f);
+  return st ? st : q;
+}
+
+/* y[r] = sum over the packed row r in stored order (T1dPackOperator::operator(),
+ * XOperator.cpp:1551-1568). */
+void xo_pack_apply(int64_t nunk, const int32_t *counts, const double *vals, const int32_t *cols, const double *x,
+                   double *y) {
+  int64_t q = 0;
+  for (int64_t r = 0; r < nunk; r++) {
+    double res = 0;
+    for (int e = 0; e < counts[r]; e++, q++) res = res + vals[q] * x[cols[q]];
+    y[r] = res;
+  }
 }
 
 /* --------------------------------------------------------- transport */

@@ -66,6 +66,10 @@ int xo_cg_hs(const xo_grid *g, const double *b, const double *x0, double *x, dou
 int xo_cg_cg(const xo_grid *g, const double *b, const double *x0, double *x, double rtol, int maxit,
              int project_mean, int *iters, double *resid);
 
+int64_t xo_pack(const xo_grid *g, int transpose, int32_t *counts, double *vals, int32_t *cols);
+void xo_pack_apply(int64_t nunk, const int32_t *counts, const double *vals, const int32_t *cols, const double *x,
+                   double *y);
+
 void xo_thomas(int M, double *u, const double *a, const double *b, const double *c, const double *f);
 int  xo_lod(const xo_grid *g, double *f, const double *f_prev, const double *u, const double *v,
             const double *w, const double *rp, const double *kappa, double tau, int scheme);

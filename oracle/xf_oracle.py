@@ -49,6 +49,8 @@ def lib():
             "xo_bicgstab": (i, [gp, p, p, p, d, i, p, p]),
             "xo_cg_hs": (i, [gp, p, p, p, d, i, i, p, p]),
             "xo_cg_cg": (i, [gp, p, p, p, d, i, i, p, p]),
+            "xo_pack": (C.c_int64, [gp, i, p, p, p]),
+            "xo_pack_apply": (None, [C.c_int64, p, p, p, p, p]),
             "xo_thomas": (None, [i, p, p, p, p, p]),
             "xo_lod": (i, [gp, p, p, p, p, p, p, p, d, i]),
             "xo_ib_delta": (d, [d, d]),
@@ -191,6 +193,28 @@ def bicgstab(g, b, x0, eps=1e-5, maxit=100000):
 
 def cg_hs(g, b, x0, rtol=1e-8, maxit=100000, project_mean=False):
     return _solve(lib().xo_cg_hs, g, b, x0, rtol, maxit, int(bool(project_mean)))
+
+
+def pack(g, transpose=False):
+    """T1dPackOperator rows: (counts[nunk], vals[nnz], cols[nnz], 0-based columns)."""
+    nunk = int((np.asarray(g.space_mask) > 0).sum())
+    counts = np.zeros(nunk, dtype=np.int32)
+    vals = np.zeros(7 * max(nunk, 1))
+    cols = np.zeros(7 * max(nunk, 1), dtype=np.int32)
+    nnz = lib().xo_pack(g.ref, int(bool(transpose)), counts.ctypes.data, vals.ctypes.data, cols.ctypes.data)
+    if nnz < 0:
+        raise ValueError("xo_pack: a row throws in the reference or is empty")
+    return counts, vals[:nnz].copy(), cols[:nnz].copy()
+
+
+def pack_apply(counts, vals, cols, x):
+    counts = np.ascontiguousarray(counts, dtype=np.int32)
+    vals, x = _f64(vals), _f64(x)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    y = np.zeros(counts.size)
+    lib().xo_pack_apply(counts.size, counts.ctypes.data, vals.ctypes.data, cols.ctypes.data, x.ctypes.data,
+                        y.ctypes.data)
+    return y
 
 
 def cg_cg(g, b, x0, rtol=1e-8, maxit=100000, project_mean=False):

@@ -108,6 +108,26 @@ def case_op():
     save("op_sphere16", inp, run("op", inp))
 
 
+def case_pack():
+    """T1dPackOperator of the Laplace operator on the mixed-border mask (PreDefined
+    and PreNormal rows make it non-symmetric) and on the uniform sphere mask."""
+    rng = np.random.default_rng(6)
+    n, l, m = 12, 10, 9
+    x = coords_from_steps(rng.uniform(0.05, 0.15, n - 1))
+    y = coords_from_steps(rng.uniform(0.05, 0.15, l - 1), 0.3)
+    z = coords_from_steps(rng.uniform(0.05, 0.15, m - 1), -0.2)
+    codes, normals, has = mixed_border_mask(n, l, m, rng)
+    inp = {"x": x, "y": y, "z": z, "codes": codes, "normals": normals, "has_normal": has,
+           "space_mask": space_mask_rule(codes), "xvec": rng.standard_normal((n, l, m))}
+    save("pack_mixed", inp, run("pack", inp))
+    nc = 10
+    codes = cases.neumann_codes(nc, [cases.sphere_obstacle((0.5, 0.5, 0.5), 0.25)]).astype(np.int32)
+    xx = cases.cell_centres(nc)
+    inp = {"x": xx, "y": xx.copy(), "z": xx.copy(), "codes": codes, "space_mask": space_mask_rule(codes),
+           "xvec": rng.standard_normal(codes.shape)}
+    save("pack_sphere10", inp, run("pack", inp))
+
+
 def case_divop():
     rng = np.random.default_rng(2)
     n, l, m = 11, 9, 10
@@ -233,3 +253,4 @@ if __name__ == "__main__":
     case_thomas()
     case_lod()
     case_ib()
+    case_pack()

@@ -97,3 +97,21 @@ def test_ib_interp_spread(axis):
     assert np.array_equal(U, outs["U"])
     F = xo.ib_spread(g, axis, nodes, ins["force_n"])
     assert np.array_equal(F, outs["force"])
+
+
+@pytest.mark.parametrize("name", ["pack_mixed", "pack_sphere10"])
+def test_pack_operator_rows_and_products(name):
+    """T1dPackOperator: packed rows of A and A^T (counts, values, 0-based
+    columns, order) and the products A x, A^T x on the Rn-ordered x."""
+    ins, outs = golden(name)
+    g = grid_from(ins)
+    rn, _ = xo.rn_number(ins["space_mask"])
+    x_rn = np.zeros(int(rn.max()))
+    x_rn[rn[rn > 0] - 1] = ins["xvec"][rn > 0]
+    for sfx, tr in (("", False), ("T", True)):
+        counts, vals, cols = xo.pack(g, transpose=tr)
+        assert np.array_equal(counts, outs["counts" + sfx])
+        assert np.array_equal(cols, outs["cols" + sfx])
+        assert np.array_equal(vals, outs["vals" + sfx])
+        y = xo.pack_apply(counts, vals, cols, x_rn)
+        assert np.array_equal(y, outs["Ax" if not tr else "ATx"])

@@ -12,7 +12,7 @@ import numpy as np
 from ._lib import (XF_BICG, XF_BICGSTAB, XF_CG, XF_CG3, XF_CG_MP, XF_FCG_MGNN, XF_FCG_NN, XF_MR, XF_OK, XF_TOL_ABS,
                    XF_TOL_REL, SolveOpts, SolveStats, XfError, lib)
 
-__all__ = ["Context", "Grid", "XfError", "XF_CG3", "XF_BICG", "XF_MR", "XF_BICGSTAB", "XF_CG", "XF_CG_MP",
+__all__ = ["Context", "Grid", "Pack", "XfError", "XF_CG3", "XF_BICG", "XF_MR", "XF_BICGSTAB", "XF_CG", "XF_CG_MP",
            "XF_FCG_NN", "XF_FCG_MGNN", "XF_TOL_ABS", "XF_TOL_REL", "solve_opts"]
 
 
@@ -256,3 +256,50 @@ def solve_opts(method, **kw):
             raise AttributeError(k)
         setattr(o, k, v)
     return o
+
+
+class Pack:
+    """T1dPackOperator on the device (xf_pack): packed rows of A and A^T over the
+    grid's Rn unknowns and their products with Rn-ordered device vectors."""
+
+    def __init__(self, grid):
+        self.ctx, self.grid = grid.ctx, grid
+        h = C.c_void_p()
+        self.ctx.check(lib().xf_pack_create(self.ctx.h, grid.h, C.byref(h)), "xf_pack_create")
+        self.h = h
+        n, a, t = C.c_int64(), C.c_int64(), C.c_int64()
+        lib().xf_pack_info(h, C.byref(n), C.byref(a), C.byref(t))
+        self.n_unknowns, self.nnz = n.value, (a.value, t.value)
+
+    def export(self, transpose=False):
+        t = 1 if transpose else 0
+        counts = np.zeros(self.n_unknowns, dtype=np.int32)
+        vals = np.zeros(max(self.nnz[t], 1))
+        cols = np.zeros(max(self.nnz[t], 1), dtype=np.int32)
+        self.ctx.check(lib().xf_pack_export(self.ctx.h, self.h, t, counts.ctypes.data, vals.ctypes.data,
+                                            cols.ctypes.data), "xf_pack_export")
+        return counts, vals[:self.nnz[t]], cols[:self.nnz[t]]
+
+    def apply(self, x, y, transpose=False):
+        self.ctx.check(lib().xf_pack_apply(self.ctx.h, self.h, 1 if transpose else 0, _ptr(x), _ptr(y)),
+                       "xf_pack_apply")
+        return y
+
+    def gather(self, field, vec):
+        self.ctx.check(lib().xf_pack_gather(self.ctx.h, self.h, _ptr(field), _ptr(vec)), "xf_pack_gather")
+        return vec
+
+    def scatter(self, vec, field):
+        self.ctx.check(lib().xf_pack_scatter(self.ctx.h, self.h, _ptr(vec), _ptr(field)), "xf_pack_scatter")
+        return field
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().xf_pack_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

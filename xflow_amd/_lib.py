@@ -74,6 +74,13 @@ SIGNATURES = {
     "xf_mg_levels": (I, [P, P, C.POINTER(I)]),
     "xf_mg_apply": (I, [P, P, P, P, P]),
     "xf_mg_level": (I, [P, P, I, C.POINTER(P)]),
+    "xf_pack_create": (I, [P, P, C.POINTER(P)]),
+    "xf_pack_destroy": (I, [P, P]),
+    "xf_pack_info": (I, [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64)]),
+    "xf_pack_export": (I, [P, P, I, P, P, P]),
+    "xf_pack_apply": (I, [P, P, I, P, P]),
+    "xf_pack_gather": (I, [P, P, P, P]),
+    "xf_pack_scatter": (I, [P, P, P, P]),
     "xf_transport_velocity": (I, [P, P, I, P, P, P, P, P, P]),
     "xf_profile_enable": (I, [P, I]),
     "xf_profile_reset": (I, [P]),

@@ -1,7 +1,8 @@
 """A short, fixed workload for hardware-counter runs (rocprofv3 --pmc).
 
-It launches a plain device copy of a known size (the calibration kernel: its
-FETCH_SIZE and WRITE_SIZE should each read the copied bytes), then a few
+It launches a plain device copy of a known size, after evicting its source
+from the Infinity Cache (the calibration kernel: its FETCH_SIZE and
+WRITE_SIZE should each read the copied bytes), then a few
 iterations of the FP64 and FP32 throughput CG on the 256^3 random-obstacle
 pressure problem, so every counter pass sees the same dispatches:
 
@@ -30,8 +31,14 @@ def main():
     nbytes = args.copy_mib << 20
     src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
+    # evict src from the 256 MB Infinity Cache first (fill 1 GiB elsewhere), so
+    # the copy's reads come from HBM
+    flush = torch.empty((1 << 30) // 8, dtype=torch.float64, device="cuda")
+    flush.fill_(2.0)
+    torch.cuda.synchronize()
     dst.copy_(src)
     torch.cuda.synchronize()
+    del flush
     print("copy: %d bytes read, %d bytes written" % (nbytes, nbytes), flush=True)
 
     case = cases.obstacle_case(args.n, seed=0)
