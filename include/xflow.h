@@ -208,6 +208,11 @@ int xf_pack_apply(xf_ctx ctx, xf_pack p, int transpose, const double *x, double 
 /* field [ncell] <-> Rn vector [n_unknowns] (TRnSpace::operator[] order, XSpace.cpp:911-920) */
 int xf_pack_gather(xf_ctx ctx, xf_pack p, const double *field, double *vec);
 int xf_pack_scatter(xf_ctx ctx, xf_pack p, const double *vec, double *field);
+/* TSLAE on the packed operator (the reference's BiCG runs on it, with operator~
+ * giving A^T, XSLAE.cpp:159-348): methods XF_CG3, XF_BICG, XF_MR, XF_BICGSTAB on
+ * Rn vectors of length n_unknowns */
+int xf_pack_solve(xf_ctx ctx, xf_pack p, const double *b, const double *x0, double *x, const xf_solve_opts *opts,
+                  xf_solve_stats *stats);
 
 /* ------------------------------------------------- learned preconditioner */
 /* No reference counterpart: the learned preconditioner of XF_FCG_NN.  A

@@ -22,6 +22,9 @@
 //   pack      T1dPackOperator built from the Laplace operator: packed rows of
 //             A and of its transpose (counts, values, 0-based columns) and
 //             A x, A^T x on the Rn-ordered x (XOperator.cpp:1135-1571)
+//   pack_bicg|pack_bicgstab|pack_mr
+//             TSLAE solvers run on that packed operator over Rn vectors
+//             (XSLAE.cpp:159-414, 667-767); sweeps counted per packed row
 //
 // File format (.xfb): repeated records
 //   u32 name_len, name bytes, u8 dtype ('d' f64 | 'i' i32), u32 ndim,
@@ -255,6 +258,10 @@ class PackAccess : public T1dPackOperator {
  public:
   PackAccess(int model, const TRnLinearOperator& A, double** v)
       : TRnLinearOperator(model), T1dPackOperator(model, A, v) {}
+  double operator()(int id, int i) const override {
+    ++g_evals;
+    return T1dPackOperator::operator()(id, i);
+  }
   void dump(bool transpose, Arr& counts, Arr& vals, Arr& cols) const {
     const int R = transpose ? fRowsCountT : fRowsCount;
     const int* cnt = transpose ? fRowsElementsCountT : fRowsElementsCount;
@@ -392,6 +399,46 @@ static void cmd_pack(const Bag& in, Bag& out) {
   out["ATx"] = b;
 }
 
+// A TSLAE solver run on the packed operator over Rn vectors (BiCG's A_ = A.Copy()
+// then ~A_ selects the transposed rows, XSLAE.cpp:205-206).
+static void cmd_pack_solve(const Bag& in, Bag& out, const std::string& which) {
+  Setup s = make_setup(in);
+  const TNodesGrid* grids[1] = {s.grid};
+  CountingLaplace* lap = new CountingLaplace(s.model_name, 1, grids);
+  const int RN = count_unknowns(s);
+  T1dNumberMask* m1 = new T1dNumberMask(RN);
+  for (int q = 0; q < RN; q++) (*m1)[q] = TActualPoint;
+  TMask** ms = new TMask*[1];
+  ms[0] = m1;
+  TRnRelease1dSpaceModel* model1 = new TRnRelease1dSpaceModel(1, ms);
+  const int name1 = g_model_counter++;
+  TRnSpace::AddModel(name1, model1, false);
+  double* nullv = nullptr;
+  PackAccess* P = new PackAccess(name1, *lap, &nullv);
+  TRnRelease3dSpace& b3 = *make_vec(s, in, "bvec");
+  TRnRelease3dSpace& x3 = *make_vec(s, in, "x0");
+  TRnRelease1dSpace* b1 = new TRnRelease1dSpace(name1);
+  TRnRelease1dSpace* x1 = new TRnRelease1dSpace(name1);
+  for (int r = 1; r <= RN; r++) {
+    (*b1)[r - 1] = ((TRnSpace&)b3)[r];
+    (*x1)[r - 1] = ((TRnSpace&)x3)[r];
+  }
+  TSLAE* slae = new TSLAE(*P, *x1, *b1);
+  g_evals = 0;
+  if (which == "pack_bicg") slae->SolveByBiConjugateGradients();
+  else if (which == "pack_bicgstab") slae->SolveByBiConjugateGradientsStab();
+  else if (which == "pack_mr") slae->SolveByMinimalResiduals();
+  else throw std::runtime_error("unknown packed solver");
+  const TRnSpace& res = slae->GetResult();
+  Arr xr = mk_d({RN});
+  for (int r = 1; r <= RN; r++) xr.d[r - 1] = const_cast<TRnSpace&>(res)[r];  // TRnSpace::operator[] is 1-based
+  out["x"] = xr;
+  Arr it = mk_i({2});
+  it.i[0] = (int)(RN ? g_evals / RN : 0);
+  it.i[1] = (int)(g_evals % (RN ? RN : 1));
+  out["sweeps"] = it;
+}
+
 static void cmd_thomas(const Bag& in, Bag& out) {
   const Arr& a = need(in, "a");
   const Arr& b = need(in, "b");
@@ -507,6 +554,7 @@ int main(int argc, char** argv) {
     else if (cmd == "lod") cmd_lod(in, out);
     else if (cmd == "ib") cmd_ib(in, out);
     else if (cmd == "pack") cmd_pack(in, out);
+    else if (cmd == "pack_bicg" || cmd == "pack_bicgstab" || cmd == "pack_mr") cmd_pack_solve(in, out, cmd);
     else { fprintf(stderr, "unknown command %s\n", cmd.c_str()); return 2; }
     write_xfb(argv[3], out);
   } catch (const char* e) {

@@ -17,7 +17,9 @@
  *   xo_cg_hs: textbook Hestenes-Stiefel CG (not in the reference), pinned by
  *       a dense numpy solve on small systems and by agreement with xo_cg3.
  *   xo_pack / xo_pack_apply: T1dPackOperator's packed rows and product,
- *       checked bit-exactly against the compiled reference's rows.
+ *       checked bit-exactly against the compiled reference's rows;
+ *   xo_pack_solve: the reference solvers on those rows, bit-exact against
+ *       the compiled reference's solutions on the packed operator.
  *   xo_cg_cg: single-reduction CG of Chronopoulos and Gear (J. Comput. Appl.
  *       Math. 25, 1989; not in the reference), the throughput solver's
  *       recurrence; pinned by a dense numpy solve and by agreement with
@@ -305,49 +307,121 @@ static double *dup(const double *s, int64_t nc) {
 static double *zeros(int64_t nc) { return (double *)calloc(nc, sizeof(double)); }
 static double rsqrt_ref(double v) { return (double)sqrtl((long double)v); }
 
+/* The operator a reference solver runs on: the grid operator over the
+ * unknown cells of an n*l*m array (A^T taken as A: the grid operator has no
+ * transpose, XOperator.h:302), or packed rows of A and A^T over Rn vectors,
+ * every entry an unknown (T1dPackOperator, XOperator.h:322-370). */
+typedef struct {
+  int64_t nc;                 /* vector length                               */
+  const int32_t *unk;         /* unknown flags per entry; NULL: every entry  */
+  const xo_grid *g;           /* grid operator, or NULL for packed rows      */
+  const int64_t *ptr[2];      /* packed rows of A ([0]) and A^T ([1])        */
+  const double *vals[2];
+  const int32_t *cols[2];
+} xo_op;
+
+#define UNK(o, c) (!(o)->unk || (o)->unk[c] > 0)
+
+/* Packed row r summed in stored order (XOperator.cpp:1551-1568). */
+static double op_row(const xo_op *o, int t, int64_t r, const double *x) {
+  double res = 0;
+  for (int64_t q = o->ptr[t][r]; q < o->ptr[t][r + 1]; q++) res = res + o->vals[t][q] * x[o->cols[t][q]];
+  return res;
+}
+
+static int op_apply(const xo_op *o, int t, const double *x, const double *b, double sign_b, double *y) {
+  if (o->g) return xo_apply(o->g, x, b, sign_b, y);
+  for (int64_t r = 0; r < o->nc; r++) {
+    double v = op_row(o, t, r, x);
+    if (sign_b > 0) y[r] = v + b[r];
+    else if (sign_b < 0) y[r] = v - b[r];
+    else y[r] = v;
+  }
+  return XO_OK;
+}
+
+static double op_apply_dot(const xo_op *o, const double *x, int *status) {
+  if (o->g) return xo_apply_dot(o->g, x, status);
+  double res = 0;
+  for (int64_t r = 0; r < o->nc; r++) res = res + op_row(o, 0, r, x) * x[r];
+  if (status) *status = XO_OK;
+  return res;
+}
+
+static void op_lincomb(const xo_op *o, double p1, const double *s1, double p2, const double *s2, double *res) {
+  if (o->g) { xo_lincomb(o->g, p1, s1, p2, s2, res); return; }
+  for (int64_t c = 0; c < o->nc; c++) {
+    if (p1 != 0 && p2 != 0) res[c] = p1 * s1[c] + p2 * s2[c];
+    else if (p1 != 0) res[c] = p1 * s1[c];
+    else if (p2 != 0) res[c] = p2 * s2[c];
+    else res[c] = 0;
+  }
+}
+
+static void op_copy(const xo_op *o, const double *s, double *res) {
+  if (o->g) { xo_copy(o->g, s, res); return; }
+  memcpy(res, s, sizeof(double) * o->nc);
+}
+
+static double op_dot(const xo_op *o, const double *a, const double *b) {
+  if (o->g) return xo_dot(o->g, a, b);
+  double res = 0;
+  for (int64_t c = 0; c < o->nc; c++) res = res + a[c] * b[c];
+  return res;
+}
+
+static xo_op grid_op(const xo_grid *g) {
+  xo_op o;
+  memset(&o, 0, sizeof(o));
+  o.nc = (int64_t)g->n * g->l * g->m;
+  o.unk = g->space_mask;
+  o.g = g;
+  return o;
+}
+
 /* Samarskii three-term CG, XSLAE.cpp:56-157.  Stops when ||r||_2 <= eps
  * (absolute; the reference hard-codes 1e-5 at XSLAE.cpp:64).  maxit bounds
  * the loop (the reference has no bound); NaN counts as failure here. */
-int xo_cg3(const xo_grid *g, const double *b, const double *x0, double *x, double eps, int maxit,
-           int *iters, double *resid) {
-  int64_t nc = (int64_t)g->n * g->l * g->m;
+static int cg3_op(const xo_op *o, const double *b, const double *x0, double *x, double eps, int maxit,
+                  int *iters, double *resid) {
+  int64_t nc = o->nc;
   int st = XO_OK, it = 0;
   double *res = dup(x0, nc);
   double *xk_1 = dup(x0, nc), *rk_1 = zeros(nc), *xk = NULL, *rk = NULL;
   double rr1 = 0, eps_cur = 0;
-  xo_lincomb(g, 0, NULL, 0, NULL, res);
-  if (xo_apply(g, xk_1, b, -1, rk_1)) { st = XO_EINVAL; goto done; }
-  rr1 = xo_dot(g, rk_1, rk_1);
-  xo_copy(g, xk_1, res);
+  op_lincomb(o, 0, NULL, 0, NULL, res);
+  if (op_apply(o, 0, xk_1, b, -1, rk_1)) { st = XO_EINVAL; goto done; }
+  rr1 = op_dot(o, rk_1, rk_1);
+  op_copy(o, xk_1, res);
   eps_cur = rsqrt_ref(rr1);
   if (rr1 != 0) {
     int s2;
-    double tauk = rr1 / xo_apply_dot(g, rk_1, &s2);
+    double tauk = rr1 / op_apply_dot(o, rk_1, &s2);
     xk = dup(x0, nc);
-    xo_lincomb(g, 1, xk_1, -tauk, rk_1, xk);
+    op_lincomb(o, 1, xk_1, -tauk, rk_1, xk);
     rk = zeros(nc);
-    xo_apply(g, xk, b, -1, rk);
-    double rr = xo_dot(g, rk, rk);
-    xo_copy(g, xk, res);
+    op_apply(o, 0, xk, b, -1, rk);
+    double rr = op_dot(o, rk, rk);
+    op_copy(o, xk, res);
     eps_cur = rsqrt_ref(rr);
     if (rr != 0) {
       double alfak = 1;
       while (eps_cur > eps) {
         if (it >= maxit) { st = XO_EMAXIT; break; }
         it++;
-        double tauk1 = rr / xo_apply_dot(g, rk, &s2);
+        double tauk1 = rr / op_apply_dot(o, rk, &s2);
         double alfak1 = 1 - (1 / alfak) * (tauk1 / tauk) * (rr / rr1);
         alfak1 = 1 / alfak1;
-        xo_lincomb(g, alfak1, xk, 1 - alfak1, xk_1, res);
-        xo_lincomb(g, 1, res, -(tauk1 * alfak1), rk, res);
-        xo_copy(g, xk, xk_1);
-        xo_copy(g, rk, rk_1);
+        op_lincomb(o, alfak1, xk, 1 - alfak1, xk_1, res);
+        op_lincomb(o, 1, res, -(tauk1 * alfak1), rk, res);
+        op_copy(o, xk, xk_1);
+        op_copy(o, rk, rk_1);
         rr1 = rr;
         tauk = tauk1;
         alfak = alfak1;
-        xo_copy(g, res, xk);
-        xo_apply(g, xk, b, -1, rk);
-        rr = xo_dot(g, rk, rk);
+        op_copy(o, res, xk);
+        op_apply(o, 0, xk, b, -1, rk);
+        rr = op_dot(o, rk, rk);
         eps_cur = rsqrt_ref(rr);
       }
       if (eps_cur != eps_cur) st = XO_ENAN;
@@ -362,59 +436,59 @@ done:
 }
 
 /* BiCG, XSLAE.cpp:159-348.  x0 is used only for its non-unknown entries
- * (the reference zeroes the unknowns, XSLAE.cpp:171); A^T is taken as A, so
- * this is defined for symmetric operators (the reference's 3-D operator
- * throws on transposition, XOperator.h:302). */
-int xo_bicg(const xo_grid *g, const double *b, const double *x0, double *x, double eps, int maxit,
-            int *iters, double *resid) {
-  int64_t nc = (int64_t)g->n * g->l * g->m;
+ * (the reference zeroes the unknowns, XSLAE.cpp:171).  The shadow sequence
+ * applies the operator's transpose (op_apply with t = 1): the packed rows of
+ * A^T, or A itself on the grid. */
+static int bicg_op(const xo_op *o, const double *b, const double *x0, double *x, double eps, int maxit,
+                   int *iters, double *resid) {
+  int64_t nc = o->nc;
   int st = XO_OK, it = 0;
   double *apk = zeros(nc);
   double *xk = dup(x0, nc), *xk_new = dup(x0, nc);
   double *rk = zeros(nc), *rt = zeros(nc), *rk_new = zeros(nc), *rt_new = zeros(nc);
   double *pk = zeros(nc), *pt = zeros(nc), *pk_new = zeros(nc), *pt_new = zeros(nc);
   double *res = dup(x0, nc);
-  xo_lincomb(g, 0, NULL, 0, NULL, res);
-  xo_lincomb(g, 0, NULL, 0, NULL, xk);
-  xo_apply(g, xk, b, -1, rk);
-  xo_lincomb(g, -1, rk, 0, NULL, rt);
-  xo_copy(g, rt, rk);
-  xo_copy(g, rk, pk);
-  xo_copy(g, rt, pt);
-  double rr = xo_dot(g, rk, rk);
-  xo_copy(g, xk, res);
+  op_lincomb(o, 0, NULL, 0, NULL, res);
+  op_lincomb(o, 0, NULL, 0, NULL, xk);
+  op_apply(o, 0, xk, b, -1, rk);
+  op_lincomb(o, -1, rk, 0, NULL, rt);
+  op_copy(o, rt, rk);
+  op_copy(o, rk, pk);
+  op_copy(o, rt, pt);
+  double rr = op_dot(o, rk, rk);
+  op_copy(o, xk, res);
   double eps_cur = rsqrt_ref(rr);
   while (eps_cur > eps) {
     if (it >= maxit) { st = XO_EMAXIT; break; }
     it++;
-    double a1 = xo_dot(g, rk, rt);
+    double a1 = op_dot(o, rk, rt);
     /* (A(pk), pt): fused product of A pk with pt, serial i->j->k sum */
-    xo_apply(g, pk, NULL, 0, apk);
+    op_apply(o, 0, pk, NULL, 0, apk);
     double a2 = 0;
     for (int64_t c = 0; c < nc; c++)
-      if (g->space_mask[c] > 0) a2 = a2 + apk[c] * pt[c];
+      if (UNK(o, c)) a2 = a2 + apk[c] * pt[c];
     double alpha = a1 / a2;
-    xo_lincomb(g, 1, xk, alpha, pk, xk_new);
-    xo_apply(g, pk, NULL, 0, rk_new);
-    xo_lincomb(g, -alpha, rk_new, 1, rk, rk_new);
-    xo_apply(g, pt, NULL, 0, rt_new);
-    xo_lincomb(g, -alpha, rt_new, 1, rt, rt_new);
-    double b1 = xo_dot(g, rk_new, rt_new);
-    double b2 = xo_dot(g, rk, rt);
+    op_lincomb(o, 1, xk, alpha, pk, xk_new);
+    op_apply(o, 0, pk, NULL, 0, rk_new);
+    op_lincomb(o, -alpha, rk_new, 1, rk, rk_new);
+    op_apply(o, 1, pt, NULL, 0, rt_new);
+    op_lincomb(o, -alpha, rt_new, 1, rt, rt_new);
+    double b1 = op_dot(o, rk_new, rt_new);
+    double b2 = op_dot(o, rk, rt);
     if (b1 == 0 || b2 == 0) { st = XO_EBREAKDOWN; break; }
     double beta = b1 / b2;
-    xo_lincomb(g, 1, rk_new, beta, pk, pk_new);
-    xo_lincomb(g, 1, rt_new, beta, pt, pt_new);
-    xo_copy(g, xk_new, xk);
-    xo_copy(g, rk_new, rk);
-    xo_copy(g, rt_new, rt);
-    xo_copy(g, pk_new, pk);
-    xo_copy(g, pt_new, pt);
-    rr = xo_dot(g, rk, rk);
+    op_lincomb(o, 1, rk_new, beta, pk, pk_new);
+    op_lincomb(o, 1, rt_new, beta, pt, pt_new);
+    op_copy(o, xk_new, xk);
+    op_copy(o, rk_new, rk);
+    op_copy(o, rt_new, rt);
+    op_copy(o, pk_new, pk);
+    op_copy(o, pt_new, pt);
+    rr = op_dot(o, rk, rk);
     eps_cur = rsqrt_ref(rr);
   }
   if (eps_cur != eps_cur) st = XO_ENAN;
-  xo_copy(g, xk, res);
+  op_copy(o, xk, res);
   memcpy(x, res, sizeof(double) * nc);
   if (iters) *iters = it;
   if (resid) *resid = eps_cur;
@@ -424,30 +498,30 @@ int xo_bicg(const xo_grid *g, const double *b, const double *x0, double *x, doub
 }
 
 /* Minimal residual iteration, XSLAE.cpp:350-414: alpha = -(Ar,r)/(Ar,Ar). */
-int xo_mr(const xo_grid *g, const double *b, const double *x0, double *x, double eps, int maxit,
-          int *iters, double *resid) {
-  int64_t nc = (int64_t)g->n * g->l * g->m;
+static int mr_op(const xo_op *o, const double *b, const double *x0, double *x, double eps, int maxit,
+                 int *iters, double *resid) {
+  int64_t nc = o->nc;
   int st = XO_OK, it = 0;
   double *xk = dup(x0, nc), *rk = zeros(nc), *ark = zeros(nc), *res = dup(x0, nc);
-  xo_lincomb(g, 0, NULL, 0, NULL, res);
-  xo_apply(g, xk, b, -1, rk);
-  double rr = xo_dot(g, rk, rk);
-  xo_copy(g, xk, res);
+  op_lincomb(o, 0, NULL, 0, NULL, res);
+  op_apply(o, 0, xk, b, -1, rk);
+  double rr = op_dot(o, rk, rk);
+  op_copy(o, xk, res);
   double eps_cur = rsqrt_ref(rr);
   if (rr != 0) {
     while (eps_cur > eps) {
       if (it >= maxit) { st = XO_EMAXIT; break; }
       it++;
-      xo_apply(g, rk, NULL, 0, ark);
-      double alfa = (-1) * xo_dot(g, ark, rk) / xo_dot(g, ark, ark);
-      xo_lincomb(g, 1, xk, alfa, rk, xk);
-      xo_apply(g, xk, b, -1, rk);
-      rr = xo_dot(g, rk, rk);
+      op_apply(o, 0, rk, NULL, 0, ark);
+      double alfa = (-1) * op_dot(o, ark, rk) / op_dot(o, ark, ark);
+      op_lincomb(o, 1, xk, alfa, rk, xk);
+      op_apply(o, 0, xk, b, -1, rk);
+      rr = op_dot(o, rk, rk);
       eps_cur = rsqrt_ref(rr);
     }
     if (eps_cur != eps_cur) st = XO_ENAN;
   }
-  xo_copy(g, xk, res);
+  op_copy(o, xk, res);
   memcpy(x, res, sizeof(double) * nc);
   if (iters) *iters = it;
   if (resid) *resid = eps_cur;
@@ -456,55 +530,111 @@ int xo_mr(const xo_grid *g, const double *b, const double *x0, double *x, double
 }
 
 /* BiCGStab, XSLAE.cpp:667-767; stops on max|r| <= eps (L-infinity). */
-int xo_bicgstab(const xo_grid *g, const double *b, const double *x0, double *x, double eps, int maxit,
-                int *iters, double *resid) {
-  int64_t nc = (int64_t)g->n * g->l * g->m;
+static int bicgstab_op(const xo_op *o, const double *b, const double *x0, double *x, double eps, int maxit,
+                       int *iters, double *resid) {
+  int64_t nc = o->nc;
   int st = XO_OK, it = 0;
   double *Un = dup(x0, nc), *tmp1 = zeros(nc), *tmp2 = zeros(nc), *Rn = zeros(nc), *sRn = zeros(nc);
   double *Pn = zeros(nc), *Pn1 = zeros(nc), *Vn = zeros(nc), *Sn = zeros(nc), *Tn = zeros(nc);
   double *res = dup(x0, nc);
-  xo_lincomb(g, 0, NULL, 0, NULL, res);
-  xo_apply(g, Un, NULL, 0, Rn);
-  xo_lincomb(g, 1, b, -1, Rn, Rn);
-  xo_copy(g, Rn, sRn);
+  op_lincomb(o, 0, NULL, 0, NULL, res);
+  op_apply(o, 0, Un, NULL, 0, Rn);
+  op_lincomb(o, 1, b, -1, Rn, Rn);
+  op_copy(o, Rn, sRn);
   double pn = 1., an = 1., bn = 1., wn = 1.;
   double norm = 0;
   {
     int64_t c;
-    for (c = 0; c < nc; c++) if (g->space_mask[c] > 0 && fabs(Rn[c]) > norm) norm = fabs(Rn[c]);
+    for (c = 0; c < nc; c++) if (UNK(o, c) && fabs(Rn[c]) > norm) norm = fabs(Rn[c]);
   }
   while (norm > eps) {
     if (it >= maxit) { st = XO_EMAXIT; break; }
     it++;
-    double pn1 = xo_dot(g, sRn, Rn);
+    double pn1 = op_dot(o, sRn, Rn);
     bn = (pn1 / pn) * (an / wn);
-    xo_lincomb(g, wn, Vn, 0, NULL, Pn1);
-    xo_lincomb(g, 1, Pn, -1, Pn1, Pn1);
-    xo_lincomb(g, bn, Pn1, 0, NULL, Pn1);
-    xo_lincomb(g, 1, Rn, 1, Pn1, Pn1);
-    xo_apply(g, Pn1, NULL, 0, Vn);
-    an = pn1 / xo_dot(g, sRn, Vn);
-    xo_lincomb(g, an, Vn, 0, NULL, Sn);
-    xo_lincomb(g, 1, Rn, -1, Sn, Sn);
-    xo_apply(g, Sn, NULL, 0, Tn);
-    wn = xo_dot(g, Tn, Sn) / xo_dot(g, Tn, Tn);
-    xo_lincomb(g, an, Pn1, 0, NULL, tmp1);
-    xo_lincomb(g, 1, tmp1, wn, Sn, tmp2);
-    xo_lincomb(g, 1, Un, 1, tmp2, Un);
-    xo_lincomb(g, wn, Tn, 0, NULL, Rn);
-    xo_lincomb(g, 1, Sn, -1, Rn, Rn);
+    op_lincomb(o, wn, Vn, 0, NULL, Pn1);
+    op_lincomb(o, 1, Pn, -1, Pn1, Pn1);
+    op_lincomb(o, bn, Pn1, 0, NULL, Pn1);
+    op_lincomb(o, 1, Rn, 1, Pn1, Pn1);
+    op_apply(o, 0, Pn1, NULL, 0, Vn);
+    an = pn1 / op_dot(o, sRn, Vn);
+    op_lincomb(o, an, Vn, 0, NULL, Sn);
+    op_lincomb(o, 1, Rn, -1, Sn, Sn);
+    op_apply(o, 0, Sn, NULL, 0, Tn);
+    wn = op_dot(o, Tn, Sn) / op_dot(o, Tn, Tn);
+    op_lincomb(o, an, Pn1, 0, NULL, tmp1);
+    op_lincomb(o, 1, tmp1, wn, Sn, tmp2);
+    op_lincomb(o, 1, Un, 1, tmp2, Un);
+    op_lincomb(o, wn, Tn, 0, NULL, Rn);
+    op_lincomb(o, 1, Sn, -1, Rn, Rn);
     pn = pn1;
-    xo_copy(g, Pn1, Pn);
+    op_copy(o, Pn1, Pn);
     norm = 0;
-    for (int64_t c = 0; c < nc; c++) if (g->space_mask[c] > 0 && fabs(Rn[c]) > norm) norm = fabs(Rn[c]);
+    for (int64_t c = 0; c < nc; c++) if (UNK(o, c) && fabs(Rn[c]) > norm) norm = fabs(Rn[c]);
   }
   if (norm != norm) st = XO_ENAN;
-  xo_copy(g, Un, res);
+  op_copy(o, Un, res);
   memcpy(x, res, sizeof(double) * nc);
   if (iters) *iters = it;
   if (resid) *resid = norm;
   free(Un); free(tmp1); free(tmp2); free(Rn); free(sRn); free(Pn); free(Pn1); free(Vn); free(Sn);
   free(Tn); free(res);
+  return st;
+}
+
+int xo_cg3(const xo_grid *g, const double *b, const double *x0, double *x, double eps, int maxit,
+           int *iters, double *resid) {
+  xo_op o = grid_op(g);
+  return cg3_op(&o, b, x0, x, eps, maxit, iters, resid);
+}
+
+int xo_bicg(const xo_grid *g, const double *b, const double *x0, double *x, double eps, int maxit,
+            int *iters, double *resid) {
+  xo_op o = grid_op(g);
+  return bicg_op(&o, b, x0, x, eps, maxit, iters, resid);
+}
+
+int xo_mr(const xo_grid *g, const double *b, const double *x0, double *x, double eps, int maxit,
+          int *iters, double *resid) {
+  xo_op o = grid_op(g);
+  return mr_op(&o, b, x0, x, eps, maxit, iters, resid);
+}
+
+int xo_bicgstab(const xo_grid *g, const double *b, const double *x0, double *x, double eps, int maxit,
+                int *iters, double *resid) {
+  xo_op o = grid_op(g);
+  return bicgstab_op(&o, b, x0, x, eps, maxit, iters, resid);
+}
+
+/* A reference solver (0 CG3, 1 BiCG, 2 minimal residuals, 3 BiCGStab) on
+ * packed rows of A and A^T (xo_pack's layout) over Rn vectors of length nunk;
+ * BiCG's shadow sequence takes the transposed rows, as the reference's copy
+ * of the packed operator toggled by operator~ does (XSLAE.cpp:205-206). */
+int xo_pack_solve(int method, int64_t nunk, const int32_t *counts, const double *vals, const int32_t *cols,
+                  const int32_t *counts_t, const double *vals_t, const int32_t *cols_t, const double *b,
+                  const double *x0, double *x, double eps, int maxit, int *iters, double *resid) {
+  int64_t *ptr = (int64_t *)malloc(sizeof(int64_t) * 2 * (nunk + 1));
+  const int32_t *cn[2] = {counts, counts_t};
+  xo_op o;
+  memset(&o, 0, sizeof(o));
+  o.nc = nunk;
+  for (int t = 0; t < 2; t++) {
+    int64_t *pt = ptr + t * (nunk + 1);
+    pt[0] = 0;
+    for (int64_t r = 0; r < nunk; r++) pt[r + 1] = pt[r] + cn[t][r];
+    o.ptr[t] = pt;
+  }
+  o.vals[0] = vals; o.vals[1] = vals_t;
+  o.cols[0] = cols; o.cols[1] = cols_t;
+  int st;
+  switch (method) {
+    case 0: st = cg3_op(&o, b, x0, x, eps, maxit, iters, resid); break;
+    case 1: st = bicg_op(&o, b, x0, x, eps, maxit, iters, resid); break;
+    case 2: st = mr_op(&o, b, x0, x, eps, maxit, iters, resid); break;
+    case 3: st = bicgstab_op(&o, b, x0, x, eps, maxit, iters, resid); break;
+    default: st = XO_EINVAL;
+  }
+  free(ptr);
   return st;
 }
 

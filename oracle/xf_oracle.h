@@ -69,6 +69,10 @@ int xo_cg_cg(const xo_grid *g, const double *b, const double *x0, double *x, dou
 int64_t xo_pack(const xo_grid *g, int transpose, int32_t *counts, double *vals, int32_t *cols);
 void xo_pack_apply(int64_t nunk, const int32_t *counts, const double *vals, const int32_t *cols, const double *x,
                    double *y);
+/* method: 0 CG3, 1 BiCG, 2 minimal residuals, 3 BiCGStab */
+int xo_pack_solve(int method, int64_t nunk, const int32_t *counts, const double *vals, const int32_t *cols,
+                  const int32_t *counts_t, const double *vals_t, const int32_t *cols_t, const double *b,
+                  const double *x0, double *x, double eps, int maxit, int *iters, double *resid);
 
 void xo_thomas(int M, double *u, const double *a, const double *b, const double *c, const double *f);
 int  xo_lod(const xo_grid *g, double *f, const double *f_prev, const double *u, const double *v,

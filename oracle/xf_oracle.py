@@ -51,6 +51,7 @@ def lib():
             "xo_cg_cg": (i, [gp, p, p, p, d, i, i, p, p]),
             "xo_pack": (C.c_int64, [gp, i, p, p, p]),
             "xo_pack_apply": (None, [C.c_int64, p, p, p, p, p]),
+            "xo_pack_solve": (i, [i, C.c_int64, p, p, p, p, p, p, p, p, p, d, i, p, p]),
             "xo_thomas": (None, [i, p, p, p, p, p]),
             "xo_lod": (i, [gp, p, p, p, p, p, p, p, d, i]),
             "xo_ib_delta": (d, [d, d]),
@@ -215,6 +216,26 @@ def pack_apply(counts, vals, cols, x):
     lib().xo_pack_apply(counts.size, counts.ctypes.data, vals.ctypes.data, cols.ctypes.data, x.ctypes.data,
                         y.ctypes.data)
     return y
+
+
+PACK_METHODS = {"cg3": 0, "bicg": 1, "mr": 2, "bicgstab": 3}
+PACK_EPS = {"cg3": 1e-5, "bicg": 1e-8, "mr": 1e-7, "bicgstab": 1e-5}  # XSLAE.cpp:64, 166, 358, 669
+
+
+def pack_solve(method, rows, rows_t, b, x0, eps=None, maxit=100000):
+    """A reference solver on packed rows (rows, rows_t = pack(g), pack(g, True))
+    over Rn vectors; returns (x, iterations, final residual norm, status)."""
+    c, v, k = (np.ascontiguousarray(a) for a in rows)
+    ct, vt, kt = (np.ascontiguousarray(a) for a in rows_t)
+    c, ct, k, kt = (a.astype(np.int32, copy=False) for a in (c, ct, k, kt))
+    v, vt, b, x0 = _f64(v), _f64(vt), _f64(b), _f64(x0)
+    x = np.zeros(c.size)
+    it, res = C.c_int(0), C.c_double(0)
+    eps = PACK_EPS[method] if eps is None else eps
+    st = lib().xo_pack_solve(PACK_METHODS[method], c.size, c.ctypes.data, v.ctypes.data, k.ctypes.data,
+                             ct.ctypes.data, vt.ctypes.data, kt.ctypes.data, b.ctypes.data, x0.ctypes.data,
+                             x.ctypes.data, eps, maxit, C.byref(it), C.byref(res))
+    return x, it.value, res.value, st
 
 
 def cg_cg(g, b, x0, rtol=1e-8, maxit=100000, project_mean=False):

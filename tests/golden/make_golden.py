@@ -128,6 +128,24 @@ def case_pack():
     save("pack_sphere10", inp, run("pack", inp))
 
 
+def case_pack_solve():
+    """BiCG, BiCGStab and minimal residuals on the packed operator of the
+    mixed-border mask: BiCG's shadow sequence uses the transposed rows."""
+    rng = np.random.default_rng(8)
+    n, l, m = 12, 10, 9
+    x = coords_from_steps(rng.uniform(0.05, 0.15, n - 1))
+    y = coords_from_steps(rng.uniform(0.05, 0.15, l - 1), 0.3)
+    z = coords_from_steps(rng.uniform(0.05, 0.15, m - 1), -0.2)
+    codes, normals, has = mixed_border_mask(n, l, m, rng)
+    sm = space_mask_rule(codes)
+    for solver, norm in (("pack_bicg", 1.0), ("pack_bicgstab", 1e3), ("pack_mr", 1e-2)):
+        b = rng.standard_normal((n, l, m)) * (sm > 0)
+        b *= norm / np.linalg.norm(b)
+        inp = {"x": x, "y": y, "z": z, "codes": codes, "normals": normals, "has_normal": has, "space_mask": sm,
+               "x0": np.zeros((n, l, m)), "bvec": b}
+        save(solver + "_mixed", inp, run(solver, inp))
+
+
 def case_divop():
     rng = np.random.default_rng(2)
     n, l, m = 11, 9, 10
@@ -247,10 +265,8 @@ def case_ib():
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
-    case_op()
-    case_divop()
-    case_solvers()
-    case_thomas()
-    case_lod()
-    case_ib()
-    case_pack()
+    CASES = {"op": case_op, "divop": case_divop, "solvers": case_solvers, "thomas": case_thomas, "lod": case_lod,
+             "ib": case_ib, "pack": case_pack, "pack_solve": case_pack_solve}
+    # python make_golden.py [case ...]: only the named cases (all by default)
+    for name in sys.argv[1:] or list(CASES):
+        CASES[name]()

@@ -285,6 +285,14 @@ class Pack:
                        "xf_pack_apply")
         return y
 
+    def solve(self, b, x0, x, opts):
+        """A reference solver (XF_CG3, XF_BICG, XF_MR, XF_BICGSTAB) on Rn vectors."""
+        st = SolveStats()
+        rc = lib().xf_pack_solve(self.ctx.h, self.h, _ptr(b), _ptr(x0), _ptr(x), C.byref(opts), C.byref(st))
+        if rc not in (XF_OK,) and st.status == XF_OK:
+            self.ctx.check(rc, "xf_pack_solve")
+        return st
+
     def gather(self, field, vec):
         self.ctx.check(lib().xf_pack_gather(self.ctx.h, self.h, _ptr(field), _ptr(vec)), "xf_pack_gather")
         return vec

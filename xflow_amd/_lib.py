@@ -81,6 +81,7 @@ SIGNATURES = {
     "xf_pack_apply": (I, [P, P, I, P, P]),
     "xf_pack_gather": (I, [P, P, P, P]),
     "xf_pack_scatter": (I, [P, P, P, P]),
+    "xf_pack_solve": (I, [P, P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "xf_transport_velocity": (I, [P, P, I, P, P, P, P, P, P]),
     "xf_profile_enable": (I, [P, I]),
     "xf_profile_reset": (I, [P]),

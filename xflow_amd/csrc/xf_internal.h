@@ -109,6 +109,20 @@ struct xf_grid_s {
   struct MgHier *mg = nullptr;        // multigrid hierarchy (XF_FCG_MGNN), built on first use
 };
 void free_cg_work(struct CgWork *w);
+
+// T1dPackOperator on the device (xf_pack.hip)
+struct xf_pack_s {
+  xf_grid g = nullptr;
+  int64_t nunk = 0;
+  int64_t *cell = nullptr;               // [nunk] array cell of each Rn index
+  int32_t *ptr[2] = {nullptr, nullptr};  // [nunk + 1] row starts of A, A^T
+  int32_t *col[2] = {nullptr, nullptr};
+  double *val[2] = {nullptr, nullptr};
+  int64_t nnz[2] = {0, 0};
+};
+void launch_pack_apply(xf_ctx c, xf_pack_s *p, int transpose, const double *x, const double *b, double sign_b,
+                       double *y, bool gated);
+void launch_pack_apply_dot(xf_ctx c, xf_pack_s *p, const double *x, const double *w, double *out);
 void free_mg(struct MgHier *h);
 void free_grid(xf_grid g);
 

@@ -14,16 +14,6 @@
 
 #include "xf_rows.h"
 
-struct xf_pack_s {
-  xf_grid g = nullptr;
-  int64_t nunk = 0;
-  int64_t *cell = nullptr;              // [nunk] array cell of each Rn index
-  int32_t *ptr[2] = {nullptr, nullptr};  // [nunk + 1] row starts of A, A^T
-  int32_t *col[2] = {nullptr, nullptr};
-  double *val[2] = {nullptr, nullptr};
-  int64_t nnz[2] = {0, 0};
-};
-
 namespace {
 
 struct UnitF {
@@ -68,13 +58,37 @@ __global__ void k_pack_rows(GridView g, const int64_t *__restrict__ cell, int64_
   }
 }
 
+// y = A x (+ b or - b with sign_b); st (may be null): return once a solver is done
 __global__ void k_pack_apply(int64_t nunk, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-                             const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y) {
+                             const double *__restrict__ val, const double *__restrict__ x,
+                             const double *__restrict__ b, double sign_b, double *__restrict__ y,
+                             const DevState *st) {
+  if (st && st->done) return;
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nunk; r += (int64_t)gridDim.x * blockDim.x) {
     double res = 0;
     for (int32_t q = ptr[r]; q < ptr[r + 1]; q++) res = res + val[q] * x[col[q]];
+    if (sign_b > 0) res = res + b[r];
+    else if (sign_b < 0) res = res - b[r];
     y[r] = res;
   }
+}
+
+// (A x, w) into *out
+__global__ void k_pack_apply_dot(int64_t nunk, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+                                 const double *__restrict__ val, const double *__restrict__ x,
+                                 const double *__restrict__ w, double *partials, unsigned *counter, double *out,
+                                 const DevState *st) {
+  __shared__ double lds[16];
+  if (st->done) return;
+  double acc = 0;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nunk; r += (int64_t)gridDim.x * blockDim.x) {
+    double res = 0;
+    for (int32_t q = ptr[r]; q < ptr[r + 1]; q++) res = res + val[q] * x[col[q]];
+    acc += res * w[r];
+  }
+  double bs = block_sum(acc, lds);
+  double tot;
+  if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) *out = tot;
 }
 
 __global__ void k_gather_rn(int64_t nunk, const int64_t *__restrict__ cell, const double *__restrict__ f,
@@ -106,6 +120,22 @@ void free_pack(xf_pack_s *p) {
 }
 
 }  // namespace
+
+void launch_pack_apply(xf_ctx c, xf_pack_s *p, int transpose, const double *x, const double *b, double sign_b,
+                       double *y, bool gated) {
+  const int t = transpose ? 1 : 0;
+  hipLaunchKernelGGL(k_pack_apply, dim3(pack_blocks(p->nunk)), dim3(kBlock), 0, c->stream, p->nunk, p->ptr[t],
+                     p->col[t], p->val[t], x, b, sign_b, y, gated ? c->state : (const DevState *)nullptr);
+  check_kernel("k_pack_apply");
+}
+
+void launch_pack_apply_dot(xf_ctx c, xf_pack_s *p, const double *x, const double *w, double *out) {
+  int nb = pack_blocks(p->nunk);
+  if (nb > 2048) nb = 2048;
+  hipLaunchKernelGGL(k_pack_apply_dot, dim3(nb), dim3(kBlock), 0, c->stream, p->nunk, p->ptr[0], p->col[0],
+                     p->val[0], x, w, c->partials, c->counters, out, c->state);
+  check_kernel("k_pack_apply_dot");
+}
 
 extern "C" int xf_pack_create(xf_ctx c, xf_grid g, xf_pack *out) {
   xf_pack_s *p = nullptr;
@@ -195,10 +225,7 @@ extern "C" int xf_pack_export(xf_ctx c, xf_pack p, int transpose, int32_t *count
 extern "C" int xf_pack_apply(xf_ctx c, xf_pack p, int transpose, const double *x, double *y) {
   return guarded(c, [&] {
     XF_REQUIRE(p && x && y && x != y, XF_EINVAL, "xf_pack_apply: bad argument");
-    const int t = transpose ? 1 : 0;
-    hipLaunchKernelGGL(k_pack_apply, dim3(pack_blocks(p->nunk)), dim3(kBlock), 0, c->stream, p->nunk, p->ptr[t],
-                       p->col[t], p->val[t], x, y);
-    check_kernel("k_pack_apply");
+    launch_pack_apply(c, p, transpose, x, nullptr, 0, y, false);
   });
 }
 

@@ -5,6 +5,7 @@
 // convergence see `done` and return at once, so the iteration count is the
 // reference's.  Dot products are deterministic device reductions (a different
 // summation order than the reference's serial loop).
+#include <algorithm>
 #include <cmath>
 
 #include "xf_internal.h"
@@ -16,7 +17,7 @@ __global__ void k_lincomb_dev(int64_t nc, const uint8_t *__restrict__ space, con
   if (st->done) return;
   const double p1 = p1p ? *p1p : 0.0, p2 = p2p ? *p2p : 0.0;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
-    if (!space[c]) continue;
+    if (space && !space[c]) continue;
     double v;
     if (p1 != 0 && p2 != 0) v = p1 * s1[c] + p2 * s2[c];
     else if (p1 != 0) v = p1 * s1[c];
@@ -30,17 +31,17 @@ __global__ void k_copy_dev(int64_t nc, const uint8_t *__restrict__ space, const 
                            const DevState *st) {
   if (st->done) return;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
-    if (space[c]) res[c] = s[c];
+    if (!space || space[c]) res[c] = s[c];
 }
 
 __global__ void k_dot_dev(int64_t nc, const uint8_t *__restrict__ space, const double *__restrict__ a,
                           const double *__restrict__ b, double *partials, unsigned *counter, double *out,
                           const DevState *st) {
   __shared__ double lds[16];
-  if (st->done) return;
+  if (st && st->done) return;  // null state: not gated
   double acc = 0;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
-    if (space[c]) acc += a[c] * b[c];
+    if (!space || space[c]) acc += a[c] * b[c];
   double bs = block_sum(acc, lds);
   double tot;
   if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) *out = tot;
@@ -53,7 +54,7 @@ __global__ void k_absmax_dev(int64_t nc, const uint8_t *__restrict__ space, cons
   if (st->done) return;
   double best = 0;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
-    if (space[c]) {
+    if (!space || space[c]) {
       double v = fabs(x[c]);
       if (v > best || v != v) best = v;  // NaN propagates
     }
@@ -199,35 +200,57 @@ namespace {
 struct Exec {
   xf_ctx c;
   xf_grid g;
+  xf_pack_s *pk = nullptr;  // set: Rn vectors of the packed operator (no mask)
+  int64_t n;                // vector length
+  const uint8_t *space;     // mask of the vectors (null: every entry)
   int nbs, nbr;
-  Exec(xf_ctx c_, xf_grid g_) : c(c_), g(g_) {
-    int64_t nb = (g->ncell + kBlock - 1) / kBlock;
-    nbs = (int)(nb > 65536 ? 65536 : nb);
-    nbr = (int)(nb > 2048 ? 2048 : nb);
+  Exec(xf_ctx c_, xf_grid g_) : c(c_), g(g_), n(g_->ncell), space(g_->d_space) { blocks(); }
+  Exec(xf_ctx c_, xf_pack_s *p) : c(c_), g(p->g), pk(p), n(p->nunk), space(nullptr) { blocks(); }
+  void blocks() {
+    int64_t nb = (n + kBlock - 1) / kBlock;
+    nbs = (int)(nb > 65536 ? 65536 : (nb < 1 ? 1 : nb));
+    nbr = (int)(nb > 2048 ? 2048 : (nb < 1 ? 1 : nb));
   }
   double *S(int slot) const { return c->state->s + slot; }
   // res = s[p1]*s1 + s[p2]*s2 (slot < 0: coefficient 0)
   void lin(int p1, const double *s1, int p2, const double *s2, double *res) {
-    hipLaunchKernelGGL(k_lincomb_dev, dim3(nbs), dim3(kBlock), 0, c->stream, g->ncell, g->d_space,
+    hipLaunchKernelGGL(k_lincomb_dev, dim3(nbs), dim3(kBlock), 0, c->stream, n, space,
                        p1 >= 0 ? S(p1) : nullptr, s1, p2 >= 0 ? S(p2) : nullptr, s2, res, c->state);
     check_kernel("k_lincomb_dev");
   }
   void copy(const double *s, double *res) {
-    hipLaunchKernelGGL(k_copy_dev, dim3(nbs), dim3(kBlock), 0, c->stream, g->ncell, g->d_space, s, res, c->state);
+    hipLaunchKernelGGL(k_copy_dev, dim3(nbs), dim3(kBlock), 0, c->stream, n, space, s, res, c->state);
     check_kernel("k_copy_dev");
   }
   void dot(const double *a, const double *b, int slot) {
-    hipLaunchKernelGGL(k_dot_dev, dim3(nbr), dim3(kBlock), 0, c->stream, g->ncell, g->d_space, a, b, c->partials,
+    hipLaunchKernelGGL(k_dot_dev, dim3(nbr), dim3(kBlock), 0, c->stream, n, space, a, b, c->partials,
                        c->counters, S(slot), c->state);
     check_kernel("k_dot_dev");
   }
   void absmax(const double *x, int slot) {
-    hipLaunchKernelGGL(k_absmax_dev, dim3(nbr), dim3(kBlock), 0, c->stream, g->ncell, g->d_space, x, c->partials,
+    hipLaunchKernelGGL(k_absmax_dev, dim3(nbr), dim3(kBlock), 0, c->stream, n, space, x, c->partials,
                        c->counters, S(slot), c->state);
     check_kernel("k_absmax_dev");
   }
-  void apply(const double *x, const double *b, double sign, double *y) { launch_apply_gated(c, g, x, b, sign, y); }
-  void apply_dot(const double *x, const double *w, int slot) { launch_apply_dot_gated(c, g, x, w, S(slot), 0); }
+  void apply(const double *x, const double *b, double sign, double *y) {
+    if (pk) launch_pack_apply(c, pk, 0, x, b, sign, y, true);
+    else launch_apply_gated(c, g, x, b, sign, y);
+  }
+  // the transposed operator (operator~): the packed A^T, or A on a grid (the
+  // symmetric masks the matrix-free path is used on)
+  void applyT(const double *x, double *y) {
+    if (pk) launch_pack_apply(c, pk, 1, x, nullptr, 0, y, true);
+    else launch_apply_gated(c, g, x, nullptr, 0, y);
+  }
+  void apply_dot(const double *x, const double *w, int slot) {
+    if (pk) launch_pack_apply_dot(c, pk, x, w, S(slot));
+    else launch_apply_dot_gated(c, g, x, w, S(slot), 0);
+  }
+  // result |= v after the loop (not gated by the convergence flag)
+  void final_copy(const double *v, double *res) {
+    if (pk) XF_HIP(hipMemcpyAsync(res, v, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    else launch_lincomb(c, g, 1.0, v, 0.0, nullptr, res);
+  }
   void one(void (*k)(DevState *)) {
     hipLaunchKernelGGL(k, dim3(1), dim3(1), 0, c->stream, c->state);
     check_kernel("scalar step");
@@ -273,7 +296,7 @@ int poll_every(const xf_solve_opts &o) { return o.check_every > 0 ? o.check_ever
 static void solve_cg3(Exec &E, const double *b, const double *x0, double *x, const xf_solve_opts &o,
                       xf_solve_stats &s) {
   xf_ctx c = E.c;
-  int64_t nc = E.g->ncell;
+  int64_t nc = E.n;
   E.init({{CG3_EPS, o.tol}});
   Buf xk_1(nc, x0, c->stream), rk_1(nc, nullptr, c->stream), xk(nc, x0, c->stream), rk(nc, nullptr, c->stream);
   XF_HIP(hipMemcpyAsync(x, x0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));  // fResult = x.Copy()
@@ -312,7 +335,7 @@ static void solve_cg3(Exec &E, const double *b, const double *x0, double *x, con
 static void solve_bicg(Exec &E, const double *b, const double *x0, double *x, const xf_solve_opts &o,
                        xf_solve_stats &s) {
   xf_ctx c = E.c;
-  int64_t nc = E.g->ncell;
+  int64_t nc = E.n;
   E.init({{BI_EPS, o.tol}});
   Buf xk(nc, x0, c->stream), xk_new(nc, x0, c->stream);
   Buf rk(nc, nullptr, c->stream), rt(nc, nullptr, c->stream), rk_new(nc, nullptr, c->stream),
@@ -338,7 +361,7 @@ static void solve_bicg(Exec &E, const double *b, const double *x0, double *x, co
       E.lin(ONE, xk.p, BI_ALPHA, pk.p, xk_new.p);
       E.apply(pk.p, nullptr, 0, rk_new.p);
       E.lin(BI_MALPHA, rk_new.p, ONE, rk.p, rk_new.p);
-      E.apply(pt.p, nullptr, 0, rt_new.p);  // A_ = A^T = A
+      E.applyT(pt.p, rt_new.p);  // A_ = A.Copy(), ~A_
       E.lin(BI_MALPHA, rt_new.p, ONE, rt.p, rt_new.p);
       E.dot(rk_new.p, rt_new.p, BI_B1);
       E.dot(rk.p, rt.p, BI_B2);
@@ -356,7 +379,7 @@ static void solve_bicg(Exec &E, const double *b, const double *x0, double *x, co
   // result |= xk (after the loop, also after a breakdown)
   XF_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
   XF_HIP(hipStreamSynchronize(c->stream));
-  launch_lincomb(c, E.g, 1.0, xk.p, 0.0, nullptr, x);
+  E.final_copy(xk.p, x);
   s.iters = c->h_state->iter;
   s.status = c->h_state->status;
   s.resid = c->h_state->s[BI_EPSCUR];
@@ -366,7 +389,7 @@ static void solve_bicg(Exec &E, const double *b, const double *x0, double *x, co
 static void solve_mr(Exec &E, const double *b, const double *x0, double *x, const xf_solve_opts &o,
                      xf_solve_stats &s) {
   xf_ctx c = E.c;
-  int64_t nc = E.g->ncell;
+  int64_t nc = E.n;
   E.init({{MR_EPS, o.tol}});
   Buf xk(nc, x0, c->stream), rk(nc, nullptr, c->stream), ark(nc, nullptr, c->stream);
   XF_HIP(hipMemcpyAsync(x, x0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
@@ -387,7 +410,7 @@ static void solve_mr(Exec &E, const double *b, const double *x0, double *x, cons
       E.dot(rk.p, rk.p, MR_RR);
     }
   }
-  launch_lincomb(c, E.g, 1.0, xk.p, 0.0, nullptr, x);
+  E.final_copy(xk.p, x);
   s.iters = c->h_state->iter;
   s.status = c->h_state->status;
   s.resid = c->h_state->s[MR_EPSCUR];
@@ -397,7 +420,7 @@ static void solve_mr(Exec &E, const double *b, const double *x0, double *x, cons
 static void solve_bicgstab(Exec &E, const double *b, const double *x0, double *x, const xf_solve_opts &o,
                            xf_solve_stats &s) {
   xf_ctx c = E.c;
-  int64_t nc = E.g->ncell;
+  int64_t nc = E.n;
   E.init({{BS_EPS, o.tol}, {BS_PN, 1.0}, {BS_AN, 1.0}, {BS_BN, 1.0}, {BS_WN, 1.0}});
   Buf Un(nc, x0, c->stream), tmp1(nc, nullptr, c->stream), tmp2(nc, nullptr, c->stream), Rn(nc, nullptr, c->stream),
       sRn(nc, nullptr, c->stream), Pn(nc, nullptr, c->stream), Pn1(nc, nullptr, c->stream), Vn(nc, nullptr, c->stream),
@@ -436,7 +459,7 @@ static void solve_bicgstab(Exec &E, const double *b, const double *x0, double *x
       E.absmax(Rn.p, BS_NORM);
     }
   }
-  launch_lincomb(c, E.g, 1.0, Un.p, 0.0, nullptr, x);
+  E.final_copy(Un.p, x);
   s.iters = c->h_state->iter;
   s.status = c->h_state->status;
   s.resid = c->h_state->s[BS_NORM];
@@ -501,6 +524,54 @@ extern "C" int xf_solve(xf_ctx c, xf_grid g, const double *b, const double *x0, 
     XF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     s.ms = ms;
     s.resid_true = true_residual(c, g, b, x, x0);
+  });
+  if (st != XF_OK) s.status = st;
+  if (stats) *stats = s;
+  return st != XF_OK ? st : s.status;
+}
+
+// ||b - A x|| / ||b - A x0|| on Rn vectors of the packed operator
+static double pack_true_residual(xf_ctx c, xf_pack_s *p, const double *b, const double *x, const double *x0) {
+  double *tmp = nullptr;
+  XF_HIP(hipMalloc(&tmp, sizeof(double) * p->nunk));
+  double v[2];
+  const double *xs[2] = {x, x0};
+  const int nb = (int)std::min<int64_t>(2048, (p->nunk + kBlock - 1) / kBlock);
+  for (int t = 0; t < 2; t++) {
+    launch_pack_apply(c, p, 0, xs[t], b, -1.0, tmp, false);
+    hipLaunchKernelGGL(k_dot_dev, dim3(nb), dim3(kBlock), 0, c->stream, p->nunk, (const uint8_t *)nullptr, tmp, tmp,
+                       c->partials, c->counters, c->d_scalar, (const DevState *)nullptr);
+    XF_HIP(hipMemcpyAsync(&v[t], c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipStreamSynchronize(c->stream));
+  }
+  hipFree(tmp);
+  return v[1] > 0 ? sqrt(v[0] / v[1]) : sqrt(v[0]);
+}
+
+extern "C" int xf_pack_solve(xf_ctx c, xf_pack p, const double *b, const double *x0, double *x,
+                             const xf_solve_opts *opts, xf_solve_stats *stats) {
+  xf_solve_stats s{};
+  int st = guarded(c, [&] {
+    XF_REQUIRE(p && b && x0 && x && opts, XF_EINVAL, "xf_pack_solve: null argument");
+    XF_REQUIRE(x != x0 && x != b, XF_EINVAL, "xf_pack_solve: x must not alias b or x0");
+    xf_solve_opts o = *opts;
+    XF_REQUIRE(o.maxit >= 0, XF_EINVAL, "xf_pack_solve: negative iteration cap");
+    XF_REQUIRE(o.tol_mode == XF_TOL_ABS, XF_EINVAL, "xf_pack_solve: the reference solvers use absolute tolerances");
+    XF_HIP(hipEventRecord(c->ev0, c->stream));
+    Exec E(c, p);
+    switch (o.method) {
+      case XF_CG3: solve_cg3(E, b, x0, x, o, s); break;
+      case XF_BICG: solve_bicg(E, b, x0, x, o, s); break;
+      case XF_MR: solve_mr(E, b, x0, x, o, s); break;
+      case XF_BICGSTAB: solve_bicgstab(E, b, x0, x, o, s); break;
+      default: XF_REQUIRE(false, XF_EINVAL, "xf_pack_solve: the packed operator runs the reference solvers only");
+    }
+    XF_HIP(hipEventRecord(c->ev1, c->stream));
+    XF_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    XF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    s.ms = ms;
+    s.resid_true = pack_true_residual(c, p, b, x, x0);
   });
   if (st != XF_OK) s.status = st;
   if (stats) *stats = s;
