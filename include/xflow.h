@@ -208,11 +208,18 @@ int xf_pack_apply(xf_ctx ctx, xf_pack p, int transpose, const double *x, double 
 /* field [ncell] <-> Rn vector [n_unknowns] (TRnSpace::operator[] order, XSpace.cpp:911-920) */
 int xf_pack_gather(xf_ctx ctx, xf_pack p, const double *field, double *vec);
 int xf_pack_scatter(xf_ctx ctx, xf_pack p, const double *vec, double *field);
-/* TSLAE on the packed operator (the reference's BiCG runs on it, with operator~
- * giving A^T, XSLAE.cpp:159-348): methods XF_CG3, XF_BICG, XF_MR, XF_BICGSTAB on
- * Rn vectors of length n_unknowns */
-int xf_pack_solve(xf_ctx ctx, xf_pack p, const double *b, const double *x0, double *x, const xf_solve_opts *opts,
-                  xf_solve_stats *stats);
+/* TSLAE on the packed operator (XSLAE.cpp:56-414, 667-767 with A a
+ * T1dPackOperator): methods XF_CG3, XF_BICG, XF_MR, XF_BICGSTAB on Rn vectors
+ * of length n_unknowns.  transpose = 1 solves with A^T (the state operator~
+ * leaves the packed operator in, XOperator.h:333); BiCG's shadow sequence uses
+ * the other matrix (its A.Copy() then ~, XSLAE.cpp:205-206). */
+int xf_pack_solve(xf_ctx ctx, xf_pack p, int transpose, const double *b, const double *x0, double *x,
+                  const xf_solve_opts *opts, xf_solve_stats *stats);
+/* Rn vectors (TRnRelease1dSpace, every entry an unknown; XSpace.cpp:1823-2125):
+ * res = p1*s1 + p2*s2, (a, b), and max |x| with its 1-based index */
+int xf_rn_lincomb(xf_ctx ctx, int64_t n, double p1, const double *s1, double p2, const double *s2, double *res);
+int xf_rn_dot(xf_ctx ctx, int64_t n, const double *a, const double *b, double *out);
+int xf_rn_absmax(xf_ctx ctx, int64_t n, const double *x, double *val, int32_t *rn_index);
 
 /* ------------------------------------------------- learned preconditioner */
 /* No reference counterpart: the learned preconditioner of XF_FCG_NN.  A

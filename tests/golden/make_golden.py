@@ -146,6 +146,23 @@ def case_pack_solve():
         save(solver + "_mixed", inp, run(solver, inp))
 
 
+def case_pack_solve_spd():
+    """BiCGStab and minimal residuals on the packed operator of the SPD
+    Dirichlet problems the grid-solver fixtures use (same inputs): smooth
+    convergence, so iteration counts are stable under rounding (on the
+    non-symmetric mask a 1e-15 relative change of b moves them by tens)."""
+    for name, solver, nc, norm in (("pack_bicgstab_dirichlet12", "pack_bicgstab", 12, 1e8),
+                                   ("pack_mr_dirichlet8", "pack_mr", 8, 10.0)):
+        codes = dirichlet_codes(nc)
+        b = cases.divergence_rhs(codes, seed=7, sigma=1.5)
+        b[codes == 1] += 0.3
+        b *= norm / np.linalg.norm(b)
+        xx = cases.cell_centres(nc)
+        inp = {"x": xx, "y": xx.copy(), "z": xx.copy(), "codes": codes, "space_mask": space_mask_rule(codes),
+               "x0": np.zeros(codes.shape), "bvec": b}
+        save(name, inp, run(solver, inp))
+
+
 def case_divop():
     rng = np.random.default_rng(2)
     n, l, m = 11, 9, 10
@@ -266,7 +283,7 @@ if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
     CASES = {"op": case_op, "divop": case_divop, "solvers": case_solvers, "thomas": case_thomas, "lod": case_lod,
-             "ib": case_ib, "pack": case_pack, "pack_solve": case_pack_solve}
+             "ib": case_ib, "pack": case_pack, "pack_solve": case_pack_solve, "pack_solve_spd": case_pack_solve_spd}
     # python make_golden.py [case ...]: only the named cases (all by default)
     for name in sys.argv[1:] or list(CASES):
         CASES[name]()

@@ -117,12 +117,14 @@ def test_pack_operator_rows_and_products(name):
         assert np.array_equal(y, outs["Ax" if not tr else "ATx"])
 
 
-@pytest.mark.parametrize("solver", ["bicg", "bicgstab", "mr"])
-def test_pack_solvers_bit_exact(solver):
-    """The reference solvers run on the packed operator of the non-symmetric
-    mixed-border mask (BiCG's shadow sequence on the transposed rows):
+@pytest.mark.parametrize("name,solver", [("pack_bicg_mixed", "bicg"), ("pack_bicgstab_mixed", "bicgstab"),
+                                         ("pack_mr_mixed", "mr"), ("pack_bicgstab_dirichlet12", "bicgstab"),
+                                         ("pack_mr_dirichlet8", "mr")])
+def test_pack_solvers_bit_exact(name, solver):
+    """The reference solvers run on the packed operator (BiCG's shadow sequence
+    on the transposed rows of the non-symmetric mixed-border mask):
     solution bit-exact, iteration count from the harness's row evaluations."""
-    ins, outs = golden("pack_%s_mixed" % solver)
+    ins, outs = golden(name)
     g = grid_from(ins)
     rn, _ = xo.rn_number(ins["space_mask"])
     m = rn > 0

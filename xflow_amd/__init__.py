@@ -285,13 +285,32 @@ class Pack:
                        "xf_pack_apply")
         return y
 
-    def solve(self, b, x0, x, opts):
-        """A reference solver (XF_CG3, XF_BICG, XF_MR, XF_BICGSTAB) on Rn vectors."""
+    def solve(self, b, x0, x, opts, transpose=False):
+        """A reference solver (XF_CG3, XF_BICG, XF_MR, XF_BICGSTAB) on Rn vectors
+        (transpose: solve with A^T)."""
         st = SolveStats()
-        rc = lib().xf_pack_solve(self.ctx.h, self.h, _ptr(b), _ptr(x0), _ptr(x), C.byref(opts), C.byref(st))
+        rc = lib().xf_pack_solve(self.ctx.h, self.h, 1 if transpose else 0, _ptr(b), _ptr(x0), _ptr(x),
+                                 C.byref(opts), C.byref(st))
         if rc not in (XF_OK,) and st.status == XF_OK:
             self.ctx.check(rc, "xf_pack_solve")
         return st
+
+    # Rn vector algebra (TRnRelease1dSpace): every entry is an unknown
+    def lincomb(self, p1, s1, p2, s2, res):
+        self.ctx.check(lib().xf_rn_lincomb(self.ctx.h, self.n_unknowns, float(p1), _ptr(s1), float(p2), _ptr(s2),
+                                           _ptr(res)), "xf_rn_lincomb")
+        return res
+
+    def dot(self, a, b):
+        out = C.c_double()
+        self.ctx.check(lib().xf_rn_dot(self.ctx.h, self.n_unknowns, _ptr(a), _ptr(b), C.byref(out)), "xf_rn_dot")
+        return out.value
+
+    def absmax(self, x):
+        v, idx = C.c_double(), C.c_int32()
+        self.ctx.check(lib().xf_rn_absmax(self.ctx.h, self.n_unknowns, _ptr(x), C.byref(v), C.byref(idx)),
+                       "xf_rn_absmax")
+        return v.value, idx.value
 
     def gather(self, field, vec):
         self.ctx.check(lib().xf_pack_gather(self.ctx.h, self.h, _ptr(field), _ptr(vec)), "xf_pack_gather")

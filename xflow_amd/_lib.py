@@ -81,7 +81,10 @@ SIGNATURES = {
     "xf_pack_apply": (I, [P, P, I, P, P]),
     "xf_pack_gather": (I, [P, P, P, P]),
     "xf_pack_scatter": (I, [P, P, P, P]),
-    "xf_pack_solve": (I, [P, P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "xf_pack_solve": (I, [P, P, I, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "xf_rn_lincomb": (I, [P, I64, D, P, D, P, P]),
+    "xf_rn_dot": (I, [P, I64, P, P, C.POINTER(D)]),
+    "xf_rn_absmax": (I, [P, I64, P, C.POINTER(D), C.POINTER(C.c_int32)]),
     "xf_transport_velocity": (I, [P, P, I, P, P, P, P, P, P]),
     "xf_profile_enable": (I, [P, I]),
     "xf_profile_reset": (I, [P]),

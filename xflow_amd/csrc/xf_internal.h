@@ -122,7 +122,7 @@ struct xf_pack_s {
 };
 void launch_pack_apply(xf_ctx c, xf_pack_s *p, int transpose, const double *x, const double *b, double sign_b,
                        double *y, bool gated);
-void launch_pack_apply_dot(xf_ctx c, xf_pack_s *p, const double *x, const double *w, double *out);
+void launch_pack_apply_dot(xf_ctx c, xf_pack_s *p, int transpose, const double *x, const double *w, double *out);
 void free_mg(struct MgHier *h);
 void free_grid(xf_grid g);
 

@@ -129,11 +129,11 @@ void launch_pack_apply(xf_ctx c, xf_pack_s *p, int transpose, const double *x, c
   check_kernel("k_pack_apply");
 }
 
-void launch_pack_apply_dot(xf_ctx c, xf_pack_s *p, const double *x, const double *w, double *out) {
+void launch_pack_apply_dot(xf_ctx c, xf_pack_s *p, int t, const double *x, const double *w, double *out) {
   int nb = pack_blocks(p->nunk);
   if (nb > 2048) nb = 2048;
-  hipLaunchKernelGGL(k_pack_apply_dot, dim3(nb), dim3(kBlock), 0, c->stream, p->nunk, p->ptr[0], p->col[0],
-                     p->val[0], x, w, c->partials, c->counters, out, c->state);
+  hipLaunchKernelGGL(k_pack_apply_dot, dim3(nb), dim3(kBlock), 0, c->stream, p->nunk, p->ptr[t], p->col[t],
+                     p->val[t], x, w, c->partials, c->counters, out, c->state);
   check_kernel("k_pack_apply_dot");
 }
 

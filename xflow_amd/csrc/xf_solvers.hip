@@ -201,11 +201,14 @@ struct Exec {
   xf_ctx c;
   xf_grid g;
   xf_pack_s *pk = nullptr;  // set: Rn vectors of the packed operator (no mask)
+  int pt = 0;               // packed operator taken transposed (its operator~ state)
   int64_t n;                // vector length
   const uint8_t *space;     // mask of the vectors (null: every entry)
   int nbs, nbr;
   Exec(xf_ctx c_, xf_grid g_) : c(c_), g(g_), n(g_->ncell), space(g_->d_space) { blocks(); }
-  Exec(xf_ctx c_, xf_pack_s *p) : c(c_), g(p->g), pk(p), n(p->nunk), space(nullptr) { blocks(); }
+  Exec(xf_ctx c_, xf_pack_s *p, int transpose) : c(c_), g(p->g), pk(p), pt(transpose), n(p->nunk), space(nullptr) {
+    blocks();
+  }
   void blocks() {
     int64_t nb = (n + kBlock - 1) / kBlock;
     nbs = (int)(nb > 65536 ? 65536 : (nb < 1 ? 1 : nb));
@@ -233,17 +236,17 @@ struct Exec {
     check_kernel("k_absmax_dev");
   }
   void apply(const double *x, const double *b, double sign, double *y) {
-    if (pk) launch_pack_apply(c, pk, 0, x, b, sign, y, true);
+    if (pk) launch_pack_apply(c, pk, pt, x, b, sign, y, true);
     else launch_apply_gated(c, g, x, b, sign, y);
   }
   // the transposed operator (operator~): the packed A^T, or A on a grid (the
   // symmetric masks the matrix-free path is used on)
   void applyT(const double *x, double *y) {
-    if (pk) launch_pack_apply(c, pk, 1, x, nullptr, 0, y, true);
+    if (pk) launch_pack_apply(c, pk, 1 - pt, x, nullptr, 0, y, true);
     else launch_apply_gated(c, g, x, nullptr, 0, y);
   }
   void apply_dot(const double *x, const double *w, int slot) {
-    if (pk) launch_pack_apply_dot(c, pk, x, w, S(slot));
+    if (pk) launch_pack_apply_dot(c, pk, pt, x, w, S(slot));
     else launch_apply_dot_gated(c, g, x, w, S(slot), 0);
   }
   // result |= v after the loop (not gated by the convergence flag)
@@ -531,14 +534,15 @@ extern "C" int xf_solve(xf_ctx c, xf_grid g, const double *b, const double *x0, 
 }
 
 // ||b - A x|| / ||b - A x0|| on Rn vectors of the packed operator
-static double pack_true_residual(xf_ctx c, xf_pack_s *p, const double *b, const double *x, const double *x0) {
+static double pack_true_residual(xf_ctx c, xf_pack_s *p, int transpose, const double *b, const double *x,
+                                 const double *x0) {
   double *tmp = nullptr;
   XF_HIP(hipMalloc(&tmp, sizeof(double) * p->nunk));
   double v[2];
   const double *xs[2] = {x, x0};
   const int nb = (int)std::min<int64_t>(2048, (p->nunk + kBlock - 1) / kBlock);
   for (int t = 0; t < 2; t++) {
-    launch_pack_apply(c, p, 0, xs[t], b, -1.0, tmp, false);
+    launch_pack_apply(c, p, transpose, xs[t], b, -1.0, tmp, false);
     hipLaunchKernelGGL(k_dot_dev, dim3(nb), dim3(kBlock), 0, c->stream, p->nunk, (const uint8_t *)nullptr, tmp, tmp,
                        c->partials, c->counters, c->d_scalar, (const DevState *)nullptr);
     XF_HIP(hipMemcpyAsync(&v[t], c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -548,7 +552,7 @@ static double pack_true_residual(xf_ctx c, xf_pack_s *p, const double *b, const 
   return v[1] > 0 ? sqrt(v[0] / v[1]) : sqrt(v[0]);
 }
 
-extern "C" int xf_pack_solve(xf_ctx c, xf_pack p, const double *b, const double *x0, double *x,
+extern "C" int xf_pack_solve(xf_ctx c, xf_pack p, int transpose, const double *b, const double *x0, double *x,
                              const xf_solve_opts *opts, xf_solve_stats *stats) {
   xf_solve_stats s{};
   int st = guarded(c, [&] {
@@ -558,7 +562,8 @@ extern "C" int xf_pack_solve(xf_ctx c, xf_pack p, const double *b, const double 
     XF_REQUIRE(o.maxit >= 0, XF_EINVAL, "xf_pack_solve: negative iteration cap");
     XF_REQUIRE(o.tol_mode == XF_TOL_ABS, XF_EINVAL, "xf_pack_solve: the reference solvers use absolute tolerances");
     XF_HIP(hipEventRecord(c->ev0, c->stream));
-    Exec E(c, p);
+    XF_REQUIRE(transpose == 0 || transpose == 1, XF_EINVAL, "xf_pack_solve: transpose must be 0 or 1");
+    Exec E(c, p, transpose);
     switch (o.method) {
       case XF_CG3: solve_cg3(E, b, x0, x, o, s); break;
       case XF_BICG: solve_bicg(E, b, x0, x, o, s); break;
@@ -571,7 +576,7 @@ extern "C" int xf_pack_solve(xf_ctx c, xf_pack p, const double *b, const double 
     float ms = 0;
     XF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     s.ms = ms;
-    s.resid_true = pack_true_residual(c, p, b, x, x0);
+    s.resid_true = pack_true_residual(c, p, transpose, b, x, x0);
   });
   if (st != XF_OK) s.status = st;
   if (stats) *stats = s;

@@ -46,7 +46,7 @@ __global__ void k_apply_dot_exact(GridView g, const double *__restrict__ x, cons
 __global__ void k_lincomb(int64_t nc, const uint8_t *__restrict__ space, double p1, const double *s1,
                           double p2, const double *s2, double *res) {
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
-    if (!space[c]) continue;
+    if (space && !space[c]) continue;  // null space: every entry (Rn vectors)
     double v;
     if (p1 != 0 && p2 != 0) v = p1 * s1[c] + p2 * s2[c];
     else if (p1 != 0) v = p1 * s1[c];
@@ -66,13 +66,14 @@ __global__ void k_dot(int64_t nc, const uint8_t *__restrict__ space, const doubl
   __shared__ double lds[16];
   double acc = 0;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
-    if (space[c]) acc += a[c] * b[c];
+    if (!space || space[c]) acc += a[c] * b[c];
   double bs = block_sum(acc, lds);
   double tot;
   if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) *out = tot;
 }
 
-// max |x| with the smallest Rn index among the maxima (the reference keeps the
+// max |x| with the smallest Rn index among the maxima (null space and rn: an
+// Rn vector, entry c has index c + 1) (the reference keeps the
 // first strict maximum in i->j->k order, which is the smallest Rn index).
 __global__ void k_absmax(int64_t nc, const uint8_t *__restrict__ space, const int32_t *__restrict__ rn,
                          const double *__restrict__ x, double *pv, int32_t *pi, unsigned *counter, double *out,
@@ -83,9 +84,10 @@ __global__ void k_absmax(int64_t nc, const uint8_t *__restrict__ space, const in
   double best = 0;
   int32_t bi = 0x7fffffff;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
-    if (!space[c]) continue;
+    if (space && !space[c]) continue;
+    const int32_t rc = rn ? rn[c] : (int32_t)(c + 1);
     double v = fabs(x[c]);
-    if (v > best || (v == best && v > 0 && rn[c] < bi)) { best = v; bi = rn[c]; }
+    if (v > best || (v == best && v > 0 && rc < bi)) { best = v; bi = rc; }
   }
   lv[threadIdx.x] = best;
   li[threadIdx.x] = bi;
@@ -223,6 +225,52 @@ extern "C" int xf_absmax(xf_ctx c, xf_grid g, const double *x, double *val, int3
     int32_t *pi = (int32_t *)(c->partials + kMaxPartials);
     hipLaunchKernelGGL(k_absmax, dim3(nb), dim3(kBlock), 0, c->stream, g->ncell, g->d_space, g->d_rn, x, pv, pi,
                        c->counters, c->d_scalar, c->d_iscalar);
+    check_kernel("k_absmax");
+    double v;
+    int32_t idx;
+    XF_HIP(hipMemcpyAsync(&v, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipMemcpyAsync(&idx, c->d_iscalar, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipStreamSynchronize(c->stream));
+    *val = v;
+    if (rn_index) *rn_index = idx;
+  });
+}
+
+// Rn vectors (TRnRelease1dSpace over an all-Actual T1dNumberMask, the vectors
+// of the packed operator): XSpace.cpp 1-D linear combination, dot, abs-max
+extern "C" int xf_rn_lincomb(xf_ctx c, int64_t n, double p1, const double *s1, double p2, const double *s2,
+                             double *res) {
+  return guarded(c, [&] {
+    XF_REQUIRE(n >= 0 && res, XF_EINVAL, "xf_rn_lincomb: null argument");
+    XF_REQUIRE((p1 == 0 || s1) && (p2 == 0 || s2), XF_EINVAL, "Incorrect combination: p<>0 but s=NULL.");
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_lincomb, dim3(stream_blocks(n)), dim3(kBlock), 0, c->stream, n, (const uint8_t *)nullptr,
+                       p1, s1, p2, s2, res);
+    check_kernel("k_lincomb");
+  });
+}
+
+extern "C" int xf_rn_dot(xf_ctx c, int64_t n, const double *a, const double *b, double *out) {
+  return guarded(c, [&] {
+    XF_REQUIRE(n > 0 && a && b && out, XF_EINVAL, "xf_rn_dot: null argument");
+    hipLaunchKernelGGL(k_dot, dim3(reduce_blocks(n)), dim3(kBlock), 0, c->stream, n, (const uint8_t *)nullptr, a, b,
+                       c->partials, c->counters, c->d_scalar);
+    check_kernel("k_dot");
+    XF_HIP(hipMemcpyAsync(out, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+extern "C" int xf_rn_absmax(xf_ctx c, int64_t n, const double *x, double *val, int32_t *rn_index) {
+  return guarded(c, [&] {
+    XF_REQUIRE(n > 0 && x && val, XF_EINVAL, "xf_rn_absmax: null argument");
+    XF_REQUIRE(n < ((int64_t)1 << 31), XF_EINVAL, "xf_rn_absmax: more than 2^31 - 1 entries");
+    int nb = reduce_blocks(n);
+    if (nb > kBlock * 8) nb = kBlock * 8;
+    double *pv = c->partials;
+    int32_t *pi = (int32_t *)(c->partials + kMaxPartials);
+    hipLaunchKernelGGL(k_absmax, dim3(nb), dim3(kBlock), 0, c->stream, n, (const uint8_t *)nullptr,
+                       (const int32_t *)nullptr, x, pv, pi, c->counters, c->d_scalar, c->d_iscalar);
     check_kernel("k_absmax");
     double v;
     int32_t idx;

@@ -148,9 +148,37 @@ const T3dNumberMask &TRnRelease3dSpaceModel::GetMask(int id) const {
   return *mask_;
 }
 
+int64_t TRnRelease3dSpaceModel::cell_of(int rn) const {
+  if (rn_cells_.empty()) {
+    const int64_t nc = ncell();
+    rn_cells_.reserve(end_index_);
+    for (int64_t c = 0; c < nc; c++)
+      if (mask_->data()[c] > 0) rn_cells_.push_back(c);
+  }
+  return rn_cells_[rn - 1];
+}
+
+TRnRelease1dSpaceModel::TRnRelease1dSpaceModel(int dataCount, const TMask *const *masks) {
+  if (dataCount != 1) fail("TRnRelease1dSpaceModel: only one data block is supported.");
+  mask_ = dynamic_cast<const T1dNumberMask *>(masks[0]);
+  if (!mask_) fail("TRnRelease1dSpaceModel constructor error: Could not cast one of the masks to T1dNumberMask * type.");
+  for (int i = 0; i < mask_->GetN(); i++) {
+    const int v = (*mask_)[i];
+    if (v != 0 && v != 1)
+      fail("TRnRelease1dSpaceModel constructor error:One of the mask element value is incorrect. Must be 0 or 1.");
+    if (v != 1) fail("TRnRelease1dSpaceModel: only all-Actual masks are supported (one entry per unknown).");
+  }
+  end_index_ = mask_->GetN();
+}
+
+const T1dNumberMask &TRnRelease1dSpaceModel::GetMask(int id) const {
+  if (id != 0) fail("Error in TRnRelease1dSpaceModel::GetMask: The index value is out of range.");
+  return *mask_;
+}
+
 namespace {
 struct ModelEntry {
-  TRnRelease3dSpaceModel *m;
+  TRnSpaceModel *m;
   bool own;
 };
 std::map<int, ModelEntry> &registry() {
@@ -169,6 +197,8 @@ std::map<const TRnRelease3dSpaceModel *, MaskGrid> &mask_grids() {
 // The device grid used for the vector kernels of a model.  The masked vector
 // kernels only read the space mask, so a grid whose point types are the model
 // mask (1 Actual, 0 Fictive) serves every model.
+const TRnRelease3dSpaceModel *as3d(const TRnSpaceModel &m) { return dynamic_cast<const TRnRelease3dSpaceModel *>(&m); }
+
 xf_grid vec_grid(const TRnRelease3dSpaceModel &m) {
   MaskGrid &mg = mask_grids()[&m];
   if (!mg.g) {
@@ -185,16 +215,15 @@ xf_grid vec_grid(const TRnRelease3dSpaceModel &m) {
 }  // namespace
 
 void TRnSpace::AddModel(int name, TRnSpaceModel *model, bool deleteModel) {
-  auto *m = dynamic_cast<TRnRelease3dSpaceModel *>(model);
-  if (!m) fail("TRnSpace::AddModel: only TRnRelease3dSpaceModel is supported.");
+  if (!model) fail("TRnSpace::AddModel: null model.");
   if (registry().count(name)) fail("TRnSpace::AddModel: model name already registered.");
-  registry()[name] = ModelEntry{m, deleteModel};
+  registry()[name] = ModelEntry{model, deleteModel};
 }
 
 void TRnSpace::RemoveModel(int name) {
   auto it = registry().find(name);
   if (it == registry().end()) return;
-  auto mg = mask_grids().find(it->second.m);
+  auto mg = mask_grids().find(as3d(*it->second.m));
   if (mg != mask_grids().end()) {
     xf_grid_destroy(context(), mg->second.g);
     mask_grids().erase(mg);
@@ -207,7 +236,7 @@ void TRnSpace::RemoveAllModels() {
   while (!registry().empty()) RemoveModel(registry().begin()->first);
 }
 
-TRnRelease3dSpaceModel *TRnSpace::model(int name) {
+TRnSpaceModel *TRnSpace::model(int name) {
   auto it = registry().find(name);
   if (it == registry().end()) fail("TRnSpace: unknown model name.");
   return it->second.m;
@@ -247,18 +276,27 @@ static void same_model(const TRnSpace &a, const TRnSpace &b) {
   if (&a.GetModel() != &b.GetModel()) fail("Error in TRnSpace::operator=: models mismatch.");
 }
 
+// masked vector algebra: the 3-D kernels over the model's space mask, or the
+// Rn kernels over every entry of a 1-D model
+static int lincomb_of(const TRnSpaceModel &m, double p1, const double *s1, double p2, const double *s2,
+                      double *res) {
+  if (const TRnRelease3dSpaceModel *m3 = as3d(m)) return xf_lincomb(context(), vec_grid(*m3), p1, s1, p2, s2, res);
+  return xf_rn_lincomb(context(), m.ncell(), p1, s1, p2, s2, res);
+}
+
 TRnSpace &TRnSpace::operator|=(const TRnSpace &s) {
   same_model(*this, s);
   const double *src = s.device();
   double *dst = device();
-  check(xf_copy(context(), vec_grid(*model_), src, dst), "operator|=");
+  if (const TRnRelease3dSpaceModel *m3 = as3d(*model_)) check(xf_copy(context(), vec_grid(*m3), src, dst), "operator|=");
+  else if (src != dst) check(xf_memcpy(context(), dst, src, sizeof(double) * host_.size(), 2), "operator|=");
   mark_device_modified();
   return *this;
 }
 
 TRnSpace &TRnSpace::operator|=(double v) {
   if (v != 0) fail("TRnSpace::operator|=: only 0 can be assigned as a number.");
-  check(xf_lincomb(context(), vec_grid(*model_), 0, nullptr, 0, nullptr, device()), "operator|=");
+  check(lincomb_of(*model_, 0, nullptr, 0, nullptr, device()), "operator|=");
   mark_device_modified();
   return *this;
 }
@@ -270,7 +308,7 @@ TRnSpace &TRnSpace::operator|=(const TLinComb &t) {
   const double *s2 = t.b.s ? t.b.s->device() : nullptr;
   if (t.a.s) same_model(*this, *t.a.s);
   if (t.b.s) same_model(*this, *t.b.s);
-  check(xf_lincomb(context(), vec_grid(*model_), t.a.p, s1, t.b.p, s2, device()), "operator|=");
+  check(lincomb_of(*model_, t.a.p, s1, t.b.p, s2, device()), "operator|=");
   mark_device_modified();
   return *this;
 }
@@ -281,7 +319,7 @@ TRnSpace &TRnSpace::operator|=(const TOpComb &t) {
   if (t.ax.x == this) fail("TRnSpace::operator|=: the operator argument must not be the result vector.");
   const double *x = t.ax.x->device();
   const double *b = t.b ? t.b->device() : nullptr;
-  check(xf_apply(context(), t.ax.A->device_grid(), x, b, t.sign, device()), "operator|=");
+  t.ax.A->apply(x, b, t.sign, device());
   mark_device_modified();
   return *this;
 }
@@ -294,22 +332,38 @@ double TRnSpace::GetAbsMaxElement() {
 double TRnSpace::GetAbsMaxElement(int &elementIndex) {
   double v;
   int32_t idx;
-  check(xf_absmax(context(), vec_grid(*model_), device(), &v, &idx), "GetAbsMaxElement");
+  if (const TRnRelease3dSpaceModel *m3 = as3d(*model_))
+    check(xf_absmax(context(), vec_grid(*m3), device(), &v, &idx), "GetAbsMaxElement");
+  else
+    check(xf_rn_absmax(context(), model_->ncell(), device(), &v, &idx), "GetAbsMaxElement");
   elementIndex = idx;
   return v;
 }
 
 TRnSpace &TRnSpace::Copy() const {
-  TRnRelease3dSpace *c = new TRnRelease3dSpace(model_name_);
-  check(xf_memcpy(context(), c->dev_, device(), sizeof(double) * host_.size(), 2), "Copy");
-  c->mark_device_modified();
-  return *c;
+  TRnSpace &c = CreateInstance();
+  check(xf_memcpy(context(), c.dev_, device(), sizeof(double) * host_.size(), 2), "Copy");
+  c.mark_device_modified();
+  return c;
 }
 
-TRnSpace &TRnSpace::CreateInstance() const { return *new TRnRelease3dSpace(model_name_); }
+TRnSpace &TRnSpace::CreateInstance() const {
+  if (as3d(*model_)) return *new TRnRelease3dSpace(model_name_);
+  return *new TRnRelease1dSpace(model_name_);
+}
+
+double &TRnSpace::operator[](int rnIndex) const {
+  if (rnIndex < 1 || rnIndex > GetEndIndex()) fail("Error in TRnSpace::operator[]: The index is out of range.");
+  sync_to_host();
+  host_dirty_ = true;  // the caller may write through the reference
+  const TRnRelease3dSpaceModel *m3 = as3d(*model_);
+  return host_[m3 ? m3->cell_of(rnIndex) : rnIndex - 1];
+}
 
 TRnRelease3dSpace::TRnRelease3dSpace(int modelName) : TRnSpace(modelName) {
-  const T3dNumberMask &mk = model_->GetMask(0);
+  const TRnRelease3dSpaceModel *m3 = as3d(*model_);
+  if (!m3) fail("TRnRelease3dSpace: the model is not a TRnRelease3dSpaceModel.");
+  const T3dNumberMask &mk = m3->GetMask(0);
   rows_.resize(mk.GetN());
   for (int i = 0; i < mk.GetN(); i++) {
     rows_[i].resize(mk.GetL());
@@ -321,6 +375,16 @@ double **TRnRelease3dSpace::operator[](int i) {
   sync_to_host();
   host_dirty_ = true;  // the caller may write through the returned rows
   return rows_[i].data();
+}
+
+TRnRelease1dSpace::TRnRelease1dSpace(int modelName) : TRnSpace(modelName) {
+  if (as3d(*model_)) fail("TRnRelease1dSpace: the model is not a TRnRelease1dSpaceModel.");
+}
+
+double &TRnRelease1dSpace::operator[](int i) {
+  sync_to_host();
+  host_dirty_ = true;
+  return host_[i];
 }
 
 TLinTerm operator*(double p, const TRnSpace &s) { return TLinTerm{p, &s}; }
@@ -338,16 +402,15 @@ TOpComb operator+(const TOpApply &a, const TRnSpace &b) { return TOpComb{a, 1.0,
 double operator,(const TRnSpace &a, const TRnSpace &b) {
   same_model(a, b);
   double v;
-  check(xf_dot(context(), vec_grid(a.GetModel()), a.device(), b.device(), &v), "operator,");
+  if (const TRnRelease3dSpaceModel *m3 = as3d(a.GetModel()))
+    check(xf_dot(context(), vec_grid(*m3), a.device(), b.device(), &v), "operator,");
+  else
+    check(xf_rn_dot(context(), a.GetModel().ncell(), a.device(), b.device(), &v), "operator,");
   return v;
 }
 
 double operator,(const TOpApply &a, const TRnSpace &b) {
-  if (a.x == &b) {
-    double v;
-    check(xf_apply_dot(context(), a.A->device_grid(), b.device(), &v), "operator,");
-    return v;
-  }
+  if (a.x == &b) return a.A->apply_dot(b.device());
   // (A x, y) with y != x: materialise A x
   TRnSpace &t = b.CreateInstance();
   t |= a;
@@ -363,7 +426,8 @@ T3dLaplaceOperator::T3dLaplaceOperator(int modelName, int gridsCount, const TNod
   grid_ = dynamic_cast<const T3dNormalGrid *>(grids[0]);
   if (!grid_) fail("Error in T3dLaplaceXOperator constructor: Could not cast grid to T3dNormalGrid* type.");
   // the model mask must be the space mask the grid's point types define
-  const TRnRelease3dSpaceModel *m = TRnSpace::model(modelName);
+  const TRnRelease3dSpaceModel *m = as3d(*TRnSpace::model(modelName));
+  if (!m) fail("T3dLaplaceOperator: the model is not a TRnRelease3dSpaceModel.");
   xf_grid g = grid_->device_grid();
   int64_t nc, nu;
   xf_grid_info(g, &nc, &nu);
@@ -404,14 +468,95 @@ double T3dLaplaceOperator::operator()(int idData, int i, int j, int k) const {
   return row_cache_[((size_t)i * mk.GetL() + j) * mk.GetM() + k];
 }
 
+void T3dLaplaceOperator::operator~() const {
+  fail("Error in T3dLaplaceOperator::operator~: Operation is not implemented");
+}
+
+TRnLinearOperator &T3dLaplaceOperator::Copy() const {
+  const TNodesGrid *g[1] = {grid_};
+  return *new T3dLaplaceOperator(model_name_, 1, g);
+}
+
+void T3dLaplaceOperator::apply(const double *x, const double *b, double sign, double *y) const {
+  check(xf_apply(context(), device_grid(), x, b, sign, y), "operator|=");
+}
+
+double T3dLaplaceOperator::apply_dot(const double *x) const {
+  double v;
+  check(xf_apply_dot(context(), device_grid(), x, &v), "operator,");
+  return v;
+}
+
 void T3dVariableDivergentedOperator::withDensity(TRnRelease3dSpace &density) {
   check(xf_grid_set_density(context(), device_grid(), density.device()), "withDensity");
 }
 
+TRnLinearOperator &T3dVariableDivergentedOperator::Copy() const {
+  const TNodesGrid *g[1] = {grid_};
+  return *new T3dVariableDivergentedOperator(model_name_, 1, g);
+}
+
+// ------------------------------------------------------------- packed operator
+T1dPackOperator::Handle::~Handle() {
+  if (h) xf_pack_destroy(context(), h);
+}
+
+T1dPackOperator::T1dPackOperator(int lModelName, const TRnLinearOperator &lA, double **lVector)
+    : model_name_(lModelName) {
+  const T3dLaplaceOperator *A = dynamic_cast<const T3dLaplaceOperator *>(&lA);
+  if (!A) fail("T1dPackOperator: the operator must be a T3dLaplaceOperator or T3dVariableDivergentedOperator.");
+  const TRnSpaceModel *m = TRnSpace::model(lModelName);
+  if (as3d(*m)) fail("T1dPackOperator: the model must be a TRnRelease1dSpaceModel.");
+  const TRnSpaceModel *ma = TRnSpace::model(A->model_name());
+  if (ma->GetEndIndex() != m->GetEndIndex())
+    fail("Error in T1dPackOperator constructor: Contradiction between size of model and size of constructing matrix.");
+  if (lVector && *lVector) fail("Error in T1dPackOperator constructor: Method is not ready for Gausse transformation.");
+  pack_ = std::make_shared<Handle>();
+  check(xf_pack_create(context(), A->device_grid(), &pack_->h), "T1dPackOperator");
+  int64_t nnz, nnzt;
+  check(xf_pack_info(pack_->h, &pack_->n, &nnz, &nnzt), "T1dPackOperator");
+}
+
+TRnLinearOperator &T1dPackOperator::Copy() const { return *new T1dPackOperator(model_name_, pack_); }
+
+void T1dPackOperator::apply(const double *x, const double *b, double sign, double *y) const {
+  if (x == y) fail("T1dPackOperator: the operator argument must not be the result vector.");
+  check(xf_pack_apply(context(), pack_->h, transpose_ ? 1 : 0, x, y), "operator|=");
+  // A x - b as (A x) + (-1) b: the same bits as the row sum minus b
+  if (sign != 0) check(xf_rn_lincomb(context(), pack_->n, 1.0, y, sign, b, y), "operator|=");
+}
+
+double T1dPackOperator::apply_dot(const double *x) const {
+  double *y = nullptr;
+  check(xf_malloc(context(), sizeof(double) * pack_->n, (void **)&y), "operator,");
+  int st = xf_pack_apply(context(), pack_->h, transpose_ ? 1 : 0, x, y);
+  double v = 0;
+  if (st == XF_OK) st = xf_rn_dot(context(), pack_->n, y, x, &v);
+  xf_free(context(), y);
+  check(st, "operator,");
+  return v;
+}
+
+double T1dPackOperator::operator()(int idData, int i) const {
+  if (idData != 0) fail("Error in T1dPackOperator::operator(): multidata index <> 0.");
+  if (!last_arg_) fail("T1dPackOperator::operator(): no argument has been given to the operator.");
+  if (i < 0 || i >= pack_->n) fail("T1dPackOperator::operator(): row index out of range.");
+  if (row_cache_src_ != last_arg_->device() || row_cache_t_ != transpose_ || row_cache_.empty()) {
+    double *tmp = nullptr;
+    check(xf_malloc(context(), sizeof(double) * pack_->n, (void **)&tmp), "operator()");
+    int st = xf_pack_apply(context(), pack_->h, transpose_ ? 1 : 0, last_arg_->device(), tmp);
+    row_cache_.assign(pack_->n, 0.0);
+    if (st == XF_OK) st = xf_memcpy(context(), row_cache_.data(), tmp, sizeof(double) * pack_->n, 1);
+    xf_free(context(), tmp);
+    check(st, "operator()");
+    row_cache_src_ = last_arg_->device();
+    row_cache_t_ = transpose_;
+  }
+  return row_cache_[i];
+}
+
 // ------------------------------------------------------------------ TSLAE
-TSLAE::TSLAE(const TRnLinearOperator &A, TRnSpace &x, TRnSpace &b) : x_(&x), b_(&b) {
-  A_ = dynamic_cast<const T3dLaplaceOperator *>(&A);
-  if (!A_) fail("TSLAE: the operator must be a T3dLaplaceOperator.");
+TSLAE::TSLAE(const TRnLinearOperator &A, TRnSpace &x, TRnSpace &b) : A_(&A), x_(&x), b_(&b) {
   same_model(x, b);
   result_.reset(&x.Copy());  // fResult = &x.Copy(); *fResult |= 0 (XSLAE.cpp:12-13)
   *result_ |= 0;
@@ -425,7 +570,17 @@ void TSLAE::solve(int method, double rel_tol, bool project_mean, void *precond) 
   if (rel_tol > 0) o.tol = rel_tol;
   o.project_mean = project_mean ? 1 : 0;
   o.precond = precond;
-  int st = xf_solve(context(), A_->device_grid(), b_->device(), x_->device(), result_->device(), &o, &stats_);
+  int st;
+  if (const T3dLaplaceOperator *A3 = dynamic_cast<const T3dLaplaceOperator *>(A_)) {
+    st = xf_solve(context(), A3->device_grid(), b_->device(), x_->device(), result_->device(), &o, &stats_);
+  } else if (const T1dPackOperator *P = dynamic_cast<const T1dPackOperator *>(A_)) {
+    if (method != XF_CG3 && method != XF_BICG && method != XF_MR && method != XF_BICGSTAB)
+      fail("TSLAE: the packed operator runs the reference solvers only.");
+    st = xf_pack_solve(context(), P->device_pack(), P->transposed() ? 1 : 0, b_->device(), x_->device(),
+                       result_->device(), &o, &stats_);
+  } else {
+    fail("TSLAE: unsupported operator type.");
+  }
   result_->mark_device_modified();
   check(st, "TSLAE::Solve");
 }

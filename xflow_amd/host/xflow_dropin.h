@@ -16,6 +16,12 @@
 //   T3dLaplaceOperator(model, gridsCount, grids),
 //   operator()(idData,i,j,k), ConstructActivityTables   XOperator.h:292-319
 //   T3dVariableDivergentedOperator::withDensity         XOperator.h:410-423
+//   T1dNumberMask(N), operator[]                        XMask.h:26-50
+//   TRnRelease1dSpaceModel(dataCount, masks)            XSpaceModel.h:288-300
+//   TRnRelease1dSpace(modelName), operator[] (0-based),
+//   TRnSpace::operator[] (1-based Rn index)             XSpace.h:504-530, XSpace.cpp:911-920
+//   T1dPackOperator(model, A, &vector), operator~, Copy,
+//   operator()(idData, i)                               XOperator.h:322-370
 //   TSLAE(A, x, b), Result / GetResult, UpdateP/UpdateRP,
 //   SolveByConjugateGradients, SolveByBiConjugateGradients,
 //   SolveByMinimalResiduals, SolveByBiConjugateGradientsStab,
@@ -24,7 +30,10 @@
 // reference's destructor deletes them, XSLAE.cpp:21-30, which double-frees at
 // XProblem.cpp:767-768); the space mask of a model used with an operator must
 // be the one XProblem.cpp:4155-4182 derives from the grid's point types;
-// solvers stop with an error instead of looping forever on NaN or stagnation.
+// solvers stop with an error instead of looping forever on NaN or stagnation;
+// a 1-D model's mask must be all Actual (the reference's only use of one,
+// XProblem.cpp:229-235, 712-718); BiCG on a T3dLaplaceOperator takes A^T = A
+// (the reference's operator~ throws there, XOperator.h:302).
 // Errors are thrown as `const char*`, the reference's convention.
 #pragma once
 
@@ -72,6 +81,19 @@ class T3dNumberMask : public TMask {
   int n_, l_, m_;
   std::vector<int> data_;
   std::vector<std::vector<int *>> rows_;
+};
+
+class T1dNumberMask : public TMask {
+ public:
+  explicit T1dNumberMask(int N) : data_((size_t)(N > 0 ? N : 0), 0) {}
+  T1dNumberMask &Copy() const { return *new T1dNumberMask(*this); }
+  int &operator[](int i) { return data_[i]; }
+  int operator[](int i) const { return data_[i]; }
+  int GetN() const { return (int)data_.size(); }
+  const int *data() const { return data_.data(); }
+
+ private:
+  std::vector<int> data_;
 };
 
 // ------------------------------------------------------------- separators
@@ -138,6 +160,7 @@ class TRnSpaceModel {
  public:
   virtual ~TRnSpaceModel() {}
   int GetEndIndex() const { return end_index_; }
+  virtual int64_t ncell() const = 0;  // entries of one vector of the model
 
  protected:
   int end_index_ = 0;
@@ -147,14 +170,28 @@ class TRnRelease3dSpaceModel : public TRnSpaceModel {
  public:
   TRnRelease3dSpaceModel(int dataCount, const TMask *const *masks);
   const T3dNumberMask &GetMask(int id) const;
-  int64_t ncell() const { return (int64_t)mask_->GetN() * mask_->GetL() * mask_->GetM(); }
+  int64_t ncell() const override { return (int64_t)mask_->GetN() * mask_->GetL() * mask_->GetM(); }
+  // array cell of the unknown with 1-based Rn index rn (i->j->k numbering)
+  int64_t cell_of(int rn) const;
 
  private:
   const T3dNumberMask *mask_;
+  mutable std::vector<int64_t> rn_cells_;
+};
+
+// Rn vectors of one data block over an all-Actual T1dNumberMask
+class TRnRelease1dSpaceModel : public TRnSpaceModel {
+ public:
+  TRnRelease1dSpaceModel(int dataCount, const TMask *const *masks);
+  const T1dNumberMask &GetMask(int id) const;
+  int64_t ncell() const override { return mask_->GetN(); }
+
+ private:
+  const T1dNumberMask *mask_;
 };
 
 // ------------------------------------------------------------- vectors
-class T3dLaplaceOperator;
+class TRnLinearOperator;
 class TRnSpace;
 class TRnRelease3dSpace;
 
@@ -166,7 +203,7 @@ struct TLinComb {  // p1*s1 + p2*s2
   TLinTerm a, b;
 };
 struct TOpApply {  // A(x)
-  const T3dLaplaceOperator *A;
+  const TRnLinearOperator *A;
   const TRnSpace *x;
 };
 struct TOpComb {  // A(x) + sign*b
@@ -181,7 +218,7 @@ class TRnSpace {
   static void AddModel(int name, TRnSpaceModel *model, bool deleteModel = false);
   static void RemoveModel(int name);
   static void RemoveAllModels();
-  static TRnRelease3dSpaceModel *model(int name);
+  static TRnSpaceModel *model(int name);
 
   int GetEndIndex() const { return model_->GetEndIndex(); }
   TRnSpace &operator|=(const TRnSpace &s);   // masked copy
@@ -194,23 +231,26 @@ class TRnSpace {
   double GetAbsMaxElement(int &elementIndex);
   TRnSpace &Copy() const;            // full copy, caller owns
   TRnSpace &CreateInstance() const;  // zero vector of the same model, caller owns
+  // host element by 1-based Rn index (XSpace.cpp:911-920)
+  double &operator[](int rnIndex) const;
 
   // device access
   double *device() const;            // uploads pending host writes
-  const TRnRelease3dSpaceModel &GetModel() const { return *model_; }
+  const TRnSpaceModel &GetModel() const { return *model_; }
   int model_name() const { return model_name_; }
   void mark_device_modified() const;
 
  protected:
   explicit TRnSpace(int modelName);
   int model_name_;
-  TRnRelease3dSpaceModel *model_;
+  TRnSpaceModel *model_;
   mutable double *dev_ = nullptr;
   mutable std::vector<double> host_;
   mutable bool host_valid_ = true;   // host copy equals the device copy
   mutable bool host_dirty_ = false;  // host copy was handed out for writing
   void sync_to_host() const;
   friend class TRnRelease3dSpace;
+  friend class TRnRelease1dSpace;
 };
 
 class TRnRelease3dSpace : public TRnSpace {
@@ -220,6 +260,12 @@ class TRnRelease3dSpace : public TRnSpace {
 
  private:
   std::vector<std::vector<double *>> rows_;
+};
+
+class TRnRelease1dSpace : public TRnSpace {
+ public:
+  explicit TRnRelease1dSpace(int modelName);
+  double &operator[](int i);  // 0-based host view; device copy refreshed on next device use
 };
 
 TLinTerm operator*(double p, const TRnSpace &s);
@@ -240,15 +286,26 @@ double operator,(const TOpApply &a, const TRnSpace &b);
 class TRnLinearOperator {
  public:
   virtual ~TRnLinearOperator() {}
+  virtual TOpApply operator()(const TRnSpace &x) const = 0;
+  virtual void operator~() const = 0;                  // XOperator.h:72
+  virtual TRnLinearOperator &Copy() const = 0;         // XOperator.h:170
+  // device products on the unknowns: y = A x + sign * b (b may be null when
+  // sign is 0), and (A x, x)
+  virtual void apply(const double *x, const double *b, double sign, double *y) const = 0;
+  virtual double apply_dot(const double *x) const = 0;
 };
 
 class T3dLaplaceOperator : public TRnLinearOperator {
  public:
   T3dLaplaceOperator(int modelName, int gridsCount, const TNodesGrid *const *grids);
-  TOpApply operator()(const TRnSpace &x) const {
+  TOpApply operator()(const TRnSpace &x) const override {
     last_arg_ = &x;
     return TOpApply{this, &x};
   }
+  void operator~() const override;  // throws, as XOperator.h:302
+  TRnLinearOperator &Copy() const override;
+  void apply(const double *x, const double *b, double sign, double *y) const override;
+  double apply_dot(const double *x) const override;
   // one row of A at (i, j, k) for the vector last passed to operator()(x)
   // (the reference's per-cell virtual call, XOperator.cpp:938-1129)
   double operator()(int idData, int i, int j, int k) const;
@@ -271,6 +328,45 @@ class T3dVariableDivergentedOperator : public T3dLaplaceOperator {
   T3dVariableDivergentedOperator(int modelName, int gridsCount, const TNodesGrid *const *grids)
       : T3dLaplaceOperator(modelName, gridsCount, grids) {}
   void withDensity(TRnRelease3dSpace &density);
+  TRnLinearOperator &Copy() const override;  // shares the device grid and its density
+};
+
+// Packed rows of a 3-D operator and of its transpose over the Rn unknowns
+// (XOperator.cpp:1135-1571), held on the device (xf_pack).  The model is a
+// 1-D model with one entry per unknown; *lVector must be NULL (the reference
+// throws otherwise, XOperator.cpp:1374-1376).  Copy() shares the rows and
+// starts untransposed; operator~ toggles which matrix products use.
+class T1dPackOperator : public TRnLinearOperator {
+ public:
+  T1dPackOperator(int lModelName, const TRnLinearOperator &lA, double **lVector);
+  TOpApply operator()(const TRnSpace &x) const override {
+    last_arg_ = &x;
+    return TOpApply{this, &x};
+  }
+  // entry i (0-based) of the product with the vector last passed to operator()(x)
+  double operator()(int idData, int i) const;
+  void operator~() const override { transpose_ = !transpose_; }
+  TRnLinearOperator &Copy() const override;
+  void apply(const double *x, const double *b, double sign, double *y) const override;
+  double apply_dot(const double *x) const override;
+  xf_pack device_pack() const { return pack_->h; }
+  bool transposed() const { return transpose_; }
+  int model_name() const { return model_name_; }
+
+ private:
+  struct Handle {
+    xf_pack h = nullptr;
+    int64_t n = 0;
+    ~Handle();
+  };
+  T1dPackOperator(int modelName, std::shared_ptr<Handle> pack) : model_name_(modelName), pack_(std::move(pack)) {}
+  int model_name_;
+  std::shared_ptr<Handle> pack_;
+  mutable bool transpose_ = false;
+  mutable const TRnSpace *last_arg_ = nullptr;
+  mutable std::vector<double> row_cache_;
+  mutable const double *row_cache_src_ = nullptr;
+  mutable bool row_cache_t_ = false;
 };
 
 // ------------------------------------------------------------------ TSLAE
@@ -309,7 +405,7 @@ class TSLAE {
 
  private:
   void solve(int method, double rel_tol = 0, bool project_mean = false, void *precond = nullptr);
-  const T3dLaplaceOperator *A_;
+  const TRnLinearOperator *A_;
   TRnSpace *x_, *b_;
   std::unique_ptr<TRnSpace> result_;
   xf_solve_stats stats_{};
