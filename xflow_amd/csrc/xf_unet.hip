@@ -52,9 +52,14 @@ struct ConvArgs {
   const float *bias; // [Cout] or null
   bf16 *out;         // [B][D][H][W][Cout]
   int C0, C1, up0, Cout, D, H, W, relu, TZ, TY, Kpad;
+  int nsplit;        // blocks per spatial tile, each taking NT of the Cout/16 channel tiles
 };
 
-template <int NT>  // Cout / 16
+// One block: a 16 x TY x TZ cell tile and NT 16-channel output tiles.  Small
+// levels (a 16^3 volume is 256 rows of 16 cells) split Cout over nsplit blocks
+// and use short tiles so that the launch still covers the chip; every output
+// sums the same k-steps in the same order whatever the split.
+template <int NT>
 __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   extern __shared__ __attribute__((aligned(16))) bf16 tile[];
   const int Cin = a.C0 + a.C1;
@@ -62,6 +67,8 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   const int TX = 16, SX = TX + 2, SY = a.TY + 2, SZ = a.TZ + 2;
   const int xt = a.W / TX, yt = a.H / a.TY, zt = a.D / a.TZ;
   int bid = blockIdx.x;
+  const int ct0 = (bid % a.nsplit) * NT;  // first output channel tile
+  bid /= a.nsplit;
   const int bx = bid % xt; bid /= xt;
   const int by = bid % yt; bid /= yt;
   const int bz = bid % zt; bid /= zt;
@@ -96,10 +103,16 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   f32x4 acc[MAXR][NT];
   for (int r = 0; r < MAXR; r++)
     for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0, 0, 0, 0};
+  // weight fragments of the next k-step are loaded before this step's MFMAs
+  const bf16 *wrow = a.wt + (int64_t)(ct0 * 16 + l16) * a.Kpad + 8 * lg;
+  bf16x8 afn[NT];
+  for (int t = 0; t < NT; t++) afn[t] = *(const bf16x8 *)(wrow + (int64_t)t * 16 * a.Kpad);
   for (int k0 = 0; k0 < a.Kpad; k0 += 32) {
     const int kk = k0 + 8 * lg;
     bf16x8 af[NT];
-    for (int t = 0; t < NT; t++) af[t] = *(const bf16x8 *)(a.wt + (int64_t)(t * 16 + l16) * a.Kpad + kk);
+    for (int t = 0; t < NT; t++) af[t] = afn[t];
+    if (k0 + 32 < a.Kpad)
+      for (int t = 0; t < NT; t++) afn[t] = *(const bf16x8 *)(wrow + (int64_t)t * 16 * a.Kpad + k0 + 32);
     const bool valid = kk < K;
     const int tap = valid ? kk / Cin : 0, ci = valid ? kk % Cin : 0;
     const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;  // offsets + 1
@@ -120,7 +133,7 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
     const int rz = rho / a.TY, ry = rho % a.TY;
     const int64_t cell = (((int64_t)b * a.D + z0 + rz) * a.H + y0 + ry) * a.W + x0 + l16;
     for (int t = 0; t < NT; t++) {
-      const int co = t * 16 + 4 * lg;
+      const int co = (ct0 + t) * 16 + 4 * lg;
       bf16 o[4];
       for (int e = 0; e < 4; e++) {
         float v = acc[r][t][e] + (a.bias ? a.bias[co + e] : 0.0f);
@@ -650,20 +663,52 @@ static void tile_of(int cin, int &TZ, int &TY) {
   else { TZ = 2; TY = 2; }
 }
 
+static size_t conv_lds(int TZ, int TY, int Cin) { return (size_t)(TZ + 2) * (TY + 2) * 18 * (Cin + 8) * sizeof(bf16); }
+
+// Launch shape: the widest tile (tile_of) and all Cout/16 channel tiles per
+// block while that gives at least kConvBlocks blocks; otherwise shorter tiles
+// (rows of TZ x TY >= 4, one per wave), then fewer channel tiles per block,
+// taking the first shape that reaches kConvBlocks, or the one with most blocks.
+constexpr int kConvBlocks = 1024;
+static void conv_shape(int Cin, int Cout, int B, int D, int H, int W, int &TZ, int &TY, int &NT) {
+  const int ntmax = Cout / 16;
+  int tz0, ty0;
+  tile_of(Cin, tz0, ty0);
+  const int tiles[4][2] = {{tz0, ty0}, {2, 4}, {1, 4}, {0, 0}};
+  int best = -1;
+  for (int nt = ntmax; nt >= 1; nt--) {
+    if (ntmax % nt) continue;
+    for (int q = 0; tiles[q][0]; q++) {
+      const int tz = tiles[q][0], ty = tiles[q][1];
+      if (q > 0 && tz * ty > tz0 * ty0) continue;
+      if (D % tz || H % ty || conv_lds(tz, ty, Cin) > 65536) continue;
+      const int nb = (W / 16) * (H / ty) * (D / tz) * B * (ntmax / nt);
+      if (nb > best) {
+        best = nb;
+        TZ = tz; TY = ty; NT = nt;
+      }
+      if (nb >= kConvBlocks) return;
+    }
+  }
+}
+
 static void conv(xf_ctx c, const bf16 *src0, int C0, int up0, const bf16 *src1, int C1, const bf16 *wt, int kpad,
                  const float *bias, bf16 *out, int Cout, int B, int D, int H, int W, bool relu) {
   ConvArgs a;
   a.src0 = src0; a.src1 = src1; a.C0 = C0; a.C1 = C1; a.up0 = up0;
   a.wt = wt; a.bias = bias; a.out = out; a.Cout = Cout; a.D = D; a.H = H; a.W = W; a.relu = relu ? 1 : 0;
   a.Kpad = kpad;
-  tile_of(C0 + C1, a.TZ, a.TY);
-  XF_REQUIRE(W % 16 == 0 && H % a.TY == 0 && D % a.TZ == 0 && (C0 + C1) % 8 == 0 && Cout % 16 == 0 && C0 % 8 == 0,
-             XF_EINVAL, "unet conv: unsupported shape");
   const int Cin = C0 + C1;
-  const size_t lds = (size_t)(a.TZ + 2) * (a.TY + 2) * 18 * (Cin + 8) * sizeof(bf16);
-  XF_REQUIRE(lds <= 65536, XF_EINVAL, "unet conv: tile does not fit in LDS");
-  const int nb = (W / 16) * (H / a.TY) * (D / a.TZ) * B;
-  switch (Cout / 16) {
+  XF_REQUIRE(W % 16 == 0 && H % 4 == 0 && D % 4 == 0 && Cin % 8 == 0 && Cout % 16 == 0 && C0 % 8 == 0, XF_EINVAL,
+             "unet conv: unsupported shape");
+  int NT = Cout / 16;
+  a.TZ = a.TY = 0;
+  conv_shape(Cin, Cout, B, D, H, W, a.TZ, a.TY, NT);
+  a.nsplit = Cout / 16 / NT;
+  const size_t lds = conv_lds(a.TZ, a.TY, Cin);
+  XF_REQUIRE(lds <= 65536 && a.TZ * a.TY >= 4 && a.TZ * a.TY <= 16, XF_EINVAL, "unet conv: no tile fits in LDS");
+  const int nb = (W / 16) * (H / a.TY) * (D / a.TZ) * B * a.nsplit;
+  switch (NT) {
     case 1: hipLaunchKernelGGL(k_conv_mfma<1>, dim3(nb), dim3(256), lds, c->stream, a); break;
     case 2: hipLaunchKernelGGL(k_conv_mfma<2>, dim3(nb), dim3(256), lds, c->stream, a); break;
     case 3: hipLaunchKernelGGL(k_conv_mfma<3>, dim3(nb), dim3(256), lds, c->stream, a); break;
