@@ -53,18 +53,26 @@ struct ConvArgs {
   bf16 *out;         // [B][D][H][W][Cout]
   int C0, C1, up0, Cout, D, H, W, relu, TZ, TY, Kpad;
   int nsplit;        // blocks per spatial tile, each taking NT of the Cout/16 channel tiles
+  uint32_t m_cpc, m_sy;  // division magics for Cin / 8 and TY + 2
 };
 
-// One block: a 16 x TY x TZ cell tile and NT 16-channel output tiles.  Small
-// levels (a 16^3 volume is 256 rows of 16 cells) split Cout over nsplit blocks
-// and use short tiles so that the launch still covers the chip; every output
-// sums the same k-steps in the same order whatever the split.
-template <int NT>
+// q / d for q * d <= 2^32 as a multiply-high by m = ceil(2^32 / d)
+__device__ inline int fdiv(int q, uint32_t m) { return (int)__umulhi((uint32_t)q, m); }
+static uint32_t fdiv_magic(int d) { return (uint32_t)((((uint64_t)1 << 32) + (uint64_t)d - 1) / (uint64_t)d); }
+
+// One block: a 16 x TY x TZ cell tile (RPW rows of 16 cells per wave, four
+// waves) and NT 16-channel output tiles.  Small levels (a 16^3 volume is 256
+// rows of 16 cells) split Cout over nsplit blocks and use short tiles so that
+// the launch still covers the chip; every output sums the same k-steps in the
+// same order whatever the split.  The k-loop walks (tap, channel) and the
+// tap's offset in the LDS tile incrementally: no division per step.
+template <int NT, int RPW>
 __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   extern __shared__ __attribute__((aligned(16))) bf16 tile[];
   const int Cin = a.C0 + a.C1;
   const int CS = Cin + 8;  // LDS cell stride (bf16), 16-byte multiple
-  const int TX = 16, SX = TX + 2, SY = a.TY + 2, SZ = a.TZ + 2;
+  constexpr int TX = 16, SX = TX + 2;
+  const int SY = a.TY + 2, SZ = a.TZ + 2;
   const int xt = a.W / TX, yt = a.H / a.TY, zt = a.D / a.TZ;
   int bid = blockIdx.x;
   const int ct0 = (bid % a.nsplit) * NT;  // first output channel tile
@@ -79,8 +87,9 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   const int nchunk = SZ * SY * SX * cpc;
   const int D0 = a.D >> a.up0, H0 = a.H >> a.up0, W0 = a.W >> a.up0;
   for (int q = threadIdx.x; q < nchunk; q += blockDim.x) {
-    const int cell = q / cpc, cc = (q % cpc) * 8;
-    const int xx = cell % SX, yy = (cell / SX) % SY, zz = cell / (SX * SY);
+    const int cell = fdiv(q, a.m_cpc), cc = (q - cell * cpc) * 8;
+    const int row = cell / SX, xx = cell - row * SX;
+    const int zz = fdiv(row, a.m_sy), yy = row - zz * SY;
     const int x = x0 - 1 + xx, y = y0 - 1 + yy, z = z0 - 1 + zz;
     bf16x8 v = {};
     if (x >= 0 && x < a.W && y >= 0 && y < a.H && z >= 0 && z < a.D) {
@@ -97,41 +106,68 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, lg = lane >> 4;
-  const int rows = a.TZ * a.TY;
-  const int K = 27 * Cin;
-  constexpr int MAXR = 4;
-  f32x4 acc[MAXR][NT];
-  for (int r = 0; r < MAXR; r++)
+  // LDS cell of (row, x = l16) at tap (0, 0, 0) for this wave's rows
+  int roff[RPW];
+#pragma unroll
+  for (int r = 0; r < RPW; r++) {
+    const int rho = wave + 4 * r, rz = rho / a.TY, ry = rho - rz * a.TY;
+    roff[r] = (rz * SY + ry) * SX + l16;
+  }
+  f32x4 acc[RPW][NT];
+#pragma unroll
+  for (int r = 0; r < RPW; r++)
+#pragma unroll
     for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0, 0, 0, 0};
+  // this lane group's k = 8 lg + 32 step = tap * Cin + ci
+  int ci = 8 * lg, tap = 0, dx = 0, dy = 0, toff = 0;
+  auto next_tap = [&]() {
+    tap++;
+    toff++;
+    if (++dx == 3) {
+      dx = 0;
+      toff += SX - 3;
+      if (++dy == 3) {
+        dy = 0;
+        toff += SX * SY - 3 * SX;
+      }
+    }
+  };
+  while (ci >= Cin) {
+    ci -= Cin;
+    next_tap();
+  }
   // weight fragments of the next k-step are loaded before this step's MFMAs
   const bf16 *wrow = a.wt + (int64_t)(ct0 * 16 + l16) * a.Kpad + 8 * lg;
   bf16x8 afn[NT];
+#pragma unroll
   for (int t = 0; t < NT; t++) afn[t] = *(const bf16x8 *)(wrow + (int64_t)t * 16 * a.Kpad);
   for (int k0 = 0; k0 < a.Kpad; k0 += 32) {
-    const int kk = k0 + 8 * lg;
     bf16x8 af[NT];
+#pragma unroll
     for (int t = 0; t < NT; t++) af[t] = afn[t];
     if (k0 + 32 < a.Kpad)
+#pragma unroll
       for (int t = 0; t < NT; t++) afn[t] = *(const bf16x8 *)(wrow + (int64_t)t * 16 * a.Kpad + k0 + 32);
-    const bool valid = kk < K;
-    const int tap = valid ? kk / Cin : 0, ci = valid ? kk % Cin : 0;
-    const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;  // offsets + 1
-    int r = 0;
-    for (int rho = wave; rho < rows; rho += 4, r++) {
-      const int rz = rho / a.TY, ry = rho % a.TY;
+    const bool valid = tap < 27;
+#pragma unroll
+    for (int r = 0; r < RPW; r++) {
       bf16x8 bf = {};
-      if (valid) {
-        const int cell = ((rz + dz) * SY + (ry + dy)) * SX + (l16 + dx);
-        bf = *(const bf16x8 *)(tile + (int64_t)cell * CS + ci);
-      }
+      if (valid) bf = *(const bf16x8 *)(tile + (int64_t)(roff[r] + toff) * CS + ci);
+#pragma unroll
       for (int t = 0; t < NT; t++) acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t], bf, acc[r][t], 0, 0, 0);
+    }
+    ci += 32;
+    while (ci >= Cin) {
+      ci -= Cin;
+      next_tap();
     }
   }
   // epilogue: lane holds D[co = 16t + 4 lg + e][cell = l16]
-  int r = 0;
-  for (int rho = wave; rho < rows; rho += 4, r++) {
-    const int rz = rho / a.TY, ry = rho % a.TY;
+#pragma unroll
+  for (int r = 0; r < RPW; r++) {
+    const int rho = wave + 4 * r, rz = rho / a.TY, ry = rho - rz * a.TY;
     const int64_t cell = (((int64_t)b * a.D + z0 + rz) * a.H + y0 + ry) * a.W + x0 + l16;
+#pragma unroll
     for (int t = 0; t < NT; t++) {
       const int co = (ct0 + t) * 16 + 4 * lg;
       bf16 o[4];
@@ -184,27 +220,44 @@ __device__ inline float src0_inv(const Src0 &s) {
   return ms > 0 ? (float)(1.0 / sqrt(ms)) : 0.0f;
 }
 
-__global__ void k_conv_first(Src0 s, const float *__restrict__ w, const float *__restrict__ bias, bf16 *out, int B) {
+// c0 (2 -> 16, FP32): a block computes a 16 x 4 x 4 cell tile from its input
+// (r * inv_s, mask) staged in LDS with a one-cell halo; per output the same
+// sum in the same order as a direct evaluation.
+__global__ __launch_bounds__(256) void k_conv_first(Src0 s, const float *__restrict__ w,
+                                                    const float *__restrict__ bias, bf16 *out, int B) {
   // w: [16][27][2] fp32 master weights
-  const int64_t n = (int64_t)B * s.D * s.H * s.W;
+  constexpr int SX = 18, SY = 6, SZ = 6;
+  __shared__ float2 in[SZ * SY * SX];
   const float inv = src0_inv(s);
-  // 32-bit index arithmetic (volumes stay below 2^31 cells; 64-bit division
-  // dominated this kernel)
-  const unsigned W = s.W, H = s.H, D = s.D;
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-    const unsigned qq = (unsigned)q, qw = qq / W, qh = qw / H;
-    const int x = (int)(qq - qw * W), y = (int)(qw - qh * H), z = (int)(qh % D), b = (int)(qh / D);
-    float in[27][2];
-    for (int t = 0; t < 27; t++)
-      src0_read(s, b, z + t / 9 - 1, y + (t / 3) % 3 - 1, x + t % 3 - 1, inv, in[t][0], in[t][1]);
-    bf16 o[16];
-    for (int co = 0; co < 16; co++) {
-      float acc = bias[co];
-      for (int t = 0; t < 27; t++) acc += w[(co * 27 + t) * 2] * in[t][0] + w[(co * 27 + t) * 2 + 1] * in[t][1];
-      o[co] = to_bf16(fmaxf(acc, 0.0f));
-    }
-    for (int co = 0; co < 16; co += 8) *(bf16x8 *)(out + q * 16 + co) = *(bf16x8 *)(o + co);
+  const int xt = s.W / 16, yt = s.H / 4, zt = s.D / 4;
+  int bid = blockIdx.x;
+  const int bx = bid % xt; bid /= xt;
+  const int by = bid % yt; bid /= yt;
+  const int bz = bid % zt;
+  const int b = bid / zt;
+  const int x0 = bx * 16, y0 = by * 4, z0 = bz * 4;
+  for (int q = threadIdx.x; q < SZ * SY * SX; q += blockDim.x) {
+    const int row = q / SX, xx = q - row * SX, zz = row / SY, yy = row - zz * SY;
+    float rv, mv;
+    src0_read(s, b, z0 - 1 + zz, y0 - 1 + yy, x0 - 1 + xx, inv, rv, mv);
+    in[q] = make_float2(rv, mv);
   }
+  __syncthreads();
+  const int tx = threadIdx.x & 15, ty = (threadIdx.x >> 4) & 3, tz = threadIdx.x >> 6;
+  float acc[16];
+#pragma unroll
+  for (int co = 0; co < 16; co++) acc[co] = bias[co];
+#pragma unroll
+  for (int t = 0; t < 27; t++) {
+    const float2 v = in[((tz + t / 9) * SY + (ty + (t / 3) % 3)) * SX + tx + t % 3];
+#pragma unroll
+    for (int co = 0; co < 16; co++) acc[co] += w[(co * 27 + t) * 2] * v.x + w[(co * 27 + t) * 2 + 1] * v.y;
+  }
+  bf16 o[16];
+#pragma unroll
+  for (int co = 0; co < 16; co++) o[co] = to_bf16(fmaxf(acc[co], 0.0f));
+  const int64_t q = (((int64_t)b * s.D + z0 + tz) * s.H + y0 + ty) * s.W + x0 + tx;
+  for (int co = 0; co < 16; co += 8) *(bf16x8 *)(out + q * 16 + co) = *(bf16x8 *)(o + co);
 }
 
 // Output of c9: y = bias + sum w * a8 (fp32).  Crop mode writes y[B][S^3];
@@ -706,16 +759,25 @@ static void conv(xf_ctx c, const bf16 *src0, int C0, int up0, const bf16 *src1, 
   conv_shape(Cin, Cout, B, D, H, W, a.TZ, a.TY, NT);
   a.nsplit = Cout / 16 / NT;
   const size_t lds = conv_lds(a.TZ, a.TY, Cin);
-  XF_REQUIRE(lds <= 65536 && a.TZ * a.TY >= 4 && a.TZ * a.TY <= 16, XF_EINVAL, "unet conv: no tile fits in LDS");
+  XF_REQUIRE(lds <= 65536 && a.TZ * a.TY >= 4 && a.TZ * a.TY <= 16 && (a.TZ * a.TY) % 4 == 0, XF_EINVAL,
+             "unet conv: no tile fits in LDS");
   const int nb = (W / 16) * (H / a.TY) * (D / a.TZ) * B * a.nsplit;
-  switch (NT) {
-    case 1: hipLaunchKernelGGL(k_conv_mfma<1>, dim3(nb), dim3(256), lds, c->stream, a); break;
-    case 2: hipLaunchKernelGGL(k_conv_mfma<2>, dim3(nb), dim3(256), lds, c->stream, a); break;
-    case 3: hipLaunchKernelGGL(k_conv_mfma<3>, dim3(nb), dim3(256), lds, c->stream, a); break;
-    case 4: hipLaunchKernelGGL(k_conv_mfma<4>, dim3(nb), dim3(256), lds, c->stream, a); break;
-    case 6: hipLaunchKernelGGL(k_conv_mfma<6>, dim3(nb), dim3(256), lds, c->stream, a); break;
-    default: XF_REQUIRE(false, XF_EINVAL, "unet conv: unsupported channel count");
+  a.m_cpc = fdiv_magic(Cin / 8);
+  a.m_sy = fdiv_magic(a.TY + 2);
+  const int rpw = a.TZ * a.TY / 4;
+#define XF_CONV_CASE(N, R) \
+  if (NT == N && rpw == R) { \
+    hipLaunchKernelGGL((k_conv_mfma<N, R>), dim3(nb), dim3(256), lds, c->stream, a); \
+    launched = true; \
   }
+  bool launched = false;
+  XF_CONV_CASE(1, 1) XF_CONV_CASE(1, 2) XF_CONV_CASE(1, 4)
+  XF_CONV_CASE(2, 1) XF_CONV_CASE(2, 2) XF_CONV_CASE(2, 4)
+  XF_CONV_CASE(3, 1) XF_CONV_CASE(3, 2) XF_CONV_CASE(3, 4)
+  XF_CONV_CASE(4, 1) XF_CONV_CASE(4, 2) XF_CONV_CASE(4, 4)
+  XF_CONV_CASE(6, 1) XF_CONV_CASE(6, 2) XF_CONV_CASE(6, 4)
+#undef XF_CONV_CASE
+  XF_REQUIRE(launched, XF_EINVAL, "unet conv: unsupported channel count or tile");
   check_kernel("k_conv_mfma");
 }
 
@@ -762,7 +824,7 @@ static void forward_body(xf_ctx c, xf_unet u, const Src0 &s, int B) {
   const int D = w.D, H = w.H, W = w.W;
   const int64_t n1 = (int64_t)B * D * H * W;
   if (!u->packed) pack(c, u);
-  hipLaunchKernelGGL(k_conv_first, dim3(eblocks(n1)), dim3(256), 0, c->stream, s, u->p + u->off_w[0],
+  hipLaunchKernelGGL(k_conv_first, dim3((int)(n1 / 256)), dim3(256), 0, c->stream, s, u->p + u->off_w[0],
                      u->p + u->off_b[0], w.a[0], B);
   check_kernel("k_conv_first");
   const float *P = u->p;
