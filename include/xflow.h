@@ -279,6 +279,35 @@ int xf_mg_apply(xf_ctx ctx, xf_grid g, xf_unet net, const float *r, float *z);
 int xf_transport_velocity(xf_ctx ctx, xf_grid g, int axis, const double *u, const double *v, const double *w,
                           double *tu, double *tv, double *tw);
 
+/* ------------------------------------------------------------ output files */
+/* The node-centred ASCII VTK structured grid of TVPProblem::SolveBySplitting
+ * (XProblem.cpp:6132-6379), same text: nodes 1..n-2 of each axis (i fastest),
+ * velocity = mean of the four faces around a node (0 where mask_grid_u's
+ * point type is Fictive), pressure / density / concentration / viscosity =
+ * sum of the eight cells around the node over the number of non-Fictive ones
+ * among them (masks of mask_grid_p, _d, _c, _c; 0 at Fictive nodes).  Fields
+ * are device arrays of the grids' shape; p, density, concentration and
+ * viscosity may be NULL (written as 0).  x, y, z: host node coordinates. */
+typedef struct {
+  xf_grid mask_grid_u, mask_grid_p, mask_grid_d, mask_grid_c;
+  const double *x, *y, *z;
+  const double *u, *v, *w, *p, *density, *concentration, *viscosity;
+} xf_vtk_flow;
+int xf_vtk_write_flow(xf_ctx ctx, const char *path, const xf_vtk_flow *fields);
+/* The immersed-boundary VTK of TImmersedBoundaryProblem::OutputBoundary
+ * (XProblem.cpp:9543-9635), same text: `circles` rings of `in_circle` nodes,
+ * positions xyz[3*nodes], start positions xyz0 (displacement = xyz0 - xyz),
+ * velocities vel[3*nodes], pressure p[nodes] and wall shear stress
+ * wss[nodes], in host or device memory.  `time` is not written (the
+ * reference puts it in the file name). */
+int xf_vtk_write_boundary(xf_ctx ctx, const char *path, int circles, int in_circle, const double *xyz,
+                          const double *xyz0, const double *vel, const double *p, const double *wss, double time);
+/* cnpy::npy_save (cnpy.h:66-125, 207-236), same bytes: a C-order array of
+ * kind 'f' (word size 4 or 8), 'i' or 'u' (1, 2, 4, 8 bytes) and the given
+ * shape, from host or device memory. */
+int xf_npy_save(xf_ctx ctx, const char *path, const void *data, char kind, int word_size, int ndims,
+                const uint32_t *shape);
+
 /* ------------------------------------------------------------ profiling */
 /* No reference counterpart (the reference only prints clock() deltas,
  * XProblem.cpp:5442-5445).  When enabled, the throughput solvers bracket

@@ -22,6 +22,9 @@
 //   pack      T1dPackOperator built from the Laplace operator: packed rows of
 //             A and of its transpose (counts, values, 0-based columns) and
 //             A x, A^T x on the Rn-ordered x (XOperator.cpp:1135-1571)
+//   npy       cnpy::npy_save of a double array (cnpy.h:66-125, 207-236): file bytes
+//   vtk_boundary
+//             TImmersedBoundaryProblem::OutputBoundary (XProblem.cpp:9543-9635): file bytes
 //   pack_bicg|pack_bicgstab|pack_mr
 //             TSLAE solvers run on that packed operator over Rn vectors
 //             (XSLAE.cpp:159-414, 667-767); sweeps counted per packed row
@@ -48,6 +51,8 @@
 #include "XSeparator.h"
 #include "XProblem.h"
 #include "XImmersedBoundary.h"
+#include "cnpy.h"
+#include <unistd.h>
 
 // ----------------------------------------------------------------- xfb I/O
 struct Arr {
@@ -490,6 +495,7 @@ class PlainBoundary : public TImmersedBoundary {
 class IbDriver : public TImmersedBoundaryProblem {
  public:
   IbDriver() : TImmersedBoundaryProblem(1.0, 1.0, 1.0, 1.0) {}
+  void output_boundary(int step, double dt) { OutputBoundary(step, dt); }
   TMaskGenerator* CreateStatement() override { return nullptr; }
   void set_boundary(TImmersedBoundary* b) { fBoundary = b; }
   void interp(TRnRelease3dSpace& vel, T3dNormalGrid* g, int axis) { Interpolate(vel, g, axis, 0); }
@@ -536,6 +542,74 @@ static void cmd_ib(const Bag& in, Bag& out) {
   out["force"] = dump_vec(s, force);
 }
 
+
+static Arr file_bytes(const std::string& path) {
+  FILE* fp = fopen(path.c_str(), "rb");
+  if (!fp) throw std::runtime_error("cannot read " + path);
+  std::vector<unsigned char> b;
+  int ch;
+  while ((ch = fgetc(fp)) != EOF) b.push_back((unsigned char)ch);
+  fclose(fp);
+  Arr a = mk_i({(int64_t)b.size()});
+  for (size_t q = 0; q < b.size(); q++) a.i[q] = b[q];
+  return a;
+}
+
+// cnpy::npy_save of a double array with the given shape (cnpy.h:66-125)
+static void cmd_npy(const Bag& in, Bag& out) {
+  const Arr& d = need(in, "data");
+  std::vector<unsigned int> shape(d.dims.begin(), d.dims.end());
+  char tmpl[] = "/tmp/xf_npy_XXXXXX";
+  int fd = mkstemp(tmpl);
+  if (fd < 0) throw std::runtime_error("mkstemp");
+  close(fd);
+  cnpy::npy_save(std::string(tmpl), d.d.data(), shape.data(), (unsigned)shape.size(), "w");
+  out["bytes"] = file_bytes(tmpl);
+  unlink(tmpl);
+}
+
+// TImmersedBoundaryProblem::OutputBoundary (XProblem.cpp:9543-9635): nodes in
+// circles of in_circle, each with position, start position, velocity, p, WSS
+static void cmd_vtk_boundary(const Bag& in, Bag& out) {
+  const Arr& X = need(in, "xyz");
+  const Arr& X0 = need(in, "xyz0");
+  const Arr& V = need(in, "vel");
+  const Arr& P = need(in, "p");
+  const Arr& S = need(in, "wss");
+  const int circles = iscalar(in, "circles"), per = iscalar(in, "in_circle");
+  const int nn = circles * per;
+  PlainBoundary* bnd = new PlainBoundary();
+  bnd->nodesCount = nn;
+  bnd->height_nodes = circles;
+  bnd->radius_nodes = per;
+  bnd->nodes = new TNode*[nn];
+  for (int q = 0; q < nn; q++) {
+    TNode* t = new TNode();
+    t->x = X.d[3 * q]; t->y = X.d[3 * q + 1]; t->z = X.d[3 * q + 2];
+    t->x_0 = X0.d[3 * q]; t->y_0 = X0.d[3 * q + 1]; t->z_0 = X0.d[3 * q + 2];
+    t->xVel = V.d[3 * q]; t->yVel = V.d[3 * q + 1]; t->zVel = V.d[3 * q + 2];
+    t->p = P.d[q];
+    t->ShearStress = S.d[q];
+    bnd->nodes[q] = t;
+  }
+  char dir[] = "/tmp/xf_vtkb_XXXXXX";
+  if (!mkdtemp(dir)) throw std::runtime_error("mkdtemp");
+  std::string cat = std::string(dir) + "/";
+  std::vector<char> catbuf(cat.begin(), cat.end());
+  catbuf.push_back('\0');
+  TProblem::GlobalCatalog = catbuf.data();
+  IbDriver& drv = *new IbDriver();
+  drv.set_boundary(bnd);
+  const int step = iscalar(in, "step");
+  const double dt = scalar(in, "dt");
+  drv.output_boundary(step, dt);
+  char name[64];
+  snprintf(name, sizeof(name), "boundT_%6.4f.vtk", step * dt);
+  out["bytes"] = file_bytes(cat + name);
+  unlink((cat + name).c_str());
+  rmdir(dir);
+}
+
 int main(int argc, char** argv) {
   if (argc != 4) {
     fprintf(stderr, "usage: %s <command> <in.xfb> <out.xfb>\n", argv[0]);
@@ -555,6 +629,8 @@ int main(int argc, char** argv) {
     else if (cmd == "ib") cmd_ib(in, out);
     else if (cmd == "pack") cmd_pack(in, out);
     else if (cmd == "pack_bicg" || cmd == "pack_bicgstab" || cmd == "pack_mr") cmd_pack_solve(in, out, cmd);
+    else if (cmd == "npy") cmd_npy(in, out);
+    else if (cmd == "vtk_boundary") cmd_vtk_boundary(in, out);
     else { fprintf(stderr, "unknown command %s\n", cmd.c_str()); return 2; }
     write_xfb(argv[3], out);
   } catch (const char* e) {

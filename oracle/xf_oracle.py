@@ -355,3 +355,103 @@ def transport_velocity(axis, u, v, w):
         o[tuple(sl_c)] = 0.25 * (((fe[sh(0, 0)] + fe[sh(0, -1)]) + fe[sh(1, 0)]) + fe[sh(1, -1)])
         out.append(o)
     return out
+
+
+# ------------------------------------------------------------ output files
+# Restatements of the reference's writers (test checkers, pure Python/numpy).
+
+def npy_bytes(a):
+    """cnpy::npy_save's bytes for a C-order array (cnpy.h:207-236 header, then
+    the raw little-endian data)."""
+    a = np.ascontiguousarray(a)
+    shape = ", ".join(str(d) for d in a.shape) + ("," if a.ndim == 1 else "")
+    d = "{'descr': '<%s%d', 'fortran_order': False, 'shape': (%s), }" % (a.dtype.kind, a.dtype.itemsize, shape)
+    rem = 16 - (10 + len(d)) % 16
+    d = d + " " * rem
+    d = d[:-1] + "\n"
+    return b"\x93NUMPY\x01\x00" + len(d).to_bytes(2, "little") + d.encode() + a.tobytes()
+
+
+def vtk_boundary_text(circles, in_circle, xyz, xyz0, vel, p, wss):
+    """TImmersedBoundaryProblem::OutputBoundary's file (XProblem.cpp:9568-9632)."""
+    nn = circles * in_circle
+    out = ["# vtk DataFile Version 1.0\nBound\nASCII\n\n", "DATASET UNSTRUCTURED_GRID\n", "POINTS %d double\n" % nn]
+    out += ["%f %f %f\n" % tuple(xyz[q]) for q in range(nn)]
+    ncell = in_circle * (circles - 1)
+    out.append("CELLS %d %d\n" % (ncell, 5 * ncell))
+    for cr in range(circles - 1):
+        for i in range(in_circle):
+            i1 = cr * in_circle + i
+            i2, i3, i4 = i1 + 1, i1 + 1 + in_circle, i1 + in_circle
+            if i == in_circle - 1:
+                i2, i3 = cr * in_circle, (cr + 1) * in_circle
+            out.append("%d %d %d %d %d\n" % (4, i1, i2, i3, i4))
+    out.append("CELL_TYPES %d\n" % ncell)
+    out += ["9\n"] * ncell
+    out.append("POINT_DATA %d\n" % nn)
+    out.append("VECTORS Velocity double\n")
+    out += ["%f %f %f\n" % tuple(vel[q]) for q in range(nn)]
+    out.append("VECTORS Displacement double\n")
+    d = np.asarray(xyz0, dtype=np.float64) - np.asarray(xyz, dtype=np.float64)
+    out += ["%f %f %f\n" % tuple(d[q]) for q in range(nn)]
+    out.append("\n\nSCALARS Pressure float\nLOOKUP_TABLE default\n")
+    out += ["%f\n" % v for v in p]
+    out.append("\n\nSCALARS WSS float\nLOOKUP_TABLE default\n")
+    out += ["%f\n" % v for v in wss]
+    return "".join(out)
+
+
+def vtk_node_average8(mask, f):
+    """Nodes 1..n-2 (i fastest): 0 where mask is Fictive, else the sum of the
+    eight cells (i..i+1, j..j+1, k..k+1) in the reference's order over the
+    count of non-Fictive ones (XProblem.cpp:6208-6239)."""
+    mask = np.asarray(mask)
+    f = np.asarray(f, dtype=np.float64)
+    s = (slice(1, -1),) * 3
+    up = lambda di, dj, dk: f[1 + di:f.shape[0] - 1 + di, 1 + dj:f.shape[1] - 1 + dj, 1 + dk:f.shape[2] - 1 + dk]
+    mu = lambda di, dj, dk: mask[1 + di:mask.shape[0] - 1 + di, 1 + dj:mask.shape[1] - 1 + dj,
+                                 1 + dk:mask.shape[2] - 1 + dk] != 0
+    order = [(1, 0, 0), (1, 1, 0), (1, 0, 1), (1, 1, 1), (0, 0, 0), (0, 1, 0), (0, 0, 1), (0, 1, 1)]
+    tot = up(*order[0]).copy()
+    for o in order[1:]:
+        tot = tot + up(*o)
+    avr = sum(mu(*o).astype(np.int64) for o in order)
+    own = mask[s] != 0
+    val = np.where(own, tot / np.where(avr > 0, avr, 1), 0.0)
+    return np.transpose(val, (2, 1, 0)).reshape(-1)  # file order: k, j, i (i fastest)
+
+
+def vtk_flow_text(codes, x, y, z, u, v, w, p=None, density=None, concentration=None, viscosity=None):
+    """TVPProblem::SolveBySplitting's VTK (XProblem.cpp:6142-6379); codes: one
+    array or (velocity, pressure, density, concentration) point types."""
+    if not isinstance(codes, (tuple, list)):
+        codes = (codes,) * 4
+    cu, cp, cd, cc = (np.asarray(c) for c in codes)
+    n, l, m = cu.shape
+    nodes = (n - 2) * (l - 2) * (m - 2)
+    zero = np.zeros(cu.shape)
+    p, density, concentration, viscosity = (zero if a is None else np.asarray(a, dtype=np.float64)
+                                            for a in (p, density, concentration, viscosity))
+    out = ["# vtk DataFile Version 3.0\n", "Flow\nASCII\n\n", "DATASET STRUCTURED_GRID\n",
+           "DIMENSIONS %d %d %d\n" % (n - 2, l - 2, m - 2), "POINTS %d double\n" % nodes]
+    for k in range(1, m - 1):
+        for j in range(1, l - 1):
+            for i in range(1, n - 1):
+                out.append("%f  %f  %f\n" % (x[i], y[j], z[k]))
+    out.append("POINT_DATA %d\n\nVECTORS Velocity double\n" % nodes)
+    u, v, w = (np.asarray(a, dtype=np.float64) for a in (u, v, w))
+    I, J, K = slice(1, n - 1), slice(1, l - 1), slice(1, m - 1)
+    Ip, Jp, Kp = slice(2, n), slice(2, l), slice(2, m)
+    uu = (u[I, J, K] + u[I, Jp, K] + u[I, J, Kp] + u[I, Jp, Kp]) / 4   # XProblem.cpp:6185-6187
+    vv = (v[I, J, K] + v[Ip, J, K] + v[I, J, Kp] + v[Ip, J, Kp]) / 4
+    ww = (w[I, J, K] + w[Ip, J, K] + w[I, Jp, K] + w[Ip, Jp, K]) / 4
+    own = cu[I, J, K] != 0
+    vel = [np.transpose(np.where(own, a, 0.0), (2, 1, 0)).reshape(-1) for a in (uu, vv, ww)]
+    out += ["%f %f %f\n " % (a, b, c) for a, b, c in zip(*vel)]
+    for head, fmt, mask, f in (("\nSCALARS Pressure float\nLOOKUP_TABLE default\n", "%f\n ", cp, p),
+                               ("\nSCALARS Density float\nLOOKUP_TABLE default\n", "%f\n", cd, density),
+                               ("\nSCALARS Concentration float\nLOOKUP_TABLE default\n", "%f\n", cc, concentration),
+                               ("\nSCALARS Viscosity float\nLOOKUP_TABLE default\n", "%f\n", cc, viscosity)):
+        out.append(head)
+        out += [fmt % val for val in vtk_node_average8(mask, f)]
+    return "".join(out)

@@ -163,6 +163,24 @@ def case_pack_solve_spd():
         save(name, inp, run(solver, inp))
 
 
+def case_io():
+    """cnpy::npy_save and TImmersedBoundaryProblem::OutputBoundary: the files'
+    bytes (stored as uint8)."""
+    rng = np.random.default_rng(9)
+    for name, a in (("npy_f8_3d", rng.standard_normal((3, 4, 5))), ("npy_f8_1d", rng.standard_normal(7) * 1e3)):
+        out = run("npy", {"data": a})
+        save(name, {"data": a}, {"bytes": np.asarray(out["bytes"], dtype=np.uint8)})
+    circles, per = 4, 6
+    nn = circles * per
+    xyz = rng.uniform(-1, 1, (nn, 3))
+    inp = {"xyz": xyz, "xyz0": xyz + 1e-3 * rng.standard_normal((nn, 3)), "vel": rng.standard_normal((nn, 3)),
+           "p": rng.standard_normal(nn) * 10, "wss": np.abs(rng.standard_normal(nn)),
+           "circles": np.array([circles], dtype=np.int32), "in_circle": np.array([per], dtype=np.int32),
+           "step": np.array([12], dtype=np.int32), "dt": np.array([0.005])}
+    out = run("vtk_boundary", inp)
+    save("vtk_boundary", inp, {"bytes": np.asarray(out["bytes"], dtype=np.uint8)})
+
+
 def case_divop():
     rng = np.random.default_rng(2)
     n, l, m = 11, 9, 10
@@ -283,7 +301,7 @@ if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
     CASES = {"op": case_op, "divop": case_divop, "solvers": case_solvers, "thomas": case_thomas, "lod": case_lod,
-             "ib": case_ib, "pack": case_pack, "pack_solve": case_pack_solve, "pack_solve_spd": case_pack_solve_spd}
+             "ib": case_ib, "pack": case_pack, "pack_solve": case_pack_solve, "pack_solve_spd": case_pack_solve_spd, "io": case_io}
     # python make_golden.py [case ...]: only the named cases (all by default)
     for name in sys.argv[1:] or list(CASES):
         CASES[name]()

@@ -10,10 +10,11 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (XF_BICG, XF_BICGSTAB, XF_CG, XF_CG3, XF_CG_MP, XF_FCG_MGNN, XF_FCG_NN, XF_MR, XF_OK, XF_TOL_ABS,
-                   XF_TOL_REL, SolveOpts, SolveStats, XfError, lib)
+                   XF_TOL_REL, SolveOpts, SolveStats, VtkFlow, XfError, lib)
 
 __all__ = ["Context", "Grid", "Pack", "XfError", "XF_CG3", "XF_BICG", "XF_MR", "XF_BICGSTAB", "XF_CG", "XF_CG_MP",
-           "XF_FCG_NN", "XF_FCG_MGNN", "XF_TOL_ABS", "XF_TOL_REL", "solve_opts"]
+           "XF_FCG_NN", "XF_FCG_MGNN", "XF_TOL_ABS", "XF_TOL_REL", "solve_opts", "write_vtk_flow",
+           "write_vtk_boundary", "npy_save"]
 
 
 def _ptr(t):
@@ -330,3 +331,53 @@ class Pack:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ output files
+def write_vtk_flow(ctx, path, grids, x, y, z, u, v, w, p=None, density=None, concentration=None, viscosity=None):
+    """The reference's node-centred VTK of a flow state (xf_vtk_write_flow).
+    grids: one Grid or (velocity, pressure, density, concentration) Grids whose
+    point types mask the averages; fields are device tensors of the grid shape."""
+    if isinstance(grids, Grid):
+        grids = (grids,) * 4
+    xs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z)]
+    f = VtkFlow()
+    f.mask_grid_u, f.mask_grid_p, f.mask_grid_d, f.mask_grid_c = (g.h.value for g in grids)
+    f.x, f.y, f.z = (a.ctypes.data for a in xs)
+    for name, t in (("u", u), ("v", v), ("w", w), ("p", p), ("density", density), ("concentration", concentration),
+                    ("viscosity", viscosity)):
+        setattr(f, name, None if t is None else grids[0]._vec(t).value)
+    ctx.check(lib().xf_vtk_write_flow(ctx.h, str(path).encode(), C.byref(f)), "xf_vtk_write_flow")
+
+
+def write_vtk_boundary(ctx, path, circles, in_circle, xyz, xyz0, vel, p, wss, time=0.0):
+    """The reference's immersed-boundary VTK (xf_vtk_write_boundary); arrays are
+    device tensors or host numpy arrays."""
+    keep = []
+
+    def ptr(a):
+        if hasattr(a, "data_ptr"):
+            return C.c_void_p(a.data_ptr())
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return C.c_void_p(a.ctypes.data)
+    ctx.check(lib().xf_vtk_write_boundary(ctx.h, str(path).encode(), int(circles), int(in_circle), ptr(xyz), ptr(xyz0),
+                                          ptr(vel), ptr(p), ptr(wss), float(time)), "xf_vtk_write_boundary")
+
+
+def npy_save(ctx, path, a):
+    """cnpy::npy_save of a device tensor or host numpy array (xf_npy_save)."""
+    if hasattr(a, "data_ptr"):
+        import torch
+        kinds = {torch.float64: ("f", 8), torch.float32: ("f", 4), torch.int32: ("i", 4), torch.int64: ("i", 8),
+                 torch.uint8: ("u", 1)}
+        kind, ws = kinds[a.dtype]
+        assert a.is_contiguous()
+        shape, ptr, keep = tuple(a.shape), C.c_void_p(a.data_ptr()), a
+    else:
+        keep = np.ascontiguousarray(a)
+        kind, ws = keep.dtype.kind, keep.dtype.itemsize
+        shape, ptr = keep.shape, C.c_void_p(keep.ctypes.data)
+    sh = (C.c_uint32 * max(len(shape), 1))(*shape)
+    ctx.check(lib().xf_npy_save(ctx.h, str(path).encode(), ptr, kind.encode(), ws, len(shape), sh), "xf_npy_save")
+    del keep

@@ -28,6 +28,12 @@ class SolveStats(C.Structure):
                 ("resid", C.c_double), ("resid_true", C.c_double), ("ms", C.c_double)]
 
 
+class VtkFlow(C.Structure):
+    """xf_vtk_flow"""
+    _fields_ = [(n, C.c_void_p) for n in ("mask_grid_u", "mask_grid_p", "mask_grid_d", "mask_grid_c", "x", "y", "z",
+                                          "u", "v", "w", "p", "density", "concentration", "viscosity")]
+
+
 # name -> (restype, argtypes); the export test checks this table against include/xflow.h.
 P, I, I64, D, SZ = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_size_t
 SIGNATURES = {
@@ -83,6 +89,9 @@ SIGNATURES = {
     "xf_pack_scatter": (I, [P, P, P, P]),
     "xf_pack_solve": (I, [P, P, I, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "xf_rn_lincomb": (I, [P, I64, D, P, D, P, P]),
+    "xf_vtk_write_flow": (I, [P, C.c_char_p, C.POINTER(VtkFlow)]),
+    "xf_vtk_write_boundary": (I, [P, C.c_char_p, I, I, P, P, P, P, P, D]),
+    "xf_npy_save": (I, [P, C.c_char_p, P, C.c_char, I, I, P]),
     "xf_rn_dot": (I, [P, I64, P, P, C.POINTER(D)]),
     "xf_rn_absmax": (I, [P, I64, P, C.POINTER(D), C.POINTER(C.c_int32)]),
     "xf_transport_velocity": (I, [P, P, I, P, P, P, P, P, P]),
