@@ -916,18 +916,8 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
     double best = c->h_state->s[HS_RR];
     int since_best = 0, it = 0;
     bool use_net = !fallback;
-    while (!c->h_state->done) {
-      if (!use_net) {
-        // plain CG from the current (e, r): restart with p = r
-        hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags,
-                           w->r32, c->state, c->partials, c->counters, (int)FC_RR2);
-        hipLaunchKernelGGL(k_restart_beta, dim3(1), dim3(1), 0, c->stream, c->state);
-        cgcg_iterations<float, float>(c, g, w->e32, w->r32, w->q32, w->p32[0], w->p32[1], g->d_cxf, g->d_cyf,
-                                      g->d_czf, std::max(1, o.maxit - inner_total - it), 16);
-        it += c->h_state->iter;
-        break;
-      }
-      // z = M(r)
+    // one network iteration: z = M(r), p = z + beta p, q = A p, e += alpha p, r -= alpha q
+    auto launch_iter = [&](int idx) {
       if (mgc) {
         mg_apply(c, g, net, w->r32, z);
       } else {
@@ -935,11 +925,11 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
                            w->r32, c->state, c->partials, c->counters, (int)FC_SUMSQ);
         unet_apply_grid(c, net, g, w->r32, z, c->state->s + FC_SUMSQ);
       }
-      if (it > 0) {
+      if (idx > 0) {
         hipLaunchKernelGGL(k_dot_part, dim3(nbS), dim3(256), 0, c->stream, nc, z, w->q32, c->state, w->part);
         hipLaunchKernelGGL(k_fcg_finish<FIN_ZQ>, dim3(1), dim3(256), 0, c->stream, w->part, nbS, c->state, 0);
       }
-      hipLaunchKernelGGL(k_fcg_dir, dim3(nbS), dim3(256), 0, c->stream, nc, z, p, w->r32, c->state, it == 0 ? 1 : 0,
+      hipLaunchKernelGGL(k_fcg_dir, dim3(nbS), dim3(256), 0, c->stream, nc, z, p, w->r32, c->state, idx == 0 ? 1 : 0,
                          w->part);
       hipLaunchKernelGGL(k_fcg_finish<FIN_PR>, dim3(1), dim3(256), 0, c->stream, w->part, nbS, c->state, 0);
       // q = A p with the partials ((p, p), (A p, p)); alpha = (p, r)/(p, q)
@@ -950,8 +940,15 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
       hipLaunchKernelGGL(k_fcg_finish<FIN_RR>, dim3(1), dim3(256), 0, c->stream, w->part, nbS, c->state,
                          o.maxit - inner_total);
       check_kernel("fcg iteration");
+    };
+    // the host reads the state after every iteration (stall test); the
+    // cycle and network kernels are not gated on the device's done flag, so
+    // no iteration is enqueued before the previous one has been checked
+    int issued = 0;
+    while (use_net && !c->h_state->done) {
+      launch_iter(issued++);
       poll();
-      it++;
+      if (c->h_state->done) break;
       const double rrn = c->h_state->s[HS_RR];
       if (rrn < 0.99 * best) {
         best = rrn;
@@ -960,6 +957,16 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
         use_net = false;
         fallback = 1;
       }
+    }
+    it = c->h_state->iter;  // network iterations applied (the device counts them)
+    if (!use_net && !c->h_state->done) {
+      // plain CG from the current (e, r): restart with p = r
+      hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags,
+                         w->r32, c->state, c->partials, c->counters, (int)FC_RR2);
+      hipLaunchKernelGGL(k_restart_beta, dim3(1), dim3(1), 0, c->stream, c->state);
+      cgcg_iterations<float, float>(c, g, w->e32, w->r32, w->q32, w->p32[0], w->p32[1], g->d_cxf, g->d_cyf,
+                                    g->d_czf, std::max(1, o.maxit - inner_total - it), 16);
+      it += c->h_state->iter;
     }
     inner_total += it;
     if (c->h_state->status == XF_ENAN) { status = XF_ENAN; break; }

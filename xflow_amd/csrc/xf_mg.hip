@@ -98,23 +98,21 @@ __global__ __launch_bounds__(256) void k_mg_sweep(Dims d, const uint8_t *__restr
   out[c] = v;
 }
 
-// The same sweeps on rows of at least 256 cells: 4 consecutive k-cells per
-// lane (16-byte loads), k-neighbours by lane shuffles, IC i-planes marched
+// The same sweeps on rows of at least 64 CPL cells: CPL (4 or 2) consecutive
+// k-cells per lane (16- or 8-byte loads), k-neighbours by lane shuffles, IC i-planes marched
 // per block with the planes i-1, i, i+1 in registers; blocks persistent over
 // the (k, j, i) tiles, consecutive tiles on one XCD.  The last k-tile of a row
 // is shifted back to end at m-2 (cells it shares with the tile before get the
 // same value twice).
 struct SweepTiles {
-  int kt, jt, it, total;
+  int kt, jt, it, total, ic;  // ic: i-planes marched per tile
 };
-constexpr int kSweepIC = 16;
 
-template <int MODE>
+template <int MODE, int CPL>
 __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, const uint8_t *__restrict__ flags,
                                                        const float *__restrict__ cx, const float *__restrict__ cy,
                                                        const float *__restrict__ cz, const float *__restrict__ r,
                                                        const float *__restrict__ z, float *__restrict__ out) {
-  constexpr int CPL = 4;
   struct V {
     float v[CPL];
   };
@@ -127,7 +125,7 @@ __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, con
     const int ktile = t % tl.kt, jtile = (t / tl.kt) % tl.jt, itile = t / (tl.kt * tl.jt);
     const int k0 = min(1 + ktile * 64 * CPL, d.m - 1 - 64 * CPL) + lane * CPL;
     const int j = 1 + jtile * 4 + (threadIdx.x >> 6);
-    const int i0 = 1 + itile * kSweepIC, i1 = min(i0 + kSweepIC, d.n - 1);
+    const int i0 = 1 + itile * tl.ic, i1 = min(i0 + tl.ic, d.n - 1);
     if (j > d.l - 2) continue;
     float czm[CPL], czp[CPL];
 #pragma unroll
@@ -210,18 +208,21 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Dims fd, Dims cd, const uin
   rc[((int64_t)I * cd.l + J) * cd.m + K] = cnt ? s / (float)cnt : 0.f;
 }
 
-// fine z += alpha * coarse z(parent) on the fine unknowns (alpha from *scale, or 1)
-__global__ __launch_bounds__(256) void k_mg_prolong(Dims fd, Dims cd, const uint8_t *__restrict__ fflags,
-                                                     const float *__restrict__ zc, float *__restrict__ z,
-                                                     const double *__restrict__ scale) {
-  const int k = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+// fine z += coarse z(parent) on the fine unknowns, 4 consecutive k-cells per thread
+__global__ __launch_bounds__(256) void k_mg_prolong4(Dims fd, Dims cd, const uint8_t *__restrict__ fflags,
+                                                     const float *__restrict__ zc, float *__restrict__ z) {
+  const int k0 = 1 + 4 * (blockIdx.x * 64 + (threadIdx.x & 63));
   const int j = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
   const int i = 1 + blockIdx.z;
-  if (k > fd.m - 2 || j > fd.l - 2) return;
-  const int64_t c = ((int64_t)i * fd.l + j) * fd.m + k;
-  if (!(fflags[c] & FL_UNKNOWN)) return;
-  const float a = scale ? (float)*scale : 1.0f;
-  z[c] += a * zc[((int64_t)((i + 1) >> 1) * cd.l + ((j + 1) >> 1)) * cd.m + ((k + 1) >> 1)];
+  if (k0 > fd.m - 2 || j > fd.l - 2) return;
+  const int64_t c = ((int64_t)i * fd.l + j) * fd.m + k0;
+  const int64_t pc = ((int64_t)((i + 1) >> 1) * cd.l + ((j + 1) >> 1)) * cd.m;
+  const int ke = min(4, fd.m - 1 - k0);
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    if (e >= ke) break;
+    if (fflags[c + e] & FL_UNKNOWN) z[c + e] += zc[pc + ((k0 + e + 1) >> 1)];
+  }
 }
 
 // per-block partials of (a, b) and (b, c) over the unknowns (c may be null: only (a, b))
@@ -282,15 +283,24 @@ static Dims dims_of(const xf_grid g) { return Dims{g->n, g->l, g->m}; }
 template <int MODE>
 static void sweep(xf_ctx c, const xf_grid g, const float *r, const float *z, float *out) {
   const Dims d = dims_of(g);
-  if (g->m - 2 >= 256) {
+  const int rows = g->m - 2;
+  if (rows >= 128) {
+    const int cpl = rows >= 256 ? 4 : 2;
     SweepTiles tl;
-    tl.kt = (g->m - 2 + 255) / 256;
+    tl.kt = (rows + 64 * cpl - 1) / (64 * cpl);
     tl.jt = (g->l - 2 + 3) / 4;
-    tl.it = (g->n - 2 + kSweepIC - 1) / kSweepIC;
+    // 16 planes per tile, fewer on small grids so that >= 1024 tiles keep the chip busy
+    tl.ic = 16;
+    while (tl.ic > 2 && (int64_t)tl.kt * tl.jt * ((g->n - 2 + tl.ic - 1) / tl.ic) < 1024) tl.ic /= 2;
+    tl.it = (g->n - 2 + tl.ic - 1) / tl.ic;
     tl.total = tl.kt * tl.jt * tl.it;
     const int nb = std::min(2048, 8 * ((tl.total + 7) / 8));
-    hipLaunchKernelGGL(k_mg_sweep_vec<MODE>, dim3(nb), dim3(256), 0, c->stream, d, tl, g->d_flags, g->d_cxf,
-                       g->d_cyf, g->d_czf, r, z, out);
+    if (cpl == 4)
+      hipLaunchKernelGGL((k_mg_sweep_vec<MODE, 4>), dim3(nb), dim3(256), 0, c->stream, d, tl, g->d_flags, g->d_cxf,
+                         g->d_cyf, g->d_czf, r, z, out);
+    else
+      hipLaunchKernelGGL((k_mg_sweep_vec<MODE, 2>), dim3(nb), dim3(256), 0, c->stream, d, tl, g->d_flags, g->d_cxf,
+                         g->d_cyf, g->d_czf, r, z, out);
   } else {
     hipLaunchKernelGGL(k_mg_sweep<MODE>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
                        g->d_czf, r, z, out);
@@ -413,22 +423,29 @@ static void smooth_from_zero(xf_ctx c, MgLevel &L, const float *r, int nu) {
   }
 }
 
-// nu sweeps of damped Jacobi continuing from L.z
-static void smooth(xf_ctx c, MgLevel &L, const float *r, int nu) {
+// nu sweeps of damped Jacobi continuing from L.z; the last into `last` when
+// given (else L.z holds the result)
+static void smooth(xf_ctx c, MgLevel &L, const float *r, int nu, float *last = nullptr) {
   const xf_grid g = L.g;
   for (int s = 0; s < nu; s++) {
+    if (last && s == nu - 1) {
+      sweep<1>(c, g, r, L.z, last);
+      return;
+    }
     sweep<1>(c, g, r, L.z, L.z2);
     std::swap(L.z, L.z2);
   }
+  if (last) XF_HIP(hipMemcpyAsync(last, L.z, sizeof(float) * g->ncell, hipMemcpyDeviceToDevice, c->stream));
 }
 
-// z of the coarsest level from its r
-static void coarse_solve(xf_ctx c, MgHier *h, struct xf_unet_s *net) {
+// z of the coarsest level from its r (into `last` when given)
+static void coarse_solve(xf_ctx c, MgHier *h, struct xf_unet_s *net, float *last) {
   MgLevel &L = h->lev[h->nlev - 1];
   const xf_grid g = L.g;
   const float *rin = L.r;
   if (!net) {
     smooth_from_zero(c, L, rin, kCoarseSweeps);
+    if (last) XF_HIP(hipMemcpyAsync(last, L.z, sizeof(float) * g->ncell, hipMemcpyDeviceToDevice, c->stream));
     return;
   }
   XF_REQUIRE((g->n - 2) % 16 == 0 && (g->l - 2) % 16 == 0 && (g->m - 2) % 64 == 0, XF_EINVAL,
@@ -446,40 +463,42 @@ static void coarse_solve(xf_ctx c, MgHier *h, struct xf_unet_s *net) {
   hipLaunchKernelGGL(k_mg_finish<1>, dim3(1), dim3(256), 0, c->stream, h->part, nb, h->scal);
   hipLaunchKernelGGL(k_mg_scale, dim3(red_blocks(g->ncell)), dim3(256), 0, c->stream, g->ncell, L.z, h->scal);
   // a few sweeps smooth what the network leaves at the grid scale
-  smooth(c, L, rin, kSweeps);
+  smooth(c, L, rin, kSweeps, last);
 }
 
-static void vcycle(xf_ctx c, MgHier *h, int k, const float *r, struct xf_unet_s *net) {
+// one cycle from level k; the level-0 result goes to `out` (levels below
+// keep theirs in lev[k].z)
+static void vcycle(xf_ctx c, MgHier *h, int k, const float *r, struct xf_unet_s *net, float *out) {
+  float *last = k == 0 ? out : nullptr;
   if (k == h->nlev - 1) {
     if (k == 0) {
       // a single level: the coarse solve reads lev[0].r, which is the caller's r
       MgLevel &L = h->lev[0];
       float *keep = L.r;
       L.r = const_cast<float *>(r);
-      coarse_solve(c, h, net);
+      coarse_solve(c, h, net, last);
       L.r = keep;
     } else {
-      coarse_solve(c, h, net);
+      coarse_solve(c, h, net, last);
     }
     return;
   }
   MgLevel &L = h->lev[k], &C = h->lev[k + 1];
   const xf_grid g = L.g;
-  const Dims d = dims_of(g);
+  const Dims d = dims_of(g), cd = dims_of(C.g);
   smooth_from_zero(c, L, r, kSweeps);
   sweep<2>(c, g, r, L.z, L.t);
-  hipLaunchKernelGGL(k_mg_restrict, cell_grid(C.g), dim3(256), 0, c->stream, d, dims_of(C.g), g->d_flags, L.t, C.r);
-  vcycle(c, h, k + 1, C.r, net);
-  hipLaunchKernelGGL(k_mg_prolong, cell_grid(g), dim3(256), 0, c->stream, d, dims_of(C.g), g->d_flags, C.z, L.z,
-                     (const double *)nullptr);
-  smooth(c, L, r, kSweeps);
+  hipLaunchKernelGGL(k_mg_restrict, cell_grid(C.g), dim3(256), 0, c->stream, d, cd, g->d_flags, L.t, C.r);
+  vcycle(c, h, k + 1, C.r, net, nullptr);
+  const dim3 pg((g->m - 2 + 255) / 256, (g->l - 2 + 3) / 4, g->n - 2);
+  hipLaunchKernelGGL(k_mg_prolong4, pg, dim3(256), 0, c->stream, d, cd, g->d_flags, C.z, L.z);
+  smooth(c, L, r, kSweeps, last);
 }
 
 // z = M(r): one V-cycle on the grid's hierarchy (r, z FP32 fields of the fine grid)
 void mg_apply(xf_ctx c, xf_grid g, struct xf_unet_s *net, const float *r, float *z) {
   MgHier *h = hierarchy(c, g);
-  vcycle(c, h, 0, r, net);
-  XF_HIP(hipMemcpyAsync(z, h->lev[0].z, sizeof(float) * g->ncell, hipMemcpyDeviceToDevice, c->stream));
+  vcycle(c, h, 0, r, net, z);
   check_kernel("multigrid cycle");
 }
 
