@@ -108,13 +108,45 @@ struct SweepTiles {
   int kt, jt, it, total, ic;  // ic: i-planes marched per tile
 };
 
+// MODE 4: MODE 1 on z + zcoarse(parent) at the unknowns (the prolongation of
+// the coarse correction folded into the first post-smoothing sweep; the
+// corrected z is formed with the same addition as k_mg_prolong4 and not stored).
 template <int MODE, int CPL>
 __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, const uint8_t *__restrict__ flags,
                                                        const float *__restrict__ cx, const float *__restrict__ cy,
                                                        const float *__restrict__ cz, const float *__restrict__ r,
-                                                       const float *__restrict__ z, float *__restrict__ out) {
+                                                       const float *__restrict__ z, float *__restrict__ out,
+                                                       const float *__restrict__ zcoarse, Dims cd) {
   struct V {
     float v[CPL];
+  };
+  // z (+ the parent's coarse correction at unknowns for MODE 4) of CPL cells from k
+  auto zrow = [&](int64_t q, int ii, int jj, int kk) {
+    V v;
+    __builtin_memcpy(&v, z + q, sizeof(V));
+    if (MODE == 4) {
+      // parents of cells kk..kk+CPL-1 are the CPL/2 + 1 coarse cells from
+      // (kk + 1) >> 1 (in bounds for an even m, which the dispatcher checks)
+      uint8_t fq[CPL];
+      __builtin_memcpy(fq, flags + q, CPL);
+      const int p0 = (kk + 1) >> 1;
+      const float *pr = zcoarse + ((int64_t)((ii + 1) >> 1) * cd.l + ((jj + 1) >> 1)) * cd.m + p0;
+      float pa[CPL / 2 + 1];
+#pragma unroll
+      for (int e = 0; e <= CPL / 2; e++) pa[e] = pr[e];
+#pragma unroll
+      for (int e = 0; e < CPL; e++) {
+        const float zc2 = pa[((kk + e + 1) >> 1) - p0];
+        v.v[e] = (fq[e] & FL_UNKNOWN) ? v.v[e] + zc2 : v.v[e];
+      }
+    }
+    return v;
+  };
+  auto zone = [&](int64_t q, int ii, int jj, int kk) {
+    float v = z[q];
+    if (MODE == 4 && (flags[q] & FL_UNKNOWN))
+      v += zcoarse[((int64_t)((ii + 1) >> 1) * cd.l + ((jj + 1) >> 1)) * cd.m + ((kk + 1) >> 1)];
+    return v;
   };
   const int per = (tl.total + 7) / 8, bx = gridDim.x >> 3, xcd = blockIdx.x & 7;
   const int lane = threadIdx.x & 63;
@@ -137,8 +169,8 @@ __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, con
     int64_t c = ((int64_t)i0 * d.l + j) * d.m + k0;
     V zm{}, zc{};
     if (MODE != 0) {
-      __builtin_memcpy(&zm, z + c - sI, sizeof(V));
-      __builtin_memcpy(&zc, z + c, sizeof(V));
+      zm = zrow(c - sI, i0 - 1, j, k0);
+      zc = zrow(c, i0, j, k0);
     }
     for (int i = i0; i < i1; i++, c += sI) {
       uint8_t fl[CPL];
@@ -147,13 +179,13 @@ __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, con
       if (MODE != 3) __builtin_memcpy(&rr, r + c, sizeof(V));
       float left = 0, right = 0;
       if (MODE != 0) {
-        __builtin_memcpy(&zp, z + c + sI, sizeof(V));
-        __builtin_memcpy(&zjm, z + c - sJ, sizeof(V));
-        __builtin_memcpy(&zjp, z + c + sJ, sizeof(V));
+        zp = zrow(c + sI, i + 1, j, k0);
+        zjm = zrow(c - sJ, i, j - 1, k0);
+        zjp = zrow(c + sJ, i, j + 1, k0);
         left = __shfl_up(zc.v[CPL - 1], 1, 64);
         right = __shfl_down(zc.v[0], 1, 64);
-        if (lane == 0) left = z[c - 1];
-        if (lane == 63) right = z[c + CPL];
+        if (lane == 0) left = zone(c - 1, i, j, k0 - 1);
+        if (lane == 63) right = zone(c + CPL, i, j, k0 + CPL);
       }
       const float cxm = cx[2 * i], cxp = cx[2 * i + 1];
 #pragma unroll
@@ -172,7 +204,7 @@ __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, con
           float ty = ((f & FL_JP) ? cyp * (zjp.v[e] - x0) : 0.f) - ((f & FL_JM) ? cym * (x0 - zjm.v[e]) : 0.f);
           float tz = ((f & FL_KP) ? czp[e] * (kp - x0) : 0.f) - ((f & FL_KM) ? czm[e] * (x0 - km) : 0.f);
           const float az = -(tx + ty + tz);
-          if (MODE == 1) v = x0 + wd * (rr.v[e] - az);
+          if (MODE == 1 || MODE == 4) v = x0 + wd * (rr.v[e] - az);
           else if (MODE == 2) v = rr.v[e] - az;
           else v = az;
         }
@@ -281,9 +313,18 @@ static Dims dims_of(const xf_grid g) { return Dims{g->n, g->l, g->m}; }
 
 // out = sweep MODE on grid g (the vector kernel where rows allow it)
 template <int MODE>
-static void sweep(xf_ctx c, const xf_grid g, const float *r, const float *z, float *out) {
+static void sweep(xf_ctx c, const xf_grid g, const float *r, const float *z, float *out,
+                  const float *zcoarse = nullptr, Dims cd = Dims{0, 0, 0}) {
   const Dims d = dims_of(g);
   const int rows = g->m - 2;
+  if (MODE == 4 && (rows < 128 || g->m % 2)) {
+    // short rows: the prolongation as its own pass, then MODE 1
+    const dim3 pg((g->m - 2 + 255) / 256, (g->l - 2 + 3) / 4, g->n - 2);
+    hipLaunchKernelGGL(k_mg_prolong4, pg, dim3(256), 0, c->stream, d, cd, g->d_flags, zcoarse, const_cast<float *>(z));
+    hipLaunchKernelGGL(k_mg_sweep<1>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
+                       g->d_czf, r, z, out);
+    return;
+  }
   if (rows >= 128) {
     const int cpl = rows >= 256 ? 4 : 2;
     SweepTiles tl;
@@ -297,13 +338,13 @@ static void sweep(xf_ctx c, const xf_grid g, const float *r, const float *z, flo
     const int nb = std::min(2048, 8 * ((tl.total + 7) / 8));
     if (cpl == 4)
       hipLaunchKernelGGL((k_mg_sweep_vec<MODE, 4>), dim3(nb), dim3(256), 0, c->stream, d, tl, g->d_flags, g->d_cxf,
-                         g->d_cyf, g->d_czf, r, z, out);
+                         g->d_cyf, g->d_czf, r, z, out, zcoarse, cd);
     else
       hipLaunchKernelGGL((k_mg_sweep_vec<MODE, 2>), dim3(nb), dim3(256), 0, c->stream, d, tl, g->d_flags, g->d_cxf,
-                         g->d_cyf, g->d_czf, r, z, out);
+                         g->d_cyf, g->d_czf, r, z, out, zcoarse, cd);
   } else {
-    hipLaunchKernelGGL(k_mg_sweep<MODE>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags, g->d_cxf, g->d_cyf,
-                       g->d_czf, r, z, out);
+    hipLaunchKernelGGL(k_mg_sweep<MODE == 4 ? 1 : MODE>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags,
+                       g->d_cxf, g->d_cyf, g->d_czf, r, z, out);
   }
 }
 
@@ -423,16 +464,16 @@ static void smooth_from_zero(xf_ctx c, MgLevel &L, const float *r, int nu) {
   }
 }
 
-// nu sweeps of damped Jacobi continuing from L.z; the last into `last` when
-// given (else L.z holds the result)
-static void smooth(xf_ctx c, MgLevel &L, const float *r, int nu, float *last = nullptr) {
+// nu sweeps of damped Jacobi continuing from L.z (plus the coarse level C's
+// correction at the unknowns when C is given, folded into the first sweep);
+// the last into `last` when given (else L.z holds the result)
+static void smooth(xf_ctx c, MgLevel &L, const float *r, int nu, float *last = nullptr, const MgLevel *C = nullptr) {
   const xf_grid g = L.g;
   for (int s = 0; s < nu; s++) {
-    if (last && s == nu - 1) {
-      sweep<1>(c, g, r, L.z, last);
-      return;
-    }
-    sweep<1>(c, g, r, L.z, L.z2);
+    float *dst = last && s == nu - 1 ? last : L.z2;
+    if (s == 0 && C) sweep<4>(c, g, r, L.z, dst, C->z, dims_of(C->g));
+    else sweep<1>(c, g, r, L.z, dst);
+    if (dst == last) return;
     std::swap(L.z, L.z2);
   }
   if (last) XF_HIP(hipMemcpyAsync(last, L.z, sizeof(float) * g->ncell, hipMemcpyDeviceToDevice, c->stream));
@@ -490,9 +531,7 @@ static void vcycle(xf_ctx c, MgHier *h, int k, const float *r, struct xf_unet_s 
   sweep<2>(c, g, r, L.z, L.t);
   hipLaunchKernelGGL(k_mg_restrict, cell_grid(C.g), dim3(256), 0, c->stream, d, cd, g->d_flags, L.t, C.r);
   vcycle(c, h, k + 1, C.r, net, nullptr);
-  const dim3 pg((g->m - 2 + 255) / 256, (g->l - 2 + 3) / 4, g->n - 2);
-  hipLaunchKernelGGL(k_mg_prolong4, pg, dim3(256), 0, c->stream, d, cd, g->d_flags, C.z, L.z);
-  smooth(c, L, r, kSweeps, last);
+  smooth(c, L, r, kSweeps, last, &C);
 }
 
 // z = M(r): one V-cycle on the grid's hierarchy (r, z FP32 fields of the fine grid)

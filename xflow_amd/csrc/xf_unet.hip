@@ -86,22 +86,37 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   const int cpc = Cin / 8;  // 16-byte chunks per cell
   const int nchunk = SZ * SY * SX * cpc;
   const int D0 = a.D >> a.up0, H0 = a.H >> a.up0, W0 = a.W >> a.up0;
-  for (int q = threadIdx.x; q < nchunk; q += blockDim.x) {
-    const int cell = fdiv(q, a.m_cpc), cc = (q - cell * cpc) * 8;
-    const int row = cell / SX, xx = cell - row * SX;
-    const int zz = fdiv(row, a.m_sy), yy = row - zz * SY;
-    const int x = x0 - 1 + xx, y = y0 - 1 + yy, z = z0 - 1 + zz;
-    bf16x8 v = {};
-    if (x >= 0 && x < a.W && y >= 0 && y < a.H && z >= 0 && z < a.D) {
-      if (cc < a.C0) {
-        const int64_t g = ((((int64_t)b * D0 + (z >> a.up0)) * H0 + (y >> a.up0)) * W0 + (x >> a.up0)) * a.C0 + cc;
-        v = *(const bf16x8 *)(a.src0 + g);
-      } else {
-        const int64_t g = ((((int64_t)b * a.D + z) * a.H + y) * a.W + x) * a.C1 + (cc - a.C0);
-        v = *(const bf16x8 *)(a.src1 + g);
+  // four chunks per thread in flight: loads first, then the LDS stores
+  constexpr int U = 4;
+  for (int qb = threadIdx.x; qb < nchunk; qb += U * blockDim.x) {
+    bf16x8 v[U];
+    int dst[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int q = qb + u * blockDim.x;
+      v[u] = bf16x8{};
+      dst[u] = -1;
+      if (q < nchunk) {
+        const int cell = fdiv(q, a.m_cpc), cc = (q - cell * cpc) * 8;
+        const int row = cell / SX, xx = cell - row * SX;
+        const int zz = fdiv(row, a.m_sy), yy = row - zz * SY;
+        const int x = x0 - 1 + xx, y = y0 - 1 + yy, z = z0 - 1 + zz;
+        dst[u] = cell * CS + cc;
+        if (x >= 0 && x < a.W && y >= 0 && y < a.H && z >= 0 && z < a.D) {
+          if (cc < a.C0) {
+            const int64_t g =
+                ((((int64_t)b * D0 + (z >> a.up0)) * H0 + (y >> a.up0)) * W0 + (x >> a.up0)) * a.C0 + cc;
+            v[u] = *(const bf16x8 *)(a.src0 + g);
+          } else {
+            const int64_t g = ((((int64_t)b * a.D + z) * a.H + y) * a.W + x) * a.C1 + (cc - a.C0);
+            v[u] = *(const bf16x8 *)(a.src1 + g);
+          }
+        }
       }
     }
-    *(bf16x8 *)(tile + (int64_t)cell * CS + cc) = v;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (dst[u] >= 0) *(bf16x8 *)(tile + dst[u]) = v[u];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -136,31 +151,44 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
     ci -= Cin;
     next_tap();
   }
-  // weight fragments of the next k-step are loaded before this step's MFMAs
+  // weight fragments (global) and input fragments (LDS) of the next k-step
+  // are loaded before this step's MFMAs
   const bf16 *wrow = a.wt + (int64_t)(ct0 * 16 + l16) * a.Kpad + 8 * lg;
   bf16x8 afn[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) afn[t] = *(const bf16x8 *)(wrow + (int64_t)t * 16 * a.Kpad);
+  auto load_b = [&](bf16x8 *bf) {
+#pragma unroll
+    for (int r = 0; r < RPW; r++) {
+      bf[r] = bf16x8{};
+      if (tap < 27) bf[r] = *(const bf16x8 *)(tile + (int64_t)(roff[r] + toff) * CS + ci);
+    }
+  };
+  bf16x8 bcur[RPW];
+  load_b(bcur);
   for (int k0 = 0; k0 < a.Kpad; k0 += 32) {
     bf16x8 af[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) af[t] = afn[t];
-    if (k0 + 32 < a.Kpad)
+    const bool more = k0 + 32 < a.Kpad;
+    if (more)
 #pragma unroll
       for (int t = 0; t < NT; t++) afn[t] = *(const bf16x8 *)(wrow + (int64_t)t * 16 * a.Kpad + k0 + 32);
-    const bool valid = tap < 27;
-#pragma unroll
-    for (int r = 0; r < RPW; r++) {
-      bf16x8 bf = {};
-      if (valid) bf = *(const bf16x8 *)(tile + (int64_t)(roff[r] + toff) * CS + ci);
-#pragma unroll
-      for (int t = 0; t < NT; t++) acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t], bf, acc[r][t], 0, 0, 0);
-    }
     ci += 32;
     while (ci >= Cin) {
       ci -= Cin;
       next_tap();
     }
+    bf16x8 bnext[RPW];
+    if (more) load_b(bnext);
+#pragma unroll
+    for (int r = 0; r < RPW; r++)
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t], bcur[r], acc[r][t], 0, 0, 0);
+    if (more)
+#pragma unroll
+      for (int r = 0; r < RPW; r++) bcur[r] = bnext[r];
   }
   // epilogue: lane holds D[co = 16t + 4 lg + e][cell = l16]
 #pragma unroll
