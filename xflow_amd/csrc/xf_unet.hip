@@ -133,35 +133,50 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   for (int r = 0; r < RPW; r++)
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0, 0, 0, 0};
-  // this lane group's k = 8 lg + 32 step = tap * Cin + ci
-  int ci = 8 * lg, tap = 0, dx = 0, dy = 0, toff = 0;
-  auto next_tap = [&]() {
-    tap++;
-    toff++;
-    if (++dx == 3) {
-      dx = 0;
-      toff += SX - 3;
-      if (++dy == 3) {
-        dy = 0;
-        toff += SX * SY - 3 * SX;
+  // this lane group's k = 8 lg + 32 step = tap * Cin + ci; `off` is that
+  // k's element offset in the LDS tile relative to a row's first cell
+  // (tap offset * CS + ci), kept up to date with additions only
+  int ci = 8 * lg, tap = 0, dx = 0, dy = 0, off = 8 * lg;
+  const int sdx = CS, sdy = (SX - 3) * CS + CS, sdz = (SX * SY - 3 * SX) * CS + (SX - 3) * CS + CS;
+  auto advance = [&]() {  // k += 32
+    ci += 32;
+    off += 32;
+    while (ci >= Cin) {
+      ci -= Cin;
+      off -= Cin;
+      tap++;
+      if (++dx < 3) {
+        off += sdx;
+      } else {
+        dx = 0;
+        if (++dy < 3) {
+          off += sdy;
+        } else {
+          dy = 0;
+          off += sdz;
+        }
       }
     }
   };
-  while (ci >= Cin) {
-    ci -= Cin;
-    next_tap();
-  }
+  // the start (8 lg may already pass Cin = 16)
+  ci -= 32;
+  off -= 32;
+  advance();
+  int rbase[RPW];
+#pragma unroll
+  for (int r = 0; r < RPW; r++) rbase[r] = roff[r] * CS;
   // weight fragments (global) and input fragments (LDS) of the next k-step
   // are loaded before this step's MFMAs
   const bf16 *wrow = a.wt + (int64_t)(ct0 * 16 + l16) * a.Kpad + 8 * lg;
+  const int64_t wstride = (int64_t)16 * a.Kpad;
   bf16x8 afn[NT];
 #pragma unroll
-  for (int t = 0; t < NT; t++) afn[t] = *(const bf16x8 *)(wrow + (int64_t)t * 16 * a.Kpad);
+  for (int t = 0; t < NT; t++) afn[t] = *(const bf16x8 *)(wrow + t * wstride);
   auto load_b = [&](bf16x8 *bf) {
 #pragma unroll
     for (int r = 0; r < RPW; r++) {
       bf[r] = bf16x8{};
-      if (tap < 27) bf[r] = *(const bf16x8 *)(tile + (int64_t)(roff[r] + toff) * CS + ci);
+      if (tap < 27) bf[r] = *(const bf16x8 *)(tile + rbase[r] + off);
     }
   };
   bf16x8 bcur[RPW];
@@ -171,14 +186,11 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
 #pragma unroll
     for (int t = 0; t < NT; t++) af[t] = afn[t];
     const bool more = k0 + 32 < a.Kpad;
+    wrow += 32;
     if (more)
 #pragma unroll
-      for (int t = 0; t < NT; t++) afn[t] = *(const bf16x8 *)(wrow + (int64_t)t * 16 * a.Kpad + k0 + 32);
-    ci += 32;
-    while (ci >= Cin) {
-      ci -= Cin;
-      next_tap();
-    }
+      for (int t = 0; t < NT; t++) afn[t] = *(const bf16x8 *)(wrow + t * wstride);
+    advance();
     bf16x8 bnext[RPW];
     if (more) load_b(bnext);
 #pragma unroll
@@ -248,9 +260,8 @@ __device__ inline float src0_inv(const Src0 &s) {
   return ms > 0 ? (float)(1.0 / sqrt(ms)) : 0.0f;
 }
 
-// c0 (2 -> 16, FP32): a block computes a 16 x 4 x 4 cell tile from its input
-// (r * inv_s, mask) staged in LDS with a one-cell halo; per output the same
-// sum in the same order as a direct evaluation.
+// c0 (2 -> 16, FP32 with fused multiply-adds): a block computes a 16 x 4 x 4
+// cell tile from its input (r * inv_s, mask) staged in LDS with a one-cell halo.
 __global__ __launch_bounds__(256) void k_conv_first(Src0 s, const float *__restrict__ w,
                                                     const float *__restrict__ bias, bf16 *out, int B) {
   // w: [16][27][2] fp32 master weights
@@ -279,7 +290,7 @@ __global__ __launch_bounds__(256) void k_conv_first(Src0 s, const float *__restr
   for (int t = 0; t < 27; t++) {
     const float2 v = in[((tz + t / 9) * SY + (ty + (t / 3) % 3)) * SX + tx + t % 3];
 #pragma unroll
-    for (int co = 0; co < 16; co++) acc[co] += w[(co * 27 + t) * 2] * v.x + w[(co * 27 + t) * 2 + 1] * v.y;
+    for (int co = 0; co < 16; co++) acc[co] = fmaf(w[(co * 27 + t) * 2 + 1], v.y, fmaf(w[(co * 27 + t) * 2], v.x, acc[co]));
   }
   bf16 o[16];
 #pragma unroll
