@@ -20,6 +20,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# measured HBM traffic of one k_cgcg_update<double,double> launch at 256^3 (mean
+# of an even and an odd iteration; rocprofv3 counters scaled by a copy of
+# known size, as recorded in profiles/cg_update_traffic_pmc.md)
+KB_TRAFFIC_BYTES = 1099.3e6
 
 
 def log(*a):
@@ -211,7 +215,10 @@ def main():
                    "fluid_cells": fluid, "mcell_updates_per_s": round(mcups, 1),
                    "solvers": solvers},
         "roofline": {"bound": "hbm", "achieved": round(kb_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(kb_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(kb_gbs / HBM_PEAK_GBS, 4), "traffic": KB_TRAFFIC_BYTES if args.n == 256 else None,
+                     "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by "
+                                     "a 512 MiB copy (profiles/cg_update_traffic_pmc.md); equal to the "
+                                     "algorithmic bytes",
                      "kernel": "k_cgcg_update<double,double> (64 B per array cell, mean of 56 and 72)",
                      "kernel_us": round(kb["ms"] * 1e3, 2), "launches": kb["count"],
                      "stencil_kernel": "k_cgcg_stencil<double> (17 B per interior cell)",

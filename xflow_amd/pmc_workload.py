@@ -6,14 +6,20 @@ WRITE_SIZE should each read the copied bytes), then a few
 iterations of the FP64 and FP32 throughput CG on the 256^3 random-obstacle
 pressure problem, so every counter pass sees the same dispatches:
 
-    rocprofv3 --pmc FETCH_SIZE -d OUT -o pmc -- python3 -m xflow_amd.pmc_workload
-    rocprofv3 --pmc WRITE_SIZE -d OUT -o pmc -- python3 -m xflow_amd.pmc_workload
+    rocprofv3 --pmc FETCH_SIZE -d OUT -o pmc -- python3 <repo>/xflow_amd/pmc_workload.py
+    rocprofv3 --pmc WRITE_SIZE -d OUT -o pmc -- python3 <repo>/xflow_amd/pmc_workload.py
 
-    python3 -m xflow_amd.pmc_workload [--n 256] [--iters 8] [--copy-mib 512]
+    python3 xflow_amd/pmc_workload.py [--n 256] [--iters 8] [--copy-mib 512]
+
+(run from any directory: the profiler is started from /tmp).
 """
 import argparse
+import os
 import sys
 import time
+
+if __package__ in (None, ""):  # run as a script: make the package importable
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
 
 def main():
