@@ -71,6 +71,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--skip-fullres", action="store_true")
+    ap.add_argument("--io", action="store_true", help="also time the output files (flow VTK, .npy) of the solution")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--nn-inner-tol", type=float, default=1e-4)
     ap.add_argument("--weights", default=os.path.join(ROOT, "xflow_amd", "weights", "unet_poisson.npy"))
@@ -177,6 +178,27 @@ def main():
     pcie_ms = (time.perf_counter() - t0) * 1e3 / 3
     log("PCIe round trip of b and x (%.0f MB each): %.2f ms" % (b.numel() * 8 / 1e6, pcie_ms))
 
+    io = None
+    if args.io:
+        # the reference's output path on the solved pressure: node-centred
+        # flow VTK (GPU node averages + threaded printf) and a .npy of p
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            coords = [np.concatenate([[0.0], np.cumsum(h)]) for h in (case["hx"], case["hy"], case["hz"])]
+            zero = torch.zeros_like(x)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            xf.write_vtk_flow(ctx, os.path.join(td, "flow.vtk"), g, *coords, zero, zero, zero, p=x)
+            vtk_s = time.perf_counter() - t0
+            vtk_mb = os.path.getsize(os.path.join(td, "flow.vtk")) / 1e6
+            t0 = time.perf_counter()
+            xf.npy_save(ctx, os.path.join(td, "p.npy"), x)
+            npy_s = time.perf_counter() - t0
+            npy_mb = os.path.getsize(os.path.join(td, "p.npy")) / 1e6
+        io = {"flow_vtk_s": round(vtk_s, 3), "flow_vtk_mb": round(vtk_mb, 1),
+              "npy_s": round(npy_s, 3), "npy_mb": round(npy_mb, 1)}
+        log("output files: flow VTK %.1f MB in %.2f s, .npy %.1f MB in %.3f s" % (vtk_mb, vtk_s, npy_mb, npy_s))
+
     interior = (args.n) ** 3
     ncell = (args.n + 2) ** 3
     ka = kstats.get("cg_stencil_fp64", {"ms": float("nan"), "count": 0})
@@ -234,6 +256,8 @@ def main():
                                           "trained network as its coarsest-grid (64^3) solve",
                                 "true_residual": nn["true_residual"],
                                 "ms_with_pcie": round(nn["ms"] + pcie_ms, 3)}
+    if io is not None:
+        out["output_files"] = io
     print(json.dumps(out), flush=True)
     g.close()
     ctx.close()

@@ -95,3 +95,47 @@ def test_pack_solvers_nonsymmetric(ctx, name, solver, tol):
     ref_iters = PACK_SWEEPS_TO_ITERS[solver](int(outs["sweeps"][0]))
     assert abs(st.iters - ref_iters) <= 0.25 * ref_iters, (st.iters, ref_iters)
     assert np.linalg.norm(got - outs["x"]) <= tol * np.linalg.norm(outs["x"])
+
+
+def test_pack_transposed_solve_matches_oracle(ctx):
+    """BiCG with the packed operator taken transposed (operator~ before the
+    solve: A^T x = b, the shadow sequence on A) against the oracle run on the
+    swapped rows; and the Rn vector algebra against numpy.  This system is
+    more sensitive to rounding than the untransposed one: in the oracle a
+    1e-15 relative change of b gives 371-378 iterations (373 unperturbed)
+    and moves the solution by up to 1.2e-9, hence 6 iterations and 5e-9."""
+    import xf_oracle as xo
+    ins, _ = golden("pack_bicg_mixed")
+    g = xf.Grid(ctx, np.diff(ins["x"]), np.diff(ins["y"]), np.diff(ins["z"]), ins["codes"], ins.get("normals"),
+                ins.get("has_normal"))
+    og = xo.Grid(np.diff(ins["x"]), np.diff(ins["y"]), np.diff(ins["z"]), ins["codes"], normals=ins.get("normals"),
+                 has_normal=ins.get("has_normal"))
+    p = xf.Pack(g)
+    b = torch.zeros(p.n_unknowns, dtype=torch.float64, device="cuda")
+    p.gather(torch.from_numpy(np.ascontiguousarray(ins["bvec"], dtype=np.float64)).cuda(), b)
+    x = torch.zeros_like(b)
+    st = p.solve(b, torch.zeros_like(b), x, xf.solve_opts(xf.XF_BICG), transpose=True)
+    assert st.status == 0
+    torch.cuda.synchronize()
+    bh = b.cpu().numpy()
+    want, it, _, s2 = xo.pack_solve("bicg", xo.pack(og, True), xo.pack(og), bh, np.zeros_like(bh))
+    assert s2 == xo.OK and abs(st.iters - it) <= 6, (st.iters, it)
+    got = x.cpu().numpy()
+    assert np.linalg.norm(got - want) <= 5e-9 * np.linalg.norm(want)
+    # A^T x = b holds for the transposed solve
+    y = torch.zeros_like(b)
+    p.apply(x, y, transpose=True)
+    torch.cuda.synchronize()
+    assert np.abs(y.cpu().numpy() - bh).max() <= 1e-7
+    # Rn vectors: res = p1 s1 + p2 s2, (a, b), max |x| with its 1-based index
+    rng = np.random.default_rng(4)
+    a1, a2 = rng.standard_normal(p.n_unknowns), rng.standard_normal(p.n_unknowns)
+    d1, d2 = torch.from_numpy(a1).cuda(), torch.from_numpy(a2).cuda()
+    out = torch.zeros_like(d1)
+    p.lincomb(0.5, d1, -2.0, d2, out)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), 0.5 * a1 + -2.0 * a2)
+    assert abs(p.dot(d1, d2) - float(a1 @ a2)) <= 1e-12 * np.abs(a1).sum() * np.abs(a2).max()
+    v, idx = p.absmax(d2)
+    assert v == np.abs(a2).max() and idx == int(np.argmax(np.abs(a2))) + 1
+    p.close()
