@@ -20,10 +20,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# measured HBM traffic of one k_cgcg_update<double,double> launch at 256^3 (mean
+# measured HBM traffic of one k_cgcg_step<double,double> launch at 256^3 (mean
 # of an even and an odd iteration; rocprofv3 counters scaled by a copy of
-# known size, as recorded in profiles/cg_update_traffic_pmc.md)
-KB_TRAFFIC_BYTES = 1099.3e6
+# known size); None until measured for this kernel
+KB_TRAFFIC_BYTES = None
 
 
 def log(*a):
@@ -200,15 +200,14 @@ def main():
         log("output files: flow VTK %.1f MB in %.2f s, .npy %.1f MB in %.3f s" % (vtk_mb, vtk_s, npy_mb, npy_s))
 
     interior = (args.n) ** 3
-    ncell = (args.n + 2) ** 3
-    ka = kstats.get("cg_stencil_fp64", {"ms": float("nan"), "count": 0})
-    kb = kstats.get("cg_update_fp64", {"ms": float("nan"), "count": 0})
-    # algorithmic bytes: KA reads r and the flag byte and writes w (17 B per
-    # interior cell); KB reads r, w, p, s and writes p, s, r, plus x read and
-    # written on odd iterations (56 / 72 B per array cell; the sampled
-    # launches alternate, so the mean is 64 B)
-    ka_bytes = 17.0 * interior
-    kb_bytes = 64.0 * ncell
+    ka = kstats.get("cg_dots_fp64", {"ms": float("nan"), "count": 0})
+    kb = kstats.get("cg_step_fp64", {"ms": float("nan"), "count": 0})
+    # algorithmic bytes per interior cell: KA reads r and the flag byte (9 B);
+    # KB reads r, p and the flag byte and writes p and r (33 B), plus x read and
+    # written on odd iterations (49 B; the sampled launches alternate, so the
+    # mean is 41 B)
+    ka_bytes = 9.0 * interior
+    kb_bytes = 41.0 * interior
     ka_gbs = ka_bytes / (ka["ms"] * 1e-3) / 1e9
     kb_gbs = kb_bytes / (kb["ms"] * 1e-3) / 1e9
     it_mean = plain["iterations"]
@@ -233,18 +232,20 @@ def main():
         "data": "synthetic",
         "config": {"workload": "pressure Poisson %d^3, random obstacles seed %d, solve to 1e-8 relative residual" %
                                (args.n, args.seed),
-                   "value_solver": "plain FP64 CG (single-reduction, two kernels per iteration)", "iterations": it_mean,
+                   "value_solver": "plain FP64 CG (single-reduction, A p recomputed, two kernels per iteration)", "iterations": it_mean,
                    "fluid_cells": fluid, "mcell_updates_per_s": round(mcups, 1),
                    "solvers": solvers},
         "roofline": {"bound": "hbm", "achieved": round(kb_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(kb_gbs / HBM_PEAK_GBS, 4), "traffic": KB_TRAFFIC_BYTES if args.n == 256 else None,
+                     "frac": round(kb_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": KB_TRAFFIC_BYTES if args.n == 256 else None,
                      "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by "
-                                     "a 512 MiB copy (profiles/cg_update_traffic_pmc.md); equal to the "
-                                     "algorithmic bytes",
-                     "kernel": "k_cgcg_update<double,double> (64 B per array cell, mean of 56 and 72)",
+                                     "a 512 MiB copy (profiles/cg_kernels_traffic_pmc.md)",
+                     "kernel": "k_cgcg_step<double,double,2> (41 B per interior cell, mean of 33 and 49)",
                      "kernel_us": round(kb["ms"] * 1e3, 2), "launches": kb["count"],
-                     "stencil_kernel": "k_cgcg_stencil<double> (17 B per interior cell)",
-                     "stencil_gbs": round(ka_gbs, 1), "stencil_us": round(ka["ms"] * 1e3, 2)},
+                     "dots_kernel": "k_cgcg_dots<double,2> (9 B per interior cell)",
+                     "dots_gbs": round(ka_gbs, 1), "dots_us": round(ka["ms"] * 1e3, 2),
+                     "iteration_bytes_per_cell": 50,
+                     "iteration_gbs": round((ka_bytes + kb_bytes) / ((ka["ms"] + kb["ms"]) * 1e-3) / 1e9, 1)},
         "cpu_baseline": cpu,
     }
     out["host_boundary"] = {"pcie_round_trip_ms": round(pcie_ms, 3),

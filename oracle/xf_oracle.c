@@ -20,10 +20,10 @@
  *       checked bit-exactly against the compiled reference's rows;
  *   xo_pack_solve: the reference solvers on those rows, bit-exact against
  *       the compiled reference's solutions on the packed operator.
- *   xo_cg_cg: single-reduction CG of Chronopoulos and Gear (J. Comput. Appl.
- *       Math. 25, 1989; not in the reference), the throughput solver's
- *       recurrence; pinned by a dense numpy solve and by agreement with
- *       xo_cg_hs (the two are the same method in exact arithmetic).
+ *   xo_cg_cg: single-reduction CG after Chronopoulos and Gear (J. Comput.
+ *       Appl. Math. 25, 1989; not in the reference) with s = A p recomputed,
+ *       the throughput solver's recurrence; pinned by a dense numpy solve and
+ *       by agreement with xo_cg_hs (the same method in exact arithmetic).
  *
  * Dot products are serial sums in i->j->k order, as in the reference
  * (XSpace.cpp:3219-3342).  Element-wise loops may run on several OpenMP
@@ -692,7 +692,9 @@ int xo_cg_hs(const xo_grid *g, const double *b, const double *x0, double *x, dou
 /* Single-reduction CG: per iteration w = A r, gamma = (r, r), delta = (w, r);
  * stop when gamma <= rtol^2 gamma_0; beta = gamma/gamma_old,
  * alpha = gamma/(delta - beta gamma/alpha_old) (alpha = gamma/delta first);
- * p = r + beta p, s = w + beta s, e += alpha p, r -= alpha s. */
+ * p = r + beta p, s = A p, e += alpha p, r -= alpha s.  (Chronopoulos and
+ * Gear carry s = w + beta s instead; s = A p holds in exact arithmetic, and
+ * the GPU solver recomputes A p rather than storing w and s.) */
 int xo_cg_cg(const xo_grid *g, const double *b, const double *x0, double *x, double rtol, int maxit,
              int project_mean, int *iters, double *resid) {
   int64_t nc = (int64_t)g->n * g->l * g->m;
@@ -727,9 +729,10 @@ int xo_cg_cg(const xo_grid *g, const double *b, const double *x0, double *x, dou
         alpha = gam / (del - beta * gam / aold);
       }
       for (int64_t c = 0; c < nc; c++)
+        if (g->space_mask[c] > 0) p[c] = r[c] + beta * p[c];
+      xo_apply(g, p, NULL, 0, s);
+      for (int64_t c = 0; c < nc; c++)
         if (g->space_mask[c] > 0) {
-          p[c] = r[c] + beta * p[c];
-          s[c] = w[c] + beta * s[c];
           e[c] += alpha * p[c];
           r[c] -= alpha * s[c];
         }

@@ -131,9 +131,13 @@ def sphere_case(ncell, seed=0):
     return cases.sphere_case(ncell, seed)
 
 
-# (17, 19, 131) and (20, 24, 300): rows long enough for the two-doubles-per-lane
-# stencil, with the last k-tile shifted back over the one before it
-@pytest.mark.parametrize("shape", [(24, 24, 24), (40, 40, 40), (30, 45, 37), (17, 19, 131), (20, 24, 300)])
+# (24, 24, 24), (40, 40, 40), (30, 45, 37): rows shorter than a wave (lanes past
+# the row end clamped); (16, 18, 100): one double per lane, last k-tile shifted
+# back; (17, 19, 131) and (20, 24, 300): rows long enough for the
+# two-doubles-per-lane stencil, with the last k-tile shifted back over the one
+# before it
+@pytest.mark.parametrize("shape", [(24, 24, 24), (40, 40, 40), (30, 45, 37), (16, 18, 100), (17, 19, 131),
+                                   (20, 24, 300)])
 def test_throughput_cg_matches_oracle(ctx, shape):
     from xflow_amd import cases
     codes = cases.neumann_codes(None, [cases.sphere_obstacle((0.5, 0.5, 0.5), 0.2)], shape=shape)

@@ -4,23 +4,26 @@
 // iterative methods for symmetric linear systems", J. Comput. Appl. Math. 25,
 // 1989): both inner products of an iteration come from one pass, so one
 // iteration is two kernels with one grid-wide reduction between them:
-//   KA  w = A r, per-tile partials of gamma = (r, r) and delta = (w, r)
+//   KA  per-tile partials of gamma = (r, r) and delta = (A r, r); no store
 //                                                   [2.5-D marching stencil]
 //   KB  every block reduces the partials itself (same order in every block,
 //       so all blocks hold bit-identical gamma, delta), then
 //       beta = gamma/gamma_old, alpha = gamma/(delta - beta gamma/alpha_old),
-//       p = r + beta p, s = w + beta s, x += alpha p, r -= alpha s
-//                                                   [streaming]
+//       p = r + beta p, r -= alpha A p (A p recomputed from r and the old p at
+//       the seven stencil points, never stored), x += alpha p
+//                                                   [2.5-D marching stencil]
+// The textbook formulation also carries s = A p by the recurrence
+// s = A r + beta s; recomputing A p instead removes w = A r and s from memory
+// (81 -> 50 B per cell and iteration in FP64).
 // No kernel needs a device-scope fence or atomic: a fence per block (the
 // "last block finishes the sum" pattern) writes back the XCD's L2 and cost
 // more than the data movement itself on this chip.  Scalars that carry over
 // between iterations (gamma, alpha) live in two parity slots of the device
 // state, written by block 0 of KB into the slot the next launch reads.
-// Algorithmic traffic per cell and iteration: KA reads r and the flag byte
-// and writes w (17 B in FP64, 9 B in FP32, interior cells); KB reads r, w, p,
-// s and writes p, s, r, and on odd iterations also reads and writes x with
-// two steps at once (56 / 72 B in FP64, mean 64 B; 28 / 36 B in FP32, all
-// array cells).
+// Algorithmic traffic per interior cell and iteration: KA reads r and the
+// flag byte (9 B in FP64, 5 B in FP32); KB reads r, p and the flag byte and
+// writes p and r, and on odd iterations also reads and writes x with two
+// steps at once (33 / 49 B in FP64, mean 41 B; 17 / 25 B in FP32, mean 21 B).
 //
 // The stencil is the reference's Actual-row formula (XOperator.cpp:971-1019)
 // with the step ratios folded into per-node coefficients
@@ -44,7 +47,6 @@ constexpr int TJ = 4;    // j extent of a tile (waves per block)
 constexpr int IC = 32;   // i planes marched per block
 constexpr int kStreamBlocks = 4096;
 constexpr int kCgTileBlocks = 2048;   // KA blocks (persistent over tiles): bounds the partials KB reads
-constexpr int kCgUpdateBlocks = 1024; // KB blocks (grid-stride)
 
 struct CgWork {
   int64_t n = 0;
@@ -90,6 +92,7 @@ enum { HS_RR = 0, HS_PQ, HS_ALPHA, HS_BETA, HS_STOP2, HS_RR0, HS_SUM, HS_CNT, HS
 struct Tiles {
   int kt, jt, it, total;
   int blocks;  // 8 * ceil(total / 8): the XCD-aware mapping is a bijection on [0, blocks)
+  int ic = IC; // i planes per tile
 };
 static Tiles tiles(xf_grid g) {
   Tiles t;
@@ -275,17 +278,168 @@ __global__ __launch_bounds__(256) void k_cgcg_stencil_vec(int n, int l, int m, T
   }
 }
 
-// KB: reduce KA's partials, decide (converged / NaN / iteration cap /
-// breakdown: the same decision in every block), then the fused update.
-// `it` is the number of updates applied before this launch; the first launch
-// (it == 0) takes p = r, s = w without reading p and s.
-template <typename T, typename X>
-__global__ __launch_bounds__(256) void k_cgcg_update(int64_t nc, int npart, const double *__restrict__ part,
-                                                      X *__restrict__ x, T *__restrict__ r, const T *__restrict__ w,
-                                                      T *__restrict__ p, T *__restrict__ s, DevState *st, int it,
-                                                      int maxit) {
+// ------------------------------------------------ plain CG: two kernels per iteration
+// The stored vectors are r and p (each ping-ponged between two buffers, so a
+// kernel never overwrites what another block still reads as a neighbour) and
+// x.  s = A p is not stored: KB recomputes it from r and p_old at the seven
+// stencil points (p = r + beta p_old evaluated there with the same operations,
+// so the neighbours' p are bit-identical to the ones their owners store).
+//   KA  k_cgcg_dots  per-block partials of gamma = (r, r), delta = (A r, r);
+//                    reads r and the flag byte, stores nothing      [9 / 5 B]
+//   KB  k_cgcg_step  every block reduces the partials (same order in every
+//                    block: bit-identical scalars), then
+//                    p = r + beta p_old, r_out = r - alpha A p, and x += alpha p
+//                    on odd iterations (two steps at once)          [41 / 21 B mean]
+// In exact arithmetic (A p, p) = delta - beta gamma/alpha_old, so alpha below
+// is the Hestenes-Stiefel step and the method is CG.
+//
+// Tiles: one wave covers 64*CPL consecutive k-cells of one (i, j) row (16-byte
+// vectors: CPL = 2 doubles or 4 floats), TJ rows per block, tl.ic planes marched
+// in i with the planes i-1, i, i+1 of the column in registers.  The last k-tile
+// of a row is shifted back to end at m-2 so every vector lies inside the row;
+// cells it shares with the tile before are computed twice and stored and
+// counted only by their owner.  Rows shorter than 64 cells (CPL = 1) clamp the
+// lanes past the row end onto its last cell and store nothing from them.
+struct CgTile {
+  int64_t c;     // array index of the lane's first cell on plane i0
+  int i0, i1, j;
+  int k0;        // first cell of the lane in its row
+  int kraw;      // nominal first cell of the lane (k0 before clamping)
+  bool own[4];
+  bool allown;
+};
+
+template <int CPL>
+__device__ inline bool cg_tile(const Tiles &tl, int t, int n, int l, int m, CgTile &w) {
+  const int lane = threadIdx.x & 63;
+  const int kspan = 64 * CPL;
+  const int ktile = t % tl.kt;
+  const int jtile = (t / tl.kt) % tl.jt;
+  const int itile = t / (tl.kt * tl.jt);
+  const int kcount = 1 + ktile * kspan;  // cells from here on are this tile's own
+  const int kstart = max(1, min(kcount, m - 1 - kspan));
+  w.kraw = kstart + lane * CPL;
+  const int k0 = min(w.kraw, m - 1 - CPL);
+  w.j = 1 + jtile * TJ + (threadIdx.x >> 6);
+  w.i0 = 1 + itile * tl.ic;
+  w.i1 = min(w.i0 + tl.ic, n - 1);
+  w.allown = true;
+#pragma unroll
+  for (int e = 0; e < CPL; e++) {
+    w.own[e] = w.kraw + e >= kcount && w.kraw + e <= m - 2;
+    w.allown = w.allown && w.own[e];
+  }
+  w.k0 = k0;
+  w.c = ((int64_t)w.i0 * l + w.j) * m + k0;
+  return w.j <= l - 2;
+}
+
+template <typename T, int CPL>
+struct CgVec {
+  T v[CPL];
+};
+
+// A f at the CPL cells of a lane from the centre plane fc, its i-neighbours
+// fm / fp, j-neighbours fjm / fjp and k-neighbours left / right (the reference's
+// Actual-row formula with folded coefficients; 0 on rows that are not Actual).
+template <typename T, int CPL>
+__device__ inline T cg_stencil_cell(uint8_t f, int e, const CgVec<T, CPL> &fc, const CgVec<T, CPL> &fm,
+                                    const CgVec<T, CPL> &fp, T fjm, T fjp, T left, T right, T cxm, T cxp, T cym,
+                                    T cyp, T czm, T czp) {
+  const T x0 = fc.v[e];
+  const T km = e == 0 ? left : fc.v[e > 0 ? e - 1 : 0];
+  const T kp = e == CPL - 1 ? right : fc.v[e < CPL - 1 ? e + 1 : 0];
+  T tx = ((f & FL_IP) ? cxp * (fp.v[e] - x0) : T(0)) - ((f & FL_IM) ? cxm * (x0 - fm.v[e]) : T(0));
+  T ty = ((f & FL_JP) ? cyp * (fjp - x0) : T(0)) - ((f & FL_JM) ? cym * (x0 - fjm) : T(0));
+  T tz = ((f & FL_KP) ? czp * (kp - x0) : T(0)) - ((f & FL_KM) ? czm * (x0 - km) : T(0));
+  return (f & FL_ACTUAL) ? -(tx + ty + tz) : T(0);
+}
+
+// KA: per-block partials (gamma, delta) = ((r, r), (A r, r)) over Actual cells.
+// Blocks are persistent over tiles (bounded partial count), consecutive tiles
+// on one XCD.
+template <typename T, int CPL>
+__global__ __launch_bounds__(256) void k_cgcg_dots(int n, int l, int m, Tiles tl, const uint8_t *__restrict__ flags,
+                                                    const T *__restrict__ cx, const T *__restrict__ cy,
+                                                    const T *__restrict__ cz, const T *__restrict__ r,
+                                                    const DevState *st, double *__restrict__ part) {
   __shared__ double lds[8];
   if (st->done) return;
+  using V = CgVec<T, CPL>;
+  const int per = (tl.total + 7) / 8;
+  const int bx = gridDim.x >> 3;
+  const int xcd = blockIdx.x & 7;
+  const int lane = threadIdx.x & 63;
+  double gam = 0, del = 0;
+  const int64_t sI = (int64_t)l * m, sJ = m;
+  for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
+    const int t = xcd * per + tq;
+    if (t >= tl.total) break;
+    CgTile w;
+    if (!cg_tile<CPL>(tl, t, n, l, m, w)) continue;
+    const int k0 = w.k0;
+    T czm[CPL], czp[CPL];
+#pragma unroll
+    for (int e = 0; e < CPL; e++) {
+      czm[e] = cz[2 * (k0 + e)];
+      czp[e] = cz[2 * (k0 + e) + 1];
+    }
+    const T cym = cy[2 * w.j], cyp = cy[2 * w.j + 1];
+    int64_t c = w.c;
+    V rm, rc;
+    __builtin_memcpy(&rm, r + c - sI, sizeof(V));
+    __builtin_memcpy(&rc, r + c, sizeof(V));
+    for (int i = w.i0; i < w.i1; i++, c += sI) {
+      V rp, rjm, rjp;
+      __builtin_memcpy(&rp, r + c + sI, sizeof(V));
+      __builtin_memcpy(&rjm, r + c - sJ, sizeof(V));
+      __builtin_memcpy(&rjp, r + c + sJ, sizeof(V));
+      uint8_t fl[CPL];
+      __builtin_memcpy(fl, flags + c, CPL);
+      T left = __shfl_up(rc.v[CPL - 1], 1, 64), right = __shfl_down(rc.v[0], 1, 64);
+      if (lane == 0) left = r[c - 1];
+      if (lane == 63 || w.kraw + CPL > m - 2) right = r[c + CPL];
+      const T cxm = cx[2 * i], cxp = cx[2 * i + 1];
+#pragma unroll
+      for (int e = 0; e < CPL; e++) {
+        const T v = cg_stencil_cell<T, CPL>(fl[e], e, rc, rm, rp, rjm.v[e], rjp.v[e], left, right, cxm, cxp, cym,
+                                            cyp, czm[e], czp[e]);
+        if ((fl[e] & FL_ACTUAL) && w.own[e]) {
+          gam += (double)rc.v[e] * (double)rc.v[e];
+          del += (double)v * (double)rc.v[e];
+        }
+      }
+      rm = rc;
+      rc = rp;
+    }
+  }
+  gam = block_sum_all(gam, lds);
+  del = block_sum_all(del, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = gam;
+    part[2 * blockIdx.x + 1] = del;
+  }
+}
+
+// KB: reduce KA's partials, decide (converged / NaN / iteration cap /
+// breakdown: the same decision in every block), then the update over the
+// interior cells.  `it` is the number of updates applied before this launch;
+// the first launch (it == 0) takes p = r without reading p_old.  x is updated
+// on odd iterations only, with both steps in the order the eager update would
+// add them, (x + a_old p_old) + a p: same bits, 16 B per cell less every other
+// iteration; k_cgcg_flush adds a pending even step at the end.
+template <typename T, typename X, int CPL>
+__global__ __launch_bounds__(256) void k_cgcg_step(int n, int l, int m, Tiles tl, int npart,
+                                                    const double *__restrict__ part, const uint8_t *__restrict__ flags,
+                                                    const T *__restrict__ cx, const T *__restrict__ cy,
+                                                    const T *__restrict__ cz, X *__restrict__ x,
+                                                    const T *__restrict__ r, T *__restrict__ rout,
+                                                    const T *__restrict__ pin, T *__restrict__ pout, DevState *st,
+                                                    int it, int maxit) {
+  __shared__ double lds[8];
+  if (st->done) return;
+  using V = CgVec<T, CPL>;
+  using VX = CgVec<X, CPL>;
   double gam = 0, del = 0;
   for (int q = threadIdx.x; q < npart; q += blockDim.x) {
     gam += part[2 * q];
@@ -328,59 +482,85 @@ __global__ __launch_bounds__(256) void k_cgcg_update(int64_t nc, int npart, cons
   }
   if (stop) return;
   const T a = (T)alpha, bt = (T)beta;
-  // x is updated on odd iterations only, with both steps in the order the
-  // eager update would add them (x + a_old p_old) + a p: same bits, 16 B per
-  // cell less every other iteration; k_cgcg_flush adds a pending even step
   const bool xup = it & 1;
+  const bool first = it == 0;
   const X aold = xup ? (X)st->s[CG_ALPHA + par] : X(0);
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  int64_t c0 = 0;  // first element of the scalar tail
-  if (sizeof(X) == sizeof(T) && ((uintptr_t)x & 15) == 0) {
-    // 16-byte vectors (2 doubles / 4 floats) over the aligned bulk (the work
-    // vectors are allocation-aligned; x is the caller's); the same
-    // per-element arithmetic as the scalar tail below
-    constexpr int V = 16 / sizeof(T);
-    struct alignas(16) Vt {
-      T v[V];
-    };
-    struct alignas(16) Vx {
-      X v[V];
-    };
-    const int64_t nv = nc / V;
-    for (int64_t q = tid; q < nv; q += stride) {
-      Vt rc = ((const Vt *)r)[q], ww = ((const Vt *)w)[q], pp, ss;
-      Vx xx;
-      if (xup) xx = ((const Vx *)x)[q];
-      if (it != 0) {
-        pp = ((const Vt *)p)[q];
-        ss = ((const Vt *)s)[q];
-      }
+  const int per = (tl.total + 7) / 8;
+  const int bx = gridDim.x >> 3;
+  const int xcd = blockIdx.x & 7;
+  const int lane = threadIdx.x & 63;
+  const int64_t sI = (int64_t)l * m, sJ = m;
+  // p = r + bt p_old at one vector (p_old taken as 0 on the first iteration)
+  auto pnew = [&](int64_t at, V &rv, V &po) {
+    V q;
+    __builtin_memcpy(&rv, r + at, sizeof(V));
+    if (first) {
 #pragma unroll
-      for (int e = 0; e < V; e++) {
-        const T pn = it == 0 ? rc.v[e] : rc.v[e] + bt * pp.v[e];
-        const T sn = it == 0 ? ww.v[e] : ww.v[e] + bt * ss.v[e];
-        if (xup) xx.v[e] = (xx.v[e] + aold * (X)pp.v[e]) + (X)a * (X)pn;
-        pp.v[e] = pn;
-        ss.v[e] = sn;
-        rc.v[e] = rc.v[e] - a * sn;
-      }
-      ((Vt *)p)[q] = pp;
-      ((Vt *)s)[q] = ss;
-      if (xup) ((Vx *)x)[q] = xx;
-      ((Vt *)r)[q] = rc;
+      for (int e = 0; e < CPL; e++) po.v[e] = T(0);
+    } else {
+      __builtin_memcpy(&po, pin + at, sizeof(V));
     }
-    c0 = nv * V;
-  }
-  for (int64_t c = c0 + tid; c < nc; c += stride) {
-    const T rc = r[c];
-    const T po = it == 0 ? T(0) : p[c];
-    const T pn = it == 0 ? rc : rc + bt * po;
-    const T sn = it == 0 ? w[c] : w[c] + bt * s[c];
-    if (xup) x[c] = (x[c] + aold * (X)po) + (X)a * (X)pn;
-    p[c] = pn;
-    s[c] = sn;
-    r[c] = rc - a * sn;
+#pragma unroll
+    for (int e = 0; e < CPL; e++) q.v[e] = rv.v[e] + bt * po.v[e];
+    return q;
+  };
+  auto pnew1 = [&](int64_t at) { return first ? r[at] : r[at] + bt * pin[at]; };
+  for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
+    const int t = xcd * per + tq;
+    if (t >= tl.total) break;
+    CgTile w;
+    if (!cg_tile<CPL>(tl, t, n, l, m, w)) continue;
+    const int k0 = w.k0;
+    T czm[CPL], czp[CPL];
+#pragma unroll
+    for (int e = 0; e < CPL; e++) {
+      czm[e] = cz[2 * (k0 + e)];
+      czp[e] = cz[2 * (k0 + e) + 1];
+    }
+    const T cym = cy[2 * w.j], cyp = cy[2 * w.j + 1];
+    int64_t c = w.c;
+    V qm, qc, rc, pc, tr, tp;
+    qm = pnew(c - sI, tr, tp);
+    qc = pnew(c, rc, pc);
+    for (int i = w.i0; i < w.i1; i++, c += sI) {
+      V rp, pp, rjm, pjm, rjp, pjp;
+      const V qp = pnew(c + sI, rp, pp);
+      const V qjm = pnew(c - sJ, rjm, pjm);
+      const V qjp = pnew(c + sJ, rjp, pjp);
+      VX xx;
+      if (xup) __builtin_memcpy(&xx, x + c, sizeof(VX));
+      uint8_t fl[CPL];
+      __builtin_memcpy(fl, flags + c, CPL);
+      T left = __shfl_up(qc.v[CPL - 1], 1, 64), right = __shfl_down(qc.v[0], 1, 64);
+      if (lane == 0) left = pnew1(c - 1);
+      if (lane == 63 || w.kraw + CPL > m - 2) right = pnew1(c + CPL);
+      const T cxm = cx[2 * i], cxp = cx[2 * i + 1];
+      V ro;
+#pragma unroll
+      for (int e = 0; e < CPL; e++) {
+        const T sv = cg_stencil_cell<T, CPL>(fl[e], e, qc, qm, qp, qjm.v[e], qjp.v[e], left, right, cxm, cxp, cym,
+                                             cyp, czm[e], czp[e]);
+        if (xup) xx.v[e] = (xx.v[e] + aold * (X)pc.v[e]) + (X)a * (X)qc.v[e];
+        ro.v[e] = rc.v[e] - a * sv;
+      }
+      if (w.allown) {
+        __builtin_memcpy(pout + c, &qc, sizeof(V));
+        __builtin_memcpy(rout + c, &ro, sizeof(V));
+        if (xup) __builtin_memcpy(x + c, &xx, sizeof(VX));
+      } else {
+#pragma unroll
+        for (int e = 0; e < CPL; e++)
+          if (w.own[e]) {
+            pout[c + e] = qc.v[e];
+            rout[c + e] = ro.v[e];
+            if (xup) x[c + e] = xx.v[e];
+          }
+      }
+      qm = qc;
+      qc = qp;
+      rc = rp;
+      pc = pp;
+    }
   }
 }
 
@@ -534,27 +714,48 @@ static int cgcg_blocks(const Tiles &tl) {
   return nb > kCgTileBlocks ? kCgTileBlocks : nb;
 }
 
-// Runs single-reduction CG iterations on A e = r from p = r (r zero-padded,
-// x the FP64 or FP32 accumulator, w and s scratch).  State must hold
-// HS_STOP2 and done = 0 (k_hs_start).  Polls the device state every `every`
-// iterations; h_state->iter counts the updates applied.
-template <typename T, typename X>
-static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r, T *w, T *p, T *s, const T *cx, const T *cy,
-                            const T *cz, int maxit, int every) {
-  CgWork *wk = g->work;
-  constexpr int CPL = 16 / sizeof(T);  // one 16-byte vector per lane and row
-  const bool vec = g->m - 2 >= 64 * CPL;
-  Tiles tl = tiles(g);
-  if (vec) {
-    tl.kt = (g->m - 2 + 64 * CPL - 1) / (64 * CPL);
-    tl.total = tl.kt * tl.jt * tl.it;
-    tl.blocks = 8 * ((tl.total + 7) / 8);
+// Planes per tile of the two plain-CG kernels: the deepest tile (at most
+// kCgMaxPlanes) that still leaves >= kCgMinTiles tiles, so a launch keeps 4096
+// waves (the step kernel's 4 waves per SIMD on 256 CUs).  Timing lab at 256^3
+// (a solve's dots/step sequence): FP64 32 planes for both, 56 + 180 us
+// (16/8 planes: 60 + 195 us); FP32 16 planes, 32 + 90 us.
+template <typename T>
+struct CgShape;
+template <>
+struct CgShape<double> {
+  static constexpr int max_planes = 32;
+};
+template <>
+struct CgShape<float> {
+  static constexpr int max_planes = 16;
+};
+constexpr int kCgMinTiles = 1024;
+constexpr int kCgStepBlocks = 4096;  // KB blocks (persistent over tiles)
+
+static Tiles march_tiles(xf_grid g, int cpl, int max_planes) {
+  Tiles t;
+  t.kt = (g->m - 2 + 64 * cpl - 1) / (64 * cpl);
+  t.jt = (g->l - 2 + TJ - 1) / TJ;
+  for (t.ic = max_planes;; t.ic /= 2) {
+    t.it = (g->n - 2 + t.ic - 1) / t.ic;
+    t.total = t.kt * t.jt * t.it;
+    if (t.total >= kCgMinTiles || t.ic <= 2) break;
   }
-  const int nbA = cgcg_blocks(tl);
-  const int nbB = std::max(1, std::min(kCgUpdateBlocks, (int)((g->ncell + 255) / 256)));
+  t.blocks = 8 * ((t.total + 7) / 8);
+  return t;
+}
+
+template <typename T, typename X, int CPL>
+static void cgcg_run(xf_ctx c, xf_grid g, X *x, T *const R[2], T *const P[2], const T *cx, const T *cy,
+                     const T *cz, int maxit, int every) {
+  CgWork *wk = g->work;
+  const Tiles ta = march_tiles(g, CPL, CgShape<T>::max_planes);
+  const Tiles &tb = ta;
+  const int nbA = std::min(ta.blocks, kCgTileBlocks);
+  const int nbB = std::min(tb.blocks, kCgStepBlocks);
   const bool f64 = sizeof(T) == 8;
-  const char *n1 = f64 ? "cg_stencil_fp64" : "cg_stencil_fp32";
-  const char *n2 = f64 ? "cg_update_fp64" : "cg_update_fp32";
+  const char *n1 = f64 ? "cg_dots_fp64" : "cg_dots_fp32";
+  const char *n2 = f64 ? "cg_step_fp64" : "cg_step_fp32";
   // per-kernel events on two of every kProfEvery iterations only (markers
   // between kernels cost a few microseconds each)
   constexpr int kProfEvery = 8;
@@ -564,20 +765,18 @@ static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r, T *w, T *p, T *s, c
   for (;;) {
     for (int q = 0; q < every; q++, it++) {
       const bool pm = c->prof && it % kProfEvery < 2;  // an even and an odd update
+      T *rin = R[it & 1], *rout = R[(it + 1) & 1];
+      T *pin = P[it & 1], *pout = P[(it + 1) & 1];
       int e1 = pm ? prof_begin(c) : -1;
-      if (vec)
-        hipLaunchKernelGGL((k_cgcg_stencil_vec<T, CPL>), dim3(nbA), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
-                           g->d_flags, cx, cy, cz, r, w, c->state, wk->part);
-      else
-        hipLaunchKernelGGL((k_cgcg_stencil<T>), dim3(nbA), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
-                           g->d_flags, cx, cy, cz, r, w, c->state, wk->part);
+      hipLaunchKernelGGL((k_cgcg_dots<T, CPL>), dim3(nbA), dim3(256), 0, c->stream, g->n, g->l, g->m, ta,
+                         g->d_flags, cx, cy, cz, rin, c->state, wk->part);
       if (pm) {
         prof_end(c, e1);
         marks.push_back({n1, e1});
       }
       int e2 = pm ? prof_begin(c) : -1;
-      hipLaunchKernelGGL((k_cgcg_update<T, X>), dim3(nbB), dim3(256), 0, c->stream, g->ncell, nbA, wk->part, x, r,
-                         w, p, s, c->state, it, maxit);
+      hipLaunchKernelGGL((k_cgcg_step<T, X, CPL>), dim3(nbB), dim3(256), 0, c->stream, g->n, g->l, g->m, tb,
+                         nbA, wk->part, g->d_flags, cx, cy, cz, x, rin, rout, pin, pout, c->state, it, maxit);
       if (pm) {
         prof_end(c, e2);
         marks.push_back({n2, e2});
@@ -588,8 +787,8 @@ static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r, T *w, T *p, T *s, c
     XF_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
     XF_HIP(hipStreamSynchronize(c->stream));
     if (c->prof) {
-      // every stencil up to the deciding one ran in full; the deciding update
-      // launch only reduced, so only the updates before it are counted
+      // every KA up to the deciding one ran in full; the deciding KB launch
+      // only reduced, so only the updates before it are counted
       const long long ran = c->h_state->iter, done = c->h_state->done ? 1 : 0;
       long long na = 0, nb = 0;
       for (int v : sampled) {
@@ -601,9 +800,30 @@ static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r, T *w, T *p, T *s, c
     }
     if (c->h_state->done) break;
   }
-  if (c->h_state->iter & 1)
-    hipLaunchKernelGGL((k_cgcg_flush<T, X>), dim3(nbB), dim3(256), 0, c->stream, g->ncell, x, p, c->state);
+  // update index applied-1 wrote its direction into P[applied & 1]
+  const int applied = c->h_state->iter;
+  if (applied & 1)
+    hipLaunchKernelGGL((k_cgcg_flush<T, X>), dim3(stream_nb(g->ncell)), dim3(256), 0, c->stream, g->ncell, x,
+                       P[applied & 1], c->state);
   check_kernel("cg flush");
+}
+
+// Runs single-reduction CG iterations on A e = r from p = r (r zero-padded,
+// in r0; x the FP64 or FP32 accumulator; r1, p0, p1 work vectors).  State
+// must hold HS_STOP2 and done = 0 (k_hs_start).  Polls the device state every
+// `every` iterations; h_state->iter counts the updates applied.  The final
+// residual is left in r0 or r1 (by the parity of the update count).
+template <typename T, typename X>
+static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r0, T *r1, T *p0, T *p1, const T *cx, const T *cy,
+                            const T *cz, int maxit, int every) {
+  T *const R[2] = {r0, r1};
+  T *const P[2] = {p0, p1};
+  // 16-byte vectors per lane where a row holds a whole wave of them
+  if constexpr (sizeof(T) == 4) {
+    if (g->m - 2 >= 64 * 4) return cgcg_run<T, X, 4>(c, g, x, R, P, cx, cy, cz, maxit, every);
+  }
+  if (g->m - 2 >= 64 * 2) return cgcg_run<T, X, 2>(c, g, x, R, P, cx, cy, cz, maxit, every);
+  cgcg_run<T, X, 1>(c, g, x, R, P, cx, cy, cz, maxit, every);
 }
 
 static void upload_state_zero(xf_ctx c) {
