@@ -123,6 +123,10 @@ typedef struct {
   int stall_window;  /* XF_FCG_NN: fall back to plain CG after this many
                         iterations without progress (20)                         */
   void *precond;     /* XF_FCG_NN, XF_FCG_MGNN: an xf_unet handle               */
+  int verify;        /* 1 (default): after the solve, recompute
+                        ||b - A x|| / ||b - A x0|| with the reference rows into
+                        stats.resid_true (two operator passes and a temporary
+                        field); 0: skip that check, resid_true = -1               */
 } xf_solve_opts;
 
 typedef struct {

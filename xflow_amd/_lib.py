@@ -20,7 +20,7 @@ STATUS_NAMES = {0: "ok", -1: "invalid", -2: "max-iterations", -3: "breakdown", -
 class SolveOpts(C.Structure):
     _fields_ = [("method", C.c_int), ("tol_mode", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
                 ("check_every", C.c_int), ("project_mean", C.c_int), ("inner_tol", C.c_double),
-                ("stall_window", C.c_int), ("precond", C.c_void_p)]
+                ("stall_window", C.c_int), ("precond", C.c_void_p), ("verify", C.c_int)]
 
 
 class SolveStats(C.Structure):

@@ -126,158 +126,6 @@ __device__ inline double block_sum_all(double v, double *lds) {
   return r;
 }
 
-// KA: w = A r over the interior, per-block partials (gamma, delta) =
-// ((r, r), (w, r)) over Actual cells.  Blocks are persistent over tiles
-// (bounded partial count), consecutive tiles on one XCD.
-template <typename T>
-__global__ __launch_bounds__(256) void k_cgcg_stencil(int n, int l, int m, Tiles tl,
-                                                       const uint8_t *__restrict__ flags, const T *__restrict__ cx,
-                                                       const T *__restrict__ cy, const T *__restrict__ cz,
-                                                       const T *__restrict__ r, T *__restrict__ w,
-                                                       const DevState *st, double *__restrict__ part) {
-  __shared__ double lds[8];
-  if (st->done) return;
-  const int per = (tl.total + 7) / 8;      // tiles per XCD
-  const int bx = gridDim.x >> 3;            // blocks per XCD
-  const int xcd = blockIdx.x & 7;
-  double gam = 0, del = 0;
-  const int64_t sI = (int64_t)l * m, sJ = m;
-  for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
-    const int t = xcd * per + tq;
-    if (t >= tl.total) break;
-    const int ktile = t % tl.kt;
-    const int jtile = (t / tl.kt) % tl.jt;
-    const int itile = t / (tl.kt * tl.jt);
-    const int k = 1 + ktile * TK + (threadIdx.x & 63);
-    const int j = 1 + jtile * TJ + (threadIdx.x >> 6);
-    const int i0 = 1 + itile * IC;
-    const int i1 = min(i0 + IC, n - 1);
-    if (k > m - 2 || j > l - 2) continue;
-    const T czm = cz[2 * k], czp = cz[2 * k + 1];
-    const T cym = cy[2 * j], cyp = cy[2 * j + 1];
-    int64_t c = ((int64_t)i0 * l + j) * m + k;
-    T rm = r[c - sI], rc = r[c];
-    for (int i = i0; i < i1; i++, c += sI) {
-      const T rp = r[c + sI];
-      const uint8_t fl = flags[c];
-      T v = 0;
-      if (fl & FL_ACTUAL) {
-        const T cxm = cx[2 * i], cxp = cx[2 * i + 1];
-        const T rjm = r[c - sJ], rjp = r[c + sJ], rkm = r[c - 1], rkp = r[c + 1];
-        T tx = ((fl & FL_IP) ? cxp * (rp - rc) : T(0)) - ((fl & FL_IM) ? cxm * (rc - rm) : T(0));
-        T ty = ((fl & FL_JP) ? cyp * (rjp - rc) : T(0)) - ((fl & FL_JM) ? cym * (rc - rjm) : T(0));
-        T tz = ((fl & FL_KP) ? czp * (rkp - rc) : T(0)) - ((fl & FL_KM) ? czm * (rc - rkm) : T(0));
-        v = -(tx + ty + tz);
-        gam += (double)rc * (double)rc;
-        del += (double)v * (double)rc;
-      }
-      w[c] = v;
-      rm = rc;
-      rc = rp;
-    }
-  }
-  gam = block_sum_all(gam, lds);
-  del = block_sum_all(del, lds);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = gam;
-    part[2 * blockIdx.x + 1] = del;
-  }
-}
-
-// KA with CPL consecutive k-cells per lane (one 16-byte load per lane and
-// row: 2 doubles or 4 floats), k-neighbours across lanes by shuffles.  A wave
-// covers 64*CPL cells of a row; the last k-tile of a row is shifted back to end
-// at m-2, so every vector lies inside the row; cells it shares with the tile
-// before are written twice with the same value and counted once (only from
-// the tile's nominal start).  Requires m - 2 >= 64*CPL.
-template <typename T, int CPL>
-struct VecT {
-  T v[CPL];
-};
-
-template <typename T, int CPL>
-__global__ __launch_bounds__(256) void k_cgcg_stencil_vec(int n, int l, int m, Tiles tl,
-                                                           const uint8_t *__restrict__ flags,
-                                                           const T *__restrict__ cx, const T *__restrict__ cy,
-                                                           const T *__restrict__ cz, const T *__restrict__ r,
-                                                           T *__restrict__ w, const DevState *st,
-                                                           double *__restrict__ part) {
-  __shared__ double lds[8];
-  if (st->done) return;
-  using V = VecT<T, CPL>;
-  const int per = (tl.total + 7) / 8;
-  const int bx = gridDim.x >> 3;
-  const int xcd = blockIdx.x & 7;
-  const int lane = threadIdx.x & 63;
-  const int kspan = 64 * CPL;
-  double gam = 0, del = 0;
-  const int64_t sI = (int64_t)l * m, sJ = m;
-  for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
-    const int t = xcd * per + tq;
-    if (t >= tl.total) break;
-    const int ktile = t % tl.kt;
-    const int jtile = (t / tl.kt) % tl.jt;
-    const int itile = t / (tl.kt * tl.jt);
-    const int kcount = 1 + ktile * kspan;  // cells from here on are this tile's own
-    const int kstart = min(kcount, m - 1 - kspan);
-    const int k0 = kstart + lane * CPL;
-    const int j = 1 + jtile * TJ + (threadIdx.x >> 6);
-    const int i0 = 1 + itile * IC;
-    const int i1 = min(i0 + IC, n - 1);
-    if (j > l - 2) continue;
-    T czm[CPL], czp[CPL];
-    bool own[CPL];
-#pragma unroll
-    for (int e = 0; e < CPL; e++) {
-      czm[e] = cz[2 * (k0 + e)];
-      czp[e] = cz[2 * (k0 + e) + 1];
-      own[e] = k0 + e >= kcount;
-    }
-    const T cym = cy[2 * j], cyp = cy[2 * j + 1];
-    int64_t c = ((int64_t)i0 * l + j) * m + k0;
-    V rm, rc;
-    __builtin_memcpy(&rm, r + c - sI, sizeof(V));
-    __builtin_memcpy(&rc, r + c, sizeof(V));
-    for (int i = i0; i < i1; i++, c += sI) {
-      V rp, rjm, rjp, out;
-      __builtin_memcpy(&rp, r + c + sI, sizeof(V));
-      __builtin_memcpy(&rjm, r + c - sJ, sizeof(V));
-      __builtin_memcpy(&rjp, r + c + sJ, sizeof(V));
-      uint8_t fl[CPL];
-      __builtin_memcpy(fl, flags + c, CPL);
-      T left = __shfl_up(rc.v[CPL - 1], 1, 64), right = __shfl_down(rc.v[0], 1, 64);
-      if (lane == 0) left = r[c - 1];
-      if (lane == 63) right = r[c + CPL];
-      const T cxm = cx[2 * i], cxp = cx[2 * i + 1];
-#pragma unroll
-      for (int e = 0; e < CPL; e++) {
-        const uint8_t f = fl[e];
-        const T x0 = rc.v[e];
-        const T km = e == 0 ? left : rc.v[e - 1];
-        const T kp = e == CPL - 1 ? right : rc.v[e + 1];
-        T tx = ((f & FL_IP) ? cxp * (rp.v[e] - x0) : T(0)) - ((f & FL_IM) ? cxm * (x0 - rm.v[e]) : T(0));
-        T ty = ((f & FL_JP) ? cyp * (rjp.v[e] - x0) : T(0)) - ((f & FL_JM) ? cym * (x0 - rjm.v[e]) : T(0));
-        T tz = ((f & FL_KP) ? czp[e] * (kp - x0) : T(0)) - ((f & FL_KM) ? czm[e] * (x0 - km) : T(0));
-        const T v = (f & FL_ACTUAL) ? -(tx + ty + tz) : T(0);
-        out.v[e] = v;
-        if ((f & FL_ACTUAL) && own[e]) {
-          gam += (double)x0 * (double)x0;
-          del += (double)v * (double)x0;
-        }
-      }
-      __builtin_memcpy(w + c, &out, sizeof(V));
-      rm = rc;
-      rc = rp;
-    }
-  }
-  gam = block_sum_all(gam, lds);
-  del = block_sum_all(del, lds);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = gam;
-    part[2 * blockIdx.x + 1] = del;
-  }
-}
-
 // ------------------------------------------------ plain CG: two kernels per iteration
 // The stored vectors are r and p (each ping-ponged between two buffers, so a
 // kernel never overwrites what another block still reads as a neighbour) and
@@ -708,12 +556,6 @@ static int reduce_nb(int64_t nc) {
   return nb > 2048 ? 2048 : nb;
 }
 
-// KA tiling for the persistent stencil: at most kCgTileBlocks blocks, a multiple of 8.
-static int cgcg_blocks(const Tiles &tl) {
-  int nb = 8 * ((tl.total + 7) / 8);
-  return nb > kCgTileBlocks ? kCgTileBlocks : nb;
-}
-
 // Planes per tile of the two plain-CG kernels: the deepest tile (at most
 // kCgMaxPlanes) that still leaves >= kCgMinTiles tiles, so a launch keeps 4096
 // waves (the step kernel's 4 waves per SIMD on 256 CUs).  Timing lab at 256^3
@@ -950,90 +792,265 @@ enum { FC_PR = 20, FC_ZQ, FC_RR2, FC_SUMSQ };
 // one partial (pair) per block, a one-block finisher adds them in block order
 // and applies the scalar step (no device-scope fences in the streaming
 // passes).
-constexpr int kFcgBlocks = 2048;
 
-// partials of (a, b) (second of the pair 0)
-__global__ __launch_bounds__(256) void k_dot_part(int64_t nc, const float *__restrict__ a, const float *__restrict__ b,
-                                                  const DevState *st, double *__restrict__ part) {
-  __shared__ double lds[16];
-  if (st->done) return;
-  double acc = 0;
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
-    acc += (double)a[c] * (double)b[c];
-  const double bs = block_sum(acc, lds);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = bs;
-    part[2 * blockIdx.x + 1] = 0;
-  }
-}
-
-// p = z + beta p, beta = -(z, q_old)/(p_old, q_old) (first: p = z), and
-// partials of (p, r)
-__global__ __launch_bounds__(256) void k_fcg_dir(int64_t nc, const float *__restrict__ z, float *__restrict__ p,
-                                                 const float *__restrict__ r, const DevState *st, int first,
+// The step's three passes are marching stencil kernels over the interior (the
+// plain-CG tiles), q = A p is never stored: each pass that needs A p (or A
+// p_old) forms it from p at the seven stencil points, with the same operations
+// as everywhere else, so every value of A p is bit-identical between passes.
+// p is ping-ponged (the kernel writing p_new never overwrites a p_old a
+// neighbouring block still reads).  FP32 bytes per interior cell: K1 9, K2 17,
+// K3 21 (the former four streaming passes with q stored moved 57).
+//   K1 k_fcg_zq      partials of (z, A p_old)                    [reads z, p_old]
+//   K2 k_fcg_dir_pq  beta = -(z, A p_old)/(p_old, A p_old) (first: 0),
+//                    p = z + beta p_old, partials of (p, r), (p, A p)
+//                                                  [reads z, p_old, r; writes p]
+//   K3 k_fcg_upd     e += alpha p, r -= alpha A p, partials of (r, r)
+//                                                  [reads e, p, r; writes e, r]
+template <int CPL>
+__global__ __launch_bounds__(256) void k_fcg_zq(int n, int l, int m, Tiles tl, const uint8_t *__restrict__ flags,
+                                                 const float *__restrict__ cx, const float *__restrict__ cy,
+                                                 const float *__restrict__ cz, const float *__restrict__ z,
+                                                 const float *__restrict__ p, const DevState *st,
                                                  double *__restrict__ part) {
-  __shared__ double lds[16];
+  __shared__ double lds[8];
   if (st->done) return;
+  using V = CgVec<float, CPL>;
+  const int per = (tl.total + 7) / 8;
+  const int bx = gridDim.x >> 3;
+  const int xcd = blockIdx.x & 7;
+  const int lane = threadIdx.x & 63;
+  double acc = 0;
+  const int64_t sI = (int64_t)l * m, sJ = m;
+  for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
+    const int t = xcd * per + tq;
+    if (t >= tl.total) break;
+    CgTile w;
+    if (!cg_tile<CPL>(tl, t, n, l, m, w)) continue;
+    float czm[CPL], czp[CPL];
+#pragma unroll
+    for (int e = 0; e < CPL; e++) {
+      czm[e] = cz[2 * (w.k0 + e)];
+      czp[e] = cz[2 * (w.k0 + e) + 1];
+    }
+    const float cym = cy[2 * w.j], cyp = cy[2 * w.j + 1];
+    int64_t c = w.c;
+    V pm, pc;
+    __builtin_memcpy(&pm, p + c - sI, sizeof(V));
+    __builtin_memcpy(&pc, p + c, sizeof(V));
+    for (int i = w.i0; i < w.i1; i++, c += sI) {
+      V pp, pjm, pjp, zc;
+      __builtin_memcpy(&pp, p + c + sI, sizeof(V));
+      __builtin_memcpy(&pjm, p + c - sJ, sizeof(V));
+      __builtin_memcpy(&pjp, p + c + sJ, sizeof(V));
+      __builtin_memcpy(&zc, z + c, sizeof(V));
+      uint8_t fl[CPL];
+      __builtin_memcpy(fl, flags + c, CPL);
+      float left = __shfl_up(pc.v[CPL - 1], 1, 64), right = __shfl_down(pc.v[0], 1, 64);
+      if (lane == 0) left = p[c - 1];
+      if (lane == 63 || w.kraw + CPL > m - 2) right = p[c + CPL];
+      const float cxm = cx[2 * i], cxp = cx[2 * i + 1];
+#pragma unroll
+      for (int e = 0; e < CPL; e++) {
+        const float q = cg_stencil_cell<float, CPL>(fl[e], e, pc, pm, pp, pjm.v[e], pjp.v[e], left, right, cxm, cxp,
+                                                    cym, cyp, czm[e], czp[e]);
+        if (w.own[e]) acc += (double)zc.v[e] * (double)q;
+      }
+      pm = pc;
+      pc = pp;
+    }
+  }
+  acc = block_sum_all(acc, lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = acc;
+    part[2 * blockIdx.x + 1] = 0;
+  }
+}
+
+template <int CPL>
+__global__ __launch_bounds__(256) void k_fcg_dir_pq(int n, int l, int m, Tiles tl, const uint8_t *__restrict__ flags,
+                                                     const float *__restrict__ cx, const float *__restrict__ cy,
+                                                     const float *__restrict__ cz, const float *__restrict__ z,
+                                                     const float *__restrict__ pin, float *__restrict__ pout,
+                                                     const float *__restrict__ r, const DevState *st, int first,
+                                                     double *__restrict__ part) {
+  __shared__ double lds[8];
+  if (st->done) return;
+  using V = CgVec<float, CPL>;
   const double pq = st->s[HS_PQ];
-  const float beta = first || pq == 0 ? 0.0f : (float)(-st->s[FC_ZQ] / pq);
-  double acc = 0;
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
-    const float pn = first ? z[c] : z[c] + beta * p[c];
-    p[c] = pn;
-    acc += (double)pn * (double)r[c];
+  const bool nob = first || pq == 0;
+  const float beta = nob ? 0.0f : (float)(-st->s[FC_ZQ] / pq);
+  const int per = (tl.total + 7) / 8;
+  const int bx = gridDim.x >> 3;
+  const int xcd = blockIdx.x & 7;
+  const int lane = threadIdx.x & 63;
+  double apr = 0, apq = 0;
+  const int64_t sI = (int64_t)l * m, sJ = m;
+  // p = z + beta p_old at one vector (p = z when there is no beta)
+  auto pnew = [&](int64_t at) {
+    V zv, q;
+    __builtin_memcpy(&zv, z + at, sizeof(V));
+    if (nob) return zv;
+    V po;
+    __builtin_memcpy(&po, pin + at, sizeof(V));
+#pragma unroll
+    for (int e = 0; e < CPL; e++) q.v[e] = zv.v[e] + beta * po.v[e];
+    return q;
+  };
+  auto pnew1 = [&](int64_t at) { return nob ? z[at] : z[at] + beta * pin[at]; };
+  for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
+    const int t = xcd * per + tq;
+    if (t >= tl.total) break;
+    CgTile w;
+    if (!cg_tile<CPL>(tl, t, n, l, m, w)) continue;
+    float czm[CPL], czp[CPL];
+#pragma unroll
+    for (int e = 0; e < CPL; e++) {
+      czm[e] = cz[2 * (w.k0 + e)];
+      czp[e] = cz[2 * (w.k0 + e) + 1];
+    }
+    const float cym = cy[2 * w.j], cyp = cy[2 * w.j + 1];
+    int64_t c = w.c;
+    V qm = pnew(c - sI), qc = pnew(c);
+    for (int i = w.i0; i < w.i1; i++, c += sI) {
+      const V qp = pnew(c + sI), qjm = pnew(c - sJ), qjp = pnew(c + sJ);
+      V rc;
+      __builtin_memcpy(&rc, r + c, sizeof(V));
+      uint8_t fl[CPL];
+      __builtin_memcpy(fl, flags + c, CPL);
+      float left = __shfl_up(qc.v[CPL - 1], 1, 64), right = __shfl_down(qc.v[0], 1, 64);
+      if (lane == 0) left = pnew1(c - 1);
+      if (lane == 63 || w.kraw + CPL > m - 2) right = pnew1(c + CPL);
+      const float cxm = cx[2 * i], cxp = cx[2 * i + 1];
+#pragma unroll
+      for (int e = 0; e < CPL; e++) {
+        const float q = cg_stencil_cell<float, CPL>(fl[e], e, qc, qm, qp, qjm.v[e], qjp.v[e], left, right, cxm, cxp,
+                                                    cym, cyp, czm[e], czp[e]);
+        if (w.own[e]) {
+          apr += (double)qc.v[e] * (double)rc.v[e];
+          apq += (double)q * (double)qc.v[e];
+        }
+      }
+      if (w.allown) {
+        __builtin_memcpy(pout + c, &qc, sizeof(V));
+      } else {
+#pragma unroll
+        for (int e = 0; e < CPL; e++)
+          if (w.own[e]) pout[c + e] = qc.v[e];
+      }
+      qm = qc;
+      qc = qp;
+    }
   }
-  const double bs = block_sum(acc, lds);
+  apr = block_sum_all(apr, lds);
+  apq = block_sum_all(apq, lds);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = bs;
-    part[2 * blockIdx.x + 1] = 0;
+    part[2 * blockIdx.x] = apr;
+    part[2 * blockIdx.x + 1] = apq;
   }
 }
 
-// e += alpha p, r -= alpha q and partials of (r, r)
-__global__ __launch_bounds__(256) void k_fcg_update(int64_t nc, float *__restrict__ e, const float *__restrict__ p,
-                                                    float *__restrict__ r, const float *__restrict__ q,
-                                                    const DevState *st, double *__restrict__ part) {
-  __shared__ double lds[16];
+template <int CPL>
+__global__ __launch_bounds__(256) void k_fcg_upd(int n, int l, int m, Tiles tl, const uint8_t *__restrict__ flags,
+                                                  const float *__restrict__ cx, const float *__restrict__ cy,
+                                                  const float *__restrict__ cz, float *__restrict__ ev,
+                                                  const float *__restrict__ p, float *__restrict__ r,
+                                                  const DevState *st, double *__restrict__ part) {
+  __shared__ double lds[8];
   if (st->done) return;
+  using V = CgVec<float, CPL>;
   const float a = (float)st->s[HS_ALPHA];
+  const int per = (tl.total + 7) / 8;
+  const int bx = gridDim.x >> 3;
+  const int xcd = blockIdx.x & 7;
+  const int lane = threadIdx.x & 63;
   double acc = 0;
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
-    const float rn = r[c] - a * q[c];
-    r[c] = rn;
-    e[c] = e[c] + a * p[c];
-    acc += (double)rn * (double)rn;
+  const int64_t sI = (int64_t)l * m, sJ = m;
+  for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
+    const int t = xcd * per + tq;
+    if (t >= tl.total) break;
+    CgTile w;
+    if (!cg_tile<CPL>(tl, t, n, l, m, w)) continue;
+    float czm[CPL], czp[CPL];
+#pragma unroll
+    for (int e = 0; e < CPL; e++) {
+      czm[e] = cz[2 * (w.k0 + e)];
+      czp[e] = cz[2 * (w.k0 + e) + 1];
+    }
+    const float cym = cy[2 * w.j], cyp = cy[2 * w.j + 1];
+    int64_t c = w.c;
+    V pm, pc;
+    __builtin_memcpy(&pm, p + c - sI, sizeof(V));
+    __builtin_memcpy(&pc, p + c, sizeof(V));
+    for (int i = w.i0; i < w.i1; i++, c += sI) {
+      V pp, pjm, pjp, rc, ec;
+      __builtin_memcpy(&pp, p + c + sI, sizeof(V));
+      __builtin_memcpy(&pjm, p + c - sJ, sizeof(V));
+      __builtin_memcpy(&pjp, p + c + sJ, sizeof(V));
+      __builtin_memcpy(&rc, r + c, sizeof(V));
+      __builtin_memcpy(&ec, ev + c, sizeof(V));
+      uint8_t fl[CPL];
+      __builtin_memcpy(fl, flags + c, CPL);
+      float left = __shfl_up(pc.v[CPL - 1], 1, 64), right = __shfl_down(pc.v[0], 1, 64);
+      if (lane == 0) left = p[c - 1];
+      if (lane == 63 || w.kraw + CPL > m - 2) right = p[c + CPL];
+      const float cxm = cx[2 * i], cxp = cx[2 * i + 1];
+#pragma unroll
+      for (int e = 0; e < CPL; e++) {
+        const float q = cg_stencil_cell<float, CPL>(fl[e], e, pc, pm, pp, pjm.v[e], pjp.v[e], left, right, cxm, cxp,
+                                                    cym, cyp, czm[e], czp[e]);
+        rc.v[e] = rc.v[e] - a * q;
+        ec.v[e] = ec.v[e] + a * pc.v[e];
+        if (w.own[e]) acc += (double)rc.v[e] * (double)rc.v[e];
+      }
+      if (w.allown) {
+        __builtin_memcpy(r + c, &rc, sizeof(V));
+        __builtin_memcpy(ev + c, &ec, sizeof(V));
+      } else {
+#pragma unroll
+        for (int e = 0; e < CPL; e++)
+          if (w.own[e]) {
+            r[c + e] = rc.v[e];
+            ev[c + e] = ec.v[e];
+          }
+      }
+      pm = pc;
+      pc = pp;
+    }
   }
-  const double bs = block_sum(acc, lds);
+  acc = block_sum_all(acc, lds);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = bs;
+    part[2 * blockIdx.x] = acc;
     part[2 * blockIdx.x + 1] = 0;
   }
 }
 
-// Finishers (one block): ZQ: s[FC_ZQ] = sum; PR: s[FC_PR] = sum; PQ: s[HS_PQ]
-// = sum of the second partials (the stencil's (A p, p)), alpha = (p, r)/(p, q)
+// Finishers (one block): ZQ: s[FC_ZQ] = sum; PRPQ: s[FC_PR] = sum of the first
+// partials (p, r), s[HS_PQ] = sum of the second (p, A p), alpha = (p, r)/(p, A p)
 // or stop on 0 / NaN; RR: s[HS_RR] = sum, iteration count and the stopping
 // test.
-enum { FIN_ZQ, FIN_PR, FIN_PQ, FIN_RR };
+enum { FIN_ZQ, FIN_PRPQ, FIN_RR };
 template <int MODE>
 __global__ void k_fcg_finish(const double *__restrict__ part, int np, DevState *st, int maxit) {
   __shared__ double lds[16];
   if (st->done) return;
-  double s0 = 0;
-  for (int q = threadIdx.x; q < np; q += blockDim.x) s0 += part[2 * q + (MODE == FIN_PQ ? 1 : 0)];
+  double s0 = 0, s1 = 0;
+  for (int q = threadIdx.x; q < np; q += blockDim.x) {
+    s0 += part[2 * q];
+    if (MODE == FIN_PRPQ) s1 += part[2 * q + 1];
+  }
   const double tot = block_sum(s0, lds);
+  const double tot1 = MODE == FIN_PRPQ ? block_sum(s1, lds) : 0.0;
   if (threadIdx.x != 0) return;
   if (MODE == FIN_ZQ) {
     st->s[FC_ZQ] = tot;
-  } else if (MODE == FIN_PR) {
+  } else if (MODE == FIN_PRPQ) {
     st->s[FC_PR] = tot;
-  } else if (MODE == FIN_PQ) {
-    st->s[HS_PQ] = tot;
-    if (tot == 0 || tot != tot) {
+    st->s[HS_PQ] = tot1;
+    if (tot1 == 0 || tot1 != tot1 || tot != tot) {
       st->done = 1;
-      if (tot != tot) st->status = XF_ENAN;
+      if (tot1 != tot1 || tot != tot) st->status = XF_ENAN;
     } else {
-      st->s[HS_ALPHA] = st->s[FC_PR] / tot;
+      st->s[HS_ALPHA] = tot / tot1;
     }
   } else {
     st->s[HS_BETA] = tot / st->s[HS_RR];
@@ -1051,25 +1068,53 @@ __global__ void k_fcg_finish(const double *__restrict__ part, int np, DevState *
   }
 }
 
-// q = A p over the interior with per-block partials ((p, p), (A p, p)) in
-// part (the plain solver's stencil, p in the role of r); returns the partial count.
-static int stencil_pq(xf_ctx c, xf_grid g, const float *p, float *q, double *part) {
-  constexpr int CPL = 4;
-  const bool vec = g->m - 2 >= 64 * CPL;
-  Tiles tl = tiles(g);
-  if (vec) {
-    tl.kt = (g->m - 2 + 64 * CPL - 1) / (64 * CPL);
-    tl.total = tl.kt * tl.jt * tl.it;
-    tl.blocks = 8 * ((tl.total + 7) / 8);
-  }
-  const int nb = cgcg_blocks(tl);
-  if (vec)
-    hipLaunchKernelGGL((k_cgcg_stencil_vec<float, CPL>), dim3(nb), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
-                       g->d_flags, g->d_cxf, g->d_cyf, g->d_czf, p, q, c->state, part);
-  else
-    hipLaunchKernelGGL((k_cgcg_stencil<float>), dim3(nb), dim3(256), 0, c->stream, g->n, g->l, g->m, tl, g->d_flags,
-                       g->d_cxf, g->d_cyf, g->d_czf, p, q, c->state, part);
-  return nb;
+// Launches of the flexible-CG step on a grid: tiles and blocks of the marching
+// kernels (16-byte vectors where a row holds a whole wave of them).
+struct FcgLaunch {
+  int cpl;
+  Tiles tl;
+  int nb;
+};
+static FcgLaunch fcg_launch(xf_grid g) {
+  FcgLaunch f;
+  f.cpl = g->m - 2 >= 64 * 4 ? 4 : (g->m - 2 >= 64 * 2 ? 2 : 1);
+  f.tl = march_tiles(g, f.cpl, CgShape<float>::max_planes);
+  f.nb = std::min(f.tl.blocks, kCgTileBlocks);
+  return f;
+}
+
+// K1: partials of (z, A p_old)
+static void fcg_zq(xf_ctx c, xf_grid g, const FcgLaunch &f, const float *z, const float *p, double *part) {
+#define XF_FCG_ZQ(CPL)                                                                                          \
+  hipLaunchKernelGGL((k_fcg_zq<CPL>), dim3(f.nb), dim3(256), 0, c->stream, g->n, g->l, g->m, f.tl, g->d_flags,     \
+                     g->d_cxf, g->d_cyf, g->d_czf, z, p, c->state, part)
+  if (f.cpl == 4) XF_FCG_ZQ(4);
+  else if (f.cpl == 2) XF_FCG_ZQ(2);
+  else XF_FCG_ZQ(1);
+#undef XF_FCG_ZQ
+}
+
+// K2: p_new = z + beta p_old, partials of (p_new, r) and (p_new, A p_new)
+static void fcg_dir_pq(xf_ctx c, xf_grid g, const FcgLaunch &f, const float *z, const float *pin, float *pout,
+                       const float *r, int first, double *part) {
+#define XF_FCG_DIR(CPL)                                                                                         \
+  hipLaunchKernelGGL((k_fcg_dir_pq<CPL>), dim3(f.nb), dim3(256), 0, c->stream, g->n, g->l, g->m, f.tl, g->d_flags, \
+                     g->d_cxf, g->d_cyf, g->d_czf, z, pin, pout, r, c->state, first, part)
+  if (f.cpl == 4) XF_FCG_DIR(4);
+  else if (f.cpl == 2) XF_FCG_DIR(2);
+  else XF_FCG_DIR(1);
+#undef XF_FCG_DIR
+}
+
+// K3: e += alpha p, r -= alpha A p, partials of (r, r)
+static void fcg_upd(xf_ctx c, xf_grid g, const FcgLaunch &f, float *e, const float *p, float *r, double *part) {
+#define XF_FCG_UPD(CPL)                                                                                         \
+  hipLaunchKernelGGL((k_fcg_upd<CPL>), dim3(f.nb), dim3(256), 0, c->stream, g->n, g->l, g->m, f.tl, g->d_flags,    \
+                     g->d_cxf, g->d_cyf, g->d_czf, e, p, r, c->state, part)
+  if (f.cpl == 4) XF_FCG_UPD(4);
+  else if (f.cpl == 2) XF_FCG_UPD(2);
+  else XF_FCG_UPD(1);
+#undef XF_FCG_UPD
 }
 
 // Flexible CG (A-orthogonalisation against the previous direction, Notay's
@@ -1086,7 +1131,7 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
   CgWork *w = work(c, g, true);
   const int64_t nc = g->ncell;
   Tiles tl = tiles(g);
-  const int nbS = std::max(1, std::min(kFcgBlocks, (int)((nc + 255) / 256)));
+  const FcgLaunch fl = fcg_launch(g);
   const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
   const int stall = o.stall_window > 0 ? o.stall_window : 20;
   float *z = nullptr;
@@ -1116,7 +1161,6 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
   const double rr0 = rr;
   const double stop2 = o.tol_mode == XF_TOL_REL ? o.tol * o.tol * rr0 : o.tol * o.tol;
   int outer = 0, inner_total = 0, status = XF_OK, fallback = 0;
-  float *p = w->p32[0];
   while (rr > stop2) {
     if (rr != rr) { status = XF_ENAN; break; }
     if (outer >= 60 || inner_total >= o.maxit) { status = XF_EMAXIT; break; }
@@ -1130,13 +1174,12 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
     hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags, w->r32,
                        c->state, c->partials, c->counters, (int)HS_AUX);
     XF_HIP(hipMemsetAsync(w->e32, 0, sizeof(float) * nc, c->stream));
-    XF_HIP(hipMemsetAsync(p, 0, sizeof(float) * nc, c->stream));
     hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, inner_tol, 1, (int)HS_AUX);
     poll();
     double best = c->h_state->s[HS_RR];
     int since_best = 0, it = 0;
     bool use_net = !fallback;
-    // one network iteration: z = M(r), p = z + beta p, q = A p, e += alpha p, r -= alpha q
+    // one network iteration: z = M(r), p = z + beta p, e += alpha p, r -= alpha A p
     auto launch_iter = [&](int idx) {
       if (mgc) {
         mg_apply(c, g, net, w->r32, z);
@@ -1145,19 +1188,16 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
                            w->r32, c->state, c->partials, c->counters, (int)FC_SUMSQ);
         unet_apply_grid(c, net, g, w->r32, z, c->state->s + FC_SUMSQ);
       }
+      float *pold = w->p32[idx & 1], *pnew = w->p32[(idx + 1) & 1];
       if (idx > 0) {
-        hipLaunchKernelGGL(k_dot_part, dim3(nbS), dim3(256), 0, c->stream, nc, z, w->q32, c->state, w->part);
-        hipLaunchKernelGGL(k_fcg_finish<FIN_ZQ>, dim3(1), dim3(256), 0, c->stream, w->part, nbS, c->state, 0);
+        fcg_zq(c, g, fl, z, pold, w->part);
+        hipLaunchKernelGGL(k_fcg_finish<FIN_ZQ>, dim3(1), dim3(256), 0, c->stream, w->part, fl.nb, c->state, 0);
       }
-      hipLaunchKernelGGL(k_fcg_dir, dim3(nbS), dim3(256), 0, c->stream, nc, z, p, w->r32, c->state, idx == 0 ? 1 : 0,
-                         w->part);
-      hipLaunchKernelGGL(k_fcg_finish<FIN_PR>, dim3(1), dim3(256), 0, c->stream, w->part, nbS, c->state, 0);
-      // q = A p with the partials ((p, p), (A p, p)); alpha = (p, r)/(p, q)
-      const int npq = stencil_pq(c, g, p, w->q32, w->part);
-      hipLaunchKernelGGL(k_fcg_finish<FIN_PQ>, dim3(1), dim3(256), 0, c->stream, w->part, npq, c->state, 0);
-      hipLaunchKernelGGL(k_fcg_update, dim3(nbS), dim3(256), 0, c->stream, nc, w->e32, p, w->r32, w->q32, c->state,
-                         w->part);
-      hipLaunchKernelGGL(k_fcg_finish<FIN_RR>, dim3(1), dim3(256), 0, c->stream, w->part, nbS, c->state,
+      // p = z + beta p_old with the partials ((p, r), (p, A p)); alpha = (p, r)/(p, A p)
+      fcg_dir_pq(c, g, fl, z, pold, pnew, w->r32, idx == 0 ? 1 : 0, w->part);
+      hipLaunchKernelGGL(k_fcg_finish<FIN_PRPQ>, dim3(1), dim3(256), 0, c->stream, w->part, fl.nb, c->state, 0);
+      fcg_upd(c, g, fl, w->e32, pnew, w->r32, w->part);
+      hipLaunchKernelGGL(k_fcg_finish<FIN_RR>, dim3(1), dim3(256), 0, c->stream, w->part, fl.nb, c->state,
                          o.maxit - inner_total);
       check_kernel("fcg iteration");
     };
