@@ -86,8 +86,8 @@ static CgWork *work(xf_ctx c, xf_grid g, bool fp32) {
 }
 
 // State slots of the throughput solvers.
-enum { HS_RR = 0, HS_PQ, HS_ALPHA, HS_BETA, HS_STOP2, HS_RR0, HS_SUM, HS_CNT, HS_AUX,
-       CG_GAMMA = 10, CG_ALPHA = 12 };  // CG_GAMMA, CG_ALPHA: two parity slots each
+enum { HS_RR = 0, HS_PQ, HS_ALPHA, HS_BETA, HS_STOP2, HS_RR0, HS_SUM, HS_CNT, HS_AUX, HS_RRW,
+       CG_GAMMA = 10, CG_ALPHA = 12 };  // HS_RRW: (r, r) of the working-precision residual  // CG_GAMMA, CG_ALPHA: two parity slots each
 
 struct Tiles {
   int kt, jt, it, total;
@@ -424,80 +424,121 @@ __global__ __launch_bounds__(256) void k_cgcg_flush(int64_t nc, X *__restrict__ 
     x[c] = x[c] + a * (X)p[c];
 }
 
+// ------------------------------------- refinement residual and mean projection
+// Fence-free reductions: each kernel writes kRefPart doubles per block into
+// the context's partial buffer, and k_ref_finish (one block) adds them in
+// block order into state slots.
+constexpr int kRefPart = 4;      // partial doubles per block
+constexpr int kRefBlocks = 2048; // blocks of the refinement kernels (partials fit the context buffer)
+
 // r = b - A x over the interior with the fast rows (x read everywhere, so
 // boundary values of Defined neighbours enter as in the reference), written
-// as type R; (r, r) and sum(r) over unknowns into slots.
+// as type R (0 off the Actual rows).  Partials per block: (r, r) in FP64
+// before rounding, the sum of the rounded r and (R r, R r), over Actual
+// cells.  Blocks are persistent over the tiles, consecutive tiles on one XCD.
 template <typename R>
 __global__ __launch_bounds__(256) void k_residual_fast(int n, int l, int m, Tiles tl,
                                                         const uint8_t *__restrict__ flags,
                                                         const double *__restrict__ cx, const double *__restrict__ cy,
                                                         const double *__restrict__ cz, const double *__restrict__ b,
                                                         const double *__restrict__ x, R *__restrict__ rout,
-                                                        DevState *st, double *partials, unsigned *counter,
-                                                        int slot_rr) {
-  __shared__ double lds[16];
+                                                        double *__restrict__ part) {
+  __shared__ double lds[8];
   const int per = (tl.total + 7) / 8;
-  const int w = blockIdx.x;
-  const int t = (w & 7) * per + (w >> 3);
-  double acc = 0;
-  if (t < tl.total) {
+  const int bx = gridDim.x >> 3;
+  const int xcd = blockIdx.x & 7;
+  double rr = 0, sum = 0, rr2 = 0;
+  const int64_t sI = (int64_t)l * m, sJ = m;
+  for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
+    const int t = xcd * per + tq;
+    if (t >= tl.total) break;
     const int ktile = t % tl.kt;
     const int jtile = (t / tl.kt) % tl.jt;
     const int itile = t / (tl.kt * tl.jt);
     const int k = 1 + ktile * TK + (threadIdx.x & 63);
     const int j = 1 + jtile * TJ + (threadIdx.x >> 6);
-    const int i0 = 1 + itile * IC;
-    const int i1 = min(i0 + IC, n - 1);
-    if (k <= m - 2 && j <= l - 2) {
-      const int64_t sI = (int64_t)l * m, sJ = m;
-      const double czm = cz[2 * k], czp = cz[2 * k + 1];
-      const double cym = cy[2 * j], cyp = cy[2 * j + 1];
-      int64_t c = ((int64_t)i0 * l + j) * m + k;
-      double pm = x[c - sI], pc = x[c];
-      for (int i = i0; i < i1; i++, c += sI) {
-        const double pp = x[c + sI];
-        const uint8_t fl = flags[c];
-        double v = 0;
-        if (fl & FL_ACTUAL) {
-          const double cxm = cx[2 * i], cxp = cx[2 * i + 1];
-          const double pjm = x[c - sJ], pjp = x[c + sJ], pkm = x[c - 1], pkp = x[c + 1];
-          double tx = ((fl & FL_IP) ? cxp * (pp - pc) : 0.0) - ((fl & FL_IM) ? cxm * (pc - pm) : 0.0);
-          double ty = ((fl & FL_JP) ? cyp * (pjp - pc) : 0.0) - ((fl & FL_JM) ? cym * (pc - pjm) : 0.0);
-          double tz = ((fl & FL_KP) ? czp * (pkp - pc) : 0.0) - ((fl & FL_KM) ? czm * (pc - pkm) : 0.0);
-          v = b[c] + (tx + ty + tz);  // b - A x
-          acc += v * v;
-        }
-        rout[c] = (R)v;
-        pm = pc;
-        pc = pp;
+    const int i0 = 1 + itile * tl.ic;
+    const int i1 = min(i0 + tl.ic, n - 1);
+    if (k > m - 2 || j > l - 2) continue;
+    const double czm = cz[2 * k], czp = cz[2 * k + 1];
+    const double cym = cy[2 * j], cyp = cy[2 * j + 1];
+    int64_t c = ((int64_t)i0 * l + j) * m + k;
+    double pm = x[c - sI], pc = x[c];
+    for (int i = i0; i < i1; i++, c += sI) {
+      const double pp = x[c + sI];
+      const uint8_t fl = flags[c];
+      double v = 0;
+      if (fl & FL_ACTUAL) {
+        const double cxm = cx[2 * i], cxp = cx[2 * i + 1];
+        const double pjm = x[c - sJ], pjp = x[c + sJ], pkm = x[c - 1], pkp = x[c + 1];
+        double tx = ((fl & FL_IP) ? cxp * (pp - pc) : 0.0) - ((fl & FL_IM) ? cxm * (pc - pm) : 0.0);
+        double ty = ((fl & FL_JP) ? cyp * (pjp - pc) : 0.0) - ((fl & FL_JM) ? cym * (pc - pjm) : 0.0);
+        double tz = ((fl & FL_KP) ? czp * (pkp - pc) : 0.0) - ((fl & FL_KM) ? czm * (pc - pkm) : 0.0);
+        v = b[c] + (tx + ty + tz);  // b - A x
+        const double vr = (double)(R)v;
+        rr += v * v;
+        sum += vr;
+        rr2 += vr * vr;
       }
+      rout[c] = (R)v;
+      pm = pc;
+      pc = pp;
     }
   }
-  double bs = block_sum(acc, lds);
-  double tot;
-  if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) st->s[slot_rr] = tot;
+  rr = block_sum_all(rr, lds);
+  sum = block_sum_all(sum, lds);
+  rr2 = block_sum_all(rr2, lds);
+  if (threadIdx.x == 0) {
+    part[kRefPart * blockIdx.x] = rr;
+    part[kRefPart * blockIdx.x + 1] = sum;
+    part[kRefPart * blockIdx.x + 2] = rr2;
+  }
 }
 
-// sum and count over unknowns (for the mean projection)
+// v -= s[slot]/count at unknowns; partials of (v, v) over unknowns after the shift
 template <typename T>
-__global__ void k_sum_unknowns(int64_t nc, const uint8_t *__restrict__ flags, const T *__restrict__ v,
-                               DevState *st, double *partials, unsigned *counter, int slot) {
-  __shared__ double lds[16];
+__global__ __launch_bounds__(256) void k_sub_mean_sq(int64_t nc, const uint8_t *__restrict__ flags,
+                                                      T *__restrict__ v, const DevState *st, int slot, double count,
+                                                      double *__restrict__ part) {
+  __shared__ double lds[8];
+  const double mean = st->s[slot] / count;
   double acc = 0;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
-    if (flags[c] & FL_UNKNOWN) acc += (double)v[c];
-  double bs = block_sum(acc, lds);
-  double tot;
-  if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) st->s[slot] = tot;
+    if (flags[c] & FL_UNKNOWN) {
+      const T nv = (T)((double)v[c] - mean);
+      v[c] = nv;
+      acc += (double)nv * (double)nv;
+    }
+  acc = block_sum_all(acc, lds);
+  if (threadIdx.x == 0) part[kRefPart * blockIdx.x] = acc;
 }
 
-// v -= s[slot]/count at unknowns
+// partials of (v, v) over unknowns
 template <typename T>
-__global__ void k_sub_mean(int64_t nc, const uint8_t *__restrict__ flags, T *__restrict__ v, const DevState *st,
-                           int slot, double count) {
-  const double mean = st->s[slot] / count;
+__global__ __launch_bounds__(256) void k_sq_unknowns(int64_t nc, const uint8_t *__restrict__ flags,
+                                                      const T *__restrict__ v, const DevState *st,
+                                                      double *__restrict__ part) {
+  __shared__ double lds[8];
+  if (st && st->done) return;
+  double acc = 0;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
-    if (flags[c] & FL_UNKNOWN) v[c] = (T)((double)v[c] - mean);
+    if (flags[c] & FL_UNKNOWN) acc += (double)v[c] * (double)v[c];
+  acc = block_sum_all(acc, lds);
+  if (threadIdx.x == 0) part[kRefPart * blockIdx.x] = acc;
+}
+
+// s[slot q] = sum over blocks of partial column q (slot < 0: column unused)
+__global__ void k_ref_finish(const double *__restrict__ part, int np, DevState *st, int slot0, int slot1,
+                             int slot2) {
+  __shared__ double lds[16];
+  const int slots[3] = {slot0, slot1, slot2};
+  for (int q = 0; q < 3; q++) {
+    if (slots[q] < 0) continue;
+    double s = 0;
+    for (int b = threadIdx.x; b < np; b += blockDim.x) s += part[kRefPart * b + q];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) st->s[slots[q]] = s;
+  }
 }
 
 __global__ void k_hs_start(DevState *st, double tol, int rel, int slot_rr) {
@@ -529,16 +570,6 @@ __global__ void k_add_correction(int64_t nc, double *__restrict__ x, const float
 }
 
 // sum of squares over unknowns
-__global__ void k_resid_sq(int64_t nc, const uint8_t *__restrict__ space, const double *__restrict__ r,
-                           double *partials, unsigned *counter, double *out) {
-  __shared__ double lds[16];
-  double acc = 0;
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
-    if (space[c]) acc += r[c] * r[c];
-  double bs = block_sum(acc, lds);
-  double tot;
-  if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) *out = tot;
-}
 
 // ------------------------------------------------------------------ driver
 static void require_fast(xf_grid g) {
@@ -668,6 +699,36 @@ static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r0, T *r1, T *p0, T *p
   cgcg_run<T, X, 1>(c, g, x, R, P, cx, cy, cz, maxit, every);
 }
 
+// r = b - A x (fast rows) into rout as type R: s[HS_AUX] = (r, r) in FP64,
+// s[HS_SUM] = sum of the rounded r, s[HS_RRW] = (R r, R r), all over unknowns.
+template <typename R>
+static void residual_fast(xf_ctx c, xf_grid g, const double *b, const double *x, R *rout) {
+  const Tiles tl = tiles(g);
+  const int nb = std::min(tl.blocks, kRefBlocks);
+  hipLaunchKernelGGL((k_residual_fast<R>), dim3(nb), dim3(256), 0, c->stream, g->n, g->l, g->m, tl, g->d_flags,
+                     g->d_cx, g->d_cy, g->d_cz, b, x, rout, c->partials);
+  hipLaunchKernelGGL(k_ref_finish, dim3(1), dim3(256), 0, c->stream, c->partials, nb, c->state, (int)HS_AUX,
+                     (int)HS_SUM, (int)HS_RRW);
+}
+
+// v -= mean of v over the unknowns (the mean from s[HS_SUM]); s[HS_RRW] = (v, v) after
+template <typename T>
+static void project_mean(xf_ctx c, xf_grid g, T *v) {
+  const int nb = std::min(stream_nb(g->ncell), kRefBlocks);
+  hipLaunchKernelGGL((k_sub_mean_sq<T>), dim3(nb), dim3(256), 0, c->stream, g->ncell, g->d_flags, v, c->state,
+                     (int)HS_SUM, (double)g->nunk, c->partials);
+  hipLaunchKernelGGL(k_ref_finish, dim3(1), dim3(256), 0, c->stream, c->partials, nb, c->state, (int)HS_RRW, -1, -1);
+}
+
+// s[slot] = (v, v) over the unknowns
+template <typename T>
+static void sq_unknowns(xf_ctx c, xf_grid g, const T *v, int slot) {
+  const int nb = std::min(stream_nb(g->ncell), kRefBlocks);
+  hipLaunchKernelGGL((k_sq_unknowns<T>), dim3(nb), dim3(256), 0, c->stream, g->ncell, g->d_flags, v,
+                     (const DevState *)nullptr, c->partials);
+  hipLaunchKernelGGL(k_ref_finish, dim3(1), dim3(256), 0, c->stream, c->partials, nb, c->state, slot, -1, -1);
+}
+
 static void upload_state_zero(xf_ctx c) {
   XF_HIP(hipMemsetAsync(c->state, 0, sizeof(DevState), c->stream));
 }
@@ -678,23 +739,13 @@ void solve_hs_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
   require_fast(g);
   CgWork *w = work(c, g, false);
   const int64_t nc = g->ncell;
-  Tiles tl = tiles(g);
   upload_state_zero(c);
   XF_HIP(hipMemcpyAsync(x, x0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
-  hipLaunchKernelGGL((k_residual_fast<double>), dim3(tl.blocks), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
-                     g->d_flags, g->d_cx, g->d_cy, g->d_cz, b, x, w->r, c->state, c->partials, c->counters,
-                     (int)HS_AUX);
-  if (o.project_mean && g->nunk > 0) {
-    hipLaunchKernelGGL((k_sum_unknowns<double>), dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags, w->r,
-                       c->state, c->partials, c->counters, (int)HS_SUM);
-    hipLaunchKernelGGL((k_sub_mean<double>), dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags, w->r,
-                       c->state, (int)HS_SUM, (double)g->nunk);
-    hipLaunchKernelGGL(k_resid_sq, dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_space, w->r, c->partials,
-                       c->counters, c->state->s + HS_AUX);
-  }
+  residual_fast<double>(c, g, b, x, w->r);
+  if (o.project_mean && g->nunk > 0) project_mean<double>(c, g, w->r);
   check_kernel("cg start");
   hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, o.tol, o.tol_mode == XF_TOL_REL ? 1 : 0,
-                     (int)HS_AUX);
+                     (int)HS_RRW);
   c->h_state->iter = 0;
   cgcg_iterations<double, double>(c, g, x, w->r, w->q, w->p[0], w->p[1], g->d_cx, g->d_cy, g->d_cz, o.maxit,
                                   o.check_every > 0 ? o.check_every : 16);
@@ -703,17 +754,6 @@ void solve_hs_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
   s.resid = c->h_state->s[HS_RR0] > 0 ? sqrt(c->h_state->s[HS_RR] / c->h_state->s[HS_RR0]) : 0;
 }
 
-template <typename T>
-__global__ void k_sq_unknowns(int64_t nc, const uint8_t *__restrict__ flags, const T *__restrict__ v, DevState *st,
-                              double *partials, unsigned *counter, int slot) {
-  __shared__ double lds[16];
-  double acc = 0;
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
-    if (flags[c] & FL_UNKNOWN) acc += (double)v[c] * (double)v[c];
-  double bs = block_sum(acc, lds);
-  double tot;
-  if (grid_sum_last(bs, partials, counter, &tot, lds) && threadIdx.x == 0) st->s[slot] = tot;
-}
 
 // Mixed precision: FP32 Hestenes-Stiefel CG sweeps on the correction
 // equation A e = r, with the residual r = b - A x recomputed in FP64 from the
@@ -724,7 +764,6 @@ void solve_mp_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
   require_fast(g);
   CgWork *w = work(c, g, true);
   const int64_t nc = g->ncell;
-  Tiles tl = tiles(g);
   const int every = o.check_every > 0 ? o.check_every : 16;
   const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
   upload_state_zero(c);
@@ -732,9 +771,7 @@ void solve_mp_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
   std::vector<ProfMark> marks;
   auto residual = [&]() {
     int e = c->prof ? prof_begin(c) : -1;
-    hipLaunchKernelGGL((k_residual_fast<float>), dim3(tl.blocks), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
-                       g->d_flags, g->d_cx, g->d_cy, g->d_cz, b, x, w->r32, c->state, c->partials, c->counters,
-                       (int)HS_AUX);
+    residual_fast<float>(c, g, b, x, w->r32);
     if (c->prof) {
       prof_end(c, e);
       marks.push_back({"mp_residual_fp64", e});
@@ -753,18 +790,11 @@ void solve_mp_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
     if (rr != rr) { status = XF_ENAN; break; }
     if (outer >= max_outer || inner_total >= o.maxit) { status = XF_EMAXIT; break; }
     outer++;
-    if (o.project_mean && g->nunk > 0) {
-      hipLaunchKernelGGL((k_sum_unknowns<float>), dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags,
-                         w->r32, c->state, c->partials, c->counters, (int)HS_SUM);
-      hipLaunchKernelGGL((k_sub_mean<float>), dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags, w->r32,
-                         c->state, (int)HS_SUM, (double)g->nunk);
-    }
-    hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags, w->r32,
-                       c->state, c->partials, c->counters, (int)HS_AUX);
+    if (o.project_mean && g->nunk > 0) project_mean<float>(c, g, w->r32);
     XF_HIP(hipMemsetAsync(w->e32, 0, sizeof(float) * nc, c->stream));
     // the last sweep only needs the factor that is still missing (with margin)
     const double itol = std::min(0.5, std::max(inner_tol, 0.3 * sqrt(stop2 / rr)));
-    hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, itol, 1, (int)HS_AUX);
+    hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, itol, 1, (int)HS_RRW);
     check_kernel("mp inner start");
     c->h_state->iter = 0;
     cgcg_iterations<float, float>(c, g, w->e32, w->r32, w->q32, w->p32[0], w->p32[1], g->d_cxf, g->d_cyf, g->d_czf,
@@ -1130,7 +1160,6 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
   struct xf_unet_s *net = (struct xf_unet_s *)o.precond;
   CgWork *w = work(c, g, true);
   const int64_t nc = g->ncell;
-  Tiles tl = tiles(g);
   const FcgLaunch fl = fcg_launch(g);
   const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
   const int stall = o.stall_window > 0 ? o.stall_window : 20;
@@ -1151,9 +1180,7 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
     XF_HIP(hipStreamSynchronize(c->stream));
   };
   auto residual = [&]() {
-    hipLaunchKernelGGL((k_residual_fast<float>), dim3(tl.blocks), dim3(256), 0, c->stream, g->n, g->l, g->m, tl,
-                       g->d_flags, g->d_cx, g->d_cy, g->d_cz, b, x, w->r32, c->state, c->partials, c->counters,
-                       (int)HS_AUX);
+    residual_fast<float>(c, g, b, x, w->r32);
     poll();
     return c->h_state->s[HS_AUX];
   };
@@ -1165,16 +1192,9 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
     if (rr != rr) { status = XF_ENAN; break; }
     if (outer >= 60 || inner_total >= o.maxit) { status = XF_EMAXIT; break; }
     outer++;
-    if (o.project_mean && g->nunk > 0) {
-      hipLaunchKernelGGL((k_sum_unknowns<float>), dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags,
-                         w->r32, c->state, c->partials, c->counters, (int)HS_SUM);
-      hipLaunchKernelGGL((k_sub_mean<float>), dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags, w->r32,
-                         c->state, (int)HS_SUM, (double)g->nunk);
-    }
-    hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags, w->r32,
-                       c->state, c->partials, c->counters, (int)HS_AUX);
+    if (o.project_mean && g->nunk > 0) project_mean<float>(c, g, w->r32);
     XF_HIP(hipMemsetAsync(w->e32, 0, sizeof(float) * nc, c->stream));
-    hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, inner_tol, 1, (int)HS_AUX);
+    hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, inner_tol, 1, (int)HS_RRW);
     poll();
     double best = c->h_state->s[HS_RR];
     int since_best = 0, it = 0;
@@ -1184,8 +1204,7 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
       if (mgc) {
         mg_apply(c, g, net, w->r32, z);
       } else {
-        hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags,
-                           w->r32, c->state, c->partials, c->counters, (int)FC_SUMSQ);
+        sq_unknowns<float>(c, g, w->r32, (int)FC_SUMSQ);
         unet_apply_grid(c, net, g, w->r32, z, c->state->s + FC_SUMSQ);
       }
       float *pold = w->p32[idx & 1], *pnew = w->p32[(idx + 1) & 1];
@@ -1221,8 +1240,7 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
     it = c->h_state->iter;  // network iterations applied (the device counts them)
     if (!use_net && !c->h_state->done) {
       // plain CG from the current (e, r): restart with p = r
-      hipLaunchKernelGGL((k_sq_unknowns<float>), dim3(reduce_nb(nc)), dim3(256), 0, c->stream, nc, g->d_flags,
-                         w->r32, c->state, c->partials, c->counters, (int)FC_RR2);
+      sq_unknowns<float>(c, g, w->r32, (int)FC_RR2);
       hipLaunchKernelGGL(k_restart_beta, dim3(1), dim3(1), 0, c->stream, c->state);
       cgcg_iterations<float, float>(c, g, w->e32, w->r32, w->q32, w->p32[0], w->p32[1], g->d_cxf, g->d_cyf,
                                     g->d_czf, std::max(1, o.maxit - inner_total - it), 16);
@@ -1249,9 +1267,8 @@ double true_residual(xf_ctx c, xf_grid g, const double *b, const double *x, cons
   const double *xs[2] = {x, x0};
   for (int t = 0; t < 2; t++) {
     launch_apply_exact(c, g, xs[t], b, -1.0, tmp);
-    hipLaunchKernelGGL(k_resid_sq, dim3(reduce_nb(g->ncell)), dim3(256), 0, c->stream, g->ncell, g->d_space, tmp,
-                       c->partials, c->counters, c->d_scalar);
-    XF_HIP(hipMemcpyAsync(&v[t], c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    sq_unknowns<double>(c, g, tmp, (int)HS_CNT);
+    XF_HIP(hipMemcpyAsync(&v[t], c->state->s + HS_CNT, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     XF_HIP(hipStreamSynchronize(c->stream));
   }
   hipFree(tmp);

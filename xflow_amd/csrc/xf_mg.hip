@@ -62,13 +62,39 @@ __device__ inline float diag_of(uint8_t f, float cxm, float cxp, float cym, floa
   return d;
 }
 
+// The hierarchy's grids have uniform steps per axis, so every interior node
+// carries the same coefficients (c-[i] == c+[i], bit for bit) and the damped
+// Jacobi weight omega / diag depends on the six face bits only: each block
+// tabulates it once in LDS (the same operations as per cell, so the same bits)
+// instead of dividing per cell.
+__device__ inline void fill_weights(float *lut, const float *__restrict__ cx, const float *__restrict__ cy,
+                                    const float *__restrict__ cz) {
+  if (threadIdx.x < 64) {
+    const float a = cx[2], b = cy[2], e = cz[2];
+    const float dg = diag_of((uint8_t)threadIdx.x, a, a, b, b, e, e);
+    lut[threadIdx.x] = dg > 0 ? kOmega / dg : 0.f;  // a cell with no open face is left alone
+  }
+  __syncthreads();
+}
+
+// omega D^-1 r at cell q: the first sweep from z = 0 (0 at cells that are not Actual)
+__device__ inline float jacobi0(const uint8_t *__restrict__ flags, const float *lut, const float *__restrict__ r,
+                                int64_t q) {
+  const uint8_t f = flags[q];
+  return (f & FL_ACTUAL) ? lut[f & 63] * r[q] : 0.f;
+}
+
 // MODE 0: out = omega D^-1 r;  MODE 1: out = z + omega D^-1 (r - A z);
-// MODE 2: out = r - A z;  MODE 3: out = A z.  0 at cells that are not Actual.
+// MODE 2: out = r - A z;  MODE 3: out = A z;  MODE 5: MODE 1 on z = omega D^-1 r
+// (the first two sweeps from z = 0 in one pass: z recomputed at the seven
+// stencil points, never stored).  0 at cells that are not Actual.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_mg_sweep(Dims d, const uint8_t *__restrict__ flags,
                                                    const float *__restrict__ cx, const float *__restrict__ cy,
                                                    const float *__restrict__ cz, const float *__restrict__ r,
                                                    const float *__restrict__ z, float *__restrict__ out) {
+  __shared__ float lut[64];
+  fill_weights(lut, cx, cy, cz);
   const int k = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
   const int i = 1 + blockIdx.z;
@@ -80,17 +106,18 @@ __global__ __launch_bounds__(256) void k_mg_sweep(Dims d, const uint8_t *__restr
   if (f & FL_ACTUAL) {
     const float cxm = cx[2 * i], cxp = cx[2 * i + 1], cym = cy[2 * j], cyp = cy[2 * j + 1];
     const float czm = cz[2 * k], czp = cz[2 * k + 1];
-    const float dg = diag_of(f, cxm, cxp, cym, cyp, czm, czp);
-    const float wd = dg > 0 ? kOmega / dg : 0.f;  // a cell with no open face is left alone
+    const float wd = lut[f & 63];
     if (MODE == 0) {
       v = wd * r[c];
     } else {
-      const float zc = z[c];
-      float tx = ((f & FL_IP) ? cxp * (z[c + sI] - zc) : 0.f) - ((f & FL_IM) ? cxm * (zc - z[c - sI]) : 0.f);
-      float ty = ((f & FL_JP) ? cyp * (z[c + sJ] - zc) : 0.f) - ((f & FL_JM) ? cym * (zc - z[c - sJ]) : 0.f);
-      float tz = ((f & FL_KP) ? czp * (z[c + 1] - zc) : 0.f) - ((f & FL_KM) ? czm * (zc - z[c - 1]) : 0.f);
+      // z at the stencil points (MODE 5: the first sweep's omega D^-1 r)
+      auto zat = [&](int64_t q) { return MODE == 5 ? jacobi0(flags, lut, r, q) : z[q]; };
+      const float zc = MODE == 5 ? wd * r[c] : z[c];
+      float tx = ((f & FL_IP) ? cxp * (zat(c + sI) - zc) : 0.f) - ((f & FL_IM) ? cxm * (zc - zat(c - sI)) : 0.f);
+      float ty = ((f & FL_JP) ? cyp * (zat(c + sJ) - zc) : 0.f) - ((f & FL_JM) ? cym * (zc - zat(c - sJ)) : 0.f);
+      float tz = ((f & FL_KP) ? czp * (zat(c + 1) - zc) : 0.f) - ((f & FL_KM) ? czm * (zc - zat(c - 1)) : 0.f);
       const float az = -(tx + ty + tz);
-      if (MODE == 1) v = zc + wd * (r[c] - az);
+      if (MODE == 1 || MODE == 5) v = zc + wd * (r[c] - az);
       else if (MODE == 2) v = r[c] - az;
       else v = az;
     }
@@ -111,6 +138,8 @@ struct SweepTiles {
 // MODE 4: MODE 1 on z + zcoarse(parent) at the unknowns (the prolongation of
 // the coarse correction folded into the first post-smoothing sweep; the
 // corrected z is formed with the same addition as k_mg_prolong4 and not stored).
+// MODE 5: MODE 1 on z = omega D^-1 r (the first two sweeps from z = 0 in one
+// pass, z formed at the stencil points as MODE 0 forms it and not stored).
 template <int MODE, int CPL>
 __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, const uint8_t *__restrict__ flags,
                                                        const float *__restrict__ cx, const float *__restrict__ cy,
@@ -120,9 +149,21 @@ __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, con
   struct V {
     float v[CPL];
   };
-  // z (+ the parent's coarse correction at unknowns for MODE 4) of CPL cells from k
+  __shared__ float lut[64];
+  fill_weights(lut, cx, cy, cz);
+  // z (+ the parent's coarse correction at unknowns for MODE 4; omega D^-1 r
+  // for MODE 5) of CPL cells from k
   auto zrow = [&](int64_t q, int ii, int jj, int kk) {
     V v;
+    if (MODE == 5) {
+      V rv;
+      uint8_t fq[CPL];
+      __builtin_memcpy(&rv, r + q, sizeof(V));
+      __builtin_memcpy(fq, flags + q, CPL);
+#pragma unroll
+      for (int e = 0; e < CPL; e++) v.v[e] = (fq[e] & FL_ACTUAL) ? lut[fq[e] & 63] * rv.v[e] : 0.f;
+      return v;
+    }
     __builtin_memcpy(&v, z + q, sizeof(V));
     if (MODE == 4) {
       // parents of cells kk..kk+CPL-1 are the CPL/2 + 1 coarse cells from
@@ -143,6 +184,7 @@ __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, con
     return v;
   };
   auto zone = [&](int64_t q, int ii, int jj, int kk) {
+    if (MODE == 5) return jacobi0(flags, lut, r, q);
     float v = z[q];
     if (MODE == 4 && (flags[q] & FL_UNKNOWN))
       v += zcoarse[((int64_t)((ii + 1) >> 1) * cd.l + ((jj + 1) >> 1)) * cd.m + ((kk + 1) >> 1)];
@@ -191,8 +233,7 @@ __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, con
 #pragma unroll
       for (int e = 0; e < CPL; e++) {
         const uint8_t f = fl[e];
-        const float dg = diag_of(f, cxm, cxp, cym, cyp, czm[e], czp[e]);
-        const float wd = dg > 0 ? kOmega / dg : 0.f;
+        const float wd = lut[f & 63];
         float v = 0;
         if (MODE == 0) {
           v = wd * rr.v[e];
@@ -204,7 +245,7 @@ __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, con
           float ty = ((f & FL_JP) ? cyp * (zjp.v[e] - x0) : 0.f) - ((f & FL_JM) ? cym * (x0 - zjm.v[e]) : 0.f);
           float tz = ((f & FL_KP) ? czp[e] * (kp - x0) : 0.f) - ((f & FL_KM) ? czm[e] * (x0 - km) : 0.f);
           const float az = -(tx + ty + tz);
-          if (MODE == 1 || MODE == 4) v = x0 + wd * (rr.v[e] - az);
+          if (MODE == 1 || MODE == 4 || MODE == 5) v = x0 + wd * (rr.v[e] - az);
           else if (MODE == 2) v = rr.v[e] - az;
           else v = az;
         }
@@ -457,8 +498,12 @@ static int red_blocks(int64_t nc) { return (int)std::max<int64_t>(1, std::min<in
 // nu sweeps of damped Jacobi on A z = r starting from z = 0; result in L.z
 static void smooth_from_zero(xf_ctx c, MgLevel &L, const float *r, int nu) {
   const xf_grid g = L.g;
-  sweep<0>(c, g, r, nullptr, L.z);
-  for (int s = 1; s < nu; s++) {
+  if (nu < 2) {
+    sweep<0>(c, g, r, nullptr, L.z);
+    return;
+  }
+  sweep<5>(c, g, r, nullptr, L.z);  // the first two sweeps in one pass
+  for (int s = 2; s < nu; s++) {
     sweep<1>(c, g, r, L.z, L.z2);
     std::swap(L.z, L.z2);
   }
