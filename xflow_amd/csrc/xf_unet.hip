@@ -165,13 +165,21 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   int rbase[RPW];
 #pragma unroll
   for (int r = 0; r < RPW; r++) rbase[r] = roff[r] * CS;
-  // weight fragments (global) and input fragments (LDS) of the next k-step
-  // are loaded before this step's MFMAs
+  // Weight fragments come from global memory (L2-resident: every block reads
+  // the same weights) and are fetched WP k-steps ahead into a ring of
+  // registers indexed at compile time (the k-loop is unrolled by WP), so an L2
+  // round trip is paid once per WP steps instead of once per step; input
+  // fragments (LDS) are loaded one step ahead.
+  constexpr int WP = NT <= 2 ? 4 : 2;
+  const int nsteps = a.Kpad / 32;
   const bf16 *wrow = a.wt + (int64_t)(ct0 * 16 + l16) * a.Kpad + 8 * lg;
   const int64_t wstride = (int64_t)16 * a.Kpad;
-  bf16x8 afn[NT];
+  bf16x8 aw[WP][NT];
 #pragma unroll
-  for (int t = 0; t < NT; t++) afn[t] = *(const bf16x8 *)(wrow + t * wstride);
+  for (int u = 0; u < WP; u++)
+    if (u < nsteps)
+#pragma unroll
+      for (int t = 0; t < NT; t++) aw[u][t] = *(const bf16x8 *)(wrow + u * 32 + t * wstride);
   auto load_b = [&](bf16x8 *bf) {
 #pragma unroll
     for (int r = 0; r < RPW; r++) {
@@ -181,26 +189,30 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   };
   bf16x8 bcur[RPW];
   load_b(bcur);
-  for (int k0 = 0; k0 < a.Kpad; k0 += 32) {
-    bf16x8 af[NT];
+  for (int s0 = 0; s0 < nsteps; s0 += WP) {
 #pragma unroll
-    for (int t = 0; t < NT; t++) af[t] = afn[t];
-    const bool more = k0 + 32 < a.Kpad;
-    wrow += 32;
-    if (more)
+    for (int u = 0; u < WP; u++) {
+      const int s = s0 + u;
+      if (s >= nsteps) break;
+      bf16x8 af[NT];
 #pragma unroll
-      for (int t = 0; t < NT; t++) afn[t] = *(const bf16x8 *)(wrow + t * wstride);
-    advance();
-    bf16x8 bnext[RPW];
-    if (more) load_b(bnext);
+      for (int t = 0; t < NT; t++) af[t] = aw[u][t];
+      if (s + WP < nsteps)
 #pragma unroll
-    for (int r = 0; r < RPW; r++)
+        for (int t = 0; t < NT; t++) aw[u][t] = *(const bf16x8 *)(wrow + (s + WP) * 32 + t * wstride);
+      const bool more = s + 1 < nsteps;
+      advance();
+      bf16x8 bnext[RPW];
+      if (more) load_b(bnext);
 #pragma unroll
-      for (int t = 0; t < NT; t++)
-        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t], bcur[r], acc[r][t], 0, 0, 0);
-    if (more)
+      for (int r = 0; r < RPW; r++)
 #pragma unroll
-      for (int r = 0; r < RPW; r++) bcur[r] = bnext[r];
+        for (int t = 0; t < NT; t++)
+          acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t], bcur[r], acc[r][t], 0, 0, 0);
+      if (more)
+#pragma unroll
+        for (int r = 0; r < RPW; r++) bcur[r] = bnext[r];
+    }
   }
   // epilogue: lane holds D[co = 16t + 4 lg + e][cell = l16]
 #pragma unroll
