@@ -582,10 +582,6 @@ static int stream_nb(int64_t nc) {
   int64_t nb = (nc + kBlock - 1) / kBlock;
   return (int)(nb > kStreamBlocks ? kStreamBlocks : (nb < 1 ? 1 : nb));
 }
-static int reduce_nb(int64_t nc) {
-  int nb = stream_nb(nc);
-  return nb > 2048 ? 2048 : nb;
-}
 
 // Planes per tile of the two plain-CG kernels: the deepest tile (at most
 // kCgMaxPlanes) that still leaves >= kCgMinTiles tiles, so a launch keeps 4096
