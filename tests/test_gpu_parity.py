@@ -174,3 +174,26 @@ def test_throughput_cg_128_sphere(ctx):
     assert st.status == 0
     assert st.resid_true <= 1.5e-8
     assert 50 < st.iters < 2000
+
+
+@pytest.mark.parametrize("method", [xf.XF_CG, xf.XF_CG_MP])
+def test_verify_flag_skips_only_the_check(ctx, method):
+    """opts.verify = 0 skips the after-solve residual check with the reference
+    rows (stats.resid_true = -1) and changes nothing else: same iterations,
+    bit-identical x."""
+    case = sphere_case(40, seed=3)
+    g = xf.Grid(ctx, case["hx"], case["hy"], case["hz"], case["codes"])
+    b = dev(case["b"])
+    out = []
+    for verify in (1, 0):
+        x = torch.zeros_like(b)
+        st = g.solve(b, torch.zeros_like(b), x, xf.solve_opts(method, tol=1e-9, tol_mode=xf.XF_TOL_REL,
+                                                                project_mean=1, verify=verify))
+        assert st.status == 0
+        out.append((st, host(x)))
+    (s1, x1), (s0, x0) = out
+    assert s1.resid_true > 0 and s1.resid_true <= 2e-9
+    assert s0.resid_true == -1.0
+    assert (s1.iters, s1.inner_iters) == (s0.iters, s0.inner_iters)
+    assert np.array_equal(x1, x0)
+    g.close()

@@ -86,8 +86,9 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   const int cpc = Cin / 8;  // 16-byte chunks per cell
   const int nchunk = SZ * SY * SX * cpc;
   const int D0 = a.D >> a.up0, H0 = a.H >> a.up0, W0 = a.W >> a.up0;
-  // four chunks per thread in flight: loads first, then the LDS stores
-  constexpr int U = 4;
+  // up to twelve chunks per thread in flight: loads first, then the LDS stores
+  // (a 48-channel 2 x 4 tile is 10 chunks per thread: one round trip)
+  constexpr int U = 12;
   for (int qb = threadIdx.x; qb < nchunk; qb += U * blockDim.x) {
     bf16x8 v[U];
     int dst[U];
@@ -287,11 +288,22 @@ __global__ __launch_bounds__(256) void k_conv_first(Src0 s, const float *__restr
   const int bz = bid % zt;
   const int b = bid / zt;
   const int x0 = bx * 16, y0 = by * 4, z0 = bz * 4;
-  for (int q = threadIdx.x; q < SZ * SY * SX; q += blockDim.x) {
-    const int row = q / SX, xx = q - row * SX, zz = row / SY, yy = row - zz * SY;
-    float rv, mv;
-    src0_read(s, b, z0 - 1 + zz, y0 - 1 + yy, x0 - 1 + xx, inv, rv, mv);
-    in[q] = make_float2(rv, mv);
+  // all of a thread's cells (at most 3) read before the LDS stores: one round trip
+  constexpr int NQ = SZ * SY * SX, U = (NQ + 255) / 256;
+  float2 v[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int q = threadIdx.x + u * 256;
+    v[u] = make_float2(0.f, 0.f);
+    if (q < NQ) {
+      const int row = q / SX, xx = q - row * SX, zz = row / SY, yy = row - zz * SY;
+      src0_read(s, b, z0 - 1 + zz, y0 - 1 + yy, x0 - 1 + xx, inv, v[u].x, v[u].y);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int q = threadIdx.x + u * 256;
+    if (q < NQ) in[q] = v[u];
   }
   __syncthreads();
   const int tx = threadIdx.x & 15, ty = (threadIdx.x >> 4) & 3, tz = threadIdx.x >> 6;
@@ -345,6 +357,71 @@ __global__ void k_conv_last(const bf16 *__restrict__ a8, const float *__restrict
     } else {
       d.y[q] = acc;
     }
+  }
+}
+
+// c9 on a block of 16 x 4 x 4 cells with their one-cell halo of a8 staged in
+// LDS (k_conv_last reads the 27 neighbour cells, 32 bytes each, straight from
+// memory for every cell); the same sums in the same order.  W % 16, H % 4 and
+// D % 4 must be 0.
+__global__ __launch_bounds__(256) void k_conv_last_tile(const bf16 *__restrict__ a8, const float *__restrict__ w,
+                                                        const float *__restrict__ bias, Dst9 d, int B, int D, int H,
+                                                        int W) {
+  constexpr int SX = 18, SY = 6, SZ = 6, CS = 24;  // LDS cell stride: 16 channels + 8 pad (48 bytes)
+  __shared__ __attribute__((aligned(16))) bf16 tile[SZ * SY * SX * CS];
+  const int xt = W / 16, yt = H / 4, zt = D / 4;
+  int bid = blockIdx.x;
+  const int bx = bid % xt; bid /= xt;
+  const int by = bid % yt; bid /= yt;
+  const int bz = bid % zt;
+  const int b = bid / zt;
+  const int x0 = bx * 16, y0 = by * 4, z0 = bz * 4;
+  // all of a thread's chunks (at most 6) loaded before the LDS stores: one round trip
+  constexpr int NQ = SZ * SY * SX * 2, U = (NQ + 255) / 256;
+  bf16x8 v[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int q = threadIdx.x + u * 256;
+    v[u] = bf16x8{};
+    if (q < NQ) {
+      const int cell = q >> 1, half = q & 1;
+      const int row = cell / SX, xx = cell - row * SX, zz = row / SY, yy = row - zz * SY;
+      const int x = x0 - 1 + xx, y = y0 - 1 + yy, z = z0 - 1 + zz;
+      if (x >= 0 && y >= 0 && z >= 0 && x < W && y < H && z < D)
+        v[u] = *(const bf16x8 *)(a8 + ((((int64_t)b * D + z) * H + y) * W + x) * 16 + half * 8);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int q = threadIdx.x + u * 256;
+    if (q < NQ) *(bf16x8 *)(tile + (q >> 1) * CS + (q & 1) * 8) = v[u];
+  }
+  __syncthreads();
+  const int tx = threadIdx.x & 15, ty = (threadIdx.x >> 4) & 3, tz = threadIdx.x >> 6;
+  const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
+  float s = 1.0f;
+  if (d.scale) {
+    const double ms = *d.scale / (d.count > 0 ? d.count : 1.0);
+    s = (float)sqrt(ms > 0 ? ms : 0.0);
+  }
+  float acc = bias[0];
+#pragma unroll
+  for (int t = 0; t < 27; t++) {
+    const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
+    const int zz = z + dz - 1, yy = y + dy - 1, xx = x + dx - 1;
+    if (xx < 0 || yy < 0 || zz < 0 || xx >= W || yy >= H || zz >= D) continue;
+    const bf16 *p = tile + (((tz + dz) * SY + (ty + dy)) * SX + tx + dx) * CS;
+    bf16 v[16];
+    *(bf16x8 *)v = *(const bf16x8 *)p;
+    *(bf16x8 *)(v + 8) = *(const bf16x8 *)(p + 8);
+#pragma unroll
+    for (int ci = 0; ci < 16; ci++) acc += w[t * 16 + ci] * from_bf16(v[ci]);
+  }
+  if (d.ring) {
+    const int64_t c = (((int64_t)(z + 1) * (H + 2)) + (y + 1)) * (W + 2) + (x + 1);
+    d.y[c] = (d.flags[c] & FL_UNKNOWN) ? acc * s : 0.0f;
+  } else {
+    d.y[(((int64_t)b * D + z) * H + y) * W + x] = acc;
   }
 }
 
@@ -798,6 +875,17 @@ static void conv_shape(int Cin, int Cout, int B, int D, int H, int W, int &TZ, i
   }
 }
 
+// c9: the LDS-tiled kernel where the volume splits into 16 x 4 x 4 tiles
+static void conv_last(xf_ctx c, const bf16 *a8, const float *w, const float *bias, const Dst9 &d, int B, int D, int H,
+                      int W) {
+  if (W % 16 == 0 && H % 4 == 0 && D % 4 == 0)
+    hipLaunchKernelGGL(k_conv_last_tile, dim3((W / 16) * (H / 4) * (D / 4) * B), dim3(256), 0, c->stream, a8, w, bias,
+                       d, B, D, H, W);
+  else
+    hipLaunchKernelGGL(k_conv_last, dim3(eblocks((int64_t)B * D * H * W)), dim3(256), 0, c->stream, a8, w, bias, d,
+                       B, D, H, W);
+}
+
 static void conv(xf_ctx c, const bf16 *src0, int C0, int up0, const bf16 *src1, int C1, const bf16 *wt, int kpad,
                  const float *bias, bf16 *out, int Cout, int B, int D, int H, int W, bool relu) {
   ConvArgs a;
@@ -983,8 +1071,7 @@ extern "C" int xf_unet_forward(xf_ctx c, xf_unet u, int B, int D, int H, int W, 
     Src0 s{r, mask, nullptr, nullptr, 0, 0, D, H, W};
     forward_body(c, u, s, B);
     Dst9 d{y, nullptr, nullptr, 0, 0};
-    hipLaunchKernelGGL(k_conv_last, dim3(eblocks((int64_t)B * D * H * W)), dim3(256), 0, c->stream, u->work.a[8],
-                       u->p + u->off_w[9], u->p + u->off_b[9], d, B, D, H, W);
+    conv_last(c, u->work.a[8], u->p + u->off_w[9], u->p + u->off_b[9], d, B, D, H, W);
     check_kernel("k_conv_last");
   });
 }
@@ -1050,8 +1137,7 @@ static double backward(xf_ctx c, xf_unet u, int B, int D, int H, int W, const fl
   forward_body(c, u, s, B);
   const int64_t n1 = (int64_t)B * D * H * W;
   Dst9 d{w.y, nullptr, nullptr, 0, 0};
-  hipLaunchKernelGGL(k_conv_last, dim3(eblocks(n1)), dim3(256), 0, c->stream, w.a[8], u->p + u->off_w[9],
-                     u->p + u->off_b[9], d, B, D, H, W);
+  conv_last(c, w.a[8], u->p + u->off_w[9], u->p + u->off_b[9], d, B, D, H, W);
   if (u->loss_mode == XF_LOSS_COSINE || u->loss_mode == XF_LOSS_ENERGY) {
     if (u->cos_B < B) {
       hipFree(u->cos_part);
@@ -1188,8 +1274,7 @@ void unet_apply_grid(xf_ctx c, xf_unet u, xf_grid g, const float *r32, float *z3
   Src0 s{r32, nullptr, g->d_flags, d_sumsq, (double)g->nunk, 1, D, H, W};
   forward_body(c, u, s, 1);
   Dst9 d{z32, g->d_flags, d_sumsq, (double)g->nunk, 1};
-  hipLaunchKernelGGL(k_conv_last, dim3(eblocks((int64_t)D * H * W)), dim3(256), 0, c->stream, u->work.a[8],
-                     u->p + u->off_w[9], u->p + u->off_b[9], d, 1, D, H, W);
+  conv_last(c, u->work.a[8], u->p + u->off_w[9], u->p + u->off_b[9], d, 1, D, H, W);
   check_kernel("k_conv_last");
 }
 
