@@ -113,3 +113,32 @@ def test_trained_network_gain(ctx):
     assert t * 3 < res["untrained"].inner_iters
     assert t * 15 < plain.iters
     assert res["trained"].ms * 1.5 < plain.ms
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 300), (16, 12, 200)])
+def test_flexible_cg_long_rows(ctx, shape):
+    """Rows of 300 cells take the flexible-CG step's 4-floats-per-lane kernels
+    with the last k-tile shifted back over the one before it, rows of 200 the
+    2-floats-per-lane ones; the solve matches the FP64 CG's field."""
+    length = tuple(s / shape[2] for s in shape)  # equal steps on all axes (the cycle needs an isotropic operator)
+    codes = cases.neumann_codes(None, [cases.box_obstacle((0.5 * length[0], 0.5 * length[1], 0.4), 0.01)],
+                                shape=shape, length=length)
+    hc = 1.0 / shape[2]
+    b = cases.divergence_rhs(codes, seed=5, h=[hc, hc, hc])
+    h = [cases.axis_steps(s, s * hc) for s in shape]
+    g = xf.Grid(ctx, h[0], h[1], h[2], codes)
+    bd = torch.from_numpy(b).cuda()
+    xr = torch.zeros_like(bd)
+    st0 = g.solve(bd, torch.zeros_like(bd), xr, xf.solve_opts(xf.XF_CG, tol=1e-11, tol_mode=xf.XF_TOL_REL,
+                                                               project_mean=1))
+    assert st0.status == 0
+    o = xf.solve_opts(xf.XF_FCG_MGNN, tol=1e-9, tol_mode=xf.XF_TOL_REL, project_mean=1, inner_tol=1e-4)
+    o.precond = None
+    x = torch.zeros_like(bd)
+    st = g.solve(bd, torch.zeros_like(bd), x, o)
+    assert st.status == 0 and st.fallback == 0, (st.status, st.fallback)
+    assert st.resid_true <= 2e-9
+    m = torch.from_numpy(codes == 1).cuda()
+    a, r = x[m] - x[m].mean(), xr[m] - xr[m].mean()
+    assert float(torch.linalg.norm(a - r) / torch.linalg.norm(r)) <= 1e-6
+    g.close()
