@@ -100,7 +100,8 @@ int xf_absmax(xf_ctx ctx, xf_grid g, const double *x, double *val, int32_t *rn_i
 #define XF_BICG 1      /* TSLAE::SolveByBiConjugateGradients, XSLAE.cpp:159-348   */
 #define XF_MR 2        /* TSLAE::SolveByMinimalResiduals, XSLAE.cpp:350-414      */
 #define XF_BICGSTAB 3  /* TSLAE::SolveByBiConjugateGradientsStab, 667-767        */
-#define XF_CG 4        /* Hestenes-Stiefel CG, fused kernels (throughput path)     */
+#define XF_CG 4        /* single-reduction CG, two fused kernels per iteration
+                          (throughput path, no reference counterpart)           */
 #define XF_CG_MP 5     /* FP32 inner CG + FP64 iterative refinement              */
 #define XF_FCG_NN 6    /* flexible CG with the learned preconditioner             */
 #define XF_FCG_MGNN 7  /* flexible CG, V-cycle preconditioner with the learned
