@@ -61,3 +61,20 @@ def test_cavity_64_runs_and_stays_divergence_free():
     assert info["div_max"] < 1e-4
     u = cav.vel[0].cpu().numpy()
     assert np.isfinite(u).all() and np.abs(u).max() <= 2.0 + 1e-9
+
+
+def test_cavity_64_ten_steps_match_oracle():
+    """The 64^3 cavity workload itself (Re = 100, dt = 1e-3): ten steps on the
+    GPU against the oracle's FP64 restatement of the same steps."""
+    ctx = xf.Context(0)
+    n, steps = 64, 10
+    cav = Cavity(ctx, n=n, tol=1e-13)
+    for _ in range(steps):
+        info = cav.step()
+    vel, p = oracle_cavity(n, steps)
+    torch.cuda.synchronize()
+    for a in range(3):
+        g = cav.vel[a].cpu().numpy()
+        d = np.linalg.norm(g - vel[a]) / max(np.linalg.norm(vel[a]), 1e-30)
+        assert d <= 1e-9, (a, d)
+    assert info["div_max"] < 1e-6
