@@ -42,7 +42,9 @@ static const int LCOUT[NL] = {16, 16, 32, 32, 64, 64, 32, 32, 16, 1};
 __device__ inline bf16 to_bf16(float f) { return (bf16)f; }
 __device__ inline float from_bf16(bf16 h) { return (float)h; }
 
-static int kpad_of(int cin) { return (27 * cin + 31) / 32 * 32; }
+// K of a convolution: 27 taps of Cin channels, zero-padded to a multiple of 96
+// (k-steps of 32, unrolled by 3)
+static int kpad_of(int cin) { return (27 * cin + 95) / 96 * 96; }
 
 // ------------------------------------------------------ MFMA convolution
 struct ConvArgs {
@@ -64,8 +66,9 @@ static uint32_t fdiv_magic(int d) { return (uint32_t)((((uint64_t)1 << 32) + (ui
 // waves) and NT 16-channel output tiles.  Small levels (a 16^3 volume is 256
 // rows of 16 cells) split Cout over nsplit blocks and use short tiles so that
 // the launch still covers the chip; every output sums the same k-steps in the
-// same order whatever the split.  The k-loop walks (tap, channel) and the
-// tap's offset in the LDS tile incrementally: no division per step.
+// same order whatever the split.  The k-loop walks (tap, channel) with
+// uniform (scalar) counters; only the lane's cell and channel group are
+// per-lane.
 template <int NT, int RPW>
 __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   extern __shared__ __attribute__((aligned(16))) bf16 tile[];
@@ -122,97 +125,94 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, lg = lane >> 4;
-  // LDS cell of (row, x = l16) at tap (0, 0, 0) for this wave's rows
-  int roff[RPW];
+  // K = 27 taps x Cin channels, k = tap * Cin + ci, zero-padded to Kpad (a
+  // multiple of 96, so the unroll by 3 below divides the step count).  Lane
+  // group lg takes k = 32 s + 8 lg + j of step s: 8 channels of one tap.  A
+  // step spans at most two taps (Cin >= 16); the walk over them is uniform
+  // (scalar): the step starts at channel c0 of tap `tap`, whose LDS offset is
+  // off_lo, and off_hi is the next tap's.  Taps past the last read tap 26
+  // (finite values against zero weights).
+  int vcell[RPW];
 #pragma unroll
   for (int r = 0; r < RPW; r++) {
     const int rho = wave + 4 * r, rz = rho / a.TY, ry = rho - rz * a.TY;
-    roff[r] = (rz * SY + ry) * SX + l16;
+    vcell[r] = ((rz * SY + ry) * SX + l16) * CS;
   }
   f32x4 acc[RPW][NT];
 #pragma unroll
   for (int r = 0; r < RPW; r++)
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0, 0, 0, 0};
-  // this lane group's k = 8 lg + 32 step = tap * Cin + ci; `off` is that
-  // k's element offset in the LDS tile relative to a row's first cell
-  // (tap offset * CS + ci), kept up to date with additions only
-  int ci = 8 * lg, tap = 0, dx = 0, dy = 0, off = 8 * lg;
-  const int sdx = CS, sdy = (SX - 3) * CS + CS, sdz = (SX * SY - 3 * SX) * CS + (SX - 3) * CS + CS;
-  auto advance = [&]() {  // k += 32
-    ci += 32;
-    off += 32;
-    while (ci >= Cin) {
-      ci -= Cin;
-      off -= Cin;
-      tap++;
-      if (++dx < 3) {
-        off += sdx;
-      } else {
-        dx = 0;
-        if (++dy < 3) {
-          off += sdy;
-        } else {
-          dy = 0;
-          off += sdz;
+  // off_hi follows tap + 1 by its digits (hx, hy) while tap + 1 <= 26 and
+  // then stays at tap 26; the offset step between consecutive taps is CS,
+  // plus jx where x wraps, plus jy where y wraps too.
+  const int jx = (SX - 3) * CS, jy = (SY - 3) * SX * CS;
+  int tap = 0, c0 = 0, off_lo = 0, off_hi = CS, hx = 1, hy = 0;
+  const int k8 = 8 * lg;
+  auto advance = [&]() {
+    c0 += 32;
+    while (c0 >= Cin) {
+      c0 -= Cin;
+      ++tap;
+      off_lo = off_hi;
+      if (tap + 1 <= 26) {
+        int inc = CS;
+        if (++hx == 3) {
+          hx = 0;
+          inc += jx;
+          if (++hy == 3) {
+            hy = 0;
+            inc += jy;
+          }
         }
+        off_hi += inc;
       }
     }
   };
-  // the start (8 lg may already pass Cin = 16)
-  ci -= 32;
-  off -= 32;
-  advance();
-  int rbase[RPW];
+  auto load_b = [&](bf16x8 *bf) {
+    const int k0 = c0 + k8;
+    const int o = k0 < Cin ? off_lo + k0 : off_hi + (k0 - Cin);
 #pragma unroll
-  for (int r = 0; r < RPW; r++) rbase[r] = roff[r] * CS;
+    for (int r = 0; r < RPW; r++) bf[r] = *(const bf16x8 *)(tile + vcell[r] + o);
+  };
   // Weight fragments come from global memory (L2-resident: every block reads
-  // the same weights) and are fetched WP k-steps ahead into a ring of
-  // registers indexed at compile time (the k-loop is unrolled by WP), so an L2
-  // round trip is paid once per WP steps instead of once per step; input
-  // fragments (LDS) are loaded one step ahead.
-  constexpr int WP = NT <= 2 ? 4 : 2;
+  // the same weights), packed in fragment order (one k-step of a 16-channel
+  // output tile is 1 KB, lane L's 16 bytes at 16 L), and fetched WP k-steps
+  // ahead into a ring of registers indexed at compile time (the k-loop is
+  // unrolled by WP, which divides Kpad / 32), so an L2 round trip is paid once
+  // per WP steps; input fragments (LDS) are loaded one step ahead.  Fetches
+  // past the last step repeat the last one.
+  constexpr int WP = 3;
   const int nsteps = a.Kpad / 32;
-  const bf16 *wrow = a.wt + (int64_t)(ct0 * 16 + l16) * a.Kpad + 8 * lg;
-  const int64_t wstride = (int64_t)16 * a.Kpad;
+  const bf16 *wbase = a.wt + (int64_t)ct0 * 16 * a.Kpad + lane * 8;
+  const int wstride = 16 * a.Kpad;
   bf16x8 aw[WP][NT];
 #pragma unroll
   for (int u = 0; u < WP; u++)
-    if (u < nsteps)
 #pragma unroll
-      for (int t = 0; t < NT; t++) aw[u][t] = *(const bf16x8 *)(wrow + u * 32 + t * wstride);
-  auto load_b = [&](bf16x8 *bf) {
-#pragma unroll
-    for (int r = 0; r < RPW; r++) {
-      bf[r] = bf16x8{};
-      if (tap < 27) bf[r] = *(const bf16x8 *)(tile + rbase[r] + off);
-    }
-  };
+    for (int t = 0; t < NT; t++) aw[u][t] = *(const bf16x8 *)(wbase + u * 512 + t * wstride);
   bf16x8 bcur[RPW];
   load_b(bcur);
   for (int s0 = 0; s0 < nsteps; s0 += WP) {
 #pragma unroll
     for (int u = 0; u < WP; u++) {
       const int s = s0 + u;
-      if (s >= nsteps) break;
       bf16x8 af[NT];
 #pragma unroll
       for (int t = 0; t < NT; t++) af[t] = aw[u][t];
-      if (s + WP < nsteps)
+      const bf16 *wq = wbase + min(s + WP, nsteps - 1) * 512;
 #pragma unroll
-        for (int t = 0; t < NT; t++) aw[u][t] = *(const bf16x8 *)(wrow + (s + WP) * 32 + t * wstride);
-      const bool more = s + 1 < nsteps;
+      for (int t = 0; t < NT; t++) aw[u][t] = *(const bf16x8 *)(wq + t * wstride);
       advance();
       bf16x8 bnext[RPW];
-      if (more) load_b(bnext);
+      load_b(bnext);
 #pragma unroll
       for (int r = 0; r < RPW; r++)
 #pragma unroll
         for (int t = 0; t < NT; t++)
           acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t], bcur[r], acc[r][t], 0, 0, 0);
-      if (more)
 #pragma unroll
-        for (int r = 0; r < RPW; r++) bcur[r] = bnext[r];
+      for (int r = 0; r < RPW; r++) bcur[r] = bnext[r];
     }
   }
   // epilogue: lane holds D[co = 16t + 4 lg + e][cell = l16]
@@ -569,26 +569,43 @@ __global__ void k_last_dx(const float *__restrict__ dy, const float *__restrict_
   }
 }
 
-// Input-gradient weights: Wt'[ci][tap'][co] = W[co][26 - tap'][ci] (bf16, K padded).
+// Packed weights are in MFMA fragment order: for 16-row tile rt, k-step s
+// (k = 32 s .. 32 s + 31) and lane L = 16 g + i, the 8 values at
+// ((rt * Kpad / 32 + s) * 64 + L) * 8 are rows 16 rt + i, k = 32 s + 8 g + j,
+// j = 0..7 (zero past the rows or K).  A wave's fragment of one step is then
+// 1 KB contiguous.
+__device__ inline void frag_index(int64_t q, int Kpad, int &row, int &k) {
+  const int64_t tile = (int64_t)16 * Kpad;
+  const int rt = (int)(q / tile);
+  const int rem = (int)(q - rt * tile);
+  const int s = rem >> 9, lane = (rem >> 3) & 63, j = rem & 7;
+  row = rt * 16 + (lane & 15);
+  k = 32 * s + 8 * (lane >> 4) + j;
+}
+
+// Input-gradient weights: Wt'[ci][k' = tap' * Cout + co] = W[co][26 - tap'][ci]
+// (bf16, fragment order)
 __global__ void k_flip_weights(const float *__restrict__ w, int Cout, int Cin, bf16 *__restrict__ wt, int Kpad) {
   const int64_t n = (int64_t)Cin * Kpad;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-    const int ci = (int)(q / Kpad), k = (int)(q % Kpad);
+    int ci, k;
+    frag_index(q, Kpad, ci, k);
     float v = 0;
-    if (k < 27 * Cout) {
-      const int tp = k / Cout, co = k % Cout;
+    if (ci < Cin && k < 27 * Cout) {
+      const int tp = k / Cout, co = k - tp * Cout;
       v = w[((int64_t)co * 27 + (26 - tp)) * Cin + ci];
     }
     wt[q] = to_bf16(v);
   }
 }
 
-// forward weights: bf16 [Cout][Kpad] from fp32 [Cout][27][Cin]
+// forward weights: bf16 W[co][k = tap * Cin + ci] (fragment order) from fp32 [Cout][27][Cin]
 __global__ void k_pack_weights(const float *__restrict__ w, int Cout, int Cin, bf16 *__restrict__ wt, int Kpad) {
   const int64_t n = (int64_t)Cout * Kpad;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-    const int co = (int)(q / Kpad), k = (int)(q % Kpad);
-    wt[q] = to_bf16(k < 27 * Cin ? w[(int64_t)co * 27 * Cin + k] : 0.0f);
+    int co, k;
+    frag_index(q, Kpad, co, k);
+    wt[q] = to_bf16(co < Cout && k < 27 * Cin ? w[(int64_t)co * 27 * Cin + k] : 0.0f);
   }
 }
 
@@ -895,6 +912,7 @@ static void conv(xf_ctx c, const bf16 *src0, int C0, int up0, const bf16 *src1, 
   const int Cin = C0 + C1;
   XF_REQUIRE(W % 16 == 0 && H % 4 == 0 && D % 4 == 0 && Cin % 8 == 0 && Cout % 16 == 0 && C0 % 8 == 0, XF_EINVAL,
              "unet conv: unsupported shape");
+  XF_REQUIRE(Cin >= 16, XF_EINVAL, "unet conv: at least 16 input channels");
   int NT = Cout / 16;
   a.TZ = a.TY = 0;
   conv_shape(Cin, Cout, B, D, H, W, a.TZ, a.TY, NT);
