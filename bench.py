@@ -30,7 +30,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(case, gpu_iters, sample_iters):
+def _size(case):
+    return "x".join(str(d - 2) for d in case["codes"].shape)
+
+
+def cpu_port(case, gpu_iters, sample_iters):
     """The CPU oracle's FP64 CG (OpenMP stencil, serial dot products), timed
     for a bounded number of iterations and scaled to the GPU's iteration count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -45,8 +49,49 @@ def cpu_baseline(case, gpu_iters, sample_iters):
     per_iter_ms = dt * 1e3 / max(it, 1)
     return {"value": per_iter_ms * gpu_iters, "unit": "ms/solve", "cores": threads, "kind": "port",
             "sample": "%d iterations of the oracle FP64 single-reduction CG (xo_cg_cg, %d OpenMP threads for the stencil, serial "
-                      "dot products) at the same 256^3 case, %.1f ms/iteration, scaled by the GPU solve's %d "
-                      "iterations" % (it, threads, per_iter_ms, gpu_iters)}
+                      "dot products) at the same %s case, %.1f ms/iteration, scaled by the GPU solve's %d "
+                      "iterations" % (it, threads, _size(case), per_iter_ms, gpu_iters)}
+
+
+REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def cpu_reference(case, gpu_iters, sample_iters):
+    """The reference's own CG (TSLAE::SolveByConjugateGradients, XSLAE.cpp:56-157,
+    compiled from its sources into oracle/_ref/ref_harness; serial: the
+    reference builds without OpenMP and its CG has no parallel region) on the
+    same case, stopped after `sample_iters` iterations, its steady-state time
+    per iteration scaled to the GPU solve's iteration count.  None when the
+    harness was not built."""
+    if not os.path.exists(REF_HARNESS):
+        return None
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import xf_oracle as xo
+    from xfb import read_xfb, write_xfb
+    codes = np.ascontiguousarray(case["codes"], dtype=np.int32)
+    nodes = lambda h: np.concatenate([[0.0], np.cumsum(h)])
+    inp = {"x": nodes(case["hx"]), "y": nodes(case["hy"]), "z": nodes(case["hz"]), "codes": codes,
+           "space_mask": xo.space_mask_of(codes), "bvec": case["b"], "x0": np.zeros_like(case["b"]),
+           "max_sweeps": np.array([3 + 2 * sample_iters], dtype=np.int32)}
+    with tempfile.TemporaryDirectory() as td:
+        fi, fo = os.path.join(td, "in.xfb"), os.path.join(td, "out.xfb")
+        write_xfb(fi, inp)
+        del inp
+        p = subprocess.run([REF_HARNESS, "cg_timed", fi, fo], stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                           text=True, timeout=600)
+        if p.returncode != 0:
+            raise RuntimeError("reference CG timing failed: %s" % p.stderr[-400:])
+        o = read_xfb(fo)
+    it = int(o["iters"][0])
+    per_iter_ms = float(o["seconds"][0]) * 1e3 / it
+    return {"value": per_iter_ms * gpu_iters, "unit": "ms/solve", "cores": 1, "kind": "reference",
+            "sample": "%d iterations of the reference's TSLAE::SolveByConjugateGradients (XSLAE.cpp:56-157, "
+                      "compiled -O2 from its sources; serial, as the reference builds it) at the same %s case "
+                      "after its first three operator sweeps, %.1f ms/iteration, scaled by the GPU solve's %d "
+                      "iterations (set-up of its masks and vectors, %.1f s, not counted)"
+                      % (it, _size(case), per_iter_ms, gpu_iters, float(o["setup_seconds"][0]))}
 
 
 def load_network(ctx, path):
@@ -69,6 +114,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--cpu-ref-iters", type=int, default=12)
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--skip-fullres", action="store_true")
     ap.add_argument("--io", action="store_true", help="also time the output files (flow VTK, .npy) of the solution")
@@ -225,9 +271,14 @@ def main():
     it_mean = plain["iterations"]
     mcups = fluid * it_mean / (wall_ms * 1e-3) / 1e6
 
-    cpu = None
+    cpu = port = None
     if not args.no_cpu:
-        cpu = cpu_baseline(case, int(it_mean), args.cpu_iters)
+        log("CPU baseline: the reference CG for %d iterations" % args.cpu_ref_iters)
+        cpu = cpu_reference(case, int(it_mean), args.cpu_ref_iters)
+        log("CPU baseline: the oracle CG for %d iterations" % args.cpu_iters)
+        port = cpu_port(case, int(it_mean), args.cpu_iters)
+        if cpu is None:
+            cpu, port = port, None
 
     out = {
         "metric": "ms per pressure solve to 1e-8 rel. residual at 256³ (plain vs NN-precond MP)",
@@ -261,6 +312,8 @@ def main():
                      "iteration_gbs": round((ka_bytes + kb_bytes) / ((ka["ms"] + kb["ms"]) * 1e-3) / 1e9, 1)},
         "cpu_baseline": cpu,
     }
+    if port is not None:
+        out["cpu_oracle_port"] = port
     out["host_boundary"] = {"pcie_round_trip_ms": round(pcie_ms, 3),
                             "plain_fp64_cg_ms_with_pcie": round(wall_ms + pcie_ms, 3)}
     nn = solvers.get("nn_preconditioned_mp")

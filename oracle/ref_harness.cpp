@@ -2,8 +2,9 @@
 //
 // Links the reference translation units (compiled from /root/reference by
 // oracle/Makefile) and drives the hot-path classes on small synthetic inputs,
-// so that tests/golden/make_golden.py can store their outputs as fixtures.
-// Nothing here is shipped or timed; the product never links it.
+// so that tests/golden/make_golden.py can store their outputs as fixtures,
+// and times the reference CG for bench.py's CPU baseline (cg_timed).  Nothing
+// here is shipped; the product never links it.
 //
 //   ref_harness <command> <in.xfb> <out.xfb>
 //
@@ -15,6 +16,11 @@
 //   cg|bicg|mr|bicgstab
 //             TSLAE solvers (XSLAE.cpp:56-414, 667-767); the iteration count is
 //             recovered by counting operator sweeps
+//   cg_timed  TSLAE::SolveByConjugateGradients stopped after max_sweeps operator
+//             sweeps (the three before the first iteration plus two per
+//             iteration); reports the wall time of the sweeps after the first
+//             three (`seconds`, steady-state iterations only), `iters` = their
+//             iteration count, and `setup_seconds` (masks, Rn model, vectors)
 //   thomas    TSLAE::SolveByScalarRunning (XSLAE.cpp:416-445)
 //   lod       THydrodynamicsProblem::SolveTransferEquation__ (XProblem.cpp:2784-3475)
 //   ib        TImmersedBoundaryProblem::Interpolate / SpreadForce / Index
@@ -41,6 +47,7 @@
 #include <string>
 #include <vector>
 #include <stdexcept>
+#include <chrono>
 
 #include "XSLAE.h"
 #include "XOperator.h"
@@ -225,13 +232,19 @@ static Arr dump_vec(const Setup& s, TRnSpace& vv) {
 // operator~ the identity so BiCG can run on the symmetric (uniform-grid,
 // Actual/Fictive) masks it is tested on.  Arithmetic is the reference's own.
 static long long g_evals = 0;
+// cg_timed: wall clock when g_evals reaches g_mark, and a budget of
+// evaluations after which the solve is abandoned by a BudgetReached throw
+static long long g_mark = -1, g_budget = -1;
+static std::chrono::steady_clock::time_point g_t_mark;
+struct BudgetReached {};
 
 class CountingLaplace : public T3dLaplaceOperator {
  public:
   CountingLaplace(int model, int gc, const TNodesGrid* const* g)
       : TRnLinearOperator(model), T3dLaplaceOperator(model, gc, g), model_(model), grid_(g[0]) {}
   double operator()(int id, int i, int j, int k) const override {
-    ++g_evals;
+    if (++g_evals == g_mark) g_t_mark = std::chrono::steady_clock::now();
+    if (g_evals == g_budget) throw BudgetReached();
     return T3dLaplaceOperator::operator()(id, i, j, k);
   }
   void operator~() const override {}
@@ -368,6 +381,42 @@ static void cmd_solver(const Bag& in, Bag& out, const std::string& which) {
   it.i[0] = (int)sweeps;
   it.i[1] = (int)(g_evals % (RN ? RN : 1));
   out["sweeps"] = it;
+}
+
+static void cmd_cg_timed(const Bag& in, Bag& out) {
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  Setup s = make_setup(in);
+  const TNodesGrid* grids[1] = {s.grid};
+  CountingLaplace* A = new CountingLaplace(s.model_name, 1, grids);
+  TRnRelease3dSpace* x0 = make_vec(s, in, "x0");
+  TRnRelease3dSpace* b = make_vec(s, in, "bvec");
+  TSLAE* slae = new TSLAE(*A, *x0, *b);
+  const long long RN = count_unknowns(s);
+  const int max_sweeps = iscalar(in, "max_sweeps");
+  if (max_sweeps < 5 || (max_sweeps - 3) % 2 != 0) throw std::runtime_error("max_sweeps must be 3 + 2 k, k >= 1");
+  const double setup = std::chrono::duration<double>(clk::now() - t0).count();
+  g_evals = 0;
+  g_mark = 3 * RN;  // the evaluation that starts the fourth sweep
+  g_budget = (long long)max_sweeps * RN + 1;  // the first evaluation past the budget
+  bool stopped = false;
+  try {
+    slae->SolveByConjugateGradients();
+  } catch (const BudgetReached&) {
+    stopped = true;
+  }
+  const auto t1 = clk::now();
+  g_mark = g_budget = -1;
+  if (!stopped) throw std::runtime_error("the solve converged inside the sweep budget");
+  Arr sec = mk_d({1});
+  sec.d[0] = std::chrono::duration<double>(t1 - g_t_mark).count();
+  out["seconds"] = sec;
+  Arr su = mk_d({1});
+  su.d[0] = setup;
+  out["setup_seconds"] = su;
+  Arr it = mk_i({1});
+  it.i[0] = (max_sweeps - 3) / 2;
+  out["iters"] = it;
 }
 
 static void cmd_pack(const Bag& in, Bag& out) {
@@ -624,6 +673,7 @@ int main(int argc, char** argv) {
     if (cmd == "op") cmd_op(in, out, false);
     else if (cmd == "divop") cmd_op(in, out, true);
     else if (cmd == "cg" || cmd == "bicg" || cmd == "mr" || cmd == "bicgstab") cmd_solver(in, out, cmd);
+    else if (cmd == "cg_timed") cmd_cg_timed(in, out);
     else if (cmd == "thomas") cmd_thomas(in, out);
     else if (cmd == "lod") cmd_lod(in, out);
     else if (cmd == "ib") cmd_ib(in, out);
