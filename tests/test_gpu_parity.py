@@ -197,3 +197,36 @@ def test_verify_flag_skips_only_the_check(ctx, method):
     assert (s1.iters, s1.inner_iters) == (s0.iters, s0.inner_iters)
     assert np.array_equal(x1, x0)
     g.close()
+
+
+CONFIG2_NORMS = {"cg3": (xf.XF_CG3, 1e3), "bicg": (xf.XF_BICG, 1.0), "mr": (xf.XF_MR, 10.0)}
+
+
+@pytest.mark.parametrize("solver", ["cg3", "bicg", "mr"])
+def test_config2_sphere128_reference_solvers(ctx, solver):
+    """The 128^3 sphere workload solved by the reference solvers (their own
+    absolute tolerances; b scaled so that each means 1e-8 relative) against
+    the oracle's summaries at the same size (tests/golden/make_config2.py:
+    809, 810 and 151234 iterations): iterations within 2, the field's norm
+    over the fluid and 512 sampled values within 1e-10 (measured: the same
+    iteration counts, differences 1e-14 to 1e-13)."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "config2_sphere128.npz"))
+    method, norm = CONFIG2_NORMS[solver]
+    case = sphere_case(128)
+    b = case["b"] * (norm / np.linalg.norm(case["b"]))
+    g = xf.Grid(ctx, case["hx"], case["hy"], case["hz"], case["codes"])
+    bd = dev(b)
+    x = torch.zeros_like(bd)
+    st = g.solve(bd, torch.zeros_like(bd), x, xf.solve_opts(method))
+    assert st.status == 0, st.status
+    it_ref = int(fx[solver + "_iters"][0])
+    assert abs(st.iters - it_ref) <= 2, (st.iters, it_ref)
+    tol = 1e-10
+    xs = host(x).ravel()
+    fluid = case["codes"].ravel() == 1
+    dn = abs(np.linalg.norm(xs[fluid]) - fx[solver + "_norm"][0]) / fx[solver + "_norm"][0]
+    dx = rel_l2(xs[fx["idx"]], fx[solver + "_x"])
+    print("%s: %d iterations (oracle %d), norm %.2e, samples %.2e" % (solver, st.iters, it_ref, dn, dx))
+    assert dn <= tol and dx <= tol, (dn, dx)
+    g.close()
