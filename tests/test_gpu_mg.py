@@ -45,7 +45,10 @@ def test_hierarchy_masks_match_numpy(ctx, shape, dirichlet):
     assert nlev == (3 if shape[2] == 256 else 2)
 
 
-@pytest.mark.parametrize("shape,dirichlet", [((8, 8, 256), False), ((16, 12, 128), True), ((20, 16, 64), True)])
+# (8, 10, 300): the residual restricted in the sweep pass with a shifted last
+# k-tile and a block whose last two rows lie past the grid
+@pytest.mark.parametrize("shape,dirichlet", [((8, 8, 256), False), ((16, 12, 128), True), ((20, 16, 64), True),
+                                             ((8, 10, 300), True)])
 def test_cycle_matches_restatement(ctx, shape, dirichlet):
     codes, h = case_of(shape, 12, dirichlet)
     g = xf.Grid(ctx, h[0], h[1], h[2], codes)

@@ -86,6 +86,7 @@ __device__ inline float jacobi0(const uint8_t *__restrict__ flags, const float *
 
 // MODE 0: out = omega D^-1 r;  MODE 1: out = z + omega D^-1 (r - A z);
 // MODE 2: out = r - A z;  MODE 3: out = A z;  MODE 5: MODE 1 on z = omega D^-1 r
+// (MODE 6, vector kernel only: MODE 2 restricted to the coarse grid in the pass)
 // (the first two sweeps from z = 0 in one pass: z recomputed at the seven
 // stencil points, never stored).  0 at cells that are not Actual.
 template <int MODE>
@@ -150,6 +151,10 @@ __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, con
     float v[CPL];
   };
   __shared__ float lut[64];
+  // MODE 6: the odd rows' residuals and unknown bits for the even rows below
+  // them, one buffer per plane parity
+  __shared__ float xr[2][2][MODE == 6 ? 64 * CPL : 1];
+  __shared__ uint8_t xu[2][2][MODE == 6 ? 64 * CPL : 1];
   fill_weights(lut, cx, cy, cz);
   // z (+ the parent's coarse correction at unknowns for MODE 4; omega D^-1 r
   // for MODE 5) of CPL cells from k
@@ -200,27 +205,36 @@ __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, con
     const int k0 = min(1 + ktile * 64 * CPL, d.m - 1 - 64 * CPL) + lane * CPL;
     const int j = 1 + jtile * 4 + (threadIdx.x >> 6);
     const int i0 = 1 + itile * tl.ic, i1 = min(i0 + tl.ic, d.n - 1);
-    if (j > d.l - 2) continue;
+    // MODE 6 exchanges rows between the waves of a block at a barrier per
+    // plane, so rows past the grid keep marching (without memory accesses);
+    // the grid's even extents make both rows of a pair active or neither
+    const bool act = j <= d.l - 2;
+    if (!act && MODE != 6) continue;
+    const int jj = act ? j : d.l - 2;  // a valid row for the coefficient reads
     float czm[CPL], czp[CPL];
 #pragma unroll
     for (int e = 0; e < CPL; e++) {
       czm[e] = cz[2 * (k0 + e)];
       czp[e] = cz[2 * (k0 + e) + 1];
     }
-    const float cym = cy[2 * j], cyp = cy[2 * j + 1];
-    int64_t c = ((int64_t)i0 * d.l + j) * d.m + k0;
+    const float cym = cy[2 * jj], cyp = cy[2 * jj + 1];
+    int64_t c = ((int64_t)i0 * d.l + jj) * d.m + k0;
     V zm{}, zc{};
-    if (MODE != 0) {
+    if (MODE != 0 && act) {
       zm = zrow(c - sI, i0 - 1, j, k0);
       zc = zrow(c, i0, j, k0);
     }
+    float rsum[CPL / 2];
+    int rcnt[CPL / 2];
     for (int i = i0; i < i1; i++, c += sI) {
-      uint8_t fl[CPL];
-      __builtin_memcpy(fl, flags + c, CPL);
+      uint8_t fl[CPL] = {};
       V rr{}, zp{}, zjm{}, zjp{}, o;
-      if (MODE != 3) __builtin_memcpy(&rr, r + c, sizeof(V));
+      if (act) {
+        __builtin_memcpy(fl, flags + c, CPL);
+        if (MODE != 3) __builtin_memcpy(&rr, r + c, sizeof(V));
+      }
       float left = 0, right = 0;
-      if (MODE != 0) {
+      if (MODE != 0 && act) {
         zp = zrow(c + sI, i + 1, j, k0);
         zjm = zrow(c - sJ, i, j - 1, k0);
         zjp = zrow(c + sJ, i, j + 1, k0);
@@ -246,12 +260,54 @@ __global__ __launch_bounds__(256) void k_mg_sweep_vec(Dims d, SweepTiles tl, con
           float tz = ((f & FL_KP) ? czp[e] * (kp - x0) : 0.f) - ((f & FL_KM) ? czm[e] * (x0 - km) : 0.f);
           const float az = -(tx + ty + tz);
           if (MODE == 1 || MODE == 4 || MODE == 5) v = x0 + wd * (rr.v[e] - az);
-          else if (MODE == 2) v = rr.v[e] - az;
+          else if (MODE == 2 || MODE == 6) v = rr.v[e] - az;
           else v = az;
         }
         o.v[e] = (f & FL_ACTUAL) ? v : 0.f;
       }
-      __builtin_memcpy(out + c, &o, sizeof(V));
+      if (MODE != 6) {
+        __builtin_memcpy(out + c, &o, sizeof(V));
+      } else {
+        // coarse r = mean of the residual over the unknown children, summed
+        // in k_mg_restrict's order: plane, then row, then k; the even row of
+        // each pair (wave 0 or 2) adds the odd row's values from LDS.  Cells
+        // of a shifted last k-tile repeat the tile before: the same coarse
+        // values are written twice.
+        const int w = threadIdx.x >> 6, par = (i - i0) & 1;
+        if (w & 1) {
+#pragma unroll
+          for (int e = 0; e < CPL; e++) {
+            xr[par][w >> 1][lane * CPL + e] = o.v[e];
+            xu[par][w >> 1][lane * CPL + e] = fl[e] & FL_UNKNOWN;
+          }
+        }
+        __syncthreads();
+        if (!(w & 1) && act) {
+#pragma unroll
+          for (int p = 0; p < CPL / 2; p++) {
+            float sp = par ? rsum[p] : 0.f;
+            int cn = par ? rcnt[p] : 0;
+#pragma unroll
+            for (int e = 2 * p; e < 2 * p + 2; e++)
+              if (fl[e] & FL_UNKNOWN) {
+                sp += o.v[e];
+                cn++;
+              }
+#pragma unroll
+            for (int e = 2 * p; e < 2 * p + 2; e++)
+              if (xu[par][w >> 1][lane * CPL + e]) {
+                sp += xr[par][w >> 1][lane * CPL + e];
+                cn++;
+              }
+            rsum[p] = sp;
+            rcnt[p] = cn;
+            if (par) {
+              const int64_t q = ((int64_t)(i >> 1) * cd.l + ((j + 1) >> 1)) * cd.m + ((k0 + 2 * p + 1) >> 1);
+              out[q] = cn ? sp / (float)cn : 0.f;
+            }
+          }
+        }
+      }
       if (MODE != 0) {
         zm = zc;
         zc = zp;
@@ -383,9 +439,11 @@ static void sweep(xf_ctx c, const xf_grid g, const float *r, const float *z, flo
     else
       hipLaunchKernelGGL((k_mg_sweep_vec<MODE, 2>), dim3(nb), dim3(256), 0, c->stream, d, tl, g->d_flags, g->d_cxf,
                          g->d_cyf, g->d_czf, r, z, out, zcoarse, cd);
-  } else {
+  } else if constexpr (MODE != 6) {
     hipLaunchKernelGGL(k_mg_sweep<MODE == 4 ? 1 : MODE>, cell_grid(g), dim3(256), 0, c->stream, d, g->d_flags,
                        g->d_cxf, g->d_cyf, g->d_czf, r, z, out);
+  } else {
+    XF_REQUIRE(false, XF_EINVAL, "multigrid: the fused restriction needs rows of >= 128 cells");
   }
 }
 
@@ -573,8 +631,13 @@ static void vcycle(xf_ctx c, MgHier *h, int k, const float *r, struct xf_unet_s 
   const xf_grid g = L.g;
   const Dims d = dims_of(g), cd = dims_of(C.g);
   smooth_from_zero(c, L, r, kSweeps);
-  sweep<2>(c, g, r, L.z, L.t);
-  hipLaunchKernelGGL(k_mg_restrict, cell_grid(C.g), dim3(256), 0, c->stream, d, cd, g->d_flags, L.t, C.r);
+  if (g->m - 2 >= 128) {
+    // the residual restricted in the same pass (the fine residual is not stored)
+    sweep<6>(c, g, r, L.z, C.r, nullptr, cd);
+  } else {
+    sweep<2>(c, g, r, L.z, L.t);
+    hipLaunchKernelGGL(k_mg_restrict, cell_grid(C.g), dim3(256), 0, c->stream, d, cd, g->d_flags, L.t, C.r);
+  }
   vcycle(c, h, k + 1, C.r, net, nullptr);
   smooth(c, L, r, kSweeps, last, &C);
 }
