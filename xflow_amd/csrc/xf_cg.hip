@@ -353,11 +353,20 @@ __global__ __launch_bounds__(256) void k_cgcg_step(int n, int l, int m, Tiles tl
     return q;
   };
   auto pnew1 = [&](int64_t at) { return first ? r[at] : r[at] + bt * pin[at]; };
+  // p at the rows j - 1 and j + 1 comes from the waves that own those rows,
+  // through LDS (slot s holds row j0 - 1 + s of the block's rows j0 ..
+  // j0 + TJ - 1; waves 0 and TJ - 1 also form the two halo rows), one buffer
+  // per plane parity and one barrier per plane.  Rows past the interior keep
+  // marching so that every wave reaches every barrier; the first of them (the
+  // array's last row) still publishes its p for the row below it.
+  __shared__ V ldsq[2][TJ + 2][64];
+  const int wv = threadIdx.x >> 6;
   for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
     const int t = xcd * per + tq;
     if (t >= tl.total) break;
     CgTile w;
-    if (!cg_tile<CPL>(tl, t, n, l, m, w)) continue;
+    const bool act = cg_tile<CPL>(tl, t, n, l, m, w);  // row inside the interior
+    const bool ld = w.j <= l - 1;                       // row inside the array
     const int k0 = w.k0;
     T czm[CPL], czp[CPL];
 #pragma unroll
@@ -365,16 +374,29 @@ __global__ __launch_bounds__(256) void k_cgcg_step(int n, int l, int m, Tiles tl
       czm[e] = cz[2 * (k0 + e)];
       czp[e] = cz[2 * (k0 + e) + 1];
     }
-    const T cym = cy[2 * w.j], cyp = cy[2 * w.j + 1];
+    const T cym = act ? cy[2 * w.j] : T(0), cyp = act ? cy[2 * w.j + 1] : T(0);
     int64_t c = w.c;
-    V qm, qc, rc, pc, tr, tp;
-    qm = pnew(c - sI, tr, tp);
-    qc = pnew(c, rc, pc);
+    V qm{}, qc{}, rc{}, pc{}, tr, tp;
+    if (ld) {
+      qm = pnew(c - sI, tr, tp);
+      qc = pnew(c, rc, pc);
+    }
     for (int i = w.i0; i < w.i1; i++, c += sI) {
-      V rp, pp, rjm, pjm, rjp, pjp;
-      const V qp = pnew(c + sI, rp, pp);
-      const V qjm = pnew(c - sJ, rjm, pjm);
-      const V qjp = pnew(c + sJ, rjp, pjp);
+      V rp{}, pp{}, rh, ph;
+      V qp{};
+      if (ld) qp = pnew(c + sI, rp, pp);
+      const int par = (i - w.i0) & 1;
+      ldsq[par][wv + 1][lane] = qc;
+      if (wv == 0) ldsq[par][0][lane] = pnew(c - sJ, rh, ph);
+      if (wv == TJ - 1 && act) ldsq[par][TJ + 1][lane] = pnew(c + sJ, rh, ph);
+      __syncthreads();
+      const V qjm = ldsq[par][wv][lane];
+      const V qjp = ldsq[par][wv + 2][lane];
+      if (!act) {
+        qm = qc;
+        qc = qp;
+        continue;
+      }
       VX xx;
       if (xup) __builtin_memcpy(&xx, x + c, sizeof(VX));
       uint8_t fl[CPL];
@@ -409,6 +431,7 @@ __global__ __launch_bounds__(256) void k_cgcg_step(int n, int l, int m, Tiles tl
       rc = rp;
       pc = pp;
     }
+    __syncthreads();  // the next tile's first plane reuses buffer 0
   }
 }
 
