@@ -23,7 +23,7 @@ HBM_PEAK_GBS = 8000.0
 # measured HBM traffic of one k_cgcg_step<double,double> launch at 256^3 (mean
 # of an even and an odd iteration; rocprofv3 counters scaled by a copy of
 # known size, as recorded in profiles/cg_kernels_traffic_pmc.md)
-KB_TRAFFIC_BYTES = 861.0e6
+KB_TRAFFIC_BYTES = 772.1e6
 
 
 def log(*a):
@@ -303,7 +303,7 @@ def main():
                      "traffic": KB_TRAFFIC_BYTES if args.n == 256 else None,
                      "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by "
                                      "a 512 MiB copy (profiles/cg_kernels_traffic_pmc.md); 688 MB algorithmic, "
-                                     "the excess is halo rows read again",
+                                     "the excess is the blocks' halo rows and planes read again",
                      "kernel": "k_cgcg_step<double,double,2> (41 B per interior cell, mean of 33 and 49)",
                      "kernel_us": round(kb["ms"] * 1e3, 2), "launches": kb["count"],
                      "dots_kernel": "k_cgcg_dots<double,2> (9 B per interior cell)",
