@@ -118,7 +118,8 @@ def test_trained_network_gain(ctx):
     assert res["trained"].ms * 1.5 < plain.ms
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 300), (16, 12, 200)])
+# (16, 14, 200): a block whose last two rows lie past the grid
+@pytest.mark.parametrize("shape", [(16, 16, 300), (16, 12, 200), (16, 14, 200)])
 def test_flexible_cg_long_rows(ctx, shape):
     """Rows of 300 cells take the flexible-CG step's 4-floats-per-lane kernels
     with the last k-tile shifted back over the one before it, rows of 200 the

@@ -946,22 +946,42 @@ __global__ __launch_bounds__(256) void k_fcg_dir_pq(int n, int l, int m, Tiles t
     return q;
   };
   auto pnew1 = [&](int64_t at) { return nob ? z[at] : z[at] + beta * pin[at]; };
+  // p at the rows j +- 1 through LDS, as in k_cgcg_step
+  __shared__ V ldsq[2][TJ + 2][64];
+  const int wv = threadIdx.x >> 6;
   for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
     const int t = xcd * per + tq;
     if (t >= tl.total) break;
     CgTile w;
-    if (!cg_tile<CPL>(tl, t, n, l, m, w)) continue;
+    const bool act = cg_tile<CPL>(tl, t, n, l, m, w);  // row inside the interior
+    const bool ld = w.j <= l - 1;                       // row inside the array
     float czm[CPL], czp[CPL];
 #pragma unroll
     for (int e = 0; e < CPL; e++) {
       czm[e] = cz[2 * (w.k0 + e)];
       czp[e] = cz[2 * (w.k0 + e) + 1];
     }
-    const float cym = cy[2 * w.j], cyp = cy[2 * w.j + 1];
+    const float cym = act ? cy[2 * w.j] : 0.f, cyp = act ? cy[2 * w.j + 1] : 0.f;
     int64_t c = w.c;
-    V qm = pnew(c - sI), qc = pnew(c);
+    V qm{}, qc{};
+    if (ld) {
+      qm = pnew(c - sI);
+      qc = pnew(c);
+    }
     for (int i = w.i0; i < w.i1; i++, c += sI) {
-      const V qp = pnew(c + sI), qjm = pnew(c - sJ), qjp = pnew(c + sJ);
+      V qp{};
+      if (ld) qp = pnew(c + sI);
+      const int par = (i - w.i0) & 1;
+      ldsq[par][wv + 1][lane] = qc;
+      if (wv == 0) ldsq[par][0][lane] = pnew(c - sJ);
+      if (wv == TJ - 1 && act) ldsq[par][TJ + 1][lane] = pnew(c + sJ);
+      __syncthreads();
+      const V qjm = ldsq[par][wv][lane], qjp = ldsq[par][wv + 2][lane];
+      if (!act) {
+        qm = qc;
+        qc = qp;
+        continue;
+      }
       V rc;
       __builtin_memcpy(&rc, r + c, sizeof(V));
       uint8_t fl[CPL];
@@ -989,6 +1009,7 @@ __global__ __launch_bounds__(256) void k_fcg_dir_pq(int n, int l, int m, Tiles t
       qm = qc;
       qc = qp;
     }
+    __syncthreads();  // the next tile's first plane reuses buffer 0
   }
   apr = block_sum_all(apr, lds);
   apq = block_sum_all(apq, lds);
