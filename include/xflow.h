@@ -146,6 +146,20 @@ void xf_solve_opts_default(int method, xf_solve_opts *o);
  * as the reference's vector copies keep them).  Device pointers. */
 int xf_solve(xf_ctx ctx, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts *opts,
              xf_solve_stats *stats);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): xf_solve with opts->method XF_CG (the plain FP64 CG,
+ * same kernels and same results), its work vectors in one fixed array per
+ * device that the library holds from load to process end (72 MiB: grids of
+ * at most XF_NOALLOC_MAX_CELLS = n*l*m cells; larger grids give XF_EINVAL).
+ * It calls no hipMalloc/hipFree and records no memset or memcpy (state
+ * cleared and x0 copied by kernels).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (two streams or threads) race and
+ * must be serialised by the caller.  opts->verify uses the same array after
+ * the solve.  Per-kernel profiling (xf_profile_enable) is not sampled. */
+#define XF_NOALLOC_MAX_CELLS 2358272
+int xf_solve_noalloc(xf_ctx ctx, xf_grid g, const double *b, const double *x0, double *x,
+                     const xf_solve_opts *opts, xf_solve_stats *stats);
 
 /* ------------------------------------------------------------ transport */
 /* One LOD transport step, THydrodynamicsProblem::SolveTransferEquation__

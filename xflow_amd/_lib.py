@@ -58,6 +58,7 @@ SIGNATURES = {
     "xf_absmax": (I, [P, P, P, C.POINTER(D), C.POINTER(C.c_int32)]),
     "xf_solve_opts_default": (None, [I, C.POINTER(SolveOpts)]),
     "xf_solve": (I, [P, P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "xf_solve_noalloc": (I, [P, P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "xf_lod_transport": (I, [P, P, P, P, P, P, P, P, P, D, I]),
     "xf_thomas_batched": (I, [P, I, I, P, P, P, P, P]),
     "xf_ib_interp": (I, [P, P, I, I, P, P, P]),

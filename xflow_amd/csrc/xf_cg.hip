@@ -39,6 +39,8 @@
 // order, partials added in block order.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "xf_internal.h"
 
@@ -639,8 +641,7 @@ static Tiles march_tiles(xf_grid g, int cpl, int max_planes) {
 
 template <typename T, typename X, int CPL>
 static void cgcg_run(xf_ctx c, xf_grid g, X *x, T *const R[2], T *const P[2], const T *cx, const T *cy,
-                     const T *cz, int maxit, int every) {
-  CgWork *wk = g->work;
+                     const T *cz, double *part, int maxit, int every) {
   const Tiles ta = march_tiles(g, CPL, CgShape<T>::max_planes);
   const Tiles &tb = ta;
   const int nbA = std::min(ta.blocks, kCgTileBlocks);
@@ -661,14 +662,14 @@ static void cgcg_run(xf_ctx c, xf_grid g, X *x, T *const R[2], T *const P[2], co
       T *pin = P[it & 1], *pout = P[(it + 1) & 1];
       int e1 = pm ? prof_begin(c) : -1;
       hipLaunchKernelGGL((k_cgcg_dots<T, CPL>), dim3(nbA), dim3(256), 0, c->stream, g->n, g->l, g->m, ta,
-                         g->d_flags, cx, cy, cz, rin, c->state, wk->part);
+                         g->d_flags, cx, cy, cz, rin, c->state, part);
       if (pm) {
         prof_end(c, e1);
         marks.push_back({n1, e1});
       }
       int e2 = pm ? prof_begin(c) : -1;
       hipLaunchKernelGGL((k_cgcg_step<T, X, CPL>), dim3(nbB), dim3(256), 0, c->stream, g->n, g->l, g->m, tb,
-                         nbA, wk->part, g->d_flags, cx, cy, cz, x, rin, rout, pin, pout, c->state, it, maxit);
+                         nbA, part, g->d_flags, cx, cy, cz, x, rin, rout, pin, pout, c->state, it, maxit);
       if (pm) {
         prof_end(c, e2);
         marks.push_back({n2, e2});
@@ -701,21 +702,22 @@ static void cgcg_run(xf_ctx c, xf_grid g, X *x, T *const R[2], T *const P[2], co
 }
 
 // Runs single-reduction CG iterations on A e = r from p = r (r zero-padded,
-// in r0; x the FP64 or FP32 accumulator; r1, p0, p1 work vectors).  State
+// in r0; x the FP64 or FP32 accumulator; r1, p0, p1 work vectors, zero on
+// the array's outer ring; part the dot kernel's partials).  State
 // must hold HS_STOP2 and done = 0 (k_hs_start).  Polls the device state every
 // `every` iterations; h_state->iter counts the updates applied.  The final
 // residual is left in r0 or r1 (by the parity of the update count).
 template <typename T, typename X>
 static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r0, T *r1, T *p0, T *p1, const T *cx, const T *cy,
-                            const T *cz, int maxit, int every) {
+                            const T *cz, double *part, int maxit, int every) {
   T *const R[2] = {r0, r1};
   T *const P[2] = {p0, p1};
   // 16-byte vectors per lane where a row holds a whole wave of them
   if constexpr (sizeof(T) == 4) {
-    if (g->m - 2 >= 64 * 4) return cgcg_run<T, X, 4>(c, g, x, R, P, cx, cy, cz, maxit, every);
+    if (g->m - 2 >= 64 * 4) return cgcg_run<T, X, 4>(c, g, x, R, P, cx, cy, cz, part, maxit, every);
   }
-  if (g->m - 2 >= 64 * 2) return cgcg_run<T, X, 2>(c, g, x, R, P, cx, cy, cz, maxit, every);
-  cgcg_run<T, X, 1>(c, g, x, R, P, cx, cy, cz, maxit, every);
+  if (g->m - 2 >= 64 * 2) return cgcg_run<T, X, 2>(c, g, x, R, P, cx, cy, cz, part, maxit, every);
+  cgcg_run<T, X, 1>(c, g, x, R, P, cx, cy, cz, part, maxit, every);
 }
 
 // r = b - A x (fast rows) into rout as type R: s[HS_AUX] = (r, r) in FP64,
@@ -752,25 +754,121 @@ static void upload_state_zero(xf_ctx c) {
   XF_HIP(hipMemsetAsync(c->state, 0, sizeof(DevState), c->stream));
 }
 
-// Plain FP64 CG from x0 on b - A x0.
-void solve_hs_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
-                 xf_solve_stats &s) {
-  require_fast(g);
-  CgWork *w = work(c, g, false);
+// Fill and copy kernels for the path that must not record memset or memcpy
+// nodes (callers that capture the stream into a graph).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(int64_t n, T *__restrict__ v, T val) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = val;
+}
+__global__ __launch_bounds__(256) void k_copy_f64(int64_t n, double *__restrict__ dst,
+                                                  const double *__restrict__ src) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// The vectors of one plain FP64 CG solve: r0/r1 and p0/p1 ping-pong pairs
+// (zero on the array's outer ring) and the dot kernel's partials.
+struct CgVecs {
+  double *r0, *r1, *p0, *p1, *part;
+};
+
+// Plain FP64 CG from x0 on b - A x0.  kernels_only: state cleared and x0
+// copied by kernels instead of memset / memcpy calls.
+static void hs_cg_core(xf_ctx c, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
+                       xf_solve_stats &s, const CgVecs &v, bool kernels_only) {
   const int64_t nc = g->ncell;
-  upload_state_zero(c);
-  XF_HIP(hipMemcpyAsync(x, x0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
-  residual_fast<double>(c, g, b, x, w->r);
-  if (o.project_mean && g->nunk > 0) project_mean<double>(c, g, w->r);
+  if (kernels_only) {
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, c->stream, (int64_t)(sizeof(DevState) / sizeof(int)),
+                       (int *)c->state, 0);
+    hipLaunchKernelGGL(k_copy_f64, dim3(stream_nb(nc)), dim3(256), 0, c->stream, nc, x, x0);
+  } else {
+    upload_state_zero(c);
+    XF_HIP(hipMemcpyAsync(x, x0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
+  }
+  residual_fast<double>(c, g, b, x, v.r0);
+  if (o.project_mean && g->nunk > 0) project_mean<double>(c, g, v.r0);
   check_kernel("cg start");
   hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, o.tol, o.tol_mode == XF_TOL_REL ? 1 : 0,
                      (int)HS_RRW);
   c->h_state->iter = 0;
-  cgcg_iterations<double, double>(c, g, x, w->r, w->q, w->p[0], w->p[1], g->d_cx, g->d_cy, g->d_cz, o.maxit,
+  cgcg_iterations<double, double>(c, g, x, v.r0, v.r1, v.p0, v.p1, g->d_cx, g->d_cy, g->d_cz, v.part, o.maxit,
                                   o.check_every > 0 ? o.check_every : 16);
   s.iters = c->h_state->iter;
   s.status = c->h_state->status;
   s.resid = c->h_state->s[HS_RR0] > 0 ? sqrt(c->h_state->s[HS_RR] / c->h_state->s[HS_RR0]) : 0;
+}
+
+void solve_hs_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
+                 xf_solve_stats &s) {
+  require_fast(g);
+  CgWork *w = work(c, g, false);
+  hs_cg_core(c, g, b, x0, x, o, s, CgVecs{w->r, w->q, w->p[0], w->p[1], w->part}, false);
+}
+
+// --------------------------------------------- plain FP64 CG without allocation
+// The work vectors of xf_solve_noalloc live in one fixed array per device,
+// held from the library's load to the end of the process.  Host code never
+// uses the array's name as an address: xf_init looks up each device's copy
+// with hipGetSymbolAddress (that device current) and keeps the pointer.
+// Layout: partials [kCgTileBlocks * 2], then r0, r1, p0, p1 [ncell] each.
+constexpr size_t kScratchBytes = size_t(72) << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+constexpr size_t kScratchPartDoubles = 2 * kCgTileBlocks;
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices];
+
+// Called by xf_init with `device` current.
+void noalloc_setup(int device) {
+  XF_REQUIRE(device >= 0 && device < kMaxDevices, XF_EINVAL, "xf_init: device index beyond the library's table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  XF_REQUIRE(e == hipSuccess && p, XF_EHIP,
+             std::string("xf_init: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = (double *)p;
+}
+
+int64_t noalloc_max_cells() {
+  return (int64_t)((kScratchBytes / sizeof(double) - kScratchPartDoubles) / 4);
+}
+static_assert((kScratchBytes / sizeof(double) - kScratchPartDoubles) / 4 == XF_NOALLOC_MAX_CELLS,
+              "XF_NOALLOC_MAX_CELLS must state the work array's capacity");
+
+void solve_hs_cg_noalloc(xf_ctx c, xf_grid g, const double *b, const double *x0, double *x,
+                         const xf_solve_opts &o, xf_solve_stats &s, double **verify_tmp) {
+  XF_REQUIRE(o.method == XF_CG, XF_EINVAL, "xf_solve_noalloc: only method XF_CG (plain FP64 CG) is supported");
+  require_fast(g);
+  XF_REQUIRE(g->ncell <= noalloc_max_cells(), XF_EINVAL,
+             "xf_solve_noalloc: grid of " + std::to_string(g->ncell) + " cells exceeds the fixed work array (" +
+                 std::to_string(noalloc_max_cells()) + " cells, 72 MiB)");
+  double *base;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : nullptr;
+  }
+  XF_REQUIRE(base, XF_EHIP, "xf_solve_noalloc: the work array of this device was not set up by xf_init");
+  const int64_t nc = g->ncell;
+  CgVecs v;
+  v.part = base;
+  v.r0 = base + kScratchPartDoubles;
+  v.r1 = v.r0 + nc;
+  v.p0 = v.r1 + nc;
+  v.p1 = v.p0 + nc;
+  // the last call's contents are still there: the rings of r and p must be 0
+  hipLaunchKernelGGL(k_fill<double>, dim3(stream_nb(4 * nc)), dim3(256), 0, c->stream, 4 * nc, v.r0, 0.0);
+  // per-kernel event sampling creates events: off on this path
+  const bool prof = c->prof;
+  c->prof = false;
+  struct Restore {
+    xf_ctx c;
+    bool p;
+    ~Restore() { c->prof = p; }
+  } rs{c, prof};
+  hs_cg_core(c, g, b, x0, x, o, s, v, true);
+  *verify_tmp = v.r0;  // free after the solve: the after-solve check may use it
 }
 
 
@@ -817,7 +915,7 @@ void solve_mp_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
     check_kernel("mp inner start");
     c->h_state->iter = 0;
     cgcg_iterations<float, float>(c, g, w->e32, w->r32, w->q32, w->p32[0], w->p32[1], g->d_cxf, g->d_cyf, g->d_czf,
-                                  o.maxit - inner_total, every);
+                                  w->part, o.maxit - inner_total, every);
     inner_total += c->h_state->iter;
     if (c->h_state->status == XF_ENAN) { status = XF_ENAN; break; }
     int e = c->prof ? prof_begin(c) : -1;
@@ -1283,7 +1381,7 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
       sq_unknowns<float>(c, g, w->r32, (int)FC_RR2);
       hipLaunchKernelGGL(k_restart_beta, dim3(1), dim3(1), 0, c->stream, c->state);
       cgcg_iterations<float, float>(c, g, w->e32, w->r32, w->q32, w->p32[0], w->p32[1], g->d_cxf, g->d_cyf,
-                                    g->d_czf, std::max(1, o.maxit - inner_total - it), 16);
+                                    g->d_czf, w->part, std::max(1, o.maxit - inner_total - it), 16);
       it += c->h_state->iter;
     }
     inner_total += it;
@@ -1298,11 +1396,18 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
   s.resid = rr0 > 0 ? sqrt(rr / rr0) : 0;
 }
 
-// ||b - A x|| / ||b - A x0|| with the reference rows.
-double true_residual(xf_ctx c, xf_grid g, const double *b, const double *x, const double *x0) {
+// ||b - A x|| / ||b - A x0|| with the reference rows; work: a field of the
+// grid's size to use (nullptr: one is allocated for the call).
+double true_residual(xf_ctx c, xf_grid g, const double *b, const double *x, const double *x0, double *work) {
   if ((g->use_rho ? g->bad_rows_div : g->bad_rows_lap) != 0) return NAN;
-  double *tmp = nullptr;
-  XF_HIP(hipMalloc(&tmp, sizeof(double) * g->ncell));
+  double *tmp = work;
+  if (!tmp) XF_HIP(hipMalloc(&tmp, sizeof(double) * g->ncell));
+  struct Free {
+    double *p;
+    ~Free() {
+      if (p) hipFree(p);
+    }
+  } ft{work ? nullptr : tmp};
   double v[2];
   const double *xs[2] = {x, x0};
   for (int t = 0; t < 2; t++) {
@@ -1311,6 +1416,5 @@ double true_residual(xf_ctx c, xf_grid g, const double *b, const double *x, cons
     XF_HIP(hipMemcpyAsync(&v[t], c->state->s + HS_CNT, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     XF_HIP(hipStreamSynchronize(c->stream));
   }
-  hipFree(tmp);
   return v[1] > 0 ? sqrt(v[0] / v[1]) : sqrt(v[0]);
 }

@@ -23,6 +23,7 @@ extern "C" int xf_init(int device, xf_ctx *out) {
     XF_REQUIRE(device >= 0 && device < count, XF_EINVAL, "xf_init: no such device");
     XF_HIP(hipSetDevice(device));
     c->device = device;
+    noalloc_setup(device);
     // blocking stream: ordered with the legacy default stream that torch uses
     XF_HIP(hipStreamCreate(&c->own_stream));
     c->stream = c->own_stream;

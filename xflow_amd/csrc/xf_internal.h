@@ -205,7 +205,12 @@ void solve_hs_cg(xf_ctx ctx, xf_grid g, const double *b, const double *x0, doubl
                  xf_solve_stats &s);
 void solve_mp_cg(xf_ctx ctx, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
                  xf_solve_stats &s);
-double true_residual(xf_ctx ctx, xf_grid g, const double *b, const double *x, const double *x0);
+double true_residual(xf_ctx ctx, xf_grid g, const double *b, const double *x, const double *x0, double *work);
+// plain FP64 CG on the library's fixed per-device work array (xf_solve_noalloc)
+void noalloc_setup(int device);
+int64_t noalloc_max_cells();
+void solve_hs_cg_noalloc(xf_ctx ctx, xf_grid g, const double *b, const double *x0, double *x,
+                         const xf_solve_opts &o, xf_solve_stats &s, double **verify_tmp);
 void solve_fcg_nn(xf_ctx ctx, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
                   xf_solve_stats &s);
 // the learned preconditioner z = M(r) on a grid (xf_unet.hip); d_sumsq = sum of r^2 over the unknowns

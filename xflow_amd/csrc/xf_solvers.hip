@@ -527,7 +527,30 @@ extern "C" int xf_solve(xf_ctx c, xf_grid g, const double *b, const double *x0, 
     float ms = 0;
     XF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     s.ms = ms;
-    s.resid_true = o.verify ? true_residual(c, g, b, x, x0) : -1.0;
+    s.resid_true = o.verify ? true_residual(c, g, b, x, x0, nullptr) : -1.0;
+  });
+  if (st != XF_OK) s.status = st;
+  if (stats) *stats = s;
+  return st != XF_OK ? st : s.status;
+}
+
+extern "C" int xf_solve_noalloc(xf_ctx c, xf_grid g, const double *b, const double *x0, double *x,
+                                const xf_solve_opts *opts, xf_solve_stats *stats) {
+  xf_solve_stats s{};
+  int st = guarded(c, [&] {
+    XF_REQUIRE(g && b && x0 && x && opts, XF_EINVAL, "xf_solve_noalloc: null argument");
+    XF_REQUIRE(x != x0 && x != b, XF_EINVAL, "xf_solve_noalloc: x must not alias b or x0");
+    xf_solve_opts o = *opts;
+    XF_REQUIRE(o.maxit >= 0, XF_EINVAL, "xf_solve_noalloc: negative iteration cap");
+    XF_HIP(hipEventRecord(c->ev0, c->stream));
+    double *tmp = nullptr;
+    solve_hs_cg_noalloc(c, g, b, x0, x, o, s, &tmp);
+    XF_HIP(hipEventRecord(c->ev1, c->stream));
+    XF_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    XF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    s.ms = ms;
+    s.resid_true = o.verify ? true_residual(c, g, b, x, x0, tmp) : -1.0;
   });
   if (st != XF_OK) s.status = st;
   if (stats) *stats = s;
