@@ -6,6 +6,7 @@ right-hand side, Neumann walls) with inputs resident in HBM.  The JSON line
 reports the plain FP64 GPU CG and, where built, the mixed-precision solver.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+    python bench.py --pmc    (the fixed workload of the counter passes only)
 """
 import argparse
 import ctypes as C
@@ -105,6 +106,44 @@ def load_network(ctx, path):
     return net
 
 
+def pmc_workload(n, iters, copy_mib):
+    """A short fixed workload for hardware-counter passes (rocprofv3 --pmc):
+    a plain device copy of known size after a 1 GiB fill that evicts its
+    source from the Infinity Cache (the calibration dispatch: its FETCH_SIZE
+    and WRITE_SIZE should each read the copied bytes), then `iters`
+    iterations each of the FP64 and FP32 throughput CG on the n^3
+    random-obstacle problem, so every counter pass sees the same dispatches."""
+    import torch
+    import xflow_amd as xf
+    from xflow_amd import cases
+    t0 = time.time()
+    nbytes = copy_mib << 20
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    flush = torch.empty((1 << 30) // 8, dtype=torch.float64, device="cuda")
+    flush.fill_(2.0)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del flush
+    log("copy: %d bytes read, %d bytes written" % (nbytes, nbytes))
+    case = cases.obstacle_case(n, seed=0)
+    ctx = xf.Context(0)
+    g = xf.Grid(ctx, case["hx"], case["hy"], case["hz"], case["codes"])
+    b = torch.from_numpy(case["b"]).cuda()
+    x0 = torch.zeros_like(b)
+    x = torch.zeros_like(b)
+    for method in (xf.XF_CG, xf.XF_CG_MP):
+        o = xf.solve_opts(method, tol=1e-30, tol_mode=xf.XF_TOL_REL, project_mean=1, maxit=iters, inner_tol=1e-30,
+                          verify=0)
+        st = g.solve(b, x0, x, o)
+        log("method %d: %d iterations (%d inner), status %d" % (method, st.iters, st.inner_iters, st.status))
+    torch.cuda.synchronize()
+    g.close()
+    ctx.close()
+    log("done in %.1f s; interior cells %d, array cells %d" % (time.time() - t0, n ** 3, (n + 2) ** 3))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,11 +161,18 @@ def main():
     ap.add_argument("--nn-inner-tol", type=float, default=1e-4)
     ap.add_argument("--weights", default=os.path.join(ROOT, "xflow_amd", "weights", "unet_poisson.npy"))
     ap.add_argument("--coarse-weights", default=os.path.join(ROOT, "xflow_amd", "weights", "unet_coarse64.npy"))
+    ap.add_argument("--pmc", action="store_true",
+                    help="run only the fixed counter workload (calibration copy + a few CG iterations) and exit")
+    ap.add_argument("--pmc-iters", type=int, default=8)
+    ap.add_argument("--pmc-copy-mib", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has a single MI355X; "
               "multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.pmc:
+        pmc_workload(args.n, args.pmc_iters, args.pmc_copy_mib)
+        return
 
     import torch
     import xflow_amd as xf
@@ -155,8 +201,7 @@ def main():
         st = solve(xf.XF_CG, verify=1)
         resid_true = st.resid_true
         log("warmup: %d iters, %.2f ms, true resid %.2e" % (st.iters, st.ms, st.resid_true))
-    # timed region: K plain-FP64 solves, per-kernel HIP events on the library's stream
-    profile.enable(ctx, True)
+    # timed region: K plain-FP64 solves (no per-kernel events inside it)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     iters = []
@@ -165,9 +210,18 @@ def main():
         iters.append(st.iters)
     torch.cuda.synchronize()
     wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    assert st.status == 0, st.status
+    # kernel durations for the roofline: one more solve with HIP events on the
+    # library's stream around a sample of the launches (two of every eight
+    # iterations), outside the timed region
+    profile.enable(ctx, True)
+    solve(xf.XF_CG)
     kstats = profile.read(ctx)
     profile.enable(ctx, False)
-    assert st.status == 0, st.status
+    # streaming ceilings of this GPU for the kernels' access patterns (1 GiB buffers)
+    ceiling = {name: round(profile.stream_ceiling(ctx, kind), 1)
+               for kind, name in ((0, "read"), (1, "copy"), (2, "read2_write2"))}
+    log("streaming ceilings (GB/s): %s" % ceiling)
     if resid_true is None:
         resid_true = solve(xf.XF_CG, verify=1).resid_true
     plain = {"ms": round(wall_ms, 3), "iterations": float(np.mean(iters)), "true_residual": resid_true}
@@ -309,7 +363,13 @@ def main():
                      "dots_kernel": "k_cgcg_dots<double,2> (9 B per interior cell)",
                      "dots_gbs": round(ka_gbs, 1), "dots_us": round(ka["ms"] * 1e3, 2),
                      "iteration_bytes_per_cell": 50,
-                     "iteration_gbs": round((ka_bytes + kb_bytes) / ((ka["ms"] + kb["ms"]) * 1e-3) / 1e9, 1)},
+                     "iteration_gbs": round((ka_bytes + kb_bytes) / ((ka["ms"] + kb["ms"]) * 1e-3) / 1e9, 1),
+                     "stream_ceiling_gbs": ceiling,
+                     "stream_ceiling_note": "grid-stride 16-byte streams over 1 GiB buffers on the same GPU in the "
+                                            "same run: read-only reduction (the dots kernel's pattern), copy, and "
+                                            "two reads + two writes in place (the update kernel's pattern)",
+                     "kernel_frac_of_read2_write2_ceiling": round(kb_gbs / ceiling["read2_write2"], 3),
+                     "dots_frac_of_read_ceiling": round(ka_gbs / ceiling["read"], 3)},
         "cpu_baseline": cpu,
     }
     if port is not None:

@@ -336,6 +336,12 @@ int xf_npy_save(xf_ctx ctx, const char *path, const void *data, char kind, int w
 int xf_profile_enable(xf_ctx ctx, int on);
 int xf_profile_reset(xf_ctx ctx);
 int xf_profile_read(xf_ctx ctx, const char *kernel, double *avg_ms, int64_t *launches);
+/* No reference counterpart: the streaming-bandwidth ceilings of the GPU for
+ * the solver kernels' access patterns, measured with two buffers of `bytes`
+ * each over `reps` launches after one warm-up launch: kind 0 a read-only
+ * reduction, 1 a copy (one read, one write), 2 two reads and two writes in
+ * place.  *gbs = bytes moved per second / 1e9.  Allocates its buffers. */
+int xf_stream_ceiling(xf_ctx ctx, int kind, size_t bytes, int reps, double *gbs);
 
 #ifdef __cplusplus
 }

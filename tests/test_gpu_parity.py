@@ -133,14 +133,24 @@ def sphere_case(ncell, seed=0):
 
 # (24, 24, 24), (40, 40, 40), (30, 45, 37): rows shorter than a wave (lanes past
 # the row end clamped); (16, 18, 100): one double per lane, last k-tile shifted
-# back; (17, 19, 131) and (20, 24, 300): rows long enough for the
-# two-doubles-per-lane stencil, with the last k-tile shifted back over the one
-# before it
-@pytest.mark.parametrize("shape", [(24, 24, 24), (40, 40, 40), (30, 45, 37), (16, 18, 100), (17, 19, 131),
-                                   (20, 24, 300)])
-def test_throughput_cg_matches_oracle(ctx, shape):
+# back; (17, 19, 131), (20, 24, 300), (20, 23, 256), (33, 10, 200): rows long
+# enough for the two-doubles-per-lane stencil, with the last k-tile shifted back
+# over the one before it where the row is not a whole number of tiles, and a
+# last block of rows that ends past the grid (23 - 2 rows); (9, 7, 130): tiles
+# of two planes.  Dirichlet variants: Defined points on the i = 0 and j = l - 1
+# faces (open faces towards the ring) and a Defined box inside the grid.
+@pytest.mark.parametrize("shape,dirichlet", [((24, 24, 24), False), ((40, 40, 40), False), ((30, 45, 37), False),
+                                             ((16, 18, 100), False), ((17, 19, 131), False), ((20, 24, 300), False),
+                                             ((20, 23, 256), False), ((33, 10, 200), False), ((9, 7, 130), False),
+                                             ((24, 24, 24), True), ((20, 23, 256), True), ((17, 19, 131), True)])
+def test_throughput_cg_matches_oracle(ctx, shape, dirichlet):
     from xflow_amd import cases
     codes = cases.neumann_codes(None, [cases.sphere_obstacle((0.5, 0.5, 0.5), 0.2)], shape=shape)
+    if dirichlet:
+        codes[0, 1:-1, 1:-1] = 3
+        codes[1:-1, -1, 1:-1] = 3
+        n, l, m = codes.shape
+        codes[n // 4:n // 4 + 2, l // 4:l // 4 + 2, m // 4:m // 4 + 3] = 3
     b = cases.divergence_rhs(codes, seed=0)
     case = {"codes": codes, "hx": cases.axis_steps(shape[0]), "hy": cases.axis_steps(shape[1]),
             "hz": cases.axis_steps(shape[2]), "b": b}
@@ -148,17 +158,22 @@ def test_throughput_cg_matches_oracle(ctx, shape):
     b = dev(case["b"])
     x0 = torch.zeros_like(b)
     x = torch.zeros_like(b)
-    st = g.solve(b, x0, x, xf.solve_opts(xf.XF_CG, tol=1e-12, tol_mode=xf.XF_TOL_REL, project_mean=1))
+    pm = 0 if dirichlet else 1
+    st = g.solve(b, x0, x, xf.solve_opts(xf.XF_CG, tol=1e-12, tol_mode=xf.XF_TOL_REL, project_mean=pm))
     assert st.status == 0
     og = xo.Grid(case["hx"], case["hy"], case["hz"], case["codes"])
-    xr, it, res, s2 = xo.cg_cg(og, case["b"], np.zeros_like(case["b"]), 1e-12, 100000, project_mean=True)
+    xr, it, res, s2 = xo.cg_cg(og, case["b"], np.zeros_like(case["b"]), 1e-12, 100000, project_mean=bool(pm))
     assert s2 == 0
+    print("%s dirichlet=%s: gpu %d iterations, oracle %d" % (shape, dirichlet, st.iters, it))
     assert abs(st.iters - it) <= 2, (st.iters, it)
     m = case["codes"] == 1
     xg = host(x)
-    # the pure-Neumann solution is defined up to a constant: compare mean-free fields
-    a = xg[m] - xg[m].mean()
-    r = xr[m] - xr[m].mean()
+    if dirichlet:
+        a, r = xg[m], xr[m]
+    else:
+        # the pure-Neumann solution is defined up to a constant: compare mean-free fields
+        a = xg[m] - xg[m].mean()
+        r = xr[m] - xr[m].mean()
     assert rel_l2(a, r) <= 1e-9
     assert st.resid_true <= 2e-12
 

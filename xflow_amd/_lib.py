@@ -99,6 +99,7 @@ SIGNATURES = {
     "xf_profile_enable": (I, [P, I]),
     "xf_profile_reset": (I, [P]),
     "xf_profile_read": (I, [P, C.c_char_p, C.POINTER(D), C.POINTER(I64)]),
+    "xf_stream_ceiling": (I, [P, I, SZ, I, C.POINTER(D)]),
 }
 
 _LIB = None

@@ -54,6 +54,7 @@ struct CgWork {
   int64_t n = 0;
   double *r = nullptr, *p[2] = {nullptr, nullptr}, *q = nullptr, *tmp = nullptr;
   float *r32 = nullptr, *p32[2] = {nullptr, nullptr}, *q32 = nullptr, *e32 = nullptr;
+  float *z32 = nullptr;    // flexible CG: the preconditioned residual
   double *part = nullptr;  // KA partials: [kCgTileBlocks][2]
 };
 
@@ -61,11 +62,11 @@ void free_cg_work(CgWork *w) {
   if (!w) return;
   hipFree(w->r); hipFree(w->p[0]); hipFree(w->p[1]); hipFree(w->q); hipFree(w->tmp);
   hipFree(w->r32); hipFree(w->p32[0]); hipFree(w->p32[1]); hipFree(w->q32); hipFree(w->e32);
-  hipFree(w->part);
+  hipFree(w->z32); hipFree(w->part);
   delete w;
 }
 
-static CgWork *work(xf_ctx c, xf_grid g, bool fp32) {
+static CgWork *work(xf_ctx c, xf_grid g, bool fp32, bool fcg = false) {
   if (!g->work) {
     g->work = new CgWork();
     g->work->n = g->ncell;
@@ -84,6 +85,7 @@ static CgWork *work(xf_ctx c, xf_grid g, bool fp32) {
   if (fp32) {
     get(&w->r32, n * 4); get(&w->p32[0], n * 4); get(&w->p32[1], n * 4); get(&w->q32, n * 4); get(&w->e32, n * 4);
   }
+  if (fcg) get(&w->z32, n * 4);
   return w;
 }
 
@@ -271,34 +273,15 @@ __global__ __launch_bounds__(256) void k_cgcg_dots(int n, int l, int m, Tiles tl
   }
 }
 
-// KB: reduce KA's partials, decide (converged / NaN / iteration cap /
-// breakdown: the same decision in every block), then the update over the
-// interior cells.  `it` is the number of updates applied before this launch;
-// the first launch (it == 0) takes p = r without reading p_old.  x is updated
-// on odd iterations only, with both steps in the order the eager update would
-// add them, (x + a_old p_old) + a p: same bits, 16 B per cell less every other
-// iteration; k_cgcg_flush adds a pending even step at the end.
-template <typename T, typename X, int CPL>
-__global__ __launch_bounds__(256) void k_cgcg_step(int n, int l, int m, Tiles tl, int npart,
-                                                    const double *__restrict__ part, const uint8_t *__restrict__ flags,
-                                                    const T *__restrict__ cx, const T *__restrict__ cy,
-                                                    const T *__restrict__ cz, X *__restrict__ x,
-                                                    const T *__restrict__ r, T *__restrict__ rout,
-                                                    const T *__restrict__ pin, T *__restrict__ pout, DevState *st,
-                                                    int it, int maxit) {
-  __shared__ double lds[8];
-  if (st->done) return;
-  using V = CgVec<T, CPL>;
-  using VX = CgVec<X, CPL>;
-  double gam = 0, del = 0;
-  for (int q = threadIdx.x; q < npart; q += blockDim.x) {
-    gam += part[2 * q];
-    del += part[2 * q + 1];
-  }
-  gam = block_sum_all(gam, lds);
-  del = block_sum_all(del, lds);
+// The step of update `it` from gamma = (r, r) and delta = (A r, r) of the
+// current residual: alpha, beta and the decision to stop (converged, NaN,
+// iteration cap, breakdown), the same in every block.  Block 0 records it in
+// the state: gamma and alpha go to the parity slot the next launch reads.
+// Returns true when the solve stops here (no update).
+__device__ inline bool cg_decide(double gam, double del, DevState *st, int it, int maxit, double &alpha,
+                                 double &beta) {
   const int par = it & 1;
-  double beta = 0, alpha;
+  beta = 0;
   if (it == 0) {
     alpha = gam / del;
   } else {
@@ -330,7 +313,38 @@ __global__ __launch_bounds__(256) void k_cgcg_step(int n, int l, int m, Tiles tl
       st->iter = it + 1;
     }
   }
-  if (stop) return;
+  return stop != 0;
+}
+
+// KB: reduce KA's partials, decide (converged / NaN / iteration cap /
+// breakdown: the same decision in every block), then the update over the
+// interior cells.  `it` is the number of updates applied before this launch;
+// the first launch (it == 0) takes p = r without reading p_old.  x is updated
+// on odd iterations only, with both steps in the order the eager update would
+// add them, (x + a_old p_old) + a p: same bits, 16 B per cell less every other
+// iteration; k_cgcg_flush adds a pending even step at the end.
+template <typename T, typename X, int CPL>
+__global__ __launch_bounds__(256) void k_cgcg_step(int n, int l, int m, Tiles tl, int npart,
+                                                    const double *__restrict__ part, const uint8_t *__restrict__ flags,
+                                                    const T *__restrict__ cx, const T *__restrict__ cy,
+                                                    const T *__restrict__ cz, X *__restrict__ x,
+                                                    const T *__restrict__ r, T *__restrict__ rout,
+                                                    const T *__restrict__ pin, T *__restrict__ pout, DevState *st,
+                                                    int it, int maxit) {
+  __shared__ double lds[8];
+  if (st->done) return;
+  using V = CgVec<T, CPL>;
+  using VX = CgVec<X, CPL>;
+  double gam = 0, del = 0;
+  for (int q = threadIdx.x; q < npart; q += blockDim.x) {
+    gam += part[2 * q];
+    del += part[2 * q + 1];
+  }
+  gam = block_sum_all(gam, lds);
+  del = block_sum_all(del, lds);
+  const int par = it & 1;
+  double alpha, beta;
+  if (cg_decide(gam, del, st, it, maxit, alpha, beta)) return;
   const T a = (T)alpha, bt = (T)beta;
   const bool xup = it & 1;
   const bool first = it == 0;
@@ -538,6 +552,37 @@ __global__ __launch_bounds__(256) void k_sub_mean_sq(int64_t nc, const uint8_t *
   if (threadIdx.x == 0) part[kRefPart * blockIdx.x] = acc;
 }
 
+// out = (float)(r - s[slot]/count) at unknowns, out = (float)r elsewhere;
+// partials of the projected residual's (r', r') in FP64 and of the rounded
+// (out, out), over unknowns.  Removing the mean before rounding keeps the
+// digits of a residual whose mean is large next to its mean-free part.
+__global__ __launch_bounds__(256) void k_sub_mean_cast(int64_t nc, const uint8_t *__restrict__ flags,
+                                                       const double *__restrict__ r, float *__restrict__ out,
+                                                       const DevState *st, int slot, double count,
+                                                       double *__restrict__ part) {
+  __shared__ double lds[8];
+  const double mean = st->s[slot] / count;
+  double a64 = 0, a32 = 0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    double v = r[c];
+    if (flags[c] & FL_UNKNOWN) {
+      v -= mean;
+      const float f = (float)v;
+      a64 += v * v;
+      a32 += (double)f * (double)f;
+      out[c] = f;
+    } else {
+      out[c] = (float)v;
+    }
+  }
+  a64 = block_sum_all(a64, lds);
+  a32 = block_sum_all(a32, lds);
+  if (threadIdx.x == 0) {
+    part[kRefPart * blockIdx.x] = a64;
+    part[kRefPart * blockIdx.x + 1] = a32;
+  }
+}
+
 // partials of (v, v) over unknowns
 template <typename T>
 __global__ __launch_bounds__(256) void k_sq_unknowns(int64_t nc, const uint8_t *__restrict__ flags,
@@ -741,6 +786,24 @@ static void project_mean(xf_ctx c, xf_grid g, T *v) {
   hipLaunchKernelGGL(k_ref_finish, dim3(1), dim3(256), 0, c->stream, c->partials, nb, c->state, (int)HS_RRW, -1, -1);
 }
 
+// Refinement residual as FP32: r = b - A x formed in FP64 and, with the mean
+// projection, its mean over the unknowns removed in FP64 before rounding
+// (r64: an FP64 work field).  s[HS_AUX] = the stopping norm (r, r) in FP64 of
+// the (projected) residual, s[HS_RRW] = (r32, r32).
+static void residual_refine(xf_ctx c, xf_grid g, const double *b, const double *x, bool proj, double *r64,
+                            float *r32) {
+  if (!proj) {
+    residual_fast<float>(c, g, b, x, r32);
+    return;
+  }
+  residual_fast<double>(c, g, b, x, r64);
+  const int nb = std::min(stream_nb(g->ncell), kRefBlocks);
+  hipLaunchKernelGGL(k_sub_mean_cast, dim3(nb), dim3(256), 0, c->stream, g->ncell, g->d_flags, r64, r32, c->state,
+                     (int)HS_SUM, (double)g->nunk, c->partials);
+  hipLaunchKernelGGL(k_ref_finish, dim3(1), dim3(256), 0, c->stream, c->partials, nb, c->state, (int)HS_AUX,
+                     (int)HS_RRW, -1);
+}
+
 // s[slot] = (v, v) over the unknowns
 template <typename T>
 static void sq_unknowns(xf_ctx c, xf_grid g, const T *v, int slot) {
@@ -886,9 +949,13 @@ void solve_mp_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
   upload_state_zero(c);
   XF_HIP(hipMemcpyAsync(x, x0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
   std::vector<ProfMark> marks;
+  // r = b - A x in FP64, rounded to FP32; with project_mean its mean over the
+  // unknowns is removed first and the stopping norm is that of the projected
+  // residual (the plain CG's rule)
+  const bool proj = o.project_mean && g->nunk > 0;
   auto residual = [&]() {
     int e = c->prof ? prof_begin(c) : -1;
-    residual_fast<float>(c, g, b, x, w->r32);
+    residual_refine(c, g, b, x, proj, w->r, w->r32);
     if (c->prof) {
       prof_end(c, e);
       marks.push_back({"mp_residual_fp64", e});
@@ -907,7 +974,6 @@ void solve_mp_cg(xf_ctx c, xf_grid g, const double *b, const double *x0, double 
     if (rr != rr) { status = XF_ENAN; break; }
     if (outer >= max_outer || inner_total >= o.maxit) { status = XF_EMAXIT; break; }
     outer++;
-    if (o.project_mean && g->nunk > 0) project_mean<float>(c, g, w->r32);
     XF_HIP(hipMemsetAsync(w->e32, 0, sizeof(float) * nc, c->stream));
     // the last sweep only needs the factor that is still missing (with margin)
     const double itol = std::min(0.5, std::max(inner_tol, 0.3 * sqrt(stop2 / rr)));
@@ -1296,29 +1362,22 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
   const bool mgc = o.method == XF_FCG_MGNN;
   XF_REQUIRE(mgc || o.precond, XF_EINVAL, "XF_FCG_NN needs opts.precond (an xf_unet)");
   struct xf_unet_s *net = (struct xf_unet_s *)o.precond;
-  CgWork *w = work(c, g, true);
+  CgWork *w = work(c, g, true, true);
   const int64_t nc = g->ncell;
   const FcgLaunch fl = fcg_launch(g);
   const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
   const int stall = o.stall_window > 0 ? o.stall_window : 20;
-  float *z = nullptr;
-  {
-    hipError_t e = hipMalloc(&z, sizeof(float) * nc);
-    XF_REQUIRE(e == hipSuccess, XF_ENOMEM, "FCG: out of device memory");
-    XF_HIP(hipMemsetAsync(z, 0, sizeof(float) * nc, c->stream));
-  }
-  struct Free {
-    float *p;
-    ~Free() { hipFree(p); }
-  } fz{z};
+  float *z = w->z32;
   upload_state_zero(c);
   XF_HIP(hipMemcpyAsync(x, x0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
   auto poll = [&]() {
     XF_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
     XF_HIP(hipStreamSynchronize(c->stream));
   };
+  // the stopping norm as in solve_mp_cg: the projected residual's with project_mean
+  const bool proj = o.project_mean && g->nunk > 0;
   auto residual = [&]() {
-    residual_fast<float>(c, g, b, x, w->r32);
+    residual_refine(c, g, b, x, proj, w->r, w->r32);
     poll();
     return c->h_state->s[HS_AUX];
   };
@@ -1330,9 +1389,10 @@ void solve_fcg_nn(xf_ctx c, xf_grid g, const double *b, const double *x0, double
     if (rr != rr) { status = XF_ENAN; break; }
     if (outer >= 60 || inner_total >= o.maxit) { status = XF_EMAXIT; break; }
     outer++;
-    if (o.project_mean && g->nunk > 0) project_mean<float>(c, g, w->r32);
     XF_HIP(hipMemsetAsync(w->e32, 0, sizeof(float) * nc, c->stream));
-    hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, inner_tol, 1, (int)HS_RRW);
+    // as in solve_mp_cg: the last sweep only needs the factor still missing
+    const double itol = std::min(0.5, std::max(inner_tol, 0.3 * sqrt(stop2 / rr)));
+    hipLaunchKernelGGL(k_hs_start, dim3(1), dim3(1), 0, c->stream, c->state, itol, 1, (int)HS_RRW);
     poll();
     double best = c->h_state->s[HS_RR];
     int since_best = 0, it = 0;

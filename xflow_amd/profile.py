@@ -20,3 +20,11 @@ def read(ctx):
         if n.value:
             out[k] = {"ms": ms.value, "count": n.value}
     return out
+
+
+def stream_ceiling(ctx, kind, nbytes=1 << 30, reps=10):
+    """GB/s of a streaming kernel on this GPU: kind 0 read-only, 1 copy,
+    2 two reads and two writes in place (xf_stream_ceiling)."""
+    out = C.c_double()
+    ctx.check(lib().xf_stream_ceiling(ctx.h, int(kind), int(nbytes), int(reps), C.byref(out)), "xf_stream_ceiling")
+    return out.value
