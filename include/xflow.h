@@ -289,14 +289,25 @@ int xf_mg_levels(xf_ctx ctx, xf_grid g, int *levels);
 int xf_mg_level(xf_ctx ctx, xf_grid g, int level, xf_grid *out);
 int xf_mg_apply(xf_ctx ctx, xf_grid g, xf_unet net, const float *r, float *z);
 
-/* Transport velocities (u, v, w) at the nodes of velocity component `axis`
- * (1..3): the component itself and the other two averaged over the four
- * surrounding faces; axis 0: at the cell centres (each component averaged
- * over its two faces).  No reference counterpart as a routine: the build's
- * interpolation for its time-step drivers (the reference's own is inline in
- * XProblem.cpp:5091-5305). */
+/* The transport terms of one transport equation as TVPProblem::SolveBySplitting
+ * builds them before its LOD step (there inline, not a routine of its own):
+ * axis 1..3, velocity component eq on that component's grid g (point-type
+ * loop XProblem.cpp:5090-5303): at Actual points the component itself and the
+ * other two averaged over the four faces around the node in the reference's
+ * operand order, tvis = the mean of vis over the two pressure nodes, rp = 0;
+ * Fictive points 0; Defined, PreDefined and Normal (eq 1, normal along x)
+ * points 0 with rp = bc; with vis, tvis and rp given, rp at Actual points
+ * then gains the viscosity-gradient term of XProblem.cpp:5357-5417 (the
+ * reference adds its external force first, XProblem.cpp:5321-5349: callers
+ * add theirs).  axis 0, the concentration on its grid (XProblem.cpp:5889-5944):
+ * cell-centre velocities (each component over its two faces), rp = bc on the
+ * PreDefined points of the plane i = 0 (the inlet) and 0 elsewhere; tvis must
+ * be NULL.  Point types the reference rejects give XF_EINVAL with its
+ * message.  Staggered fields u, v, w and vis (pressure nodes) on g's array
+ * shape; vis, bc, tvis, rp may be NULL.  Device pointers. */
 int xf_transport_velocity(xf_ctx ctx, xf_grid g, int axis, const double *u, const double *v, const double *w,
-                          double *tu, double *tv, double *tw);
+                          const double *vis, const double *bc, double *tu, double *tv, double *tw, double *tvis,
+                          double *rp);
 
 /* ------------------------------------------------------------ output files */
 /* The node-centred ASCII VTK structured grid of TVPProblem::SolveBySplitting

@@ -1165,3 +1165,116 @@ double xo_divergence_max(int n, int l, int m, const int32_t *codes, const double
   if (cell) *cell = best_c;
   return best;
 }
+
+/* Transport velocities, viscosity and boundary right-hand side of one
+ * transport equation, as TVPProblem::SolveBySplitting builds them before the
+ * LOD step.  axis 1..3 (velocity component eq, on that component's grid):
+ * the point-type loop XProblem.cpp:5090-5303 and the viscosity-gradient term
+ * at Actual points XProblem.cpp:5357-5417 (with vis; the external force the
+ * reference adds at Actual points first, XProblem.cpp:5321-5349, is the
+ * caller's).  axis 0 (concentration, on its grid): XProblem.cpp:5889-5944.
+ * Staggered fields on the pressure array shape; vis at the pressure nodes
+ * (NULL: tvis and the gradient term skipped); bc the boundary values (NULL:
+ * 0); tvis and rp may be NULL.  Returns XO_EINVAL where the reference throws
+ * (and where an Actual point's stencil would leave the array). */
+int xo_transport_velocity(int n, int l, int m, int axis, const int32_t *codes, const double *normals,
+                          const double *u, const double *v, const double *w, const double *vis, const double *bc,
+                          const double *hx, const double *hy, const double *hz, double *tu, double *tv, double *tw,
+                          double *tvis, double *rp) {
+  const int64_t S[3] = {(int64_t)l * m, m, 1};
+  const int E[3] = {n, l, m};
+  const double *f[3] = {u, v, w};
+  double *o[3] = {tu, tv, tw};
+  const int a = axis - 1;
+  if (axis < 0 || axis > 3) return XO_EINVAL;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < l; j++)
+      for (int k = 0; k < m; k++) {
+        const int64_t c = ((int64_t)i * l + j) * m + k;
+        const int idx[3] = {i, j, k};
+        const int code = codes[c];
+        const double bcv = bc ? bc[c] : 0.0;
+        if (axis == 0) {
+          /* XProblem.cpp:5896-5940 */
+          if (code == 0) {
+            tu[c] = 0; tv[c] = 0; tw[c] = 0;
+            if (rp) rp[c] = 0;
+          } else if (code == 1) {
+            if (i < 1 || j < 1 || k < 1) return XO_EINVAL;
+            double uu = (u[c - S[0]] + u[c]) / 2;
+            tu[c] = uu;
+            double vv = (v[c - S[1]] + v[c]) / 2;
+            tv[c] = vv;
+            double ww = (w[c - S[2]] + w[c]) / 2;
+            tw[c] = ww;
+            if (rp) rp[c] = 0;
+          } else if (code == 4) {
+            tu[c] = 0; tv[c] = 0; tw[c] = 0;
+            if (rp) rp[c] = i == 0 ? bcv : 0; /* the inlet condition on i = 0 only */
+          } else if (code == 6) {
+            tu[c] = 0; tv[c] = 0; tw[c] = 0;
+            if (rp) rp[c] = 0;
+          } else {
+            return XO_EINVAL;
+          }
+          continue;
+        }
+        if (code == 0) {
+          tu[c] = 0; tv[c] = 0; tw[c] = 0;
+          if (rp) rp[c] = 0;
+          if (tvis) tvis[c] = 0;
+        } else if (code == 1) {
+          /* the component itself; each other one over the four faces around
+           * the node, in the reference's order (c, +a, -e, +a-e) */
+          if (idx[a] + 1 >= E[a]) return XO_EINVAL;
+          for (int e = 0; e < 3; e++) {
+            if (e == a) {
+              o[e][c] = f[e][c];
+              continue;
+            }
+            if (idx[e] < 1) return XO_EINVAL;
+            const double *fe = f[e];
+            o[e][c] = (fe[c] + fe[c + S[a]] + fe[c - S[e]] + fe[c + S[a] - S[e]]) / 4;
+          }
+          if (tvis && vis) tvis[c] = (vis[c] + vis[c + S[a]]) / 2;
+          if (rp) rp[c] = 0;
+        } else if (code == 3 || code == 4 || code == 5) {
+          if (code == 5) {
+            /* Normal points: only for eq 1 with the normal along x (XProblem.cpp:5163-5197) */
+            if (axis != 1 || !normals) return XO_EINVAL;
+            const double nx = normals[3 * c], ny = normals[3 * c + 1], nz = normals[3 * c + 2];
+            if ((nx == 0) || (ny != 0) || (nz != 0)) return XO_EINVAL;
+          }
+          tu[c] = 0; tv[c] = 0; tw[c] = 0;
+          if (tvis) tvis[c] = 0;
+          if (rp) rp[c] = bcv;
+        } else {
+          return XO_EINVAL;
+        }
+      }
+  if (axis == 0 || !vis || !tvis || !rp) return XO_OK;
+  /* -nabla sigma at Actual points (XProblem.cpp:5357-5417) */
+  const double *tr[3] = {tu, tv, tw};
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < l; j++)
+      for (int k = 0; k < m; k++) {
+        const int64_t c = ((int64_t)i * l + j) * m + k;
+        if (codes[c] != 1) continue;
+        if (i < 1 || j < 1 || k < 1 || i + 1 >= n || j + 1 >= l || k + 1 >= m) return XO_EINVAL;
+        double dVisX = (tvis[c + S[0]] - tvis[c - S[0]]) / (hx[i] + hx[i - 1]);
+        double dVisY = (tvis[c + S[1]] - tvis[c - S[1]]) / (hy[j] + hy[j - 1]);
+        double dVisZ = (tvis[c + S[2]] - tvis[c - S[2]]) / (hz[k] + hz[k - 1]);
+        /* the derivatives of the three transport velocities along the equation's axis */
+        const double *ha = a == 0 ? hx : (a == 1 ? hy : hz);
+        const int ia = a == 0 ? i : (a == 1 ? j : k);
+        double dU = (tu[c + S[a]] - tu[c - S[a]]) / (ha[ia] + ha[ia - 1]);
+        double dV = (tv[c + S[a]] - tv[c - S[a]]) / (ha[ia] + ha[ia - 1]);
+        double dW = (tw[c + S[a]] - tw[c - S[a]]) / (ha[ia] + ha[ia - 1]);
+        const double *velTr = tr[a];
+        double dVelX = (velTr[c + S[0]] - velTr[c - S[0]]) / (hx[i] + hx[i - 1]);
+        double dVelY = (velTr[c + S[1]] - velTr[c - S[1]]) / (hy[j] + hy[j - 1]);
+        double dVelZ = (velTr[c + S[2]] - velTr[c - S[2]]) / (hz[k] + hz[k - 1]);
+        rp[c] = rp[c] + (dVisX * (dVelX + dU) + dVisY * (dVelY + dV) + dVisZ * (dVelZ + dW));
+      }
+  return XO_OK;
+}

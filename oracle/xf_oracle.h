@@ -90,6 +90,13 @@ int xo_velocity_correct(int n, int l, int m, int axis, const int32_t *codes, con
                         const double *rho, const double *hP, const double *bc, double tau, double *out);
 double xo_divergence_max(int n, int l, int m, const int32_t *codes, const double *u, const double *v, const double *w,
                          const double *hxU, const double *hyV, const double *hzW, int64_t *cell);
+/* Transport velocities, viscosity and boundary right-hand side of one
+ * transport equation: XProblem.cpp:5090-5303 and 5357-5417 (axis 1..3),
+ * 5889-5944 (axis 0, concentration). */
+int xo_transport_velocity(int n, int l, int m, int axis, const int32_t *codes, const double *normals,
+                          const double *u, const double *v, const double *w, const double *vis, const double *bc,
+                          const double *hx, const double *hy, const double *hz, double *tu, double *tv, double *tw,
+                          double *tvis, double *rp);
 
 #ifdef __cplusplus
 }

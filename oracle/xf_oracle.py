@@ -61,6 +61,7 @@ def lib():
             "xo_pressure_rhs": (i, [i, i, i, p, p, p, p, p, p, p, p, p, d, p]),
             "xo_velocity_correct": (i, [i, i, i, i, p, p, p, p, p, p, d, p]),
             "xo_divergence_max": (d, [i, i, i, p, p, p, p, p, p, p, p]),
+            "xo_transport_velocity": (i, [i, i, i, i, p, p, p, p, p, p, p, p, p, p, p, p, p, p, p]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -320,41 +321,30 @@ def divergence_max(codes, u, v, w, hxU, hyV, hzW):
     return val, cell.value
 
 
-def transport_velocity(axis, u, v, w):
-    """numpy restatement of k_transport_vel (xflow_amd/csrc/xf_projection.hip)."""
-    f = [np.asarray(u, dtype=np.float64), np.asarray(v, dtype=np.float64), np.asarray(w, dtype=np.float64)]
-    a = axis - 1
-    N = f[0].shape
-    if axis == 0:
-        out = []
-        for e in range(3):
-            o = np.zeros(N)
-            s1 = [slice(None)] * 3
-            s0 = [slice(None)] * 3
-            s1[e] = slice(1, N[e])
-            s0[e] = slice(0, N[e] - 1)
-            o[tuple(s1)] = 0.5 * (f[e][tuple(s0)] + f[e][tuple(s1)])
-            out.append(o)
-        return out
-    out = []
-    for e in range(3):
-        if e == a:
-            out.append(f[e].copy())
-            continue
-        o = np.zeros(N)
-        sl_c = [slice(None)] * 3
-        sl_c[a] = slice(0, N[a] - 1)
-        sl_c[e] = slice(1, N[e])
-
-        def sh(da, de):
-            s = [slice(None)] * 3
-            s[a] = slice(0 + da, N[a] - 1 + da)
-            s[e] = slice(1 + de, N[e] + de)
-            return tuple(s)
-        fe = f[e]
-        o[tuple(sl_c)] = 0.25 * (((fe[sh(0, 0)] + fe[sh(0, -1)]) + fe[sh(1, 0)]) + fe[sh(1, -1)])
-        out.append(o)
-    return out
+def transport_velocity(axis, codes, u, v, w, vis=None, bc=None, h=None, normals=None):
+    """xo_transport_velocity: the transport velocities (tu, tv, tw), viscosity
+    tvis and boundary right-hand side rp of one transport equation, as
+    TVPProblem::SolveBySplitting builds them (XProblem.cpp:5090-5303 and
+    5357-5417 for velocity component axis 1..3 on its grid, 5889-5944 for the
+    concentration, axis 0).  h = (hx, hy, hz) separator steps (needed with vis).
+    Returns (tu, tv, tw, tvis, rp); raises ValueError where the reference throws."""
+    codes = np.ascontiguousarray(codes, dtype=np.int32)
+    n, l, m = codes.shape
+    f = [_f64(a) for a in (u, v, w)]
+    out = [np.zeros(codes.shape) for _ in range(5)]
+    vis_a = _f64(vis) if vis is not None else None
+    bc_a = _f64(bc) if bc is not None else None
+    if h is None:
+        h = [np.ones(n - 1), np.ones(l - 1), np.ones(m - 1)]
+    hs = [_f64(x) for x in h]
+    nrm = _f64(normals) if normals is not None else None
+    ptr = lambda a: a.ctypes.data if a is not None else None
+    st = lib().xo_transport_velocity(n, l, m, int(axis), codes.ctypes.data, ptr(nrm), *[x.ctypes.data for x in f],
+                                     ptr(vis_a), ptr(bc_a), *[x.ctypes.data for x in hs],
+                                     *[x.ctypes.data for x in out])
+    if st != OK:
+        raise ValueError("transport velocities: a point type or position the reference rejects")
+    return tuple(out)
 
 
 # ------------------------------------------------------------ output files

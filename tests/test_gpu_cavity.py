@@ -23,15 +23,15 @@ def oracle_cavity(n, steps, dt=1e-3, re=100.0):
     vel = [b.copy() for b in bc]
     for a in range(3):
         vel[a][vc[a] == cases.PREDEFINED] *= 2.0
-    kappa = np.full(pcodes.shape, 1.0 / re)
+    vis = np.full(pcodes.shape, 1.0 / re)
     pg = xo.Grid(st, st, st, pcodes)
     vgs = [xo.Grid(st, st, st, vc[a]) for a in range(3)]
     p = np.zeros(pcodes.shape)
     for _ in range(steps):
         star = []
         for a in range(3):
-            tr = xo.transport_velocity(a + 1, *vel)
-            star.append(xo.lod(vgs[a], vel[a], vel[a], tr[0], tr[1], tr[2], bc[a], kappa, dt, 0))
+            tu, tv, tw, tvis, rp = xo.transport_velocity(a + 1, vc[a], *vel, vis=vis, bc=bc[a], h=[st, st, st])
+            star.append(xo.lod(vgs[a], vel[a], vel[a], tu, tv, tw, rp, tvis, dt, 0))
         rhs = xo.pressure_rhs(pcodes, star[0], star[1], star[2], st, st, st, dt)
         p, it, _, s = xo.cg_cg(pg, rhs, p, 1e-13, 100000, project_mean=True)
         vel = [xo.velocity_correct(vc[a], a + 1, star[a], p, st, dt, bc=bc[a], out=vel[a]) for a in range(3)]

@@ -47,8 +47,8 @@ def oracle_channel(ny, steps, subdivisions):
     for _ in range(steps):
         star = []
         for a in range(3):
-            tr = xo.transport_velocity(a + 1, *vel)
-            star.append(xo.lod(vgs[a], vel[a], vel[a], tr[0], tr[1], tr[2], bc[a], nuf, dt, 0))
+            tu, tv, tw, tvis, rp = xo.transport_velocity(a + 1, vc[a], *vel, vis=nuf, bc=bc[a], h=st)
+            star.append(xo.lod(vgs[a], vel[a], vel[a], tu, tv, tw, rp, tvis, dt, 0))
         for a in range(3):
             U = xo.ib_interp(vgs[a], a + 1, V, star[a])
             force = xo.ib_spread(vgs[a], a + 1, V, (-U / dt) * vol)
@@ -58,8 +58,8 @@ def oracle_channel(ny, steps, subdivisions):
         rhs = xo.pressure_rhs(pcodes, *star, *st, dt)
         p, it, _, s = xo.cg_cg(pg, rhs, p, 1e-13, 100000, project_mean=True)
         vel = [xo.velocity_correct(vc[a], a + 1, star[a], p, st[a], dt, bc=bc[a], out=vel[a]) for a in range(3)]
-        tr = xo.transport_velocity(0, *vel)
-        c = xo.lod(sg, c, c, tr[0], tr[1], tr[2], np.zeros(shape), kf, dt, 1)
+        tu, tv, tw, _, rp = xo.transport_velocity(0, scodes, *vel, bc=np.zeros(shape))
+        c = xo.lod(sg, c, c, tu, tv, tw, rp, kf, dt, 1)
     return vel, c
 
 

@@ -126,3 +126,51 @@ def test_mixed_precision_long_rows(ctx, shape):
     xg = host(x)
     a, r = xg[m] - xg[m].mean(), xr[m] - xr[m].mean()
     assert np.linalg.norm(a - r) / np.linalg.norm(r) <= 1e-6
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2, 3])
+def test_transport_terms_bit_exact(ctx, axis):
+    """xf_transport_velocity against the oracle's restatement of the
+    reference's inline construction (XProblem.cpp:5090-5303, 5357-5417 for
+    the velocity components, 5889-5944 for the concentration): transport
+    velocities, viscosity mean and right-hand side bit for bit, on random
+    fields, non-uniform steps and every point type the reference accepts for
+    the equation (Normal points with an x normal for the first component)."""
+    from xflow_amd import drivers
+    rng = np.random.default_rng(40 + axis)
+    normals = None
+    if axis == 0:
+        codes = drivers.channel_scalar_codes(12, 10, 8)
+    else:
+        codes, _ = drivers.velocity_codes(12, axis)
+        if axis == 1:
+            codes[12, 4:7, 3:9] = 5  # Normal points on the outflow wall of u (normal along x)
+            normals = np.zeros(codes.shape + (3,))
+            normals[..., 0] = 1.0
+    shape = codes.shape
+    hs = [rng.uniform(0.05, 0.1, s - 1) for s in shape]
+    u, v, w, vis, bc = (rng.standard_normal(shape) for _ in range(5))
+    vis = 0.01 + np.abs(vis)
+    g = xf.Grid(ctx, hs[0], hs[1], hs[2], codes, normals=normals)
+    out = [dev(np.full(shape, 9.0)) for _ in range(5)]
+    if axis == 0:
+        g.transport_terms(0, [dev(u), dev(v), dev(w)], out[:3], bc=dev(bc), rp=out[4])
+        want = xo.transport_velocity(0, codes, u, v, w, bc=bc, h=hs)
+        names = (0, 1, 2, 4)
+    else:
+        g.transport_terms(axis, [dev(u), dev(v), dev(w)], out[:3], vis=dev(vis), bc=dev(bc), tvis=out[3], rp=out[4])
+        want = xo.transport_velocity(axis, codes, u, v, w, vis=vis, bc=bc, h=hs, normals=normals)
+        names = range(5)
+    for q in names:
+        got = host(out[q])
+        assert np.array_equal(got, want[q]), (axis, q, np.abs(got - want[q]).max())
+    # point types the reference rejects for this equation
+    bad = codes.copy()
+    bad[5, 5, 5] = 3 if axis == 0 else 7  # Defined concentration point / Equation velocity point
+    gb = xf.Grid(ctx, hs[0], hs[1], hs[2], bad)
+    with pytest.raises(xf.XfError):
+        gb.transport_terms(axis, [dev(u), dev(v), dev(w)], out[:3])
+    with pytest.raises(ValueError):
+        xo.transport_velocity(axis, bad, u, v, w)
+    g.close()
+    gb.close()

@@ -152,6 +152,16 @@ class Grid:
             self.ctx.check(rc, "xf_solve")
         return st
 
+    def transport_terms(self, axis, vel, tr, vis=None, bc=None, tvis=None, rp=None):
+        """xf_transport_velocity: transport velocities into tr (three fields),
+        and optionally the viscosity mean tvis and the boundary right-hand
+        side rp, of equation `axis` (0 concentration, 1..3 velocity) on this
+        grid's point types."""
+        v = lambda t: self._vec(t) if t is not None else None
+        self.ctx.check(lib().xf_transport_velocity(self.ctx.h, self.h, int(axis), *[self._vec(x) for x in vel], v(vis),
+                                                   v(bc), *[self._vec(t) for t in tr], v(tvis), v(rp)),
+                       "xf_transport_velocity")
+
     def lod_transport(self, f, f_prev, u, v, w, rp, kappa, tau, scheme):
         self.ctx.check(lib().xf_lod_transport(self.ctx.h, self.h, self._vec(f), self._vec(f_prev), self._vec(u),
                                               self._vec(v), self._vec(w), self._vec(rp), self._vec(kappa),

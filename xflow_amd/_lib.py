@@ -95,7 +95,7 @@ SIGNATURES = {
     "xf_npy_save": (I, [P, C.c_char_p, P, C.c_char, I, I, P]),
     "xf_rn_dot": (I, [P, I64, P, P, C.POINTER(D)]),
     "xf_rn_absmax": (I, [P, I64, P, C.POINTER(D), C.POINTER(C.c_int32)]),
-    "xf_transport_velocity": (I, [P, P, I, P, P, P, P, P, P]),
+    "xf_transport_velocity": (I, [P, P, I, P, P, P, P, P, P, P, P, P, P]),
     "xf_profile_enable": (I, [P, I]),
     "xf_profile_reset": (I, [P]),
     "xf_profile_read": (I, [P, C.c_char_p, C.POINTER(D), C.POINTER(I64)]),

@@ -208,6 +208,64 @@ void free_grid(xf_grid g) {
   delete g;
 }
 
+// Where TVPProblem::SolveBySplitting would throw while building the
+// transport terms of each equation on a grid with these point types
+// (XProblem.cpp:5096-5300 velocity components, 5896-5940 concentration), and
+// Actual points whose stencil would leave the array.
+static void transport_checks(xf_grid g, const int32_t *codes, const double *normals, const int32_t *has_normal) {
+  const int n = g->n, l = g->l, m = g->m;
+  const int E[3] = {n, l, m};
+  const char *vel_type = "Error in TVPProblem::SolveBySplitting: Method is not implemented for this point border "
+                         "type of velocity.";
+  const char *vel_eq = "Error in TVPProblem::SolveBySplitting: Method is not implemented for this point border type "
+                       "of velocity and this equation number.";
+  const char *vel_normal = "Error in TVPProblem::SolveBySplitting: Method is not implemented for this point border "
+                           "type of velocity and this equation number and this normal vector.";
+  const char *conc = "Error in TVPProblem::SolveBySplitting: Unknown concentration point mask type.";
+  const char *edge = "xf_transport_velocity: an Actual point on the array border (its averages leave the array)";
+  for (int ax = 0; ax < 4; ax++) {
+    g->tv_err[ax].clear();
+    g->tv_visterm_ok[ax] = ax > 0;
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < l; j++)
+      for (int k = 0; k < m; k++) {
+        const int64_t c = ((int64_t)i * l + j) * m + k;
+        const int code = codes[c];
+        const int idx[3] = {i, j, k};
+        auto fail = [&](int ax, const char *msg) {
+          if (g->tv_err[ax].empty()) g->tv_err[ax] = msg;
+        };
+        // concentration
+        if (code == PT_ACTUAL) {
+          if (i < 1 || j < 1 || k < 1) fail(0, edge);
+        } else if (!(code == PT_FICTIVE || code == PT_PREDEFINED || code == PT_PRENORMAL)) {
+          fail(0, conc);
+        }
+        // velocity components
+        for (int a = 0; a < 3; a++) {
+          const int ax = a + 1;
+          if (code == PT_ACTUAL) {
+            bool ok = idx[a] + 1 < E[a];
+            for (int e = 0; e < 3; e++)
+              if (e != a && idx[e] < 1) ok = false;
+            if (!ok) fail(ax, edge);
+            if (i < 1 || j < 1 || k < 1 || i + 1 >= n || j + 1 >= l || k + 1 >= m) g->tv_visterm_ok[ax] = false;
+          } else if (code == PT_NORMAL) {
+            if (ax != 1) {
+              fail(ax, vel_eq);
+            } else {
+              const bool have = normals && (!has_normal || has_normal[c]);
+              if (!have || normals[3 * c] == 0 || normals[3 * c + 1] != 0 || normals[3 * c + 2] != 0)
+                fail(ax, vel_normal);
+            }
+          } else if (code == PT_EQUATION || code == PT_PRENORMAL || code == PT_PREEQUATION) {
+            fail(ax, vel_type);
+          }
+        }
+      }
+}
+
 extern "C" int xf_grid_create(xf_ctx c, int n, int l, int m, const double *hx, const double *hy,
                               const double *hz, const int32_t *codes, const double *normals,
                               const int32_t *has_normal, xf_grid *out) {
@@ -229,6 +287,7 @@ extern "C" int xf_grid_create(xf_ctx c, int n, int l, int m, const double *hx, c
                  "xf_grid_create: unknown point type in mask");
       cb[q] = (uint8_t)codes[q];
     }
+    transport_checks(g, codes, normals, has_normal);
     g->d_hx = upload(g->hx); g->d_hy = upload(g->hy); g->d_hz = upload(g->hz);
     std::vector<double> cd;
     std::vector<float> cf;

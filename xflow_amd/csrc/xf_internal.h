@@ -105,6 +105,12 @@ struct xf_grid_s {
   int64_t bad_rows_lap = 0;           // rows the Laplace operator rejects
   int64_t bad_rows_div = 0;           // rows the divergented operator rejects
   int64_t special_rows = 0;           // unknown rows that are not Actual (codes 4, 5, 6, 8)
+  // xf_transport_velocity per equation (0 concentration, 1..3 velocity
+  // components): why the reference would throw on this grid's point types
+  // (empty: it would not), and whether every Actual point has both
+  // neighbours on each axis (the viscosity-gradient term reads them)
+  std::string tv_err[4];
+  bool tv_visterm_ok[4] = {false, false, false, false};
   struct CgWork *work = nullptr;      // throughput-solver vectors, allocated on first use
   struct MgHier *mg = nullptr;        // multigrid hierarchy (XF_FCG_MGNN), built on first use
 };

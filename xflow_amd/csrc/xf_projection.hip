@@ -247,50 +247,105 @@ extern "C" int xf_divergence_max(xf_ctx c, xf_grid gp, const double *u, const do
   });
 }
 
-// Transport velocities at the nodes of velocity component `axis` (1..3) on
-// the staggered pressure-shape layout: the component itself, and each other
-// component averaged over the four faces around the node (the build's
-// interpolation; the reference interpolates to the component's faces in
-// XProblem.cpp:5091-5305).  Ring entries copy the component's value.
-__global__ void k_transport_vel(int n, int l, int m, int axis, const double *__restrict__ u,
-                                const double *__restrict__ v, const double *__restrict__ w, double *__restrict__ tu,
-                                double *__restrict__ tv, double *__restrict__ tw) {
+// Transport velocities, viscosity and boundary right-hand side of one
+// transport equation, as TVPProblem::SolveBySplitting builds them before the
+// LOD step: velocity component axis 1..3 on its own grid (point-type loop
+// XProblem.cpp:5090-5303), the concentration (axis 0) on its grid
+// (XProblem.cpp:5889-5944).  Operands in the reference's order; point types
+// the reference rejects were ruled out on the host (xf_grid_s::tv_err).
+__global__ void k_transport_vel(int n, int l, int m, int axis, const uint8_t *__restrict__ codes,
+                                const double *__restrict__ u, const double *__restrict__ v,
+                                const double *__restrict__ w, const double *__restrict__ vis,
+                                const double *__restrict__ bc, double *__restrict__ tu, double *__restrict__ tv,
+                                double *__restrict__ tw, double *__restrict__ tvis, double *__restrict__ rp) {
   const int64_t nc = (int64_t)n * l * m;
   const int64_t S[3] = {(int64_t)l * m, (int64_t)m, 1};
-  const int E[3] = {n, l, m};
   const double *f[3] = {u, v, w};
   double *o[3] = {tu, tv, tw};
   const int a = axis - 1;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
-    const int idx[3] = {(int)(c / S[0]), (int)((c / m) % l), (int)(c % m)};
-    if (a < 0) {  // cell centres: each component averaged over its two faces
-      for (int e = 0; e < 3; e++) o[e][c] = idx[e] >= 1 ? 0.5 * (f[e][c - S[e]] + f[e][c]) : 0.0;
-      continue;
-    }
-    for (int e = 0; e < 3; e++) {
-      if (e == a) {
-        o[e][c] = f[e][c];
-        continue;
+    const int code = codes[c];
+    const double bcv = bc ? bc[c] : 0.0;
+    if (code == PT_ACTUAL) {
+      if (a < 0) {  // cell centres: each component over its two faces
+        tu[c] = (u[c - S[0]] + u[c]) / 2;
+        tv[c] = (v[c - S[1]] + v[c]) / 2;
+        tw[c] = (w[c - S[2]] + w[c]) / 2;
+      } else {
+        for (int e = 0; e < 3; e++) {
+          const double *fe = f[e];
+          o[e][c] = e == a ? fe[c] : (fe[c] + fe[c + S[a]] + fe[c - S[e]] + fe[c + S[a] - S[e]]) / 4;
+        }
+        if (tvis) tvis[c] = vis ? (vis[c] + vis[c + S[a]]) / 2 : 0.0;
       }
-      // node of component a sits at +1/2 along a; component e at +1/2 along e
-      const bool ok = idx[a] + 1 < E[a] && idx[e] >= 1;
-      if (!ok) {
-        o[e][c] = 0;
-        continue;
+      if (rp) rp[c] = 0;
+    } else {
+      // Fictive: 0; Defined, PreDefined, Normal: 0 and the boundary value in
+      // rp (the concentration's PreDefined points: the inlet value on i = 0
+      // only; its PreNormal points: 0)
+      tu[c] = 0;
+      tv[c] = 0;
+      tw[c] = 0;
+      if (tvis && a >= 0) tvis[c] = 0;
+      if (rp) {
+        double r = 0;
+        if (a >= 0) r = code == PT_FICTIVE ? 0.0 : bcv;
+        else if (code == PT_PREDEFINED && c < S[0]) r = bcv;
+        rp[c] = r;
       }
-      const double s = f[e][c] + f[e][c - S[e]] + f[e][c + S[a]] + f[e][c + S[a] - S[e]];
-      o[e][c] = 0.25 * s;
     }
   }
 }
 
+// rp += the viscosity-gradient term at Actual points (XProblem.cpp:5357-5417)
+// from the transport velocities and viscosity of k_transport_vel.
+__global__ void k_transport_visterm(int n, int l, int m, int axis, const uint8_t *__restrict__ codes,
+                                    const double *__restrict__ hx, const double *__restrict__ hy,
+                                    const double *__restrict__ hz, const double *__restrict__ tu,
+                                    const double *__restrict__ tv, const double *__restrict__ tw,
+                                    const double *__restrict__ tvis, double *__restrict__ rp) {
+  const int64_t nc = (int64_t)n * l * m;
+  const int64_t S[3] = {(int64_t)l * m, (int64_t)m, 1};
+  const double *tr[3] = {tu, tv, tw};
+  const double *hs[3] = {hx, hy, hz};
+  const int a = axis - 1;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    if (codes[c] != PT_ACTUAL) continue;
+    const int idx[3] = {(int)(c / S[0]), (int)((c / m) % l), (int)(c % m)};
+    const int i = idx[0], j = idx[1], k = idx[2];
+    double dVisX = (tvis[c + S[0]] - tvis[c - S[0]]) / (hx[i] + hx[i - 1]);
+    double dVisY = (tvis[c + S[1]] - tvis[c - S[1]]) / (hy[j] + hy[j - 1]);
+    double dVisZ = (tvis[c + S[2]] - tvis[c - S[2]]) / (hz[k] + hz[k - 1]);
+    const double *ha = hs[a];
+    const int ia = idx[a];
+    double dU = (tu[c + S[a]] - tu[c - S[a]]) / (ha[ia] + ha[ia - 1]);
+    double dV = (tv[c + S[a]] - tv[c - S[a]]) / (ha[ia] + ha[ia - 1]);
+    double dW = (tw[c + S[a]] - tw[c - S[a]]) / (ha[ia] + ha[ia - 1]);
+    const double *velTr = tr[a];
+    double dVelX = (velTr[c + S[0]] - velTr[c - S[0]]) / (hx[i] + hx[i - 1]);
+    double dVelY = (velTr[c + S[1]] - velTr[c - S[1]]) / (hy[j] + hy[j - 1]);
+    double dVelZ = (velTr[c + S[2]] - velTr[c - S[2]]) / (hz[k] + hz[k - 1]);
+    rp[c] = rp[c] + (dVisX * (dVelX + dU) + dVisY * (dVelY + dV) + dVisZ * (dVelZ + dW));
+  }
+}
+
 extern "C" int xf_transport_velocity(xf_ctx c, xf_grid g, int axis, const double *u, const double *v,
-                                     const double *w, double *tu, double *tv, double *tw) {
+                                     const double *w, const double *vis, const double *bc, double *tu, double *tv,
+                                     double *tw, double *tvis, double *rp) {
   return guarded(c, [&] {
     XF_REQUIRE(g && u && v && w && tu && tv && tw, XF_EINVAL, "xf_transport_velocity: null argument");
-    XF_REQUIRE(axis >= 0 && axis <= 3, XF_EINVAL, "Unknown velocity number.");
-    hipLaunchKernelGGL(k_transport_vel, dim3(nblocks(g->ncell)), dim3(kBlock), 0, c->stream, g->n, g->l, g->m, axis, u,
-                       v, w, tu, tv, tw);
+    XF_REQUIRE(axis >= 0 && axis <= 3, XF_EINVAL, "Error in TVPProblem::SolveBySplitting: Unknown equation number.");
+    XF_REQUIRE(axis > 0 || !tvis, XF_EINVAL, "xf_transport_velocity: the concentration equation has no viscosity");
+    XF_REQUIRE(g->tv_err[axis].empty(), XF_EINVAL, g->tv_err[axis]);
+    hipLaunchKernelGGL(k_transport_vel, dim3(nblocks(g->ncell)), dim3(kBlock), 0, c->stream, g->n, g->l, g->m, axis,
+                       g->d_codes, u, v, w, vis, bc, tu, tv, tw, tvis, rp);
+    if (axis > 0 && vis && tvis && rp) {
+      XF_REQUIRE(g->tv_visterm_ok[axis], XF_EINVAL,
+                 "xf_transport_velocity: an Actual point on the array border (the viscosity term reads both "
+                 "neighbours)");
+      hipLaunchKernelGGL(k_transport_visterm, dim3(nblocks(g->ncell)), dim3(kBlock), 0, c->stream, g->n, g->l, g->m,
+                         axis, g->d_codes, g->d_hx, g->d_hy, g->d_hz, tu, tv, tw, tvis, rp);
+    }
     check_kernel("k_transport_vel");
   });
 }

@@ -68,7 +68,11 @@ class Cavity:
             self.vel[a][torch.from_numpy(self.vcodes[a] == cases.PREDEFINED).cuda()] *= 2.0
         self.star = [z() for _ in range(3)]
         self.tr = [z() for _ in range(3)]
-        self.kappa = torch.full(shape, 1.0 / re, dtype=torch.float64, device="cuda")
+        # viscosity at the pressure nodes; the LOD step takes its two-node mean
+        # and the boundary right-hand side built as the reference builds them
+        self.vis = torch.full(shape, 1.0 / re, dtype=torch.float64, device="cuda")
+        self.tvis = z()
+        self.rp = z()
         self.p = z()
         self.p0 = z()
         self.rhs = z()
@@ -78,13 +82,11 @@ class Cavity:
         n = self.n
         # velocity predictor: LOD transport per component (XProblem.cpp:5017-5446)
         for a in range(3):
-            lib_ok = xf.lib().xf_transport_velocity(self.ctx.h, self.vgrids[a].h, a + 1,
-                                                     *[xf._ptr(v) for v in self.vel],
-                                                     *[xf._ptr(t) for t in self.tr])
-            self.ctx.check(lib_ok, "xf_transport_velocity")
+            self.vgrids[a].transport_terms(a + 1, self.vel, self.tr, vis=self.vis, bc=self.bc[a], tvis=self.tvis,
+                                           rp=self.rp)
             self.star[a].copy_(self.vel[a])
-            self.vgrids[a].lod_transport(self.star[a], self.vel[a], self.tr[0], self.tr[1], self.tr[2], self.bc[a],
-                                         self.kappa, self.dt, 0)
+            self.vgrids[a].lod_transport(self.star[a], self.vel[a], self.tr[0], self.tr[1], self.tr[2], self.rp,
+                                         self.tvis, self.dt, 0)
         # pressure equation (XProblem.cpp:5474-5591)
         self.gp.pressure_rhs(self.star[0], self.star[1], self.star[2], self.steps, self.steps, self.steps, self.dt,
                              self.rhs)
@@ -127,14 +129,15 @@ def channel_velocity_codes(nx, ny, nz, axis):
 
 def channel_scalar_codes(nx, ny, nz):
     """Cell-centred scalar: inflow ghost PreDefined (c = 0), outflow and wall
-    ghosts Normal (zero flux), edges Fictive."""
+    ghosts PreNormal (zero flux: the reference's concentration boundary,
+    XProblem.cpp:5925-5929), edges Fictive."""
     shape = (nx + 2, ny + 2, nz + 2)
     E = (nx, ny, nz)
     idx = np.indices(shape)
     ghost = [(idx[e] == 0) | (idx[e] == E[e] + 1) for e in range(3)]
     count = sum(g.astype(int) for g in ghost)
     codes = np.full(shape, cases.ACTUAL, dtype=np.int32)
-    codes[count == 1] = cases.NORMAL
+    codes[count == 1] = cases.PRENORMAL
     codes[(count == 1) & (idx[0] == 0)] = cases.PREDEFINED
     codes[count >= 2] = cases.FICTIVE
     return codes
@@ -194,6 +197,8 @@ class IBChannel:
         self.force = z()
         self.nu_f = torch.full(shape, self.nu, dtype=torch.float64, device="cuda")
         self.kap_f = torch.full(shape, self.kap, dtype=torch.float64, device="cuda")
+        self.tvis = z()
+        self.rp = z()
         X = (np.arange(nx + 2) - 0.5) * h
         Y = (np.arange(ny + 2) - 0.5) * h
         Zc = (np.arange(nz + 2) - 0.5) * h
@@ -207,17 +212,14 @@ class IBChannel:
         self.U = torch.zeros(self.nodes.shape[0], dtype=torch.float64, device="cuda")
         self.last = None
 
-    def _transport(self, axis):
-        self.ctx.check(xf.lib().xf_transport_velocity(self.ctx.h, self.gp.h, axis, *[xf._ptr(v) for v in self.vel],
-                                                      *[xf._ptr(t) for t in self.tr]), "xf_transport_velocity")
-
     def step(self):
         dt = self.dt
         for a in range(3):  # velocity predictor
-            self._transport(a + 1)
+            self.vgrids[a].transport_terms(a + 1, self.vel, self.tr, vis=self.nu_f, bc=self.bc[a], tvis=self.tvis,
+                                           rp=self.rp)
             self.star[a].copy_(self.vel[a])
-            self.vgrids[a].lod_transport(self.star[a], self.vel[a], self.tr[0], self.tr[1], self.tr[2], self.bc[a],
-                                         self.nu_f, dt, 0)
+            self.vgrids[a].lod_transport(self.star[a], self.vel[a], self.tr[0], self.tr[1], self.tr[2], self.rp,
+                                         self.tvis, dt, 0)
         for a in range(3):  # direct forcing at the sphere's vertices
             g = self.vgrids[a]
             g.ib_interp(a + 1, self.nodes, self.star[a], self.U)
@@ -235,9 +237,9 @@ class IBChannel:
         for a in range(3):
             self.vgrids[a].velocity_correct(a + 1, self.star[a], self.p, self.steps[a], dt, self.vel[a], bc=self.bc[a])
         # passive scalar, upwind LOD with cell-centred velocities
-        self._transport(0)
+        self.gs.transport_terms(0, self.vel, self.tr, bc=self.zero, rp=self.rp)
         self.c_new.copy_(self.c)
-        self.gs.lod_transport(self.c_new, self.c, self.tr[0], self.tr[1], self.tr[2], self.zero, self.kap_f, dt, 1)
+        self.gs.lod_transport(self.c_new, self.c, self.tr[0], self.tr[1], self.tr[2], self.rp, self.kap_f, dt, 1)
         self.c, self.c_new = self.c_new, self.c
         div, _ = self.gp.divergence_max(self.vel[0], self.vel[1], self.vel[2], *self.steps)
         self.last = {"pressure_iters": st.iters, "inner_iters": st.inner_iters, "pressure_ms": st.ms,
