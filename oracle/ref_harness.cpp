@@ -34,6 +34,11 @@
 //   pack_bicg|pack_bicgstab|pack_mr
 //             TSLAE solvers run on that packed operator over Rn vectors
 //             (XSLAE.cpp:159-414, 667-767); sweeps counted per packed row
+//   vp_masks  a TVPProblem with the live cylinder-channel statement: the U, V, W,
+//             P grids' point types, normals and node coordinates as the
+//             reference generates them (XStatement.cpp:7717-7977), the space
+//             models' Rn numbering from LocalPrepareVariables
+//             (XProblem.cpp:4116-4206), and on P the activity table and A x
 //
 // File format (.xfb): repeated records
 //   u32 name_len, name bytes, u8 dtype ('d' f64 | 'i' i32), u32 ndim,
@@ -659,6 +664,122 @@ static void cmd_vtk_boundary(const Bag& in, Bag& out) {
   rmdir(dir);
 }
 
+// ------------------------------------------------ vp_masks: the reference's own masks
+// A TVPProblem whose statement is the cylinder-channel generator of the live
+// problem (TCanalIBMWithElasticBoundary::CreateStatement, XProblem.cpp:10340-10373,
+// with its pressure condition, 10360-10366).  Initialize() (XProblem.h:293-306)
+// builds the U, V, W, P grids from the generator (XStatement.cpp:7717-7977,
+// XProblem.cpp:3989-4108) and runs LocalPrepareVariables (XProblem.cpp:4116-4206),
+// which registers each grid's space model; they are read back from the registry.
+struct ModelPeek : public TRnRelease3dSpace {
+  static TRnSpaceModel* get(int name) { return GetModel(name); }
+};
+
+class HarnessVP : public TVPProblem {
+ public:
+  HarnessVP(double lx, double ly, double lz, int nx, int ny, int nz, double pin, double pout)
+      : TVPProblem(1.0, 0.1, 1.0), lx_(lx), ly_(ly), lz_(lz), nx_(nx), ny_(ny), nz_(nz), pin_(pin), pout_(pout) {}
+  TMaskGenerator* CreateStatement() override {
+    std::function<bool(int, int, int, int, int, int)> condition = [](int i, int j, int k, int L, int M, int K) {
+      return (double(j) - double(M) / 2.) * (double(j) - double(M) / 2.) +
+                 (double(k) - double(K) / 2.) * (double(k) - double(K) / 2.) >
+             (double(M) / 10.1) * (double(K) / 10.1);
+    };
+    return TSimpleWithCylinderPressureMaskGenerator::CreateInstance(lx_, ly_, lz_, nx_, ny_, nz_, pin_, pout_,
+                                                                     condition);
+  }
+  void run(const Bag& in, Bag& out) {
+    Initialize();
+    const char* names[4] = {"U", "V", "W", "P"};
+    TNodesGrid* grids[4] = {fGridU, fGridV, fGridW, fGridP};
+    const int models[4] = {fSpaceModelNameU, fSpaceModelNameV, fSpaceModelNameW, fSpaceModelNameP};
+    for (int q = 0; q < 4; q++) {
+      T3dNormalGrid& g = dynamic_cast<T3dNormalGrid&>(*grids[q]);
+      const int n = g.GetSeparator1().EndIndex + 1, l = g.GetSeparator2().EndIndex + 1,
+                m = g.GetSeparator3().EndIndex + 1;
+      const std::string nm = names[q];
+      const TSeparator* seps[3] = {&g.GetSeparator1(), &g.GetSeparator2(), &g.GetSeparator3()};
+      const char* ax[3] = {"_x", "_y", "_z"};
+      for (int a = 0; a < 3; a++) {
+        const int cnt = seps[a]->EndIndex + 1;
+        Arr c = mk_d({cnt});
+        const double* dim = seps[a]->GetDimension();
+        for (int t = 0; t < cnt; t++) c.d[t] = dim[t];
+        out[nm + ax[a]] = c;
+      }
+      Arr codes = mk_i({n, l, m}), hn = mk_i({n, l, m}), nv = mk_d({n, l, m, 3}), rn = mk_i({n, l, m});
+      const T3dNumberMask& mask = g.Mask;
+      const T3dVector* const* const* const* nr = g.Normals;
+      const TRnRelease3dSpaceModel& model = dynamic_cast<const TRnRelease3dSpaceModel&>(*ModelPeek::get(models[q]));
+      const T3dInverseCorrespondence& ic3 =
+          dynamic_cast<const T3dInverseCorrespondence&>(model.GetInverseCorrespondence());
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < l; j++)
+          for (int k = 0; k < m; k++) {
+            const int64_t c = ((int64_t)i * l + j) * m + k;
+            codes.i[c] = mask[i][j][k];
+            rn.i[c] = ic3[0][i][j][k];
+            if (nr[i][j][k]) {
+              hn.i[c] = 1;
+              nv.d[3 * c] = nr[i][j][k]->X;
+              nv.d[3 * c + 1] = nr[i][j][k]->Y;
+              nv.d[3 * c + 2] = nr[i][j][k]->Z;
+            }
+          }
+      out[nm + "_codes"] = codes;
+      out[nm + "_has_normal"] = hn;
+      out[nm + "_normals"] = nv;
+      out[nm + "_rn"] = rn;
+      if (q == 3 && in.count("xvec")) {
+        // the pressure operator's rows on this grid and model: activity
+        // table and A x for the caller's x (XOperator.cpp:837-1129)
+        const TNodesGrid* gp[1] = {&g};
+        CountingLaplace* lap = new CountingLaplace(models[q], 1, gp);
+        TRnRelease3dSpace& x = *new TRnRelease3dSpace(models[q]);
+        TRnRelease3dSpace& y = *new TRnRelease3dSpace(models[q]);
+        const Arr& xv = need(in, "xvec");
+        if (xv.size() != (int64_t)n * l * m) throw std::runtime_error("vp_masks: xvec must have the P grid's shape");
+        for (int i = 0; i < n; i++)
+          for (int j = 0; j < l; j++)
+            for (int k = 0; k < m; k++) {
+              x[i][j][k] = xv.d[((int64_t)i * l + j) * m + k];
+              y[i][j][k] = 0;
+            }
+        TRnLinearOperator* A = lap;
+        y |= (*A)(x);
+        Arr ax_ = mk_d({n, l, m});
+        for (int i = 0; i < n; i++)
+          for (int j = 0; j < l; j++)
+            for (int k = 0; k < m; k++) ax_.d[((int64_t)i * l + j) * m + k] = y[i][j][k];
+        out["P_Ax"] = ax_;
+        lap->ConstructActivityTables();
+        const int RN = model.EndIndex;
+        int** tab = lap->table();
+        Arr at = mk_i({RN, 8});
+        for (int r = 0; r < RN; r++) {
+          const int cnt = tab[r][0];
+          at.i[r * 8 + 0] = cnt;
+          for (int t = 1; t <= cnt; t++) at.i[r * 8 + t] = tab[r][t];
+        }
+        out["P_act"] = at;
+      }
+    }
+  }
+
+ private:
+  double lx_, ly_, lz_;
+  int nx_, ny_, nz_;
+  double pin_, pout_;
+};
+
+static void cmd_vp_masks(const Bag& in, Bag& out) {
+  const Arr& d = need(in, "dims");
+  const Arr& len = need(in, "lengths");
+  HarnessVP* vp = new HarnessVP(len.d.at(0), len.d.at(1), len.d.at(2), d.i.at(0), d.i.at(1), d.i.at(2),
+                                scalar(in, "p_in"), scalar(in, "p_out"));
+  vp->run(in, out);
+}
+
 int main(int argc, char** argv) {
   if (argc != 4) {
     fprintf(stderr, "usage: %s <command> <in.xfb> <out.xfb>\n", argv[0]);
@@ -681,6 +802,7 @@ int main(int argc, char** argv) {
     else if (cmd == "pack_bicg" || cmd == "pack_bicgstab" || cmd == "pack_mr") cmd_pack_solve(in, out, cmd);
     else if (cmd == "npy") cmd_npy(in, out);
     else if (cmd == "vtk_boundary") cmd_vtk_boundary(in, out);
+    else if (cmd == "vp_masks") cmd_vp_masks(in, out);
     else { fprintf(stderr, "unknown command %s\n", cmd.c_str()); return 2; }
     write_xfb(argv[3], out);
   } catch (const char* e) {

@@ -280,6 +280,21 @@ def case_lod():
         save("lod_scheme%d" % scheme, inp, run("lod", inp))
 
 
+def case_vp_masks():
+    """The reference's own masks: a TVPProblem with the live cylinder-channel
+    statement (TSimpleWithCylinderPressureMaskGenerator with the pressure
+    condition of TCanalIBMWithElasticBoundary, XProblem.cpp:10340-10373)
+    builds its U, V, W, P grids (XStatement.cpp:7717-7977) and space models
+    (LocalPrepareVariables, XProblem.cpp:4116-4206); on P also the activity
+    table and A x for a seeded x."""
+    base = {"dims": np.array([31, 25, 25], dtype=np.int32), "lengths": np.array([1.5, 1.2, 1.2]),
+            "p_in": np.array([6.0]), "p_out": np.array([1.0])}
+    shape = run("vp_masks", base)["P_codes"].shape
+    inp = dict(base)
+    inp["xvec"] = np.random.default_rng(11).standard_normal(shape)
+    save("vp_cylinder", inp, run("vp_masks", inp))
+
+
 def case_ib():
     rng = np.random.default_rng(5)
     nc = 16
@@ -301,7 +316,8 @@ if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
     CASES = {"op": case_op, "divop": case_divop, "solvers": case_solvers, "thomas": case_thomas, "lod": case_lod,
-             "ib": case_ib, "pack": case_pack, "pack_solve": case_pack_solve, "pack_solve_spd": case_pack_solve_spd, "io": case_io}
+             "ib": case_ib, "pack": case_pack, "pack_solve": case_pack_solve, "pack_solve_spd": case_pack_solve_spd, "io": case_io,
+             "vp_masks": case_vp_masks}
     # python make_golden.py [case ...]: only the named cases (all by default)
     for name in sys.argv[1:] or list(CASES):
         CASES[name]()

@@ -55,6 +55,26 @@ def test_grid_export_bit_exact(ctx, name):
     assert np.array_equal(act, outs["act"])
 
 
+@pytest.mark.parametrize("g", ["U", "V", "W", "P"])
+def test_reference_made_masks(ctx, g):
+    """Grids from the reference's own cylinder-channel generator and
+    LocalPrepareVariables (XStatement.cpp:7717-7977, XProblem.cpp:4116-4206):
+    space mask and Rn numbering bit-exact on all four grids; on the pressure
+    grid (with its normals) the activity table and the operator rows too."""
+    ins, outs = golden("vp_cylinder")
+    grid = xf.Grid(ctx, np.diff(outs[g + "_x"]), np.diff(outs[g + "_y"]), np.diff(outs[g + "_z"]),
+                   outs[g + "_codes"], outs[g + "_normals"], outs[g + "_has_normal"])
+    sm, rn, act = grid.export(activity=g == "P")
+    assert np.array_equal(sm, (outs[g + "_rn"] > 0).astype(np.int32))
+    assert np.array_equal(rn, outs[g + "_rn"])
+    if g == "P":
+        assert np.array_equal(act, outs["P_act"])
+        y = dev(np.zeros(outs["P_codes"].shape))
+        grid.apply(dev(ins["xvec"]), y)
+        assert np.array_equal(host(y), outs["P_Ax"])
+    grid.close()
+
+
 @pytest.mark.parametrize("name,rho", [("op_mixed", False), ("op_sphere16", False), ("divop_mixed", True)])
 def test_operator_rows_bit_exact(ctx, name, rho):
     ins, outs = golden(name)

@@ -137,3 +137,31 @@ def test_pack_solvers_bit_exact(name, solver):
     # whose row evaluations the harness does not count: two counted sweeps per iteration
     assert (1 + 2 * it if solver == "bicg" else SWEEPS[solver](it)) == outs["sweeps"][0]
     assert np.array_equal(x, outs["x"])
+
+
+@pytest.mark.parametrize("g", ["U", "V", "W", "P"])
+def test_reference_made_masks(g):
+    """Masks the reference generates itself: the cylinder-channel statement of
+    its live problem and LocalPrepareVariables (XStatement.cpp:7717-7977,
+    XProblem.cpp:4116-4206), run by the harness.  The oracle's space-mask rule
+    and Rn numbering reproduce the reference's registered space models on all
+    four grids; on the pressure grid (with its normals) the activity table and
+    A x too."""
+    ins, outs = golden("vp_cylinder")
+    codes = outs[g + "_codes"]
+    sm = xo.space_mask_of(codes)
+    assert np.array_equal(sm, (outs[g + "_rn"] > 0).astype(np.int32))
+    rn, cnt = xo.rn_number(sm)
+    assert np.array_equal(rn, outs[g + "_rn"])
+    if g != "P":
+        return
+    act = xo.activity(sm, rn, cnt)
+    assert np.array_equal(act, outs["P_act"])
+    grid = xo.Grid(np.diff(outs["P_x"]), np.diff(outs["P_y"]), np.diff(outs["P_z"]), codes, None,
+                   outs["P_normals"], outs["P_has_normal"])
+    y = xo.apply(grid, ins["xvec"], y=np.zeros(codes.shape))
+    assert np.array_equal(y, outs["P_Ax"])
+    # the generator's point types: Actual interior, PreDefined inflow (inside
+    # the cylinder opening) and outflow planes, Fictive walls and edges
+    assert set(np.unique(codes)) == {0, 1, 4}
+    assert (codes[0] == 4).sum() > 0 and (codes[-1, 1:-1, 1:-1] == 4).all()

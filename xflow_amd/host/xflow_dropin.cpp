@@ -44,9 +44,11 @@ using xflow_dropin::fail;
 // ------------------------------------------------------------------ masks
 T3dNumberMask::T3dNumberMask(int N, int L, int M) : n_(N), l_(L), m_(M), data_((size_t)N * L * M, 0) {
   rows_.resize(N);
+  planes_.resize(N);
   for (int i = 0; i < N; i++) {
     rows_[i].resize(L);
     for (int j = 0; j < L; j++) rows_[i][j] = data_.data() + ((size_t)i * L + j) * M;
+    planes_[i] = rows_[i].data();
   }
 }
 
@@ -109,6 +111,27 @@ T3dNormalGrid::T3dNormalGrid(TSeparator *s1, TSeparator *s2, TSeparator *s3, con
           has_normal_[c] = 1;
         }
   }
+}
+
+const T3dVector *const *const *const *T3dNormalGrid::GetNormals() const {
+  if (nplane_.empty()) {
+    const int n = s_[0]->GetEndIndex() + 1, l = s_[1]->GetEndIndex() + 1, m = s_[2]->GetEndIndex() + 1;
+    const size_t nc = (size_t)n * l * m;
+    nvec_.assign(nc, T3dVector{});
+    nptr_.assign(nc, nullptr);
+    for (size_t c = 0; c < nc && !has_normal_.empty(); c++)
+      if (has_normal_[c]) {
+        nvec_[c].X = normals_[3 * c];
+        nvec_[c].Y = normals_[3 * c + 1];
+        nvec_[c].Z = normals_[3 * c + 2];
+        nptr_[c] = &nvec_[c];
+      }
+    nrow_.resize((size_t)n * l);
+    for (size_t r = 0; r < nrow_.size(); r++) nrow_[r] = nptr_.data() + r * m;
+    nplane_.resize(n);
+    for (int i = 0; i < n; i++) nplane_[i] = nrow_.data() + (size_t)i * l;
+  }
+  return nplane_.data();
 }
 
 T3dNormalGrid::~T3dNormalGrid() {
@@ -340,6 +363,15 @@ double TRnSpace::GetAbsMaxElement(int &elementIndex) {
   return v;
 }
 
+TRnSpace &TRnSpace::Assign(const TRnSpace &s) {
+  if (model_ != s.model_) fail("Error in TRnSpace::Assign: models mismatch.");
+  if (this != &s) {
+    check(xf_memcpy(context(), device(), s.device(), sizeof(double) * host_.size(), 2), "Assign");
+    mark_device_modified();
+  }
+  return *this;
+}
+
 TRnSpace &TRnSpace::Copy() const {
   TRnSpace &c = CreateInstance();
   check(xf_memcpy(context(), c.dev_, device(), sizeof(double) * host_.size(), 2), "Copy");
@@ -562,7 +594,22 @@ TSLAE::TSLAE(const TRnLinearOperator &A, TRnSpace &x, TRnSpace &b) : A_(&A), x_(
   *result_ |= 0;
 }
 
+TSLAE::TSLAE(const TRnLinearOperator &A, TRnSpace &x, TRnSpace &u, TRnSpace &v, TRnSpace &w, TRnSpace &b)
+    : TSLAE(A, x, b) {
+  u_ = &u;
+  v_ = &v;
+  w_ = &w;
+}
+
 TSLAE::~TSLAE() {}
+
+void TSLAE::UpdateA(TRnLinearOperator &A) {
+  int name = -1;
+  if (const T3dLaplaceOperator *A3 = dynamic_cast<const T3dLaplaceOperator *>(&A)) name = A3->model_name();
+  else if (const T1dPackOperator *Ap = dynamic_cast<const T1dPackOperator *>(&A)) name = Ap->model_name();
+  if (name != x_->model_name()) fail("Error in TSLAE::UpdateA: the operator acts on another space model.");
+  A_ = &A;
+}
 
 void TSLAE::solve(int method, double rel_tol, bool project_mean, void *precond) {
   xf_solve_opts o;

@@ -22,7 +22,14 @@
 //   TRnSpace::operator[] (1-based Rn index)             XSpace.h:504-530, XSpace.cpp:911-920
 //   T1dPackOperator(model, A, &vector), operator~, Copy,
 //   operator()(idData, i)                               XOperator.h:322-370
-//   TSLAE(A, x, b), Result / GetResult, UpdateP/UpdateRP,
+//   Property accessors: mask.N/L/M/Mask, sep.EndIndex/CountNodes/StartValue/
+//   EndValue/Dimension/Separation, grid.Separator1-3/Mask/Normals,
+//   model.EndIndex, x.EndIndex, slae.Result           property.h, XMask.h:102-120,
+//                                                       XSeparator.h:18-54, XGrid.h,
+//                                                       XSpaceModel.h:50-54, XSpace.h:288-292
+//   TRnSpace::Assign                                    XSpace.cpp:813-819
+//   TSLAE(A, x, b) and TSLAE(A, x, u, v, w, b), Result / GetResult,
+//   UpdateP/UpdateRP (Assign), UpdateA,
 //   SolveByConjugateGradients, SolveByBiConjugateGradients,
 //   SolveByMinimalResiduals, SolveByBiConjugateGradientsStab,
 //   static SolveByScalarRunning                         XSLAE.h:9-48
@@ -61,6 +68,25 @@ xf_ctx context();                                // one per host thread, device 
 void check(int status, const char *what);
 }  // namespace xflow_dropin
 
+// Read-only member that reads like a field and calls its owner's getter, as
+// the reference's Property<C, T, getter, setter> does (property.h:4-41) for
+// `mask.N`, `x.EndIndex`, `slae.Result`.  Its setters all throw "Not
+// implemented" in the reference; here assignment does not compile.  Owners
+// are not copyable (the reference's copy constructors are private or throw),
+// so the owner pointer cannot go stale.
+template <class C, class T, T (C::*Get)() const>
+class Property {
+ public:
+  explicit Property(const C *owner) : owner_(owner) {}
+  Property(const Property &) = delete;
+  Property &operator=(const Property &) = delete;
+  operator T() const { return (owner_->*Get)(); }
+  T operator()() const { return (owner_->*Get)(); }
+
+ private:
+  const C *owner_;
+};
+
 // ------------------------------------------------------------------ masks
 class TMask {
  public:
@@ -70,26 +96,42 @@ class TMask {
 class T3dNumberMask : public TMask {
  public:
   T3dNumberMask(int N, int L, int M);
+  T3dNumberMask(const T3dNumberMask &) = delete;
+  T3dNumberMask &operator=(const T3dNumberMask &) = delete;
   T3dNumberMask &Copy() const;
   int *const *operator[](int i) const { return rows_[i].data(); }
   int GetN() const { return n_; }
   int GetL() const { return l_; }
   int GetM() const { return m_; }
+  int ***GetMask() const { return const_cast<int ***>(planes_.data()); }
+  // XMask.h:102-120
+  Property<T3dNumberMask, int, &T3dNumberMask::GetN> N{this};
+  Property<T3dNumberMask, int, &T3dNumberMask::GetL> L{this};
+  Property<T3dNumberMask, int, &T3dNumberMask::GetM> M{this};
+  Property<T3dNumberMask, int ***, &T3dNumberMask::GetMask> Mask{this};
   const int *data() const { return data_.data(); }
 
  private:
   int n_, l_, m_;
   std::vector<int> data_;
   std::vector<std::vector<int *>> rows_;
+  std::vector<int **> planes_;
 };
 
 class T1dNumberMask : public TMask {
  public:
   explicit T1dNumberMask(int N) : data_((size_t)(N > 0 ? N : 0), 0) {}
-  T1dNumberMask &Copy() const { return *new T1dNumberMask(*this); }
+  T1dNumberMask(const T1dNumberMask &) = delete;
+  T1dNumberMask &operator=(const T1dNumberMask &) = delete;
+  T1dNumberMask &Copy() const {
+    T1dNumberMask *c = new T1dNumberMask(GetN());
+    c->data_ = data_;
+    return *c;
+  }
   int &operator[](int i) { return data_[i]; }
   int operator[](int i) const { return data_[i]; }
   int GetN() const { return (int)data_.size(); }
+  Property<T1dNumberMask, int, &T1dNumberMask::GetN> N{this};  // XMask.h:30-35
   const int *data() const { return data_.data(); }
 
  private:
@@ -99,10 +141,23 @@ class T1dNumberMask : public TMask {
 // ------------------------------------------------------------- separators
 class TSeparator {
  public:
+  TSeparator() = default;
+  TSeparator(const TSeparator &) = delete;
+  TSeparator &operator=(const TSeparator &) = delete;
   virtual ~TSeparator() {}
   int GetEndIndex() const { return end_index_; }
+  int GetCountNodes() const { return end_index_ + 1; }
+  double GetStartValue() const { return dim_.empty() ? 0.0 : dim_.front(); }
+  double GetEndValue() const { return dim_.empty() ? 0.0 : dim_.back(); }
   const double *GetSeparation() const { return sep_.data(); }
   const double *GetDimension() const { return dim_.data(); }
+  // XSeparator.h:18-54
+  Property<TSeparator, double, &TSeparator::GetStartValue> StartValue{this};
+  Property<TSeparator, double, &TSeparator::GetEndValue> EndValue{this};
+  Property<TSeparator, int, &TSeparator::GetEndIndex> EndIndex{this};
+  Property<TSeparator, int, &TSeparator::GetCountNodes> CountNodes{this};
+  Property<TSeparator, const double *, &TSeparator::GetDimension> Dimension{this};
+  Property<TSeparator, const double *, &TSeparator::GetSeparation> Separation{this};
 
  protected:
   int end_index_ = 0;
@@ -138,11 +193,21 @@ class T3dNormalGrid : public TNodesGrid {
  public:
   T3dNormalGrid(TSeparator *s1, TSeparator *s2, TSeparator *s3, const TMask &mask,
                 const T3dVector *const *const *const *normals, bool deleteSeparators = true);
+  T3dNormalGrid(const T3dNormalGrid &) = delete;
+  T3dNormalGrid &operator=(const T3dNormalGrid &) = delete;
   ~T3dNormalGrid();
   const TSeparator &GetSeparator1() const { return *s_[0]; }
   const TSeparator &GetSeparator2() const { return *s_[1]; }
   const TSeparator &GetSeparator3() const { return *s_[2]; }
   const T3dNumberMask &GetMask() const { return *mask_; }
+  // per-node outward normal or NULL (XGrid.h:279-299)
+  const T3dVector *const *const *const *GetNormals() const;
+  // XGrid.h:45-47, 71-73, 96-98, 204, 282-299
+  Property<T3dNormalGrid, const TSeparator &, &T3dNormalGrid::GetSeparator1> Separator1{this};
+  Property<T3dNormalGrid, const TSeparator &, &T3dNormalGrid::GetSeparator2> Separator2{this};
+  Property<T3dNormalGrid, const TSeparator &, &T3dNormalGrid::GetSeparator3> Separator3{this};
+  Property<T3dNormalGrid, const T3dNumberMask &, &T3dNormalGrid::GetMask> Mask{this};
+  Property<T3dNormalGrid, const T3dVector *const *const *const *, &T3dNormalGrid::GetNormals> Normals{this};
   // device grid for a given space mask (built on first use)
   xf_grid device_grid() const;
 
@@ -153,13 +218,22 @@ class T3dNormalGrid : public TNodesGrid {
   std::vector<double> normals_;
   std::vector<int32_t> has_normal_;
   mutable xf_grid dev_ = nullptr;
+  // GetNormals' pointer view of normals_ (built on first use)
+  mutable std::vector<T3dVector> nvec_;
+  mutable std::vector<const T3dVector *> nptr_;
+  mutable std::vector<const T3dVector *const *> nrow_;
+  mutable std::vector<const T3dVector *const *const *> nplane_;
 };
 
 // ------------------------------------------------------------ space model
 class TRnSpaceModel {
  public:
+  TRnSpaceModel() = default;
+  TRnSpaceModel(const TRnSpaceModel &) = delete;
+  TRnSpaceModel &operator=(const TRnSpaceModel &) = delete;
   virtual ~TRnSpaceModel() {}
   int GetEndIndex() const { return end_index_; }
+  Property<TRnSpaceModel, int, &TRnSpaceModel::GetEndIndex> EndIndex{this};  // XSpaceModel.h:50-54
   virtual int64_t ncell() const = 0;  // entries of one vector of the model
 
  protected:
@@ -220,8 +294,13 @@ class TRnSpace {
   static void RemoveAllModels();
   static TRnSpaceModel *model(int name);
 
+  TRnSpace(const TRnSpace &) = delete;
+  TRnSpace &operator=(const TRnSpace &) = delete;
   int GetEndIndex() const { return model_->GetEndIndex(); }
+  Property<TRnSpace, int, &TRnSpace::GetEndIndex> EndIndex{this};  // XSpace.h:288-292
   TRnSpace &operator|=(const TRnSpace &s);   // masked copy
+  // every entry, unknowns or not; the models must match (XSpace.cpp:813-819, 33-41)
+  TRnSpace &Assign(const TRnSpace &s);
   TRnSpace &operator|=(double zero);         // only 0 is supported (XSpace.h:92-102)
   TRnSpace &operator|=(const TLinTerm &t);
   TRnSpace &operator|=(const TLinComb &t);
@@ -373,17 +452,22 @@ class T1dPackOperator : public TRnLinearOperator {
 class TSLAE {
  public:
   TSLAE(const TRnLinearOperator &A, TRnSpace &x, TRnSpace &b);
+  // XSLAE.h:13: u, v, w are held and unused, as in the reference (no solver reads them)
+  TSLAE(const TRnLinearOperator &A, TRnSpace &x, TRnSpace &u, TRnSpace &v, TRnSpace &w, TRnSpace &b);
+  TSLAE(const TSLAE &) = delete;
+  TSLAE &operator=(const TSLAE &) = delete;
   virtual ~TSLAE();
   const TRnSpace &GetResult() const { return *result_; }
-  // `slae.Result` reads like the reference's Property (XSLAE.h:20)
-  struct ResultProperty {
-    const TSLAE *owner;
-    operator const TRnSpace &() const { return owner->GetResult(); }
-  } Result{this};
+  Property<TSLAE, const TRnSpace &, &TSLAE::GetResult> Result{this};  // XSLAE.h:16-20
   static void SolveByScalarRunning(int M, double *u, const double *a, const double *b, const double *c,
                                    const double *f);
-  void UpdateP(TRnSpace &p) { *x_ |= p; }
-  void UpdateRP(TRnSpace &rp) { *b_ |= rp; }
+  // x and b take every entry of p and rp (TRnSpace::Assign, XSLAE.cpp:40-50)
+  void UpdateP(TRnSpace &p) { x_->Assign(p); }
+  void UpdateRP(TRnSpace &rp) { b_->Assign(rp); }
+  // the operator of later solves (the reference declares it, XSLAE.h:30, and
+  // throws "too lazy to make this", XSLAE.cpp:52-55; here it replaces A, which
+  // must act on the same model as x)
+  void UpdateA(TRnLinearOperator &A);
   void SolveByConjugateGradients() { solve(XF_CG3); }
   void SolveByBiConjugateGradients() { solve(XF_BICG); }
   void SolveByMinimalResiduals() { solve(XF_MR); }
@@ -407,6 +491,7 @@ class TSLAE {
   void solve(int method, double rel_tol = 0, bool project_mean = false, void *precond = nullptr);
   const TRnLinearOperator *A_;
   TRnSpace *x_, *b_;
+  TRnSpace *u_ = nullptr, *v_ = nullptr, *w_ = nullptr;
   std::unique_ptr<TRnSpace> result_;
   xf_solve_stats stats_{};
 };
