@@ -165,3 +165,19 @@ def test_reference_made_masks(g):
     # the cylinder opening) and outflow planes, Fictive walls and edges
     assert set(np.unique(codes)) == {0, 1, 4}
     assert (codes[0] == 4).sum() > 0 and (codes[-1, 1:-1, 1:-1] == 4).all()
+
+
+def test_config2_sphere128_oracle_equals_reference_run():
+    """At 128^3 the oracle's summaries of CG and BiCG (make_config2.py) are
+    the compiled reference's own results (make_config2_reference.py runs
+    TSLAE::SolveByConjugateGradients and SolveByBiConjugateGradients on the
+    same case): same iteration counts, norms and sampled values, bit for bit."""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    o = np.load(os.path.join(here, "config2_sphere128.npz"))
+    r = np.load(os.path.join(here, "config2_sphere128_reference.npz"))
+    assert np.array_equal(o["idx"], r["idx"])
+    for s in ("cg3", "bicg"):
+        for q in ("_iters", "_norm", "_x"):
+            assert np.array_equal(o[s + q], r[s + q]), s + q
+    assert int(r["cg3_iters"][0]) == 809 and int(r["bicg_iters"][0]) == 810
