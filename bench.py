@@ -268,10 +268,10 @@ def main():
 
         coarse = load_network(ctx, args.coarse_weights)
         solvers["mp_fcg_cycle_no_network"] = timed(opts(xf.XF_FCG_MGNN, None),
-                                                   "MP flexible CG, V-cycle, Jacobi coarsest solve")
+                                                   "MP flexible CG, full-depth V-cycle, CG on the 8^3 grid")
         if coarse is not None:
             solvers["nn_preconditioned_mp"] = timed(opts(xf.XF_FCG_MGNN, coarse),
-                                                    "MP flexible CG, V-cycle, network coarsest solve")
+                                                    "MP flexible CG, V-cycle to 64^3, network solve there")
         full = load_network(ctx, args.weights)
         if full is not None and not args.skip_fullres:
             solvers["nn_fullres_mp"] = timed(opts(xf.XF_FCG_NN, full), "MP flexible CG, full-resolution network")

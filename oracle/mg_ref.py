@@ -1,5 +1,6 @@
 """Test infrastructure: a numpy restatement of the V-cycle preconditioner of
-XF_FCG_MGNN (xflow_amd/csrc/xf_mg.hip) without the network, used only to
+XF_FCG_MGNN (xflow_amd/csrc/xf_mg.hip) without the network (the full-depth
+cycle with a one-workgroup CG on the deepest grid), used only to
 check the GPU cycle.  Not a reference routine: the reference has no
 multigrid; the cycle's parts are pinned by what the mathematics fixes (a
 smooth error is carried to the coarse grid and back by restriction and
@@ -12,6 +13,7 @@ import numpy as np
 
 FICTIVE, ACTUAL = 0, 1
 OMEGA, SWEEPS, COARSE_SWEEPS = np.float32(0.8), 2, 8
+COARSE_CELLS, COARSE_ITERS, COARSE_TOL = 4096, 64, 1e-6
 
 
 def coefficients(h, cnt):
@@ -86,13 +88,47 @@ def coarsen(codes):
 
 
 def hierarchy(codes, h):
+    """Every level down to the first whose grid (ring included) has at most
+    COARSE_CELLS cells, or whose interior extents cannot be halved again
+    (odd, or a half below 2)."""
     levels = [Level(np.asarray(codes, dtype=np.int32), h)]
     while True:
         n, l, m = (s - 2 for s in levels[-1].shape)
-        if n % 2 or l % 2 or m % 2 or m // 2 < 64 or n // 2 < 2 or l // 2 < 2:
+        if n % 2 or l % 2 or m % 2 or n // 2 < 2 or l // 2 < 2 or m // 2 < 2:
+            break
+        if levels[-1].codes.size <= COARSE_CELLS:
             break
         levels.append(Level(coarsen(levels[-1].codes), tuple(2 * x for x in levels[-1].h)))
     return levels
+
+
+def coarse_cg(L, r):
+    """CG on L z = r from z = 0 in FP32 with FP64 dots, until ||r_k|| <=
+    COARSE_TOL ||r_0|| or COARSE_ITERS iterations; r's mean over the unknowns
+    is removed first when the grid has no boundary-value cell (pure Neumann)."""
+    neumann = bool(np.all((L.codes == ACTUAL) | (L.codes == FICTIVE)))
+    r = np.where(L.unknown, r, np.float32(0)).astype(np.float32)
+    if neumann and L.unknown.any():
+        mean = np.float32(r[L.unknown].astype(np.float64).sum() / L.unknown.sum())
+        r = np.where(L.unknown, r - mean, np.float32(0)).astype(np.float32)
+    z = np.zeros_like(r)
+    p = r.copy()
+    rr = float(np.sum(r.astype(np.float64) ** 2))
+    stop = COARSE_TOL * COARSE_TOL * rr
+    for _ in range(COARSE_ITERS):
+        if not rr > stop:
+            break
+        q = L.apply(p)
+        pq = float(np.sum(p.astype(np.float64) * q.astype(np.float64)))
+        if not pq > 0:
+            break
+        alpha = np.float32(rr / pq)
+        z = np.where(L.act_in, z + alpha * p, z).astype(np.float32)
+        r = np.where(L.act_in, r - alpha * q, r).astype(np.float32)
+        rn = float(np.sum(r.astype(np.float64) ** 2))
+        p = (r + np.float32(rn / rr) * p).astype(np.float32)
+        rr = rn
+    return z
 
 
 def restrict(fine, t, coarse):
@@ -116,6 +152,8 @@ def prolong(fine, zc):
 def vcycle(levels, k, r):
     L = levels[k]
     if k == len(levels) - 1:
+        if L.codes.size <= COARSE_CELLS:
+            return coarse_cg(L, r)
         z = L.jacobi(r)
         for _ in range(COARSE_SWEEPS - 1):
             z = L.jacobi(r, z)

@@ -224,7 +224,9 @@ class Grid:
         return g
 
     def mg_apply(self, net, r, z):
-        """z = M(r): one V-cycle (FP32 fields), the network as the coarsest solve (net may be None)."""
+        """z = M(r): one V-cycle (FP32 fields): with a network, the cycle down to
+        the first grid with rows under 128 cells and the network as its solve;
+        with net None, the full-depth cycle and CG on its deepest grid."""
         import torch
         self.ctx.check(lib().xf_mg_apply(self.ctx.h, self.h, net.h if net is not None else None,
                                          self._vec(r, torch.float32), self._vec(z, torch.float32)), "xf_mg_apply")

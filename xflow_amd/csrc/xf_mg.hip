@@ -2,8 +2,11 @@
 // coarsest-grid solve (XF_FCG_MGNN).
 //
 // The pressure grid is halved per axis while every interior extent stays
-// even and the next x extent is at least 64 (the network's minimum), down to
-// a coarsest grid (64^3 for a 256^3 problem).  Coarse cells are unknowns if
+// even.  With the network the cycle stops where the next row (k extent)
+// would fall below 64, the network's minimum (64^3 for a 256^3 problem); the
+// cycle without a network goes on until the grid, ring included, fits one
+// workgroup (<= kCoarseCells cells: 8^3 for a 256^3 problem) and solves there
+// with CG in that one workgroup.  Coarse cells are unknowns if
 // any of their 8 children is an unknown, "boundary value" cells (a zero
 // correction, open face) if any child is a Defined/boundary point, Fictive
 // otherwise; coarse steps are the doubled fine steps, so each coarse grid
@@ -16,8 +19,9 @@
 //   coarsest level: y = net(r) (the network applied as on any grid: input
 //     r / rms(r) and the unknown mask, output times rms(r)), scaled by the
 //     energy-optimal factor (r, y) / (y, A y), then nu Jacobi sweeps; without
-//     a network nu_c Jacobi sweeps instead (the classical baseline of the
-//     same cycle).
+//     a network CG on the deepest grid to a 1e-6 relative residual (the
+//     classical full-depth cycle), or nu_c Jacobi sweeps where no grid is
+//     small enough.
 // The rediscretised coarse equation A_H e_H = mean(t) is consistent for
 // smooth errors (both sides approximate -lap e); the flexible CG around the
 // cycle absorbs any remaining scale.
@@ -31,6 +35,10 @@ constexpr float kOmega = 0.8f;
 constexpr int kSweeps = 2;        // pre- and post-smoothing sweeps per level
 constexpr int kCoarseSweeps = 8;  // coarsest-level sweeps without a network
 constexpr int kMgBlocks = 512;    // blocks of the reduction kernels
+constexpr int kCoarseCells = 4096;  // cells (ring included) of a grid solved in one workgroup
+constexpr int kCoarseIters = 64;    // at most this many CG iterations in that solve
+constexpr double kCoarseTol = 1e-6; // relative 2-norm residual at which that solve stops
+constexpr int kMaxLevels = 12;
 
 struct MgLevel {
   xf_grid g = nullptr;  // the rediscretised grid of this level (owned, except level 0)
@@ -38,8 +46,11 @@ struct MgLevel {
 };
 
 struct MgHier {
-  int nlev = 0;
-  MgLevel lev[8];
+  int nlev = 0;         // levels of the cycle without a network
+  int net_lev = 0;      // levels of the cycle with the network (its coarsest has rows >= 64)
+  bool deep_cg = false; // the deepest grid fits one workgroup: CG there
+  bool neumann = false; // the deepest grid has no boundary-value cells: project the mean
+  MgLevel lev[kMaxLevels];
   double *part = nullptr;  // [kMgBlocks][2] dot partials
   double *scal = nullptr;  // [4]: sum r^2, (r, y), (y, A y), alpha
 };
@@ -404,6 +415,117 @@ __global__ void k_mg_scale(int64_t nc, float *__restrict__ z, const double *__re
     z[q] *= a;
 }
 
+// CG on A z = r over a whole coarse grid in one workgroup: r, z and p in LDS
+// (<= kCoarseCells cells, ring included), FP32 vectors, FP64 dots summed in a
+// fixed order (every thread holds the same scalars), from z = 0 until
+// (r, r) <= kCoarseTol^2 (r0, r0) or kCoarseIters iterations, r's mean over
+// the unknowns removed first on a grid without boundary-value cells.
+__global__ __launch_bounds__(1024) void k_mg_coarse_cg(Dims d, const uint8_t *__restrict__ flags,
+                                                        const float *__restrict__ cx, const float *__restrict__ cy,
+                                                        const float *__restrict__ cz, int neumann,
+                                                        const float *__restrict__ rin, float *__restrict__ zout) {
+  __shared__ float zs[kCoarseCells], rs[kCoarseCells], ps[kCoarseCells];
+  __shared__ uint8_t fs[kCoarseCells];
+  __shared__ double red[16];
+  constexpr int kPer = kCoarseCells / 1024;  // cells per thread
+  const int nc = d.n * d.l * d.m;
+  const int sI = d.l * d.m, sJ = d.m;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+  auto bsum = [&](double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double t = 0;
+    for (int w = 0; w < nw; w++) t += red[w];
+    return t;
+  };
+  float cw[kPer][6];  // (c-, c+) per axis of this thread's cells
+  double s = 0, cnt = 0;
+  for (int t = 0; t < kPer; t++) {
+    const int q = tid + t * (int)blockDim.x;
+    for (int u = 0; u < 6; u++) cw[t][u] = 0.f;
+    if (q >= nc) continue;
+    const int i = q / sI, j = (q / sJ) % d.l, k = q % d.m;
+    cw[t][0] = cx[2 * i];
+    cw[t][1] = cx[2 * i + 1];
+    cw[t][2] = cy[2 * j];
+    cw[t][3] = cy[2 * j + 1];
+    cw[t][4] = cz[2 * k];
+    cw[t][5] = cz[2 * k + 1];
+    const uint8_t f = flags[q];
+    fs[q] = f;
+    const float v = (f & FL_UNKNOWN) ? rin[q] : 0.f;
+    rs[q] = v;
+    zs[q] = 0.f;
+    if (f & FL_UNKNOWN) {
+      s += v;
+      cnt += 1;
+    }
+  }
+  s = bsum(s);
+  cnt = bsum(cnt);
+  const float mean = (neumann && cnt > 0) ? (float)(s / cnt) : 0.f;
+  double rr = 0;
+  for (int t = 0; t < kPer; t++) {
+    const int q = tid + t * (int)blockDim.x;
+    if (q >= nc) break;
+    if (fs[q] & FL_UNKNOWN) rs[q] -= mean;
+    ps[q] = rs[q];
+    rr += (double)rs[q] * (double)rs[q];
+  }
+  rr = bsum(rr);
+  const double stop = kCoarseTol * kCoarseTol * rr;
+  for (int it = 0; it < kCoarseIters && rr > stop; it++) {
+    float aq[kPer];
+    double pq = 0;
+    for (int t = 0; t < kPer; t++) {
+      const int q = tid + t * (int)blockDim.x;
+      aq[t] = 0.f;
+      if (q >= nc) continue;
+      const uint8_t f = fs[q];
+      if (f & FL_ACTUAL) {
+        const float pc = ps[q];
+        const float tx = ((f & FL_IP) ? cw[t][1] * (ps[q + sI] - pc) : 0.f) -
+                         ((f & FL_IM) ? cw[t][0] * (pc - ps[q - sI]) : 0.f);
+        const float ty = ((f & FL_JP) ? cw[t][3] * (ps[q + sJ] - pc) : 0.f) -
+                         ((f & FL_JM) ? cw[t][2] * (pc - ps[q - sJ]) : 0.f);
+        const float tz = ((f & FL_KP) ? cw[t][5] * (ps[q + 1] - pc) : 0.f) -
+                         ((f & FL_KM) ? cw[t][4] * (pc - ps[q - 1]) : 0.f);
+        aq[t] = -(tx + ty + tz);
+        pq += (double)pc * (double)aq[t];
+      }
+    }
+    pq = bsum(pq);
+    if (!(pq > 0)) break;
+    const float alpha = (float)(rr / pq);
+    double rn = 0;
+    for (int t = 0; t < kPer; t++) {
+      const int q = tid + t * (int)blockDim.x;
+      if (q >= nc) break;
+      if (fs[q] & FL_ACTUAL) {
+        zs[q] += alpha * ps[q];
+        rs[q] -= alpha * aq[t];
+        rn += (double)rs[q] * (double)rs[q];
+      }
+    }
+    rn = bsum(rn);
+    const float beta = (float)(rn / rr);
+    rr = rn;
+    for (int t = 0; t < kPer; t++) {
+      const int q = tid + t * (int)blockDim.x;
+      if (q >= nc) break;
+      ps[q] = rs[q] + beta * ps[q];
+    }
+    __syncthreads();
+  }
+  for (int t = 0; t < kPer; t++) {
+    const int q = tid + t * (int)blockDim.x;
+    if (q >= nc) break;
+    zout[q] = zs[q];
+  }
+}
+
 // ------------------------------------------------------------------ host
 static dim3 cell_grid(const xf_grid g) { return dim3((g->m - 2 + 63) / 64, (g->l - 2 + 3) / 4, g->n - 2); }
 static Dims dims_of(const xf_grid g) { return Dims{g->n, g->l, g->m}; }
@@ -491,10 +613,13 @@ static MgHier *hierarchy(xf_ctx c, xf_grid g) {
   std::vector<uint8_t> fcodes((size_t)g->ncell);
   XF_HIP(hipMemcpy(fcodes.data(), g->d_codes, g->ncell, hipMemcpyDeviceToHost));
   int nlev = 1;
-  while (nlev < 8) {
+  h->net_lev = 0;
+  while (nlev < kMaxLevels) {
     const xf_grid f = h->lev[nlev - 1].g;
     const int N = f->n - 2, L = f->l - 2, M = f->m - 2;
-    if (N % 2 || L % 2 || M % 2 || M / 2 < 64 || N / 2 < 2 || L / 2 < 2) break;
+    if (N % 2 || L % 2 || M % 2 || N / 2 < 2 || L / 2 < 2 || M / 2 < 2) break;
+    if (f->ncell <= kCoarseCells) break;  // already small enough for the one-workgroup solve
+    if (!h->net_lev && M / 2 < 64) h->net_lev = nlev;  // the network's cycle ends here
     const int cn = N / 2 + 2, cl = L / 2 + 2, cm = M / 2 + 2;
     std::vector<int32_t> cc((size_t)cn * cl * cm, 0);
     auto kids = [](int I, int Nc, int *out) {  // fine indices of coarse index I (with ring)
@@ -533,6 +658,14 @@ static MgHier *hierarchy(xf_ctx c, xf_grid g) {
     for (size_t q = 0; q < fcodes.size(); q++) fcodes[q] = (uint8_t)cc[q];
   }
   h->nlev = nlev;
+  if (!h->net_lev) h->net_lev = nlev;
+  {
+    const xf_grid cg = h->lev[nlev - 1].g;
+    h->deep_cg = cg->ncell <= kCoarseCells;
+    h->neumann = true;
+    for (uint8_t v : fcodes)  // a boundary-value cell makes the deepest operator regular
+      if (v != PT_ACTUAL && v != PT_FICTIVE) h->neumann = false;
+  }
   for (int k = 0; k < nlev; k++) {
     MgLevel &L = h->lev[k];
     const int64_t nc = L.g->ncell;
@@ -582,12 +715,20 @@ static void smooth(xf_ctx c, MgLevel &L, const float *r, int nu, float *last = n
   if (last) XF_HIP(hipMemcpyAsync(last, L.z, sizeof(float) * g->ncell, hipMemcpyDeviceToDevice, c->stream));
 }
 
-// z of the coarsest level from its r (into `last` when given)
+// the deepest level of the cycle (with or without the network)
+static int last_level(const MgHier *h, const struct xf_unet_s *net) { return (net ? h->net_lev : h->nlev) - 1; }
+
+// z of the cycle's coarsest level from its r (into `last` when given)
 static void coarse_solve(xf_ctx c, MgHier *h, struct xf_unet_s *net, float *last) {
-  MgLevel &L = h->lev[h->nlev - 1];
+  MgLevel &L = h->lev[last_level(h, net)];
   const xf_grid g = L.g;
   const float *rin = L.r;
   if (!net) {
+    if (h->deep_cg) {
+      hipLaunchKernelGGL(k_mg_coarse_cg, dim3(1), dim3(1024), 0, c->stream, dims_of(g), g->d_flags, g->d_cxf,
+                         g->d_cyf, g->d_czf, h->neumann ? 1 : 0, rin, last ? last : L.z);
+      return;
+    }
     smooth_from_zero(c, L, rin, kCoarseSweeps);
     if (last) XF_HIP(hipMemcpyAsync(last, L.z, sizeof(float) * g->ncell, hipMemcpyDeviceToDevice, c->stream));
     return;
@@ -614,7 +755,7 @@ static void coarse_solve(xf_ctx c, MgHier *h, struct xf_unet_s *net, float *last
 // keep theirs in lev[k].z)
 static void vcycle(xf_ctx c, MgHier *h, int k, const float *r, struct xf_unet_s *net, float *out) {
   float *last = k == 0 ? out : nullptr;
-  if (k == h->nlev - 1) {
+  if (k == last_level(h, net)) {
     if (k == 0) {
       // a single level: the coarse solve reads lev[0].r, which is the caller's r
       MgLevel &L = h->lev[0];
