@@ -268,7 +268,7 @@ def main():
 
         coarse = load_network(ctx, args.coarse_weights)
         solvers["mp_fcg_cycle_no_network"] = timed(opts(xf.XF_FCG_MGNN, None),
-                                                   "MP flexible CG, full-depth V-cycle, CG on the 8^3 grid")
+                                                   "MP flexible CG, full-depth V-cycle, exact solve on the 4^3 grid")
         if coarse is not None:
             solvers["nn_preconditioned_mp"] = timed(opts(xf.XF_FCG_MGNN, coarse),
                                                     "MP flexible CG, V-cycle to 64^3, network solve there")

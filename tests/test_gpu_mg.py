@@ -42,15 +42,17 @@ def test_hierarchy_masks_match_numpy(ctx, shape, dirichlet):
         want = cases.coarsen_codes(want)
         sm, _, _ = g.mg_level(k).export(activity=False)
         assert np.array_equal(sm, (want == 1).astype(np.int32)), k
-    # halved to the first grid of at most 4096 cells (ring included) or odd extents:
+    # halved to the first grid of at most 512 cells (ring included) or odd extents:
     # (8, 8, 256) -> (2, 2, 64); (16, 12, 128) -> (8, 6, 64) -> (4, 3, 32)
     assert nlev == 3 == len(mg_ref.hierarchy(codes, tuple(1.0 / s for s in shape)))
 
 
 # (8, 10, 300): the residual restricted in the sweep pass with a shifted last
-# k-tile and a block whose last two rows lie past the grid
+# k-tile and a block whose last two rows lie past the grid; (16, 16, 64) and
+# (16, 16, 32): halved down to a grid of at most 512 cells, solved with its
+# pseudo-inverse (pure Neumann: singular; with a boundary-value plane: regular)
 @pytest.mark.parametrize("shape,dirichlet", [((8, 8, 256), False), ((16, 12, 128), True), ((20, 16, 64), True),
-                                             ((8, 10, 300), True)])
+                                             ((8, 10, 300), True), ((16, 16, 64), False), ((16, 16, 32), True)])
 def test_cycle_matches_restatement(ctx, shape, dirichlet):
     codes, h = case_of(shape, 12, dirichlet)
     g = xf.Grid(ctx, h[0], h[1], h[2], codes)
@@ -87,12 +89,12 @@ def test_flexible_cg_with_cycle_converges(ctx):
 def test_trained_network_gain(ctx):
     """What the trained network does against the method without it, on a
     held-out problem (training used seeds 100+, this is seed 7): the flexible
-    CG with the full-depth classical V-cycle (no network: down to 8^3 and a
-    one-workgroup CG there), with the cycle cut at 64^3 and an untrained
-    network there, and with the trained one (xflow_amd/weights), and plain CG
-    for reference.  Every solve must reach the true residual.  The trained
-    network is far better than an untrained one; against the full-depth
-    classical cycle it needs more iterations, each cheaper (the times are
+    CG with the full-depth classical V-cycle (no network: down to 4^3 and an
+    exact pseudo-inverse solve there), with the cycle cut at 64^3 and an
+    untrained network there, and with the trained one (xflow_amd/weights),
+    and plain CG for reference.  Every solve must reach the true residual.
+    The trained network is far better than an untrained one; against the
+    full-depth classical cycle it needs more iterations (the times are
     printed, not asserted)."""
     import os
     path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "xflow_amd", "weights",

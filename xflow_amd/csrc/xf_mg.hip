@@ -4,9 +4,10 @@
 // The pressure grid is halved per axis while every interior extent stays
 // even.  With the network the cycle stops where the next row (k extent)
 // would fall below 64, the network's minimum (64^3 for a 256^3 problem); the
-// cycle without a network goes on until the grid, ring included, fits one
-// workgroup (<= kCoarseCells cells: 8^3 for a 256^3 problem) and solves there
-// with CG in that one workgroup.  Coarse cells are unknowns if
+// cycle without a network goes on until the grid, ring included, has at most
+// kCoarseCells cells (4^3 for a 256^3 problem) and solves there exactly: a
+// dense pseudo-inverse of that grid's operator, formed on the host when the
+// hierarchy is built.  Coarse cells are unknowns if
 // any of their 8 children is an unknown, "boundary value" cells (a zero
 // correction, open face) if any child is a Defined/boundary point, Fictive
 // otherwise; coarse steps are the doubled fine steps, so each coarse grid
@@ -19,9 +20,10 @@
 //   coarsest level: y = net(r) (the network applied as on any grid: input
 //     r / rms(r) and the unknown mask, output times rms(r)), scaled by the
 //     energy-optimal factor (r, y) / (y, A y), then nu Jacobi sweeps; without
-//     a network CG on the deepest grid to a 1e-6 relative residual (the
-//     classical full-depth cycle), or nu_c Jacobi sweeps where no grid is
-//     small enough.
+//     a network z = A^+ r on the deepest grid (the classical full-depth
+//     cycle: the least-squares solution of least norm, which on a grid with
+//     no boundary-value cell is the mean-free one), or nu_c Jacobi sweeps
+//     where no grid is small enough.
 // The rediscretised coarse equation A_H e_H = mean(t) is consistent for
 // smooth errors (both sides approximate -lap e); the flexible CG around the
 // cycle absorbs any remaining scale.
@@ -35,9 +37,7 @@ constexpr float kOmega = 0.8f;
 constexpr int kSweeps = 2;        // pre- and post-smoothing sweeps per level
 constexpr int kCoarseSweeps = 8;  // coarsest-level sweeps without a network
 constexpr int kMgBlocks = 512;    // blocks of the reduction kernels
-constexpr int kCoarseCells = 4096;  // cells (ring included) of a grid solved in one workgroup
-constexpr int kCoarseIters = 64;    // at most this many CG iterations in that solve
-constexpr double kCoarseTol = 1e-6; // relative 2-norm residual at which that solve stops
+constexpr int kCoarseCells = 512;   // cells (ring included) of a grid solved densely
 constexpr int kMaxLevels = 12;
 
 struct MgLevel {
@@ -48,8 +48,9 @@ struct MgLevel {
 struct MgHier {
   int nlev = 0;         // levels of the cycle without a network
   int net_lev = 0;      // levels of the cycle with the network (its coarsest has rows >= 64)
-  bool deep_cg = false; // the deepest grid fits one workgroup: CG there
-  bool neumann = false; // the deepest grid has no boundary-value cells: project the mean
+  int ndense = 0;       // unknowns of the deepest grid's dense solve (0: Jacobi sweeps there)
+  float *minv = nullptr;     // [ndense][ndense] its pseudo-inverse (symmetric)
+  int32_t *uidx = nullptr;   // [ncell] cell -> unknown index, -1 elsewhere
   MgLevel lev[kMaxLevels];
   double *part = nullptr;  // [kMgBlocks][2] dot partials
   double *scal = nullptr;  // [4]: sum r^2, (r, y), (y, A y), alpha
@@ -415,115 +416,23 @@ __global__ void k_mg_scale(int64_t nc, float *__restrict__ z, const double *__re
     z[q] *= a;
 }
 
-// CG on A z = r over a whole coarse grid in one workgroup: r, z and p in LDS
-// (<= kCoarseCells cells, ring included), FP32 vectors, FP64 dots summed in a
-// fixed order (every thread holds the same scalars), from z = 0 until
-// (r, r) <= kCoarseTol^2 (r0, r0) or kCoarseIters iterations, r's mean over
-// the unknowns removed first on a grid without boundary-value cells.
-__global__ __launch_bounds__(1024) void k_mg_coarse_cg(Dims d, const uint8_t *__restrict__ flags,
-                                                        const float *__restrict__ cx, const float *__restrict__ cy,
-                                                        const float *__restrict__ cz, int neumann,
-                                                        const float *__restrict__ rin, float *__restrict__ zout) {
-  __shared__ float zs[kCoarseCells], rs[kCoarseCells], ps[kCoarseCells];
-  __shared__ uint8_t fs[kCoarseCells];
-  __shared__ double red[16];
-  constexpr int kPer = kCoarseCells / 1024;  // cells per thread
-  const int nc = d.n * d.l * d.m;
-  const int sI = d.l * d.m, sJ = d.m;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
-  auto bsum = [&](double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    __syncthreads();
-    if (lane == 0) red[wid] = v;
-    __syncthreads();
-    double t = 0;
-    for (int w = 0; w < nw; w++) t += red[w];
-    return t;
-  };
-  float cw[kPer][6];  // (c-, c+) per axis of this thread's cells
-  double s = 0, cnt = 0;
-  for (int t = 0; t < kPer; t++) {
-    const int q = tid + t * (int)blockDim.x;
-    for (int u = 0; u < 6; u++) cw[t][u] = 0.f;
-    if (q >= nc) continue;
-    const int i = q / sI, j = (q / sJ) % d.l, k = q % d.m;
-    cw[t][0] = cx[2 * i];
-    cw[t][1] = cx[2 * i + 1];
-    cw[t][2] = cy[2 * j];
-    cw[t][3] = cy[2 * j + 1];
-    cw[t][4] = cz[2 * k];
-    cw[t][5] = cz[2 * k + 1];
-    const uint8_t f = flags[q];
-    fs[q] = f;
-    const float v = (f & FL_UNKNOWN) ? rin[q] : 0.f;
-    rs[q] = v;
-    zs[q] = 0.f;
-    if (f & FL_UNKNOWN) {
-      s += v;
-      cnt += 1;
-    }
-  }
-  s = bsum(s);
-  cnt = bsum(cnt);
-  const float mean = (neumann && cnt > 0) ? (float)(s / cnt) : 0.f;
-  double rr = 0;
-  for (int t = 0; t < kPer; t++) {
-    const int q = tid + t * (int)blockDim.x;
-    if (q >= nc) break;
-    if (fs[q] & FL_UNKNOWN) rs[q] -= mean;
-    ps[q] = rs[q];
-    rr += (double)rs[q] * (double)rs[q];
-  }
-  rr = bsum(rr);
-  const double stop = kCoarseTol * kCoarseTol * rr;
-  for (int it = 0; it < kCoarseIters && rr > stop; it++) {
-    float aq[kPer];
-    double pq = 0;
-    for (int t = 0; t < kPer; t++) {
-      const int q = tid + t * (int)blockDim.x;
-      aq[t] = 0.f;
-      if (q >= nc) continue;
-      const uint8_t f = fs[q];
-      if (f & FL_ACTUAL) {
-        const float pc = ps[q];
-        const float tx = ((f & FL_IP) ? cw[t][1] * (ps[q + sI] - pc) : 0.f) -
-                         ((f & FL_IM) ? cw[t][0] * (pc - ps[q - sI]) : 0.f);
-        const float ty = ((f & FL_JP) ? cw[t][3] * (ps[q + sJ] - pc) : 0.f) -
-                         ((f & FL_JM) ? cw[t][2] * (pc - ps[q - sJ]) : 0.f);
-        const float tz = ((f & FL_KP) ? cw[t][5] * (ps[q + 1] - pc) : 0.f) -
-                         ((f & FL_KM) ? cw[t][4] * (pc - ps[q - 1]) : 0.f);
-        aq[t] = -(tx + ty + tz);
-        pq += (double)pc * (double)aq[t];
-      }
-    }
-    pq = bsum(pq);
-    if (!(pq > 0)) break;
-    const float alpha = (float)(rr / pq);
-    double rn = 0;
-    for (int t = 0; t < kPer; t++) {
-      const int q = tid + t * (int)blockDim.x;
-      if (q >= nc) break;
-      if (fs[q] & FL_ACTUAL) {
-        zs[q] += alpha * ps[q];
-        rs[q] -= alpha * aq[t];
-        rn += (double)rs[q] * (double)rs[q];
-      }
-    }
-    rn = bsum(rn);
-    const float beta = (float)(rn / rr);
-    rr = rn;
-    for (int t = 0; t < kPer; t++) {
-      const int q = tid + t * (int)blockDim.x;
-      if (q >= nc) break;
-      ps[q] = rs[q] + beta * ps[q];
-    }
-    __syncthreads();
-  }
-  for (int t = 0; t < kPer; t++) {
-    const int q = tid + t * (int)blockDim.x;
-    if (q >= nc) break;
-    zout[q] = zs[q];
-  }
+// z = A^+ r on the deepest grid, one thread per cell (<= kCoarseCells): r at
+// the unknowns staged in LDS, each unknown's row of the pseudo-inverse
+// (stored symmetric, read column-wise across threads) summed in FP64.
+__global__ __launch_bounds__(kCoarseCells) void k_mg_coarse_dense(int ncell, int nu, const int32_t *__restrict__ uidx,
+                                                                   const float *__restrict__ minv,
+                                                                   const float *__restrict__ rin,
+                                                                   float *__restrict__ zout) {
+  __shared__ float rs[kCoarseCells];
+  const int q = threadIdx.x;
+  const int u = q < ncell ? uidx[q] : -1;
+  if (u >= 0) rs[u] = rin[q];
+  __syncthreads();
+  if (q >= ncell) return;
+  double acc = 0;
+  if (u >= 0)
+    for (int v = 0; v < nu; v++) acc += (double)minv[(size_t)v * nu + u] * (double)rs[v];
+  zout[q] = (float)acc;
 }
 
 // ------------------------------------------------------------------ host
@@ -595,7 +504,120 @@ void free_mg(MgHier *h) {
   }
   hipFree(h->part);
   hipFree(h->scal);
+  hipFree(h->minv);
+  hipFree(h->uidx);
   delete h;
+}
+
+// The pseudo-inverse of the deepest grid's operator (the sweeps' A on its
+// Actual interior cells, coefficients rounded to FP32 as the kernels use
+// them).  Unknowns joined by open faces form components; a component with no
+// open face to a boundary-value cell is singular with the constant as its
+// null vector.  With P = sum over those components of 1 1^T / |component|,
+// A + P is SPD and A^+ = (A + P)^-1 - P.
+static void build_dense(xf_ctx c, MgHier *h, const std::vector<uint8_t> &codes) {
+  const xf_grid g = h->lev[h->nlev - 1].g;
+  const int n = g->n, l = g->l, m = g->m;
+  const int64_t S[3] = {(int64_t)l * m, (int64_t)m, 1};
+  const std::vector<double> *hs[3] = {&g->hx, &g->hy, &g->hz};
+  std::vector<int32_t> uidx((size_t)g->ncell, -1);
+  std::vector<int64_t> cells;
+  for (int i = 1; i < n - 1; i++)
+    for (int j = 1; j < l - 1; j++)
+      for (int k = 1; k < m - 1; k++) {
+        const int64_t q = ((int64_t)i * l + j) * m + k;
+        if (codes[q] == PT_ACTUAL) {
+          uidx[q] = (int32_t)cells.size();
+          cells.push_back(q);
+        }
+      }
+  const int nu = (int)cells.size();
+  h->ndense = nu;
+  if (nu == 0) return;
+  // per-node (c-, c+) of each axis, as the grid's FP32 coefficients
+  auto coef = [&](int axis, int i, int side) {
+    const std::vector<double> &hh = *hs[axis];
+    const double mid = (hh[i - 1] + hh[i]) / 2;
+    return (double)(float)(1.0 / ((side ? hh[i] : hh[i - 1]) * mid));
+  };
+  std::vector<double> A((size_t)nu * nu, 0.0);
+  std::vector<int> comp(nu, -1);
+  std::vector<char> regular;
+  for (int u = 0; u < nu; u++) {
+    const int64_t q = cells[u];
+    const int idx[3] = {(int)(q / S[0]), (int)((q / m) % l), (int)(q % m)};
+    for (int a = 0; a < 3; a++)
+      for (int side = 0; side < 2; side++) {
+        const int64_t q2 = side ? q + S[a] : q - S[a];
+        if (codes[q2] == PT_FICTIVE) continue;
+        const double cf = coef(a, idx[a], side);
+        A[(size_t)u * nu + u] += cf;
+        if (uidx[q2] >= 0) A[(size_t)u * nu + uidx[q2]] -= cf;
+      }
+  }
+  // components (breadth-first over open faces between unknowns)
+  for (int u0 = 0; u0 < nu; u0++) {
+    if (comp[u0] >= 0) continue;
+    const int id = (int)regular.size();
+    regular.push_back(0);
+    std::vector<int> stack{u0};
+    comp[u0] = id;
+    while (!stack.empty()) {
+      const int u = stack.back();
+      stack.pop_back();
+      const int64_t q = cells[u];
+      for (int a = 0; a < 3; a++)
+        for (int side = 0; side < 2; side++) {
+          const int64_t q2 = side ? q + S[a] : q - S[a];
+          if (codes[q2] == PT_FICTIVE) continue;
+          const int v = uidx[q2];
+          if (v < 0) {
+            regular[id] = 1;  // an open face to a boundary-value cell
+          } else if (comp[v] < 0) {
+            comp[v] = id;
+            stack.push_back(v);
+          }
+        }
+    }
+  }
+  std::vector<int> csize(regular.size(), 0);
+  for (int u = 0; u < nu; u++) csize[comp[u]]++;
+  std::vector<double> P((size_t)nu * nu, 0.0);
+  for (int u = 0; u < nu; u++)
+    for (int v = 0; v < nu; v++)
+      if (comp[u] == comp[v] && !regular[comp[u]]) P[(size_t)u * nu + v] = 1.0 / csize[comp[u]];
+  // Cholesky of A + P, then its inverse column by column
+  std::vector<double> Lc((size_t)nu * nu, 0.0);
+  for (int i = 0; i < nu; i++)
+    for (int j = 0; j <= i; j++) {
+      double sum = A[(size_t)i * nu + j] + P[(size_t)i * nu + j];
+      for (int k = 0; k < j; k++) sum -= Lc[(size_t)i * nu + k] * Lc[(size_t)j * nu + k];
+      if (i == j) {
+        XF_REQUIRE(sum > 0, XF_EINVAL, "multigrid: the deepest grid's operator is not positive definite");
+        Lc[(size_t)i * nu + i] = std::sqrt(sum);
+      } else {
+        Lc[(size_t)i * nu + j] = sum / Lc[(size_t)j * nu + j];
+      }
+    }
+  std::vector<float> minv((size_t)nu * nu);
+  std::vector<double> col(nu);
+  for (int e = 0; e < nu; e++) {
+    for (int i = 0; i < nu; i++) {  // L y = e_e
+      double sum = i == e ? 1.0 : 0.0;
+      for (int k = 0; k < i; k++) sum -= Lc[(size_t)i * nu + k] * col[k];
+      col[i] = sum / Lc[(size_t)i * nu + i];
+    }
+    for (int i = nu - 1; i >= 0; i--) {  // L^T x = y
+      double sum = col[i];
+      for (int k = i + 1; k < nu; k++) sum -= Lc[(size_t)k * nu + i] * col[k];
+      col[i] = sum / Lc[(size_t)i * nu + i];
+    }
+    for (int i = 0; i < nu; i++) minv[(size_t)e * nu + i] = (float)(col[i] - P[(size_t)e * nu + i]);
+  }
+  XF_HIP(hipMalloc(&h->minv, sizeof(float) * minv.size()));
+  XF_HIP(hipMemcpy(h->minv, minv.data(), sizeof(float) * minv.size(), hipMemcpyHostToDevice));
+  XF_HIP(hipMalloc(&h->uidx, sizeof(int32_t) * uidx.size()));
+  XF_HIP(hipMemcpy(h->uidx, uidx.data(), sizeof(int32_t) * uidx.size(), hipMemcpyHostToDevice));
 }
 
 // Builds the hierarchy of g on first use (level 0 is g itself; its r is the
@@ -618,7 +640,7 @@ static MgHier *hierarchy(xf_ctx c, xf_grid g) {
     const xf_grid f = h->lev[nlev - 1].g;
     const int N = f->n - 2, L = f->l - 2, M = f->m - 2;
     if (N % 2 || L % 2 || M % 2 || N / 2 < 2 || L / 2 < 2 || M / 2 < 2) break;
-    if (f->ncell <= kCoarseCells) break;  // already small enough for the one-workgroup solve
+    if (f->ncell <= kCoarseCells) break;  // already small enough for the dense solve
     if (!h->net_lev && M / 2 < 64) h->net_lev = nlev;  // the network's cycle ends here
     const int cn = N / 2 + 2, cl = L / 2 + 2, cm = M / 2 + 2;
     std::vector<int32_t> cc((size_t)cn * cl * cm, 0);
@@ -659,13 +681,7 @@ static MgHier *hierarchy(xf_ctx c, xf_grid g) {
   }
   h->nlev = nlev;
   if (!h->net_lev) h->net_lev = nlev;
-  {
-    const xf_grid cg = h->lev[nlev - 1].g;
-    h->deep_cg = cg->ncell <= kCoarseCells;
-    h->neumann = true;
-    for (uint8_t v : fcodes)  // a boundary-value cell makes the deepest operator regular
-      if (v != PT_ACTUAL && v != PT_FICTIVE) h->neumann = false;
-  }
+  if (h->lev[nlev - 1].g->ncell <= kCoarseCells) build_dense(c, h, fcodes);
   for (int k = 0; k < nlev; k++) {
     MgLevel &L = h->lev[k];
     const int64_t nc = L.g->ncell;
@@ -724,9 +740,9 @@ static void coarse_solve(xf_ctx c, MgHier *h, struct xf_unet_s *net, float *last
   const xf_grid g = L.g;
   const float *rin = L.r;
   if (!net) {
-    if (h->deep_cg) {
-      hipLaunchKernelGGL(k_mg_coarse_cg, dim3(1), dim3(1024), 0, c->stream, dims_of(g), g->d_flags, g->d_cxf,
-                         g->d_cyf, g->d_czf, h->neumann ? 1 : 0, rin, last ? last : L.z);
+    if (h->ndense > 0) {
+      hipLaunchKernelGGL(k_mg_coarse_dense, dim3(1), dim3(kCoarseCells), 0, c->stream, (int)g->ncell, h->ndense,
+                         h->uidx, h->minv, rin, last ? last : L.z);
       return;
     }
     smooth_from_zero(c, L, rin, kCoarseSweeps);
