@@ -46,6 +46,14 @@ int xf_shutdown(xf_ctx ctx);
 const char *xf_last_error(xf_ctx ctx);
 int xf_set_stream(xf_ctx ctx, void *hip_stream); /* NULL: the context's own stream */
 int xf_synchronize(xf_ctx ctx);
+/* Checks that only the data can fail (xf_velocity_correct's boundary-value
+ * checks, where the reference throws) are read back by the call itself by
+ * default, which waits for the stream.  With deferred errors on, calls only
+ * record them on the device; xf_check_errors waits for the stream, returns
+ * XF_EINVAL with the first recorded error's message (xf_last_error) and
+ * clears the record. */
+int xf_set_deferred_errors(xf_ctx ctx, int on);
+int xf_check_errors(xf_ctx ctx);
 int xf_malloc(xf_ctx ctx, size_t bytes, void **dptr);
 int xf_free(xf_ctx ctx, void *dptr);
 /* kind: 0 host->device, 1 device->host, 2 device->device; synchronous. */

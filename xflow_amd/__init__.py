@@ -58,6 +58,14 @@ class Context:
     def synchronize(self):
         self.check(lib().xf_synchronize(self.h), "xf_synchronize")
 
+    def set_deferred_errors(self, on=True):
+        """Data checks (the velocity correction's boundary values) recorded on
+        the device and raised by check_errors() instead of by each call."""
+        self.check(lib().xf_set_deferred_errors(self.h, 1 if on else 0), "xf_set_deferred_errors")
+
+    def check_errors(self):
+        self.check(lib().xf_check_errors(self.h), "xf_check_errors")
+
     def close(self):
         if getattr(self, "h", None):
             lib().xf_shutdown(self.h)
@@ -226,7 +234,7 @@ class Grid:
     def mg_apply(self, net, r, z):
         """z = M(r): one V-cycle (FP32 fields): with a network, the cycle down to
         the first grid with rows under 128 cells and the network as its solve;
-        with net None, the full-depth cycle and CG on its deepest grid."""
+        with net None, the full-depth cycle and the pseudo-inverse of its deepest grid."""
         import torch
         self.ctx.check(lib().xf_mg_apply(self.ctx.h, self.h, net.h if net is not None else None,
                                          self._vec(r, torch.float32), self._vec(z, torch.float32)), "xf_mg_apply")

@@ -42,6 +42,8 @@ SIGNATURES = {
     "xf_last_error": (C.c_char_p, [P]),
     "xf_set_stream": (I, [P, P]),
     "xf_synchronize": (I, [P]),
+    "xf_set_deferred_errors": (I, [P, I]),
+    "xf_check_errors": (I, [P]),
     "xf_malloc": (I, [P, SZ, C.POINTER(P)]),
     "xf_free": (I, [P, P]),
     "xf_memcpy": (I, [P, P, P, SZ, I]),

@@ -2,6 +2,7 @@
 // activity table) for libxflow_hip.so.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -37,6 +38,8 @@ extern "C" int xf_init(int device, xf_ctx *out) {
     XF_HIP(hipHostMalloc((void **)&c->h_state, sizeof(DevState), hipHostMallocDefault));
     XF_HIP(hipEventCreate(&c->ev0));
     XF_HIP(hipEventCreate(&c->ev1));
+    XF_HIP(hipMalloc(&c->d_err, sizeof(int)));
+    XF_HIP(hipMemset(c->d_err, 0, sizeof(int)));
   });
   if (st != XF_OK) {
     fprintf(stderr, "xf_init: %s\n", c->err.c_str());
@@ -59,6 +62,8 @@ extern "C" int xf_shutdown(xf_ctx c) {
   hipHostFree(c->h_state);
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
+  hipFree(c->d_err);
+  for (int k = 0; k < WS_COUNT; k++) hipFree(c->ws[k]);
   for (auto e : c->prof_events) hipEventDestroy(e);
   if (c->own_stream) hipStreamDestroy(c->own_stream);
   delete c;
@@ -69,8 +74,87 @@ extern "C" const char *xf_last_error(xf_ctx c) { return c ? c->err.c_str() : "nu
 
 extern "C" int xf_set_stream(xf_ctx c, void *s) {
   if (!c) return XF_EINVAL;
+  // the workspaces are ordered by the stream: let the old stream's work finish
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return XF_EHIP;
   c->stream = s ? (hipStream_t)s : c->own_stream;
   return XF_OK;
+}
+
+void *ws_get(xf_ctx c, int slot, size_t bytes) {
+  if (bytes == 0) bytes = 1;
+  if (c->ws_cap[slot] < bytes) {
+    if (c->ws[slot]) {
+      XF_HIP(hipStreamSynchronize(c->stream));
+      XF_HIP(hipFree(c->ws[slot]));
+      c->ws[slot] = nullptr;
+      c->ws_cap[slot] = 0;
+    }
+    XF_HIP(hipMalloc(&c->ws[slot], bytes));
+    c->ws_cap[slot] = bytes;
+  }
+  return c->ws[slot];
+}
+
+const std::vector<uint8_t> &host_codes(xf_grid g) {
+  if (g->codes_h.empty()) {
+    g->codes_h.resize((size_t)g->ncell);
+    XF_HIP(hipMemcpy(g->codes_h.data(), g->d_codes, (size_t)g->ncell, hipMemcpyDeviceToHost));
+  }
+  return g->codes_h;
+}
+
+// A device copy of the host steps h[0..cnt) for the grid's slot, uploaded
+// only when they differ from the last ones passed.
+const double *device_steps(xf_ctx c, xf_grid g, int slot, const double *h, int cnt) {
+  std::vector<double> &hs = g->steps_h[slot];
+  if ((int)hs.size() == cnt && g->steps_d[slot] && std::equal(hs.begin(), hs.end(), h)) return g->steps_d[slot];
+  if ((int)hs.size() != cnt || !g->steps_d[slot]) {
+    if (g->steps_d[slot]) {
+      XF_HIP(hipStreamSynchronize(c->stream));
+      XF_HIP(hipFree(g->steps_d[slot]));
+      g->steps_d[slot] = nullptr;
+    }
+    XF_HIP(hipMalloc(&g->steps_d[slot], sizeof(double) * (cnt > 0 ? cnt : 1)));
+  }
+  hs.assign(h, h + cnt);
+  // from the cached host vector, which outlives the stream-ordered copy
+  XF_HIP(hipMemcpyAsync(g->steps_d[slot], hs.data(), sizeof(double) * cnt, hipMemcpyHostToDevice, c->stream));
+  return g->steps_d[slot];
+}
+
+const char *err_bit_message(int bits) {
+  if (bits & EB_VC_DEFINED) return "TVPProblem::SolveBySplitting: Defined border condition checking error.";
+  if (bits & EB_VC_PREDEF)
+    return "TVPProblem::SolveBySplitting: PreDefined border condition checking error - value incorrect.";
+  return "unknown device error";
+}
+
+// Reads and clears the context's error word (the stream is synchronised).
+static int take_errors(xf_ctx c) {
+  int e = 0;
+  XF_HIP(hipMemcpyAsync(&e, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  XF_HIP(hipStreamSynchronize(c->stream));
+  if (e) XF_HIP(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+  return e;
+}
+
+extern "C" int xf_set_deferred_errors(xf_ctx c, int on) {
+  if (!c) return XF_EINVAL;
+  c->defer_errors = on != 0;
+  return XF_OK;
+}
+
+extern "C" int xf_check_errors(xf_ctx c) {
+  return guarded(c, [&] {
+    const int e = take_errors(c);
+    XF_REQUIRE(e == 0, XF_EINVAL, err_bit_message(e));
+  });
+}
+
+void raise_device_errors(xf_ctx c) {
+  if (c->defer_errors) return;
+  const int e = take_errors(c);
+  XF_REQUIRE(e == 0, XF_EINVAL, err_bit_message(e));
 }
 
 extern "C" int xf_synchronize(xf_ctx c) {
@@ -204,6 +288,8 @@ void free_grid(xf_grid g) {
   hipFree(g->d_cxf); hipFree(g->d_cyf); hipFree(g->d_czf);
   hipFree(g->d_codes); hipFree(g->d_flags); hipFree(g->d_space); hipFree(g->d_rn);
   hipFree(g->d_normals); hipFree(g->d_has_normal); hipFree(g->d_rho);
+  hipFree(g->d_codes_kt);
+  for (int k = 0; k < 4; k++) hipFree(g->steps_d[k]);
   free_cg_work(g->work);
   delete g;
 }

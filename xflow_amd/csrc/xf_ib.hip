@@ -146,24 +146,19 @@ extern "C" int xf_ib_spread(xf_ctx c, xf_grid g, int axis, int nn, const double 
     XF_HIP(hipMemsetAsync(force, 0, sizeof(double) * g->ncell, c->stream));
     if (nn == 0) return;
     const int64_t np = (int64_t)nn * 343;
-    uint32_t *k0 = nullptr, *k1 = nullptr;
-    double *v0 = nullptr, *v1 = nullptr;
-    XF_HIP(hipMalloc(&k0, sizeof(uint32_t) * np));
-    XF_HIP(hipMalloc(&k1, sizeof(uint32_t) * np));
-    XF_HIP(hipMalloc(&v0, sizeof(double) * np));
-    XF_HIP(hipMalloc(&v1, sizeof(double) * np));
+    uint32_t *k0 = (uint32_t *)ws_get(c, WS_IB_K0, sizeof(uint32_t) * np);
+    uint32_t *k1 = (uint32_t *)ws_get(c, WS_IB_K1, sizeof(uint32_t) * np);
+    double *v0 = (double *)ws_get(c, WS_IB_V0, sizeof(double) * np);
+    double *v1 = (double *)ws_get(c, WS_IB_V1, sizeof(double) * np);
     hipLaunchKernelGGL(k_ib_products, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, geom(g), axis, nn,
                        xyz, F, k0, v0);
     check_kernel("k_ib_products");
     size_t tb = 0;
     XF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, (int)np, 0, 32, c->stream));
-    void *tmp = nullptr;
-    XF_HIP(hipMalloc(&tmp, tb));
+    void *tmp = ws_get(c, WS_IB_TMP, tb);
     XF_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, (int)np, 0, 32, c->stream));
     hipLaunchKernelGGL(k_ib_segsum, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, np, k1, v1, force);
     check_kernel("k_ib_segsum");
-    XF_HIP(hipStreamSynchronize(c->stream));
-    hipFree(tmp); hipFree(k0); hipFree(k1); hipFree(v0); hipFree(v1);
   });
 }
 

@@ -57,8 +57,31 @@ struct DevState {
   unsigned counter[kSlots];
 };
 
+// Grow-only device workspaces of a context, one per use; every call that
+// uses one runs on the context's stream, so a later call's reuse is ordered
+// after an earlier call's kernels.
+enum WsSlot {
+  WS_LOD_ST0, WS_LOD_ST1, WS_LOD_ROWS, WS_LOD_SCRA, WS_LOD_SCRB,
+  WS_IB_K0, WS_IB_K1, WS_IB_V0, WS_IB_V1, WS_IB_TMP,
+  WS_THOMAS_A, WS_THOMAS_B,
+  WS_COUNT
+};
+
+// Errors that only the data can show (a boundary value that does not match
+// the predictor): kernels OR these bits into the context's error word; the
+// call reads it back, or with deferred errors on, xf_check_errors does.
+enum ErrBit {
+  EB_VC_DEFINED = 1,     // velocity correction: u* differs from the Defined value
+  EB_VC_PREDEF = 2,      // velocity correction: u* at a PreDefined point is not the mirror
+};
+const char *err_bit_message(int bits);
+
 struct xf_ctx_s {
   int device = 0;
+  void *ws[WS_COUNT] = {};
+  size_t ws_cap[WS_COUNT] = {};
+  int *d_err = nullptr;           // ErrBit word
+  bool defer_errors = false;
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
   std::string err;
@@ -113,7 +136,25 @@ struct xf_grid_s {
   bool tv_visterm_ok[4] = {false, false, false, false};
   struct CgWork *work = nullptr;      // throughput-solver vectors, allocated on first use
   struct MgHier *mg = nullptr;        // multigrid hierarchy (XF_FCG_MGNN), built on first use
+  // LOD transport: sector check of the point types (done on first use; 0 =
+  // every line valid, else the failing rule's bit) and, for lines along k,
+  // the point types in line-interleaved order (type of cell k of line t at
+  // k * (n * l) + t)
+  bool lod_checked = false;
+  int lod_err = 0;
+  uint8_t *d_codes_kt = nullptr;
+  // projection glue: point-type errors per call form (computed on first use,
+  // -1 = not yet), and device copies of the face steps last passed
+  int glue_rhs_err = -1;
+  int glue_vc_err[3] = {-1, -1, -1};
+  std::vector<double> steps_h[4];     // hxU, hyV, hzW, and the velocity correction's hP
+  double *steps_d[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<uint8_t> codes_h;       // host copy of the point types, made on first need
 };
+void *ws_get(xf_ctx c, int slot, size_t bytes);
+void raise_device_errors(xf_ctx c);  // unless errors are deferred: read, clear and raise the error word
+const std::vector<uint8_t> &host_codes(xf_grid g);
+const double *device_steps(xf_ctx c, xf_grid g, int slot, const double *h, int cnt);
 void free_cg_work(struct CgWork *w);
 
 // T1dPackOperator on the device (xf_pack.hip)

@@ -3,15 +3,26 @@
 // THydrodynamicsProblem::SolveTransferEquation__ (XProblem.cpp:2784-3475) runs,
 // for each global direction, one implicit sweep along that direction (a
 // tridiagonal solve per "sector" of Actual points between two border points)
-// followed by two border-only passes along the other directions.  Lines are
-// independent within a sweep or pass (a sweep reads only the previous time
-// level and the stages of earlier directions; a border pass reads sector
-// interiors that no pass writes), so each line is one GPU thread; sweeps and
-// passes are separate launches, in the reference's order.  The Thomas
-// elimination (XSLAE.cpp:416-445) keeps its A[m], B[m] in a line-interleaved
-// scratch array (element m of line t at m*nlines + t) so neighbouring threads
-// touch neighbouring addresses.  Arithmetic follows the reference's operand
-// order (built with -ffp-contract=off), so results are bit-identical.
+// followed by two border-only passes along the other directions.  Here each
+// direction is three launches:
+//   k_lod_rows   the tridiagonal row (a, b, c, F) of every Actual cell, one
+//                thread per cell (a row reads only the previous level and the
+//                stages of earlier directions); rows of lines along k are
+//                written line-interleaved through an LDS transpose, so every
+//                line solve reads its rows coalesced;
+//   k_lod_solve  one thread per line: the sector walk, the boundary rows and
+//                the Thomas elimination (XSLAE.cpp:416-445) in the
+//                reference's order, rows prefetched eight cells ahead, the
+//                elimination's A[m], B[m] line-interleaved in scratch;
+//   k_lod_border the two border-only passes, one thread per border cell (a
+//                sector end reads only its Actual neighbour, which no pass
+//                writes; where a cell ends one sector and starts the next, or
+//                borders sectors of both passes, the later rule wins, as in
+//                the line walk).
+// The sector rules the reference enforces per line (XProblem.cpp:2932-2960)
+// depend only on the point types, so they are checked once per grid on the
+// host.  Arithmetic follows the reference's operand order (built with
+// -ffp-contract=off), so results are bit-identical.
 #include <algorithm>
 
 #include "xf_internal.h"
@@ -26,7 +37,6 @@ struct LodArgs {
   double tau;
   int scheme;
   double *scrA, *scrB;       // [maxlen][nlines]
-  int *err;
 };
 
 __device__ inline void line_geometry(const LodArgs &a, int axis, int t, int64_t &base, int64_t &st, int &len,
@@ -64,31 +74,20 @@ __device__ inline void side_terms(const LodArgs &a, int e, int64_t c, int ii, co
   FD = fdd;
 }
 
-// Row `s` of the sector system: a, b, c, f (XProblem.cpp:3058-3222, 3224-3419).
+// Row of an Actual cell c of a sector along `axis` (XProblem.cpp:3198-3222,
+// 3224-3419); idx3 = (i, j, k) of c, dir = axis + 1 = the sweep's direction.
 struct Row {
   double a, b, c, f;
 };
+struct alignas(32) Row4 {
+  double a, b, c, f;
+};
 
-__device__ inline Row sector_row(const LodArgs &A, int axis, int dir, int64_t base, int64_t sa, int beg, int end,
-                                 int mbeg, int mend, int fl, int fr, int sd, int s) {
+__device__ inline Row cell_row(const LodArgs &A, int axis, int dir, int64_t c, const int idx3[3]) {
   Row r{0, 0, 0, 0};
   const double *h = A.h[axis];
-  if (fl && s == 1) {
-    const int64_t cb = base + (int64_t)beg * sa;
-    if (mbeg == PT_PREDEFINED) r = Row{1, 0.5, 0.5, A.rp[cb]};
-    else r = Row{1, -1 / h[beg], 1 / h[beg], A.rp[cb]};
-    return r;
-  }
-  if (fr && s == sd - 1) {
-    const int64_t ce = base + (int64_t)end * sa;
-    if (mend == PT_PREDEFINED) r = Row{0.5, 0.5, 1, A.rp[ce]};
-    else r = Row{-1 / h[end - 1], 1 / h[end - 1], 1, A.rp[ce]};
-    return r;
-  }
-  const int da = beg + s - fl;
-  const int64_t c = base + (int64_t)da * sa;
   const int64_t S[3] = {(int64_t)A.l * A.m, (int64_t)A.m, 1};
-  const int idx3[3] = {(int)(c / S[0]), (int)((c / S[1]) % A.l), (int)(c % A.m)};
+  const int64_t sa = S[axis];
   const int ii = idx3[axis];
   const double h_1 = h[ii - 1], hh = h[ii], hx2 = h[ii - 1] + h[ii];
   const double hxhxh = h[ii - 1] * h[ii] * (h[ii - 1] + h[ii]) / 2;
@@ -126,146 +125,315 @@ __device__ inline Row sector_row(const LodArgs &A, int axis, int dir, int64_t ba
   return r;
 }
 
-// One sweep (implicit solve along `axis`, writes stage[dg-1]) or one border
-// pass (rewrites sector ends of stage[dg-1]) over every line along `axis`.
-__global__ void k_lod_lines(LodArgs A, int axis, int dg, int border_only) {
-  int64_t base, sa;
-  int len, nlines;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  line_geometry(A, axis, 0, base, sa, len, nlines);
-  if (t >= nlines) return;
-  line_geometry(A, axis, t, base, sa, len, nlines);
-  const uint8_t *mk = A.codes;
-  const double *h = A.h[axis];
-  double *fa = A.stage[dg - 1];
-  const int dir = axis + 1;
-  int beg = -1, mbeg = 0;
-  for (int q = 0; q < len; q++) {
-    const int mq = mk[base + (int64_t)q * sa];
-    if (beg < 0) {
-      if (mq == PT_ACTUAL) {
-        if (q == 0) { atomicOr(A.err, 1); return; }  // XProblem.cpp:2932-2935
-        beg = q - 1;
-        mbeg = mk[base + (int64_t)beg * sa];
+// Rows of every Actual cell for the sweep along AXIS.  AXIS 0 and 1: one
+// thread per cell, rows in cell order (the line solves of those axes read
+// them coalesced already).  AXIS 2: 16 (k) x 16 (j) tiles of a plane,
+// transposed in LDS and written with the row of cell k of line t = i*l + j at
+// k * (n*l) + t.
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_lod_rows(LodArgs A, Row4 *__restrict__ rows) {
+  const int n = A.n, l = A.l, m = A.m;
+  if (AXIS < 2) {
+    const int64_t nc = (int64_t)n * l * m;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+      if (A.codes[c] != PT_ACTUAL) continue;
+      const int idx3[3] = {(int)(c / ((int64_t)l * m)), (int)((c / m) % l), (int)(c % m)};
+      const Row r = cell_row(A, AXIS, AXIS + 1, c, idx3);
+      rows[c] = Row4{r.a, r.b, r.c, r.f};
+    }
+  } else {
+    __shared__ Row4 tile[16][16];
+    __shared__ uint8_t act[16][16];
+    const int kk = threadIdx.x & 15, jj = threadIdx.x >> 4;
+    const int k0 = blockIdx.x * 16, j0 = blockIdx.y * 16, i = blockIdx.z;
+    const int k = k0 + kk, j = j0 + jj;
+    uint8_t on = 0;
+    if (k < m && j < l) {
+      const int64_t c = ((int64_t)i * l + j) * m + k;
+      if (A.codes[c] == PT_ACTUAL) {
+        const int idx3[3] = {i, j, k};
+        const Row r = cell_row(A, 2, 3, c, idx3);
+        tile[kk][jj] = Row4{r.a, r.b, r.c, r.f};
+        on = 1;
       }
-      continue;
     }
-    if (mq == PT_ACTUAL) continue;
-    const int end = q, mend = mq;
-    // XProblem.cpp:2952-2960
-    if (mbeg == PT_ACTUAL || mbeg == PT_FICTIVE || mend == PT_ACTUAL || mend == PT_FICTIVE || end - beg < 2) {
-      atomicOr(A.err, 2);
-      return;
-    }
-    const int64_t cb = base + (int64_t)beg * sa, ce = base + (int64_t)end * sa;
-    if (border_only) {  // XProblem.cpp:2965-3033
-      if (mbeg == PT_DEFINED || mbeg == PT_EQUATION) fa[cb] = A.rp[cb];
-      else if (mbeg == PT_PREDEFINED) fa[cb] = 2 * A.rp[cb] - fa[cb + sa];
-      else if (mbeg == PT_NORMAL || mbeg == PT_PRENORMAL) fa[cb] = -(h[beg]) * A.rp[cb] + fa[cb + sa];
-      else { atomicOr(A.err, 4); return; }
-      if (mend == PT_DEFINED || mend == PT_EQUATION) fa[ce] = A.rp[ce];
-      else if (mend == PT_PREDEFINED) fa[ce] = 2 * A.rp[ce] - fa[ce - sa];
-      else if (mend == PT_NORMAL || mend == PT_PRENORMAL) fa[ce] = (h[end - 1]) * A.rp[ce] + fa[ce - sa];
-      else { atomicOr(A.err, 4); return; }
-      beg = -1;
-      continue;
-    }
-    const bool okb = mbeg == PT_DEFINED || mbeg == PT_EQUATION || mbeg == PT_PREDEFINED || mbeg == PT_NORMAL ||
-                     mbeg == PT_PRENORMAL;
-    const bool oke = mend == PT_DEFINED || mend == PT_EQUATION || mend == PT_PREDEFINED || mend == PT_NORMAL ||
-                     mend == PT_PRENORMAL;
-    if (!okb || !oke) { atomicOr(A.err, 4); return; }
-    const int fl = (mbeg != PT_DEFINED && mbeg != PT_EQUATION);
-    const int fr = (mend != PT_DEFINED && mend != PT_EQUATION);
-    const int sd = end - beg + fl + fr;  // dim - 1
-    const double u0 = fl ? 0.0 : A.rp[cb];
-    const double uM = fr ? 0.0 : A.rp[ce];
-    // forward elimination (XSLAE.cpp:428-436)
-    double Am1 = 0, Bm1 = u0;
-    for (int s = 1; s < sd; s++) {
-      const Row rw = sector_row(A, axis, dir, base, sa, beg, end, mbeg, mend, fl, fr, sd, s);
-      const double Am = (-rw.c) / (rw.a * Am1 + rw.b);
-      const double Bm = (rw.f - rw.a * Bm1) / (rw.a * Am1 + rw.b);
-      A.scrA[(int64_t)s * nlines + t] = Am;
-      A.scrB[(int64_t)s * nlines + t] = Bm;
-      Am1 = Am;
-      Bm1 = Bm;
-    }
-    // back substitution (XSLAE.cpp:438-441) and assignment (XProblem.cpp:3439-3449)
-    double u = uM;
-    if (!fr) fa[base + (int64_t)(beg + sd - fl) * sa] = u;
-    for (int s = sd - 1; s > 0; s--) {
-      u = A.scrA[(int64_t)s * nlines + t] * u + A.scrB[(int64_t)s * nlines + t];
-      if (s >= fl && s <= sd - fr) fa[base + (int64_t)(beg + s - fl) * sa] = u;
-    }
-    if (!fl) fa[cb] = u0;
-    beg = -1;
+    act[kk][jj] = on;
+    __syncthreads();
+    const int kk2 = threadIdx.x >> 4, jj2 = threadIdx.x & 15;
+    const int k2 = k0 + kk2, j2 = j0 + jj2;
+    if (k2 < m && j2 < l && act[kk2][jj2])
+      rows[(int64_t)k2 * ((int64_t)n * l) + (int64_t)i * l + j2] = tile[kk2][jj2];
   }
 }
 
-namespace {
-struct LodWork {
-  int64_t ncell = 0, scr = 0;
-  double *st0 = nullptr, *st1 = nullptr, *scrA = nullptr, *scrB = nullptr;
-  int *err = nullptr;
-  ~LodWork() {
-    hipFree(st0); hipFree(st1); hipFree(scrA); hipFree(scrB); hipFree(err);
+// The sweep along AXIS over every line (one thread per line): sectors of
+// Actual cells between border points, the boundary rows of PreDefined /
+// Normal / PreNormal ends (XProblem.cpp:3064-3155), forward elimination and
+// back substitution (XSLAE.cpp:428-441), written into stage[dg-1]
+// (XProblem.cpp:3439-3449).  The host has checked every line's sectors.
+template <int AXIS>
+__global__ __launch_bounds__(64) void k_lod_solve(LodArgs A, const Row4 *__restrict__ rows,
+                                                  const uint8_t *__restrict__ codes_kt, int dg) {
+  int64_t base, sa;
+  int len, nl;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  line_geometry(A, AXIS, 0, base, sa, len, nl);
+  if (t >= nl) return;
+  line_geometry(A, AXIS, t, base, sa, len, nl);
+  const double *h = A.h[AXIS];
+  double *fa = A.stage[dg - 1];
+  double *__restrict__ sA = A.scrA;
+  double *__restrict__ sB = A.scrB;
+  auto load = [&](int q, Row4 &r, int &cd) {
+    if (q < len) {
+      if (AXIS == 2) {
+        cd = codes_kt[(int64_t)q * nl + t];
+        r = rows[(int64_t)q * nl + t];
+      } else {
+        cd = A.codes[base + (int64_t)q * sa];
+        r = rows[base + (int64_t)q * sa];
+      }
+    } else {
+      cd = PT_FICTIVE;
+    }
+  };
+  bool in = false;
+  int beg = 0, mbeg = 0, fl = 0, s = 0, pc = PT_FICTIVE;
+  double Am1 = 0, Bm1 = 0, u0 = 0;
+  auto fwd = [&](double ra, double rb, double rc, double rf) {
+    const double den = ra * Am1 + rb;
+    const double Am = (-rc) / den;
+    const double Bm = (rf - ra * Bm1) / den;
+    sA[(int64_t)s * nl + t] = Am;
+    sB[(int64_t)s * nl + t] = Bm;
+    Am1 = Am;
+    Bm1 = Bm;
+    s++;
+  };
+  auto visit = [&](int q, int cq, const Row4 &rw) {
+    if (!in) {
+      if (cq != PT_ACTUAL) {
+        pc = cq;
+        return;
+      }
+      beg = q - 1;
+      mbeg = pc;
+      fl = (mbeg != PT_DEFINED && mbeg != PT_EQUATION);
+      const int64_t cb = base + (int64_t)beg * sa;
+      u0 = fl ? 0.0 : A.rp[cb];
+      Am1 = 0;
+      Bm1 = u0;
+      s = 1;
+      in = true;
+      if (fl) {
+        if (mbeg == PT_PREDEFINED) fwd(1, 0.5, 0.5, A.rp[cb]);
+        else fwd(1, -1 / h[beg], 1 / h[beg], A.rp[cb]);
+      }
+    }
+    pc = cq;
+    if (cq == PT_ACTUAL) {
+      fwd(rw.a, rw.b, rw.c, rw.f);
+      return;
+    }
+    const int end = q, mend = cq;
+    const int fr = (mend != PT_DEFINED && mend != PT_EQUATION);
+    const int sd = end - beg + fl + fr;  // dim - 1
+    const int64_t cb = base + (int64_t)beg * sa, ce = base + (int64_t)end * sa;
+    if (fr) {
+      if (mend == PT_PREDEFINED) fwd(0.5, 0.5, 1, A.rp[ce]);
+      else fwd(-1 / h[end - 1], 1 / h[end - 1], 1, A.rp[ce]);
+    }
+    double u = fr ? 0.0 : A.rp[ce];
+    if (!fr) fa[base + (int64_t)(beg + sd - fl) * sa] = u;
+    int s2 = sd - 1;
+    for (; s2 >= 4; s2 -= 4) {
+      double a4[4], b4[4];
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        a4[v] = sA[(int64_t)(s2 - v) * nl + t];
+        b4[v] = sB[(int64_t)(s2 - v) * nl + t];
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        u = a4[v] * u + b4[v];
+        const int ss = s2 - v;
+        if (ss >= fl && ss <= sd - fr) fa[base + (int64_t)(beg + ss - fl) * sa] = u;
+      }
+    }
+    for (; s2 > 0; s2--) {
+      u = sA[(int64_t)s2 * nl + t] * u + sB[(int64_t)s2 * nl + t];
+      if (s2 >= fl && s2 <= sd - fr) fa[base + (int64_t)(beg + s2 - fl) * sa] = u;
+    }
+    if (!fl) fa[cb] = u0;
+    in = false;
+  };
+  constexpr int P = 8;
+  Row4 nxt[P];
+  int ncd[P];
+#pragma unroll
+  for (int v = 0; v < P; v++) load(v, nxt[v], ncd[v]);
+  for (int q0 = 0; q0 < len; q0 += P) {
+    Row4 cur[P];
+    int cc[P];
+#pragma unroll
+    for (int v = 0; v < P; v++) {
+      cur[v] = nxt[v];
+      cc[v] = ncd[v];
+    }
+#pragma unroll
+    for (int v = 0; v < P; v++) load(q0 + P + v, nxt[v], ncd[v]);
+#pragma unroll
+    for (int v = 0; v < P; v++)
+      if (q0 + v < len) visit(q0 + v, cc[v], cur[v]);
   }
-};
-}  // namespace
+}
+
+// The border-only passes of direction group dg along ax1, then ax2
+// (XProblem.cpp:2965-3033), one thread per cell.
+__global__ __launch_bounds__(256) void k_lod_border(LodArgs A, int dg, int ax1, int ax2) {
+  const int n = A.n, l = A.l, m = A.m;
+  const int64_t nc = (int64_t)n * l * m;
+  const int64_t S[3] = {(int64_t)l * m, (int64_t)m, 1};
+  const int E[3] = {n, l, m};
+  double *fa = A.stage[dg - 1];
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    const int cc = A.codes[c];
+    if (cc == PT_ACTUAL || cc == PT_FICTIVE) continue;
+    const int idx3[3] = {(int)(c / S[0]), (int)((c / m) % l), (int)(c % m)};
+    bool any = false;
+    double v = 0;
+    for (int pass = 0; pass < 2; pass++) {
+      const int ax = pass ? ax2 : ax1;
+      const int64_t sa = S[ax];
+      const int q = idx3[ax];
+      const double *h = A.h[ax];
+      if (q >= 1 && A.codes[c - sa] == PT_ACTUAL) {  // the end of the sector before c
+        any = true;
+        if (cc == PT_DEFINED || cc == PT_EQUATION) v = A.rp[c];
+        else if (cc == PT_PREDEFINED) v = 2 * A.rp[c] - fa[c - sa];
+        else v = (h[q - 1]) * A.rp[c] + fa[c - sa];
+      }
+      if (q + 1 < E[ax] && A.codes[c + sa] == PT_ACTUAL) {  // the start of the sector after c
+        any = true;
+        if (cc == PT_DEFINED || cc == PT_EQUATION) v = A.rp[c];
+        else if (cc == PT_PREDEFINED) v = 2 * A.rp[c] - fa[c + sa];
+        else v = -(h[q]) * A.rp[c] + fa[c + sa];
+      }
+    }
+    if (any) fa[c] = v;
+  }
+}
+
+// The reference's per-line sector rules (XProblem.cpp:2928-2962) over every
+// line of every axis: 1 = an Actual point first on its line ("Initial border
+// not found"), 2 = a Fictive sector border, 4 = a border type the LOD has no
+// rule for, 8 = a sector still open at the line's end (the reference would
+// read past the array there).
+static int lod_check(xf_grid g) {
+  if (g->lod_checked) return g->lod_err;
+  const std::vector<uint8_t> &cd = host_codes(g);
+  const int E[3] = {g->n, g->l, g->m};
+  const int64_t S[3] = {(int64_t)g->l * g->m, (int64_t)g->m, 1};
+  auto okb = [](int v) {
+    return v == PT_DEFINED || v == PT_EQUATION || v == PT_PREDEFINED || v == PT_NORMAL || v == PT_PRENORMAL;
+  };
+  int err = 0;
+  for (int axis = 0; axis < 3; axis++) {
+    const int ax1 = axis == 0 ? 1 : 0, ax2 = axis == 2 ? 1 : 2;
+    for (int d1 = 0; d1 < E[ax1]; d1++)
+      for (int d2 = 0; d2 < E[ax2]; d2++) {
+        const int64_t base = d1 * S[ax1] + d2 * S[ax2];
+        bool in = false;
+        int mbeg = 0;
+        for (int q = 0; q < E[axis]; q++) {
+          const int v = cd[base + q * S[axis]];
+          if (!in) {
+            if (v != PT_ACTUAL) continue;
+            if (q == 0) {
+              err |= 1;
+              break;
+            }
+            mbeg = cd[base + (q - 1) * S[axis]];
+            in = true;
+            continue;
+          }
+          if (v == PT_ACTUAL) continue;
+          if (mbeg == PT_FICTIVE || v == PT_FICTIVE) err |= 2;
+          else if (!okb(mbeg) || !okb(v)) err |= 4;
+          in = false;
+        }
+        if (in) err |= 8;
+      }
+  }
+  if (!err && !g->d_codes_kt) {
+    // point types of lines along k in line-interleaved order
+    const int64_t nl = (int64_t)g->n * g->l;
+    std::vector<uint8_t> kt((size_t)g->ncell);
+    for (int64_t t = 0; t < nl; t++)
+      for (int k = 0; k < g->m; k++) kt[(size_t)(k * nl + t)] = cd[(size_t)(t * g->m + k)];
+    XF_HIP(hipMalloc(&g->d_codes_kt, (size_t)g->ncell));
+    XF_HIP(hipMemcpy(g->d_codes_kt, kt.data(), (size_t)g->ncell, hipMemcpyHostToDevice));
+  }
+  g->lod_checked = true;
+  g->lod_err = err;
+  return err;
+}
 
 static void run_lod(xf_ctx c, xf_grid g, double *f, const double *f_prev, const double *u, const double *v,
                     const double *w, const double *rp, const double *kappa, double tau, int scheme) {
+  const int err = lod_check(g);
+  XF_REQUIRE(!(err & 1), XF_EINVAL, "SolveTransferEquation: Initial border not found.");
+  XF_REQUIRE(!(err & 8), XF_EINVAL,
+             "SolveTransferEquation: a sector of Actual points reaches the end of a grid line (no closing border "
+             "point).");
+  XF_REQUIRE(err == 0, XF_EINVAL, "SolveTransferEquation: Incorrect sector border or size.");
   const int64_t nc = g->ncell;
   const int E[3] = {g->n, g->l, g->m};
   int64_t maxlines = 0;
   int maxlen = 0;
   for (int a = 0; a < 3; a++) {
-    int64_t nl = nc / E[a];
-    maxlines = std::max(maxlines, nl);
+    maxlines = std::max(maxlines, nc / E[a]);
     maxlen = std::max(maxlen, E[a]);
   }
-  LodWork wk;
-  XF_HIP(hipMalloc(&wk.st0, sizeof(double) * nc));
-  XF_HIP(hipMalloc(&wk.st1, sizeof(double) * nc));
-  XF_HIP(hipMalloc(&wk.scrA, sizeof(double) * maxlines * (maxlen + 2)));
-  XF_HIP(hipMalloc(&wk.scrB, sizeof(double) * maxlines * (maxlen + 2)));
-  XF_HIP(hipMalloc(&wk.err, sizeof(int)));
-  XF_HIP(hipMemsetAsync(wk.err, 0, sizeof(int), c->stream));
+  double *st0 = (double *)ws_get(c, WS_LOD_ST0, sizeof(double) * nc);
+  double *st1 = (double *)ws_get(c, WS_LOD_ST1, sizeof(double) * nc);
+  Row4 *rows = (Row4 *)ws_get(c, WS_LOD_ROWS, sizeof(Row4) * nc);
+  double *scrA = (double *)ws_get(c, WS_LOD_SCRA, sizeof(double) * maxlines * (maxlen + 2));
+  double *scrB = (double *)ws_get(c, WS_LOD_SCRB, sizeof(double) * maxlines * (maxlen + 2));
   // f_1/3 and f_2/3 start as copies of the previous level (XProblem.cpp:2796-2797)
-  XF_HIP(hipMemcpyAsync(wk.st0, f_prev, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
-  XF_HIP(hipMemcpyAsync(wk.st1, f_prev, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
+  XF_HIP(hipMemcpyAsync(st0, f_prev, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
+  XF_HIP(hipMemcpyAsync(st1, f_prev, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
   LodArgs A;
   A.n = g->n; A.l = g->l; A.m = g->m;
   A.codes = g->d_codes;
   A.h[0] = g->d_hx; A.h[1] = g->d_hy; A.h[2] = g->d_hz;
   A.vel[0] = u; A.vel[1] = v; A.vel[2] = w;
   A.f_prev = f_prev; A.rp = rp; A.kappa = kappa;
-  A.stage[0] = wk.st0; A.stage[1] = wk.st1; A.stage[2] = f;
+  A.stage[0] = st0; A.stage[1] = st1; A.stage[2] = f;
   A.tau = tau;
   A.scheme = scheme;
-  A.scrA = wk.scrA; A.scrB = wk.scrB;
-  A.err = wk.err;
-  for (int dg = 1; dg < 4; dg++) {
-    for (int dfk = 1; dfk < 4; dfk++) {  // XProblem.cpp:2849-2864
-      int dir;
-      if (dfk == 1) dir = dg;
-      else if (dg == 1) dir = dfk;
-      else if (dg == 2) dir = dfk == 2 ? 1 : 3;
-      else dir = dfk - 1;
-      const int axis = dir - 1;
-      const int nl = (int)(nc / E[axis]);
-      hipLaunchKernelGGL(k_lod_lines, dim3((nl + 127) / 128), dim3(128), 0, c->stream, A, axis, dg,
-                         dir != dg ? 1 : 0);
-      check_kernel("k_lod_lines");
+  A.scrA = scrA; A.scrB = scrB;
+  const int cb = (int)std::min<int64_t>((nc + 255) / 256, 16384);
+  for (int dg = 1; dg < 4; dg++) {  // XProblem.cpp:2847-2864
+    const int axis = dg - 1;
+    const int nl = (int)(nc / E[axis]);
+    const dim3 lb((nl + 63) / 64);
+    if (axis == 0) {
+      hipLaunchKernelGGL(k_lod_rows<0>, dim3(cb), dim3(256), 0, c->stream, A, rows);
+      hipLaunchKernelGGL(k_lod_solve<0>, lb, dim3(64), 0, c->stream, A, rows, g->d_codes_kt, dg);
+      hipLaunchKernelGGL(k_lod_border, dim3(cb), dim3(256), 0, c->stream, A, dg, 1, 2);
+    } else if (axis == 1) {
+      hipLaunchKernelGGL(k_lod_rows<1>, dim3(cb), dim3(256), 0, c->stream, A, rows);
+      hipLaunchKernelGGL(k_lod_solve<1>, lb, dim3(64), 0, c->stream, A, rows, g->d_codes_kt, dg);
+      hipLaunchKernelGGL(k_lod_border, dim3(cb), dim3(256), 0, c->stream, A, dg, 0, 2);
+    } else {
+      hipLaunchKernelGGL(k_lod_rows<2>, dim3((g->m + 15) / 16, (g->l + 15) / 16, g->n), dim3(256), 0, c->stream, A,
+                         rows);
+      hipLaunchKernelGGL(k_lod_solve<2>, lb, dim3(64), 0, c->stream, A, rows, g->d_codes_kt, dg);
+      hipLaunchKernelGGL(k_lod_border, dim3(cb), dim3(256), 0, c->stream, A, dg, 0, 1);
     }
+    check_kernel("LOD sweep");
   }
-  int err = 0;
-  XF_HIP(hipMemcpyAsync(&err, wk.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  XF_HIP(hipStreamSynchronize(c->stream));
-  XF_REQUIRE(err == 0, XF_EINVAL,
-             err & 1 ? "SolveTransferEquation: Initial border not found."
-                     : "SolveTransferEquation: Incorrect sector border or size.");
 }
 
 extern "C" int xf_lod_transport(xf_ctx c, xf_grid g, double *f, const double *f_prev, const double *u,
@@ -307,14 +475,10 @@ extern "C" int xf_thomas_batched(xf_ctx c, int nsys, int M, double *u, const dou
   return guarded(c, [&] {
     XF_REQUIRE(nsys >= 0 && M >= 1 && u && a && b && cc && f, XF_EINVAL, "xf_thomas_batched: bad argument");
     if (nsys == 0) return;
-    double *sa = nullptr, *sb = nullptr;
-    XF_HIP(hipMalloc(&sa, sizeof(double) * nsys * (M + 1)));
-    XF_HIP(hipMalloc(&sb, sizeof(double) * nsys * (M + 1)));
+    double *sa = (double *)ws_get(c, WS_THOMAS_A, sizeof(double) * nsys * (M + 1));
+    double *sb = (double *)ws_get(c, WS_THOMAS_B, sizeof(double) * nsys * (M + 1));
     hipLaunchKernelGGL(k_thomas_batched, dim3((nsys + 127) / 128), dim3(128), 0, c->stream, nsys, M, u, a, b, cc, f,
                        sa, sb);
     check_kernel("k_thomas_batched");
-    XF_HIP(hipStreamSynchronize(c->stream));
-    hipFree(sa);
-    hipFree(sb);
   });
 }

@@ -17,7 +17,9 @@ __global__ void k_pressure_rhs(int n, int l, int m, const uint8_t *__restrict__ 
                                const double *__restrict__ v, const double *__restrict__ w,
                                const double *__restrict__ rho, const double *__restrict__ hxU,
                                const double *__restrict__ hyV, const double *__restrict__ hzW,
-                               const double *__restrict__ pbc, double tau, double *__restrict__ rhs, int *err) {
+                               const double *__restrict__ pbc, double tau, double *__restrict__ rhs) {
+  // point types other than Fictive, Actual and PreDefined, and Actual points
+  // on the low faces, were ruled out on the host (pressure_rhs_check)
   const int64_t nc = (int64_t)n * l * m, sI = (int64_t)l * m, sJ = m;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
     const uint8_t code = codes[c];
@@ -25,15 +27,12 @@ __global__ void k_pressure_rhs(int n, int l, int m, const uint8_t *__restrict__ 
       rhs[c] = 0;
     } else if (code == PT_ACTUAL) {
       const int k = (int)(c % m), j = (int)((c / m) % l), i = (int)(c / sI);
-      if (i < 1 || j < 1 || k < 1) { atomicOr(err, 2); continue; }
       const double d = rho ? rho[c] : 1.0;
       const double div_ = d * d * ((u[c] - u[c - sI]) / hxU[i - 1]) + ((v[c] - v[c - sJ]) / hyV[j - 1]) +
                           ((w[c] - w[c - 1]) / hzW[k - 1]);
       rhs[c] = -(div_ / tau);
     } else if (code == PT_PREDEFINED) {
       rhs[c] = pbc ? pbc[c] : 0.0;
-    } else {
-      atomicOr(err, 1);  // XProblem.cpp:5502-5505
     }
   }
 }
@@ -43,6 +42,8 @@ __global__ void k_vel_correct(int n, int l, int m, int axis, const uint8_t *__re
                               const double *__restrict__ us, const double *__restrict__ p,
                               const double *__restrict__ rho, const double *__restrict__ hP,
                               const double *__restrict__ bc, double tau, double *__restrict__ out, int *err) {
+  // point types and stencils that leave the array were ruled out on the host
+  // (vel_correct_check); err collects the data check of Defined points
   const int64_t nc = (int64_t)n * l * m;
   const int64_t S[3] = {(int64_t)l * m, (int64_t)m, 1};
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
@@ -52,16 +53,12 @@ __global__ void k_vel_correct(int n, int l, int m, int axis, const uint8_t *__re
       out[c] = 0;
     } else if (code == PT_ACTUAL || code == PT_NORMAL || code == PT_EQUATION) {
       const int idx[3] = {(int)(c / S[0]), (int)((c / m) % l), (int)(c % m)};
-      const int E[3] = {n, l, m};
-      if (idx[axis] + 1 >= E[axis]) { atomicOr(err, 2); continue; }
       const double gradP = (p[c + S[axis]] - p[c]) / hP[idx[axis]];
       out[c] = us[c] - tau * gradP / (rho ? rho[c] : 1.0);
     } else if (code == PT_DEFINED) {
       const double dv = bc ? bc[c] : 0.0;
-      if (us[c] != dv) atomicOr(err, 4);  // XProblem.cpp:5684-5688
+      if (us[c] != dv) atomicOr(err, EB_VC_DEFINED);  // XProblem.cpp:5684-5688
       out[c] = dv;
-    } else {
-      atomicOr(err, 1);
     }
   }
 }
@@ -88,9 +85,9 @@ __global__ void k_vel_correct_predef(int n, int l, int m, const uint8_t *__restr
           }
         }
       }
-    if (count != 1) { atomicOr(err, 8); continue; }
+    if (count != 1) continue;  // ruled out on the host
     const double dv = bc ? bc[c] : 0.0;
-    if (fabs(us[c] - (2 * dv - us[nb])) > 1e-10) { atomicOr(err, 16); continue; }
+    if (fabs(us[c] - (2 * dv - us[nb])) > 1e-10) { atomicOr(err, EB_VC_PREDEF); continue; }
     out[c] = 2 * dv - out[nb];
   }
 }
@@ -136,11 +133,13 @@ __global__ void k_divmax(int n, int l, int m, const uint8_t *__restrict__ codes,
     am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
   }
   __syncthreads();
-  if (!am_last || threadIdx.x != 0) return;
+  if (!am_last) return;
   __threadfence();
+  // the last block: every thread folds a strided share of the partials, then
+  // the block folds the threads' results (ties to the lowest cell index)
   double b = 0;
   int64_t idx = -1;
-  for (unsigned q = 0; q < gridDim.x; q++) {
+  for (unsigned q = threadIdx.x; q < gridDim.x; q += blockDim.x) {
     const double v2 = __hip_atomic_load(&pv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int64_t i2 = __hip_atomic_load(&pi[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (v2 > b || (v2 == b && i2 >= 0 && (idx < 0 || i2 < idx))) {
@@ -148,38 +147,82 @@ __global__ void k_divmax(int n, int l, int m, const uint8_t *__restrict__ codes,
       idx = i2;
     }
   }
-  *out = b;
-  *out_idx = b > 0 ? idx : -1;
+  __syncthreads();
+  lv[threadIdx.x] = b;
+  li[threadIdx.x] = idx;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double v2 = lv[threadIdx.x + s];
+      const int64_t i2 = li[threadIdx.x + s];
+      if (v2 > lv[threadIdx.x] || (v2 == lv[threadIdx.x] && i2 >= 0 && (li[threadIdx.x] < 0 || i2 < li[threadIdx.x]))) {
+        lv[threadIdx.x] = v2;
+        li[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  *out = lv[0];
+  *out_idx = lv[0] > 0 ? li[0] : -1;
   *counter = 0;
 }
 
 namespace {
-struct Steps {
-  double *d = nullptr;
-  Steps(const double *h, int cnt, hipStream_t s) {
-    XF_HIP(hipMalloc(&d, sizeof(double) * (cnt > 0 ? cnt : 1)));
-    XF_HIP(hipMemcpyAsync(d, h, sizeof(double) * cnt, hipMemcpyHostToDevice, s));
-  }
-  ~Steps() { hipFree(d); }
-};
-struct ErrFlag {
-  int *d = nullptr;
-  hipStream_t s;
-  explicit ErrFlag(hipStream_t st) : s(st) {
-    XF_HIP(hipMalloc(&d, sizeof(int)));
-    XF_HIP(hipMemsetAsync(d, 0, sizeof(int), s));
-  }
-  int read() {
-    int h = 0;
-    XF_HIP(hipMemcpyAsync(&h, d, sizeof(int), hipMemcpyDeviceToHost, s));
-    XF_HIP(hipStreamSynchronize(s));
-    return h;
-  }
-  ~ErrFlag() { hipFree(d); }
-};
 int nblocks(int64_t nc) {
   int64_t b = (nc + kBlock - 1) / kBlock;
   return (int)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
+}
+
+// Point types the pressure right-hand side rejects (XProblem.cpp:5502-5505):
+// 1 = a type other than Fictive, Actual or PreDefined; 2 = an Actual point on
+// a low face (its backward differences would leave the array).
+int pressure_rhs_check(xf_grid g) {
+  if (g->glue_rhs_err >= 0) return g->glue_rhs_err;
+  const std::vector<uint8_t> &cd = host_codes(g);
+  int e = 0;
+  for (int i = 0; i < g->n; i++)
+    for (int j = 0; j < g->l; j++)
+      for (int k = 0; k < g->m; k++) {
+        const uint8_t v = cd[((size_t)i * g->l + j) * g->m + k];
+        if (v == PT_ACTUAL && (i < 1 || j < 1 || k < 1)) e |= 2;
+        else if (v != PT_FICTIVE && v != PT_ACTUAL && v != PT_PREDEFINED) e |= 1;
+      }
+  return g->glue_rhs_err = e;
+}
+
+// Point types the velocity correction along `axis` (0..2) rejects
+// (XProblem.cpp:5636-5758): 1 = an unknown type, 2 = an Actual, Normal or
+// Equation point on the high face of the axis, 8 = a PreDefined point
+// without exactly one Actual neighbour.
+int vel_correct_check(xf_grid g, int axis) {
+  if (g->glue_vc_err[axis] >= 0) return g->glue_vc_err[axis];
+  const std::vector<uint8_t> &cd = host_codes(g);
+  const int n = g->n, l = g->l, m = g->m;
+  const int E[3] = {n, l, m};
+  int e = 0;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < l; j++)
+      for (int k = 0; k < m; k++) {
+        const uint8_t v = cd[((size_t)i * l + j) * m + k];
+        const int idx[3] = {i, j, k};
+        if (v == PT_ACTUAL || v == PT_NORMAL || v == PT_EQUATION) {
+          if (idx[axis] + 1 >= E[axis]) e |= 2;
+        } else if (v == PT_PREDEFINED) {
+          int count = 0;
+          for (int d = 0; d < 3; d++)
+            for (int sh = -1; sh <= 1; sh += 2) {
+              int q[3] = {i, j, k};
+              q[d] += sh;
+              if (q[d] < 0 || q[d] >= E[d]) continue;
+              if (cd[((size_t)q[0] * l + q[1]) * m + q[2]] == PT_ACTUAL) count++;
+            }
+          if (count != 1) e |= 8;
+        } else if (v != PT_FICTIVE && v != PT_DEFINED) {
+          e |= 1;
+        }
+      }
+  return g->glue_vc_err[axis] = e;
 }
 }  // namespace
 
@@ -189,14 +232,13 @@ extern "C" int xf_pressure_rhs(xf_ctx c, xf_grid gp, const double *u, const doub
   return guarded(c, [&] {
     XF_REQUIRE(gp && u && v && w && hxU && hyV && hzW && rhs, XF_EINVAL, "xf_pressure_rhs: null argument");
     XF_REQUIRE(tau > 0, XF_EINVAL, "xf_pressure_rhs: tau must be positive");
-    Steps sx(hxU, gp->n - 1, c->stream), sy(hyV, gp->l - 1, c->stream), sz(hzW, gp->m - 1, c->stream);
-    ErrFlag err(c->stream);
-    hipLaunchKernelGGL(k_pressure_rhs, dim3(nblocks(gp->ncell)), dim3(kBlock), 0, c->stream, gp->n, gp->l, gp->m,
-                       gp->d_codes, u, v, w, rho, sx.d, sy.d, sz.d, pbc, tau, rhs, err.d);
-    check_kernel("k_pressure_rhs");
-    const int e = err.read();
-    XF_REQUIRE(e == 0, XF_EINVAL,
+    XF_REQUIRE(pressure_rhs_check(gp) == 0, XF_EINVAL,
                "TVPProblem::SolveBySplitting: Method is not implemented for this border point type of pressure.");
+    const double *sx = device_steps(c, gp, 0, hxU, gp->n - 1), *sy = device_steps(c, gp, 1, hyV, gp->l - 1),
+                 *sz = device_steps(c, gp, 2, hzW, gp->m - 1);
+    hipLaunchKernelGGL(k_pressure_rhs, dim3(nblocks(gp->ncell)), dim3(kBlock), 0, c->stream, gp->n, gp->l, gp->m,
+                       gp->d_codes, u, v, w, rho, sx, sy, sz, pbc, tau, rhs);
+    check_kernel("k_pressure_rhs");
   });
 }
 
@@ -206,22 +248,19 @@ extern "C" int xf_velocity_correct(xf_ctx c, xf_grid gv, int axis, const double 
     XF_REQUIRE(gv && u_star && p && hP && u_out, XF_EINVAL, "xf_velocity_correct: null argument");
     XF_REQUIRE(axis >= 1 && axis <= 3, XF_EINVAL, "Unknown velocity number.");
     XF_REQUIRE(u_out != u_star, XF_EINVAL, "xf_velocity_correct: u_out must not alias u_star");
-    const int E[3] = {gv->n, gv->l, gv->m};
-    Steps sp(hP, E[axis - 1] - 1, c->stream);
-    ErrFlag err(c->stream);
-    hipLaunchKernelGGL(k_vel_correct, dim3(nblocks(gv->ncell)), dim3(kBlock), 0, c->stream, gv->n, gv->l, gv->m,
-                       axis - 1, gv->d_codes, u_star, p, rho, sp.d, bc, tau, u_out, err.d);
-    hipLaunchKernelGGL(k_vel_correct_predef, dim3(nblocks(gv->ncell)), dim3(kBlock), 0, c->stream, gv->n, gv->l,
-                       gv->m, gv->d_codes, u_star, bc, u_out, err.d);
-    check_kernel("k_vel_correct");
-    const int e = err.read();
-    XF_REQUIRE(!(e & 4), XF_EINVAL, "TVPProblem::SolveBySplitting: Defined border condition checking error.");
+    const int e = vel_correct_check(gv, axis - 1);
     XF_REQUIRE(!(e & 8), XF_EINVAL, "TVPProblem::SolveBySplitting: PreDefined border condition checking error - "
                                     "actual values count must be 1.");
-    XF_REQUIRE(!(e & 16), XF_EINVAL,
-               "TVPProblem::SolveBySplitting: PreDefined border condition checking error - value incorrect.");
     XF_REQUIRE(e == 0, XF_EINVAL,
                "TVPProblem::SolveBySplitting: Method is not implemented for this point border type of velocity.");
+    const int E[3] = {gv->n, gv->l, gv->m};
+    const double *sp = device_steps(c, gv, 3, hP, E[axis - 1] - 1);
+    hipLaunchKernelGGL(k_vel_correct, dim3(nblocks(gv->ncell)), dim3(kBlock), 0, c->stream, gv->n, gv->l, gv->m,
+                       axis - 1, gv->d_codes, u_star, p, rho, sp, bc, tau, u_out, c->d_err);
+    hipLaunchKernelGGL(k_vel_correct_predef, dim3(nblocks(gv->ncell)), dim3(kBlock), 0, c->stream, gv->n, gv->l,
+                       gv->m, gv->d_codes, u_star, bc, u_out, c->d_err);
+    check_kernel("k_vel_correct");
+    raise_device_errors(c);
   });
 }
 
@@ -230,14 +269,15 @@ extern "C" int xf_divergence_max(xf_ctx c, xf_grid gp, const double *u, const do
                                  int64_t *cell) {
   return guarded(c, [&] {
     XF_REQUIRE(gp && u && v && w && hxU && hyV && hzW && max_abs, XF_EINVAL, "xf_divergence_max: null argument");
-    Steps sx(hxU, gp->n - 1, c->stream), sy(hyV, gp->l - 1, c->stream), sz(hzW, gp->m - 1, c->stream);
+    const double *sx = device_steps(c, gp, 0, hxU, gp->n - 1), *sy = device_steps(c, gp, 1, hyV, gp->l - 1),
+                 *sz = device_steps(c, gp, 2, hzW, gp->m - 1);
     int nb = nblocks(gp->ncell);
     if (nb > 2048) nb = 2048;
     double *pv = c->partials;
     int64_t *pi = (int64_t *)(c->partials + kMaxPartials);
     int64_t *d_idx = (int64_t *)(c->partials + 2 * kMaxPartials);
     hipLaunchKernelGGL(k_divmax, dim3(nb), dim3(kBlock), 0, c->stream, gp->n, gp->l, gp->m, gp->d_codes, u, v, w,
-                       sx.d, sy.d, sz.d, pv, pi, c->counters, c->d_scalar, d_idx);
+                       sx, sy, sz, pv, pi, c->counters, c->d_scalar, d_idx);
     check_kernel("k_divmax");
     int64_t idx;
     XF_HIP(hipMemcpyAsync(max_abs, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));

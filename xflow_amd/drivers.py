@@ -95,11 +95,18 @@ class Cavity:
         if st.status != 0:
             raise xf.XfError(st.status, "pressure solve failed")
         self.p0.copy_(self.p)
-        # velocity correction (XProblem.cpp:5598-5778)
-        for a in range(3):
-            self.vgrids[a].velocity_correct(a + 1, self.star[a], self.p, self.steps, self.dt, self.vel[a],
-                                            bc=self.bc[a])
-        div, _ = self.gp.divergence_max(self.vel[0], self.vel[1], self.vel[2], self.steps, self.steps, self.steps)
+        # velocity correction (XProblem.cpp:5598-5778); its boundary-value
+        # checks are read back once, after the divergence monitor's wait
+        self.ctx.set_deferred_errors(True)
+        try:
+            for a in range(3):
+                self.vgrids[a].velocity_correct(a + 1, self.star[a], self.p, self.steps, self.dt, self.vel[a],
+                                                bc=self.bc[a])
+            div, _ = self.gp.divergence_max(self.vel[0], self.vel[1], self.vel[2], self.steps, self.steps,
+                                            self.steps)
+        finally:
+            self.ctx.set_deferred_errors(False)
+        self.ctx.check_errors()
         self.last = {"pressure_iters": st.iters, "pressure_ms": st.ms, "div_max": div,
                      "resid_true": st.resid_true}
         return self.last
@@ -167,6 +174,7 @@ class IBChannel:
         self.tol = tol
         self.solver = xf.XF_CG if solver is None else solver
         self.precond = precond
+        self.verify = False  # report each solve's true residual (one more FP64 operator pass)
         self.steps = [np.full(e + 1, h) for e in self.E]
         shape = (nx + 2, ny + 2, nz + 2)
         self.pcodes = np.zeros(shape, dtype=np.int32)
@@ -228,20 +236,29 @@ class IBChannel:
             g.lincomb(1.0, self.star[a], dt, self.force, self.star[a])
         self.gp.pressure_rhs(self.star[0], self.star[1], self.star[2], *self.steps, dt, self.rhs)
         o = xf.solve_opts(self.solver, tol=self.tol, tol_mode=xf.XF_TOL_REL, project_mean=1)
+        o.verify = 1 if self.verify else 0
         if self.precond is not None:
             o.precond = self.precond
         st = self.gp.solve(self.rhs, self.p0, self.p, o)
         if st.status != 0:
             raise xf.XfError(st.status, "pressure solve failed")
         self.p0.copy_(self.p)
-        for a in range(3):
-            self.vgrids[a].velocity_correct(a + 1, self.star[a], self.p, self.steps[a], dt, self.vel[a], bc=self.bc[a])
-        # passive scalar, upwind LOD with cell-centred velocities
-        self.gs.transport_terms(0, self.vel, self.tr, bc=self.zero, rp=self.rp)
-        self.c_new.copy_(self.c)
-        self.gs.lod_transport(self.c_new, self.c, self.tr[0], self.tr[1], self.tr[2], self.rp, self.kap_f, dt, 1)
-        self.c, self.c_new = self.c_new, self.c
-        div, _ = self.gp.divergence_max(self.vel[0], self.vel[1], self.vel[2], *self.steps)
+        # the velocity correction's boundary-value checks are read back once,
+        # after the divergence monitor's wait
+        self.ctx.set_deferred_errors(True)
+        try:
+            for a in range(3):
+                self.vgrids[a].velocity_correct(a + 1, self.star[a], self.p, self.steps[a], dt, self.vel[a],
+                                                bc=self.bc[a])
+            # passive scalar, upwind LOD with cell-centred velocities
+            self.gs.transport_terms(0, self.vel, self.tr, bc=self.zero, rp=self.rp)
+            self.c_new.copy_(self.c)
+            self.gs.lod_transport(self.c_new, self.c, self.tr[0], self.tr[1], self.tr[2], self.rp, self.kap_f, dt, 1)
+            self.c, self.c_new = self.c_new, self.c
+            div, _ = self.gp.divergence_max(self.vel[0], self.vel[1], self.vel[2], *self.steps)
+        finally:
+            self.ctx.set_deferred_errors(False)
+        self.ctx.check_errors()
         self.last = {"pressure_iters": st.iters, "inner_iters": st.inner_iters, "pressure_ms": st.ms,
                      "div_max": div, "resid_true": st.resid_true}
         return self.last

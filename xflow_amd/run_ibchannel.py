@@ -1,10 +1,13 @@
 """The immersed-boundary channel (256x128x128 cells, icosphere of 2562
-vertices, direct forcing, passive scalar) advanced twice from the same state:
-once with the plain FP64 CG pressure solve, once with the V-cycle/network
-flexible CG in mixed precision; reports ms per step and the relative L2
-difference of velocity and scalar between the two runs.
+vertices, direct forcing, passive scalar) advanced side by side from the same
+state with different pressure solvers: the plain FP64 CG ("fp64"), the
+flexible CG in mixed precision with the full-depth classical V-cycle
+("cycle") and with the cycle cut at 64^3 and the trained network there
+("network").  Reports ms per step, the pressure solve's ms, iterations and
+true residual per step, and the relative L2 difference of velocity and scalar
+of each run against the FP64 one.
 
-    python -m xflow_amd.run_ibchannel [--ny 128] [--steps 200] [--tol 1e-8]
+    python -m xflow_amd.run_ibchannel [--ny 128] [--steps 200] [--tol 1e-8] [--solvers fp64,cycle,network]
 """
 import argparse
 import json
@@ -28,36 +31,73 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--every", type=int, default=20)
+    ap.add_argument("--solvers", default="fp64,cycle,network")
+    ap.add_argument("--no-verify", action="store_true",
+                    help="skip the per-step true residual (one FP64 operator pass after each solve)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "xflow_amd", "weights", "unet_coarse64.npy"),
                     help="coarsest-grid network")
     args = ap.parse_args()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    names = [s for s in args.solvers.split(",") if s]
+    assert names and names[0] == "fp64" and set(names) <= {"fp64", "cycle", "network"}, \
+        "--solvers: fp64 first, then any of cycle, network"
     ctx = xf.Context(0)
-    net = UNet(ctx, seed=0)
-    log("coarsest-grid network: %s" % args.weights)
-    net.set_params(np.load(args.weights))
-    ref = IBChannel(ctx, ny=args.ny, subdivisions=4, tol=args.tol)
-    nn = IBChannel(ctx, ny=args.ny, subdivisions=4, tol=args.tol, solver=xf.XF_FCG_MGNN, precond=net.h)
-    times = {"fp64": 0.0, "nn": 0.0}
+    runs = {}
+    for name in names:
+        if name == "fp64":
+            ch = IBChannel(ctx, ny=args.ny, subdivisions=4, tol=args.tol)
+        elif name == "cycle":
+            ch = IBChannel(ctx, ny=args.ny, subdivisions=4, tol=args.tol, solver=xf.XF_FCG_MGNN)
+        else:
+            net = UNet(ctx, seed=0)
+            log("coarsest-grid network: %s" % args.weights)
+            net.set_params(np.load(args.weights))
+            ch = IBChannel(ctx, ny=args.ny, subdivisions=4, tol=args.tol, solver=xf.XF_FCG_MGNN, precond=net.h)
+            ch.net = net  # keep the network alive with its run
+        ch.verify = not args.no_verify
+        runs[name] = ch
+    times = {k: 0.0 for k in names}
+    acc = {k: {"pressure_ms": 0.0, "iters": 0, "inner": 0, "resid_true_max": 0.0} for k in names}
     hist = []
+    ref = runs["fp64"]
     for s in range(args.steps):
-        for name, ch in (("fp64", ref), ("nn", nn)):
+        for name, ch in runs.items():
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             ch.step()
             torch.cuda.synchronize()
             times[name] += time.perf_counter() - t0
+            a = acc[name]
+            a["pressure_ms"] += ch.last["pressure_ms"]
+            a["iters"] += ch.last["pressure_iters"]
+            a["inner"] += ch.last["inner_iters"]
+            a["resid_true_max"] = max(a["resid_true_max"], ch.last["resid_true"])
         if (s + 1) % args.every == 0 or s + 1 == args.steps:
-            dv = [float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) for a, b in zip(nn.vel, ref.vel)]
-            dc = float(torch.linalg.norm(nn.c - ref.c) / torch.linalg.norm(ref.c))
-            rec = {"step": s + 1, "vel_rel_l2": dv, "scalar_rel_l2": dc, "fp64": ref.last, "nn": nn.last}
+            rec = {"step": s + 1}
+            for name, ch in runs.items():
+                rec[name] = dict(ch.last)
+                if name != "fp64":
+                    rec[name]["vel_rel_l2"] = [float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+                                               for a, b in zip(ch.vel, ref.vel)]
+                    rec[name]["scalar_rel_l2"] = float(torch.linalg.norm(ch.c - ref.c) / torch.linalg.norm(ref.c))
             hist.append(rec)
-            log("step %d: velocity rel L2 %s, scalar %.2e; pressure fp64 %d it %.1f ms, nn %d inner %.1f ms, "
-                "div max %.2e / %.2e" % (s + 1, ["%.2e" % v for v in dv], dc, ref.last["pressure_iters"],
-                                           ref.last["pressure_ms"], nn.last["inner_iters"], nn.last["pressure_ms"],
-                                           ref.last["div_max"], nn.last["div_max"]))
-    out = {"cells": [2 * args.ny, args.ny, args.ny], "steps": args.steps, "tol": args.tol,
-           "ms_per_step": {k: round(v * 1e3 / args.steps, 2) for k, v in times.items()}, "history": hist}
+            log("step %d: " % (s + 1) + "; ".join(
+                "%s %d it (%d inner) %.1f ms, true resid %.1e, div max %.1e%s" % (
+                    n, r["pressure_iters"], r["inner_iters"], r["pressure_ms"], r["resid_true"], r["div_max"],
+                    "" if n == "fp64" else ", vel diff %.1e, scalar diff %.1e" % (
+                        max(r["vel_rel_l2"]), r["scalar_rel_l2"]))
+                for n, r in rec.items() if n != "step"))
+    per = {}
+    for name in names:
+        a = acc[name]
+        per[name] = {"ms_per_step": round(times[name] * 1e3 / args.steps, 2),
+                     "pressure_ms_per_step": round(a["pressure_ms"] / args.steps, 2),
+                     "non_pressure_ms_per_step": round((times[name] * 1e3 - a["pressure_ms"]) / args.steps, 2),
+                     "iters_per_step": round(a["iters"] / args.steps, 1),
+                     "inner_iters_per_step": round(a["inner"] / args.steps, 1),
+                     "resid_true_max": a["resid_true_max"] if ch.verify else None}
+    out = {"cells": [2 * args.ny, args.ny, args.ny], "steps": args.steps, "tol": args.tol, "solvers": per,
+           "history": hist}
     print(json.dumps(out))
 
 
