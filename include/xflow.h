@@ -179,6 +179,14 @@ int xf_solve_noalloc(xf_ctx ctx, xf_grid g, const double *b, const double *x0, d
  * rejects give XF_EINVAL.  Device pointers, all [ncell]. */
 int xf_lod_transport(xf_ctx ctx, xf_grid g, double *f, const double *f_prev, const double *u, const double *v,
                      const double *w, const double *rp, const double *kappa, double tau, int scheme);
+/* No reference counterpart (the reference advances the velocity components
+ * one after another): xf_lod_transport for count (1..4) independent problems,
+ * problem p on grids[p] with the fields f[p], f_prev[p], ..., in one set of
+ * launches, with the same results as count separate calls.  No problem's
+ * output may be another problem's field. */
+int xf_lod_transport_batch(xf_ctx ctx, int count, const xf_grid *grids, double *const *f, const double *const *f_prev,
+                           const double *const *u, const double *const *v, const double *const *w,
+                           const double *const *rp, const double *const *kappa, double tau, int scheme);
 /* nsys independent tridiagonal systems a u[m-1] + b u[m] + c u[m+1] = f,
  * m = 1..M-1, u[0] and u[M] given (TSLAE::SolveByScalarRunning,
  * XSLAE.cpp:416-445); system s at [s*(M+1), (s+1)*(M+1)).  Device pointers. */

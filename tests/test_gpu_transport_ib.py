@@ -68,6 +68,40 @@ def test_lod_transport_large_matches_oracle(ctx):
         assert np.array_equal(host(f), want)
 
 
+def test_lod_transport_batch_equals_single_calls(ctx):
+    """Three problems on grids of different shapes (the golden grid, a
+    48x40x36 grid with obstacles, and its transpose-like 36x40x48 variant)
+    advanced in one batched call: each equals its own single call bit for bit."""
+    rng = np.random.default_rng(5)
+    probs = []
+    ins, _ = golden("lod_scheme0")
+    probs.append((np.diff(ins["x"]), np.diff(ins["y"]), np.diff(ins["z"]), ins["codes"]))
+    for (n, l, m) in ((48, 40, 36), (36, 40, 48)):
+        codes = np.full((n, l, m), 3, dtype=np.int32)
+        codes[1:-1, 1:-1, 1:-1] = 1
+        codes[0], codes[-1] = 4, 5
+        codes[:, :, 0], codes[:, :, -1] = 7, 4
+        codes[10:14, 8:20, 5:9] = 3
+        probs.append((rng.uniform(0.02, 0.03, n - 1), rng.uniform(0.02, 0.03, l - 1), rng.uniform(0.02, 0.03, m - 1),
+                      codes))
+    grids, fields = [], []
+    for hx, hy, hz, codes in probs:
+        grids.append(xf.Grid(ctx, hx, hy, hz, codes))
+        fields.append({k: dev(rng.standard_normal(codes.shape)) for k in ("f", "f_prev", "u", "v", "w", "rp")})
+        fields[-1]["kappa"] = dev(rng.uniform(0.001, 0.01, codes.shape))
+    for scheme in (0, 1):
+        single = []
+        for g, fl in zip(grids, fields):
+            f = fl["f"].clone()
+            g.lod_transport(f, fl["f_prev"], fl["u"], fl["v"], fl["w"], fl["rp"], fl["kappa"], 0.004, scheme)
+            single.append(host(f))
+        outs = [fl["f"].clone() for fl in fields]
+        xf.lod_transport_batch(ctx, grids, outs, *[[fl[k] for fl in fields] for k in ("f_prev", "u", "v", "w", "rp",
+                                                                                       "kappa")], 0.004, scheme)
+        for got, want in zip(outs, single):
+            assert np.array_equal(host(got), want)
+
+
 def test_lod_rejects_bad_sector(ctx):
     codes = np.full((6, 6, 6), 3, dtype=np.int32)
     codes[1:-1, 1:-1, 1:-1] = 1

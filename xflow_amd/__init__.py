@@ -255,6 +255,17 @@ class Grid:
             pass
 
 
+def lod_transport_batch(ctx, grids, f, f_prev, u, v, w, rp, kappa, tau, scheme):
+    """Grid.lod_transport for several independent problems (lists of the same
+    length, 1..4) in one set of launches."""
+    n = len(grids)
+    arr = lambda xs: (C.c_void_p * n)(*[grids[i]._vec(x) for i, x in enumerate(xs)])
+    gh = (C.c_void_p * n)(*[g.h for g in grids])
+    ctx.check(lib().xf_lod_transport_batch(ctx.h, n, gh, arr(f), arr(f_prev), arr(u), arr(v), arr(w), arr(rp),
+                                           arr(kappa), float(tau), int(scheme)), "xf_lod_transport_batch")
+    return f
+
+
 def thomas_batched(ctx, u, a, b, c, f):
     """Solves u.shape[0] tridiagonal systems of size u.shape[1] in place."""
     nsys, M1 = u.shape

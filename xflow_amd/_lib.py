@@ -62,6 +62,7 @@ SIGNATURES = {
     "xf_solve": (I, [P, P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "xf_solve_noalloc": (I, [P, P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "xf_lod_transport": (I, [P, P, P, P, P, P, P, P, P, D, I]),
+    "xf_lod_transport_batch": (I, [P, I, P, P, P, P, P, P, P, P, D, I]),
     "xf_thomas_batched": (I, [P, I, I, P, P, P, P, P]),
     "xf_ib_interp": (I, [P, P, I, I, P, P, P]),
     "xf_ib_spread": (I, [P, P, I, I, P, P, P]),

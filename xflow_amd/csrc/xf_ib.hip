@@ -2,13 +2,15 @@
 // (TImmersedBoundaryProblem::Interpolate / SpreadForce / Index / Coord /
 // OptimalDirichlet, XProblem.cpp:8550-8791, 8931-9046).
 //
-// Interpolation: one thread per Lagrangian node adds its 7^3 window in the
+// Interpolation: one thread per Lagrangian node adds its window in the
 // reference's i->j->k order, so each node's value is the reference's sum.
-// Spreading is a deterministic gather: every (node, window cell) product is
-// keyed by its cell, a stable radix sort groups the keys while keeping node
-// order inside each cell, and one thread per cell adds its products in node
-// order - the order of the reference's serial scatter.  Products with a zero
-// weight (r >= 2h, 3 of the 7 cells per axis) are skipped: they add +-0.
+// Only the cells with a non-zero weight are visited (r < 2h: 4 of the 7 per
+// axis, 5 where rounding puts a cell at 2h just inside; always consecutive);
+// the others add +-0 to the sum and leave it unchanged.  Spreading is a deterministic gather: every (node, non-zero
+// window cell) product is keyed by its cell, a stable radix sort over the
+// bits a cell index needs groups the keys while keeping node order inside
+// each cell, and one thread per cell adds its products in node order - the
+// order of the reference's serial scatter.
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
@@ -39,52 +41,77 @@ __device__ inline double ib_delta(double r, double h) {  // XProblem.cpp:9026-90
   return 0;
 }
 
+// The non-zero weights of one axis: the window's first index with r < 2h,
+// the count and the weights.
+constexpr int kIbPer = 5;  // non-zero weights per axis kept (4 of 7, 5 when rounding lets one more in)
+struct AxisWeights {
+  int first, count;
+  double w[kIbPer];
+};
+
+__device__ inline AxisWeights axis_weights(double X, int type, int axis, double h, int end_index) {
+  const int b = ib_index(X, type, axis, h, end_index);
+  AxisWeights a{b - RANGE, 0, {0, 0, 0, 0, 0}};
+  bool seen = false;
+  for (int t = 0; t < 7; t++) {
+    const int i = b - RANGE + t;
+    const double wt = ib_delta(fabs(X - ib_coord(i, type, axis, h)), h);
+    if (wt == 0) {
+      if (seen) break;
+      continue;
+    }
+    if (!seen) {
+      a.first = i;
+      seen = true;
+    }
+    if (a.count < kIbPer) a.w[a.count++] = wt;
+  }
+  return a;
+}
+
 __global__ void k_ib_interp(IbGeom G, int axis, int nn, const double *__restrict__ xyz, const double *__restrict__ vel,
                             double *__restrict__ U) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nn) return;
-  const double X = xyz[3 * q], Y = xyz[3 * q + 1], Z = xyz[3 * q + 2];
-  const int b0 = ib_index(X, 1, axis, G.h[0], G.E[0]);
-  const int b1 = ib_index(Y, 2, axis, G.h[1], G.E[1]);
-  const int b2 = ib_index(Z, 3, axis, G.h[2], G.E[2]);
-  double wy[7], wz[7];
-  for (int t = 0; t < 7; t++) {
-    wy[t] = ib_delta(fabs(Y - ib_coord(b1 - RANGE + t, 2, axis, G.h[1])), G.h[1]);
-    wz[t] = ib_delta(fabs(Z - ib_coord(b2 - RANGE + t, 3, axis, G.h[2])), G.h[2]);
-  }
+  const AxisWeights ax = axis_weights(xyz[3 * q], 1, axis, G.h[0], G.E[0]);
+  const AxisWeights ay = axis_weights(xyz[3 * q + 1], 2, axis, G.h[1], G.E[1]);
+  const AxisWeights az = axis_weights(xyz[3 * q + 2], 3, axis, G.h[2], G.E[2]);
   const double ha = G.h[axis - 1];
   const double cb = ha * ha * ha;  // CB(separator->Separation[0]), XProblem.cpp:8782
   double acc = 0;
-  for (int i = b0 - RANGE; i <= b0 + RANGE; i++) {
-    const double wx = ib_delta(fabs(X - ib_coord(i, 1, axis, G.h[0])), G.h[0]);
-    for (int j = 0; j < 7; j++)
-      for (int k = 0; k < 7; k++) {
-        const int64_t c = ((int64_t)i * G.l + (b1 - RANGE + j)) * G.m + (b2 - RANGE + k);
-        acc += (vel[c] * wx * wy[j] * wz[k]) * cb;
+  for (int a = 0; a < ax.count; a++)
+    for (int b = 0; b < ay.count; b++)
+      for (int e = 0; e < az.count; e++) {
+        const int64_t c = ((int64_t)(ax.first + a) * G.l + (ay.first + b)) * G.m + (az.first + e);
+        acc += (vel[c] * ax.w[a] * ay.w[b] * az.w[e]) * cb;  // XProblem.cpp:8742-8783
       }
-  }
   U[q] = acc;
 }
 
-// Emits 343 (cell, value) products per node; zero-weight products get the
-// key 0xffffffff and sort to the end.
+// 125 (cell, value) slots per node: the products of its non-zero window
+// (at most 5 x 5 x 5) in i->j->k order; unused slots get the key 0xffffffff,
+// which sorts after every cell.
 __global__ void k_ib_products(IbGeom G, int axis, int nn, const double *__restrict__ xyz,
                               const double *__restrict__ F, uint32_t *__restrict__ keys, double *__restrict__ vals) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nn * 343) return;
-  const int q = (int)(t / 343), w = (int)(t % 343);
-  const int di = w / 49, dj = (w / 7) % 7, dk = w % 7;
-  const double X = xyz[3 * q], Y = xyz[3 * q + 1], Z = xyz[3 * q + 2];
-  const int i = ib_index(X, 1, axis, G.h[0], G.E[0]) - RANGE + di;
-  const int j = ib_index(Y, 2, axis, G.h[1], G.E[1]) - RANGE + dj;
-  const int k = ib_index(Z, 3, axis, G.h[2], G.E[2]) - RANGE + dk;
-  const double wx = ib_delta(fabs(X - ib_coord(i, 1, axis, G.h[0])), G.h[0]);
-  const double wy = ib_delta(fabs(Y - ib_coord(j, 2, axis, G.h[1])), G.h[1]);
-  const double wz = ib_delta(fabs(Z - ib_coord(k, 3, axis, G.h[2])), G.h[2]);
-  const double v = (F[q] * wx * wy * wz);  // XProblem.cpp:8596
-  const bool zero = wx == 0 || wy == 0 || wz == 0;
-  keys[t] = zero ? 0xffffffffu : (uint32_t)(((int64_t)i * G.l + j) * G.m + k);
-  vals[t] = v;
+  constexpr int kSlots = kIbPer * kIbPer * kIbPer;
+  if (t >= (int64_t)nn * kSlots) return;
+  const int q = (int)(t / kSlots), w = (int)(t % kSlots);
+  const int di = w / (kIbPer * kIbPer), dj = (w / kIbPer) % kIbPer, dk = w % kIbPer;
+  const AxisWeights ax = axis_weights(xyz[3 * q], 1, axis, G.h[0], G.E[0]);
+  const AxisWeights ay = axis_weights(xyz[3 * q + 1], 2, axis, G.h[1], G.E[1]);
+  const AxisWeights az = axis_weights(xyz[3 * q + 2], 3, axis, G.h[2], G.E[2]);
+  if (di >= ax.count || dj >= ay.count || dk >= az.count) {
+    keys[t] = 0xffffffffu;
+    vals[t] = 0;
+    return;
+  }
+  vals[t] = (F[q] * ax.w[di] * ay.w[dj] * az.w[dk]);  // XProblem.cpp:8596
+  keys[t] = (uint32_t)(((int64_t)(ax.first + di) * G.l + (ay.first + dj)) * G.m + (az.first + dk));
+}
+
+__global__ void k_ib_zero(int64_t n, double *__restrict__ f) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) f[c] = 0;
 }
 
 __global__ void k_ib_segsum(int64_t np, const uint32_t *__restrict__ keys, const double *__restrict__ vals,
@@ -143,9 +170,13 @@ extern "C" int xf_ib_spread(xf_ctx c, xf_grid g, int axis, int nn, const double 
   return guarded(c, [&] {
     XF_REQUIRE(g && force && (nn == 0 || (xyz && F)), XF_EINVAL, "xf_ib_spread: null argument");
     ib_check(g, axis, nn);
-    XF_HIP(hipMemsetAsync(force, 0, sizeof(double) * g->ncell, c->stream));
+    // a fill kernel rather than a memset: callers may capture this call in a graph
+    hipLaunchKernelGGL(k_ib_zero, dim3((unsigned)std::min<int64_t>((g->ncell + 255) / 256, 8192)), dim3(256), 0,
+                       c->stream, g->ncell, force);
     if (nn == 0) return;
-    const int64_t np = (int64_t)nn * 343;
+    const int64_t np = (int64_t)nn * kIbPer * kIbPer * kIbPer;
+    int bits = 1;  // cell keys < 2^bits - 1, so that the unused slots' key sorts after them
+    while (((int64_t)1 << bits) <= g->ncell) bits++;
     uint32_t *k0 = (uint32_t *)ws_get(c, WS_IB_K0, sizeof(uint32_t) * np);
     uint32_t *k1 = (uint32_t *)ws_get(c, WS_IB_K1, sizeof(uint32_t) * np);
     double *v0 = (double *)ws_get(c, WS_IB_V0, sizeof(double) * np);
@@ -154,9 +185,9 @@ extern "C" int xf_ib_spread(xf_ctx c, xf_grid g, int axis, int nn, const double 
                        xyz, F, k0, v0);
     check_kernel("k_ib_products");
     size_t tb = 0;
-    XF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, (int)np, 0, 32, c->stream));
+    XF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, (int)np, 0, bits, c->stream));
     void *tmp = ws_get(c, WS_IB_TMP, tb);
-    XF_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, (int)np, 0, 32, c->stream));
+    XF_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, (int)np, 0, bits, c->stream));
     hipLaunchKernelGGL(k_ib_segsum, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, np, k1, v1, force);
     check_kernel("k_ib_segsum");
   });

@@ -39,6 +39,16 @@ struct LodArgs {
   double *scrA, *scrB;       // [maxlen][nlines]
 };
 
+// Up to kLodBatch independent LOD problems advanced by the same launches
+// (blockIdx.y, or blockIdx.z for the k-line rows, picks the problem).
+constexpr int kLodBatch = 4;
+struct LodSet {
+  LodArgs a[kLodBatch];
+  struct Row4 *rows[kLodBatch];
+  const uint8_t *codes_kt[kLodBatch];
+  int count;
+};
+
 __device__ inline void line_geometry(const LodArgs &a, int axis, int t, int64_t &base, int64_t &st, int &len,
                                      int &nlines) {
   const int64_t S[3] = {(int64_t)a.l * a.m, (int64_t)a.m, 1};
@@ -131,7 +141,10 @@ __device__ inline Row cell_row(const LodArgs &A, int axis, int dir, int64_t c, c
 // transposed in LDS and written with the row of cell k of line t = i*l + j at
 // k * (n*l) + t.
 template <int AXIS>
-__global__ __launch_bounds__(256) void k_lod_rows(LodArgs A, Row4 *__restrict__ rows) {
+__global__ __launch_bounds__(256) void k_lod_rows(LodSet S) {
+  const int p = AXIS < 2 ? (int)blockIdx.y : (int)(blockIdx.z % S.count);
+  const LodArgs &A = S.a[p];
+  Row4 *__restrict__ rows = S.rows[p];
   const int n = A.n, l = A.l, m = A.m;
   if (AXIS < 2) {
     const int64_t nc = (int64_t)n * l * m;
@@ -145,7 +158,8 @@ __global__ __launch_bounds__(256) void k_lod_rows(LodArgs A, Row4 *__restrict__ 
     __shared__ Row4 tile[16][16];
     __shared__ uint8_t act[16][16];
     const int kk = threadIdx.x & 15, jj = threadIdx.x >> 4;
-    const int k0 = blockIdx.x * 16, j0 = blockIdx.y * 16, i = blockIdx.z;
+    const int k0 = blockIdx.x * 16, j0 = blockIdx.y * 16, i = blockIdx.z / S.count;
+    if (i >= n) return;
     const int k = k0 + kk, j = j0 + jj;
     uint8_t on = 0;
     if (k < m && j < l) {
@@ -172,8 +186,10 @@ __global__ __launch_bounds__(256) void k_lod_rows(LodArgs A, Row4 *__restrict__ 
 // back substitution (XSLAE.cpp:428-441), written into stage[dg-1]
 // (XProblem.cpp:3439-3449).  The host has checked every line's sectors.
 template <int AXIS>
-__global__ __launch_bounds__(64) void k_lod_solve(LodArgs A, const Row4 *__restrict__ rows,
-                                                  const uint8_t *__restrict__ codes_kt, int dg) {
+__global__ __launch_bounds__(64) void k_lod_solve(LodSet S, int dg) {
+  const LodArgs &A = S.a[blockIdx.y];
+  const Row4 *__restrict__ rows = S.rows[blockIdx.y];
+  const uint8_t *__restrict__ codes_kt = S.codes_kt[blockIdx.y];
   int64_t base, sa;
   int len, nl;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -245,24 +261,38 @@ __global__ __launch_bounds__(64) void k_lod_solve(LodArgs A, const Row4 *__restr
     }
     double u = fr ? 0.0 : A.rp[ce];
     if (!fr) fa[base + (int64_t)(beg + sd - fl) * sa] = u;
+    // blocks of Q rows from the top, the next block's A, B loaded before
+    // the current one is used
+    constexpr int Q = 8;
+    double ca[Q], cbk[Q], na[Q], nbk[Q];
+    auto ldblk = [&](int top, double *xa, double *xb) {
+#pragma unroll
+      for (int v = 0; v < Q; v++) {
+        const int ss = top - v;
+        if (ss > 0) {
+          xa[v] = sA[(int64_t)ss * nl + t];
+          xb[v] = sB[(int64_t)ss * nl + t];
+        }
+      }
+    };
     int s2 = sd - 1;
-    for (; s2 >= 4; s2 -= 4) {
-      double a4[4], b4[4];
+    ldblk(s2, ca, cbk);
+    while (s2 > 0) {
+      ldblk(s2 - Q, na, nbk);
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        a4[v] = sA[(int64_t)(s2 - v) * nl + t];
-        b4[v] = sB[(int64_t)(s2 - v) * nl + t];
-      }
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        u = a4[v] * u + b4[v];
+      for (int v = 0; v < Q; v++) {
         const int ss = s2 - v;
-        if (ss >= fl && ss <= sd - fr) fa[base + (int64_t)(beg + ss - fl) * sa] = u;
+        if (ss > 0) {
+          u = ca[v] * u + cbk[v];
+          if (ss >= fl && ss <= sd - fr) fa[base + (int64_t)(beg + ss - fl) * sa] = u;
+        }
       }
-    }
-    for (; s2 > 0; s2--) {
-      u = sA[(int64_t)s2 * nl + t] * u + sB[(int64_t)s2 * nl + t];
-      if (s2 >= fl && s2 <= sd - fr) fa[base + (int64_t)(beg + s2 - fl) * sa] = u;
+#pragma unroll
+      for (int v = 0; v < Q; v++) {
+        ca[v] = na[v];
+        cbk[v] = nbk[v];
+      }
+      s2 -= Q;
     }
     if (!fl) fa[cb] = u0;
     in = false;
@@ -290,7 +320,8 @@ __global__ __launch_bounds__(64) void k_lod_solve(LodArgs A, const Row4 *__restr
 
 // The border-only passes of direction group dg along ax1, then ax2
 // (XProblem.cpp:2965-3033), one thread per cell.
-__global__ __launch_bounds__(256) void k_lod_border(LodArgs A, int dg, int ax1, int ax2) {
+__global__ __launch_bounds__(256) void k_lod_border(LodSet set, int dg, int ax1, int ax2) {
+  const LodArgs &A = set.a[blockIdx.y];
   const int n = A.n, l = A.l, m = A.m;
   const int64_t nc = (int64_t)n * l * m;
   const int64_t S[3] = {(int64_t)l * m, (int64_t)m, 1};
@@ -321,6 +352,17 @@ __global__ __launch_bounds__(256) void k_lod_border(LodArgs A, int dg, int ax1, 
       }
     }
     if (any) fa[c] = v;
+  }
+}
+
+// f_1/3 and f_2/3 start as copies of the previous level (XProblem.cpp:2796-2797)
+__global__ __launch_bounds__(256) void k_lod_stages(LodSet S) {
+  const LodArgs &A = S.a[blockIdx.y];
+  const int64_t n = (int64_t)A.n * A.l * A.m;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const double v = A.f_prev[c];
+    A.stage[0][c] = v;
+    A.stage[1][c] = v;
   }
 }
 
@@ -379,58 +421,91 @@ static int lod_check(xf_grid g) {
   return err;
 }
 
-static void run_lod(xf_ctx c, xf_grid g, double *f, const double *f_prev, const double *u, const double *v,
-                    const double *w, const double *rp, const double *kappa, double tau, int scheme) {
-  const int err = lod_check(g);
-  XF_REQUIRE(!(err & 1), XF_EINVAL, "SolveTransferEquation: Initial border not found.");
-  XF_REQUIRE(!(err & 8), XF_EINVAL,
-             "SolveTransferEquation: a sector of Actual points reaches the end of a grid line (no closing border "
-             "point).");
-  XF_REQUIRE(err == 0, XF_EINVAL, "SolveTransferEquation: Incorrect sector border or size.");
-  const int64_t nc = g->ncell;
-  const int E[3] = {g->n, g->l, g->m};
-  int64_t maxlines = 0;
-  int maxlen = 0;
-  for (int a = 0; a < 3; a++) {
-    maxlines = std::max(maxlines, nc / E[a]);
-    maxlen = std::max(maxlen, E[a]);
+// The LOD step of `count` independent problems (xf_lod_transport's
+// arguments per problem).
+static void run_lod(xf_ctx c, int count, const xf_grid *grids, double *const *f, const double *const *f_prev,
+                    const double *const *u, const double *const *v, const double *const *w,
+                    const double *const *rp, const double *const *kappa, double tau, int scheme) {
+  XF_REQUIRE(count >= 1 && count <= kLodBatch, XF_EINVAL, "xf_lod_transport_batch: 1 to 4 problems");
+  int64_t ncs = 0, scs = 0, ncmax = 0, nlmax[3] = {0, 0, 0};
+  int nmax = 0, lmax = 0, mmax = 0;
+  for (int p = 0; p < count; p++) {
+    const xf_grid g = grids[p];
+    XF_REQUIRE(g && f[p] && f_prev[p] && u[p] && v[p] && w[p] && rp[p] && kappa[p], XF_EINVAL,
+               "xf_lod_transport: null argument");
+    const int err = lod_check(g);
+    XF_REQUIRE(!(err & 1), XF_EINVAL, "SolveTransferEquation: Initial border not found.");
+    XF_REQUIRE(!(err & 8), XF_EINVAL,
+               "SolveTransferEquation: a sector of Actual points reaches the end of a grid line (no closing border "
+               "point).");
+    XF_REQUIRE(err == 0, XF_EINVAL, "SolveTransferEquation: Incorrect sector border or size.");
+    const int64_t nc = g->ncell;
+    const int E[3] = {g->n, g->l, g->m};
+    int64_t maxlines = 0;
+    int maxlen = 0;
+    for (int a = 0; a < 3; a++) {
+      maxlines = std::max(maxlines, nc / E[a]);
+      maxlen = std::max(maxlen, E[a]);
+      nlmax[a] = std::max(nlmax[a], nc / E[a]);
+    }
+    ncs += nc;
+    scs += maxlines * (maxlen + 2);
+    ncmax = std::max(ncmax, nc);
+    nmax = std::max(nmax, g->n);
+    lmax = std::max(lmax, g->l);
+    mmax = std::max(mmax, g->m);
   }
-  double *st0 = (double *)ws_get(c, WS_LOD_ST0, sizeof(double) * nc);
-  double *st1 = (double *)ws_get(c, WS_LOD_ST1, sizeof(double) * nc);
-  Row4 *rows = (Row4 *)ws_get(c, WS_LOD_ROWS, sizeof(Row4) * nc);
-  double *scrA = (double *)ws_get(c, WS_LOD_SCRA, sizeof(double) * maxlines * (maxlen + 2));
-  double *scrB = (double *)ws_get(c, WS_LOD_SCRB, sizeof(double) * maxlines * (maxlen + 2));
-  // f_1/3 and f_2/3 start as copies of the previous level (XProblem.cpp:2796-2797)
-  XF_HIP(hipMemcpyAsync(st0, f_prev, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
-  XF_HIP(hipMemcpyAsync(st1, f_prev, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
-  LodArgs A;
-  A.n = g->n; A.l = g->l; A.m = g->m;
-  A.codes = g->d_codes;
-  A.h[0] = g->d_hx; A.h[1] = g->d_hy; A.h[2] = g->d_hz;
-  A.vel[0] = u; A.vel[1] = v; A.vel[2] = w;
-  A.f_prev = f_prev; A.rp = rp; A.kappa = kappa;
-  A.stage[0] = st0; A.stage[1] = st1; A.stage[2] = f;
-  A.tau = tau;
-  A.scheme = scheme;
-  A.scrA = scrA; A.scrB = scrB;
-  const int cb = (int)std::min<int64_t>((nc + 255) / 256, 16384);
+  double *st0 = (double *)ws_get(c, WS_LOD_ST0, sizeof(double) * ncs);
+  double *st1 = (double *)ws_get(c, WS_LOD_ST1, sizeof(double) * ncs);
+  Row4 *rows = (Row4 *)ws_get(c, WS_LOD_ROWS, sizeof(Row4) * ncs);
+  double *scrA = (double *)ws_get(c, WS_LOD_SCRA, sizeof(double) * scs);
+  double *scrB = (double *)ws_get(c, WS_LOD_SCRB, sizeof(double) * scs);
+  LodSet S{};
+  S.count = count;
+  int64_t oc = 0, os = 0;
+  for (int p = 0; p < count; p++) {
+    const xf_grid g = grids[p];
+    LodArgs &A = S.a[p];
+    A.n = g->n; A.l = g->l; A.m = g->m;
+    A.codes = g->d_codes;
+    A.h[0] = g->d_hx; A.h[1] = g->d_hy; A.h[2] = g->d_hz;
+    A.vel[0] = u[p]; A.vel[1] = v[p]; A.vel[2] = w[p];
+    A.f_prev = f_prev[p]; A.rp = rp[p]; A.kappa = kappa[p];
+    A.stage[0] = st0 + oc; A.stage[1] = st1 + oc; A.stage[2] = f[p];
+    A.tau = tau;
+    A.scheme = scheme;
+    A.scrA = scrA + os; A.scrB = scrB + os;
+    S.rows[p] = rows + oc;
+    S.codes_kt[p] = g->d_codes_kt;
+    const int E[3] = {g->n, g->l, g->m};
+    int64_t maxlines = 0;
+    int maxlen = 0;
+    for (int a = 0; a < 3; a++) {
+      maxlines = std::max(maxlines, g->ncell / E[a]);
+      maxlen = std::max(maxlen, E[a]);
+    }
+    oc += g->ncell;
+    os += maxlines * (maxlen + 2);
+  }
+  const unsigned cb = (unsigned)std::min<int64_t>((ncmax + 255) / 256, 16384);
+  const dim3 cells(cb, count);
+  hipLaunchKernelGGL(k_lod_stages, cells, dim3(256), 0, c->stream, S);
   for (int dg = 1; dg < 4; dg++) {  // XProblem.cpp:2847-2864
     const int axis = dg - 1;
-    const int nl = (int)(nc / E[axis]);
-    const dim3 lb((nl + 63) / 64);
+    const dim3 lines((unsigned)((nlmax[axis] + 63) / 64), count);
     if (axis == 0) {
-      hipLaunchKernelGGL(k_lod_rows<0>, dim3(cb), dim3(256), 0, c->stream, A, rows);
-      hipLaunchKernelGGL(k_lod_solve<0>, lb, dim3(64), 0, c->stream, A, rows, g->d_codes_kt, dg);
-      hipLaunchKernelGGL(k_lod_border, dim3(cb), dim3(256), 0, c->stream, A, dg, 1, 2);
+      hipLaunchKernelGGL(k_lod_rows<0>, cells, dim3(256), 0, c->stream, S);
+      hipLaunchKernelGGL(k_lod_solve<0>, lines, dim3(64), 0, c->stream, S, dg);
+      hipLaunchKernelGGL(k_lod_border, cells, dim3(256), 0, c->stream, S, dg, 1, 2);
     } else if (axis == 1) {
-      hipLaunchKernelGGL(k_lod_rows<1>, dim3(cb), dim3(256), 0, c->stream, A, rows);
-      hipLaunchKernelGGL(k_lod_solve<1>, lb, dim3(64), 0, c->stream, A, rows, g->d_codes_kt, dg);
-      hipLaunchKernelGGL(k_lod_border, dim3(cb), dim3(256), 0, c->stream, A, dg, 0, 2);
+      hipLaunchKernelGGL(k_lod_rows<1>, cells, dim3(256), 0, c->stream, S);
+      hipLaunchKernelGGL(k_lod_solve<1>, lines, dim3(64), 0, c->stream, S, dg);
+      hipLaunchKernelGGL(k_lod_border, cells, dim3(256), 0, c->stream, S, dg, 0, 2);
     } else {
-      hipLaunchKernelGGL(k_lod_rows<2>, dim3((g->m + 15) / 16, (g->l + 15) / 16, g->n), dim3(256), 0, c->stream, A,
-                         rows);
-      hipLaunchKernelGGL(k_lod_solve<2>, lb, dim3(64), 0, c->stream, A, rows, g->d_codes_kt, dg);
-      hipLaunchKernelGGL(k_lod_border, dim3(cb), dim3(256), 0, c->stream, A, dg, 0, 1);
+      hipLaunchKernelGGL(k_lod_rows<2>, dim3((mmax + 15) / 16, (lmax + 15) / 16, nmax * count), dim3(256), 0,
+                         c->stream, S);
+      hipLaunchKernelGGL(k_lod_solve<2>, lines, dim3(64), 0, c->stream, S, dg);
+      hipLaunchKernelGGL(k_lod_border, cells, dim3(256), 0, c->stream, S, dg, 0, 1);
     }
     check_kernel("LOD sweep");
   }
@@ -443,7 +518,24 @@ extern "C" int xf_lod_transport(xf_ctx c, xf_grid g, double *f, const double *f_
     XF_REQUIRE(g && f && f_prev && u && v && w && rp && kappa, XF_EINVAL, "xf_lod_transport: null argument");
     XF_REQUIRE(scheme == 0 || scheme == 1, XF_EINVAL, "Unknown scheme type.");
     XF_REQUIRE(tau > 0, XF_EINVAL, "xf_lod_transport: time step must be positive");
-    run_lod(c, g, f, f_prev, u, v, w, rp, kappa, tau, scheme);
+    run_lod(c, 1, &g, &f, &f_prev, &u, &v, &w, &rp, &kappa, tau, scheme);
+  });
+}
+
+extern "C" int xf_lod_transport_batch(xf_ctx c, int count, const xf_grid *grids, double *const *f,
+                                      const double *const *f_prev, const double *const *u, const double *const *v,
+                                      const double *const *w, const double *const *rp, const double *const *kappa,
+                                      double tau, int scheme) {
+  return guarded(c, [&] {
+    XF_REQUIRE(grids && f && f_prev && u && v && w && rp && kappa, XF_EINVAL,
+               "xf_lod_transport_batch: null argument");
+    XF_REQUIRE(scheme == 0 || scheme == 1, XF_EINVAL, "Unknown scheme type.");
+    XF_REQUIRE(tau > 0, XF_EINVAL, "xf_lod_transport: time step must be positive");
+    for (int p = 0; p < count; p++)
+      for (int q = 0; q < count; q++)
+        XF_REQUIRE(p == q || (f[p] != f[q] && f[p] != f_prev[q]), XF_EINVAL,
+                   "xf_lod_transport_batch: a problem's output is another problem's field");
+    run_lod(c, count, grids, f, f_prev, u, v, w, rp, kappa, tau, scheme);
   });
 }
 

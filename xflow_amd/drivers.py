@@ -22,6 +22,50 @@ import xflow_amd as xf
 from xflow_amd import cases
 
 
+class StepGraphs:
+    """The parts of a time step around the pressure solve, captured once as
+    HIP graphs and replayed (no host work or launch gaps per kernel).  With
+    graphs on, a whole step runs on one stream of its own, the library's
+    context pointed at it too (step()), so that replays, solves and torch
+    operations are ordered on that stream alone.  The first call of each part
+    runs eagerly (the library allocates its workspaces and checks point types
+    then); the second captures it and replays.  The solve itself stays eager:
+    its loop is steered from the host."""
+
+    def __init__(self, ctx, enabled):
+        self.ctx, self.enabled = ctx, enabled
+        self.graphs, self.seen = {}, set()
+        self.stream = torch.cuda.Stream() if enabled else None
+
+    def step(self, body):
+        if not self.enabled:
+            return body()
+        torch.cuda.current_stream().synchronize()
+        self.ctx.set_stream(self.stream.cuda_stream)
+        try:
+            with torch.cuda.stream(self.stream):
+                return body()
+        finally:
+            self.ctx.set_stream(None)
+
+    def run(self, name, fn):
+        if not self.enabled:
+            fn()
+            return
+        g = self.graphs.get(name)
+        if g is None:
+            if name not in self.seen:
+                self.seen.add(name)
+                fn()
+                return
+            g = torch.cuda.CUDAGraph()
+            self.stream.synchronize()
+            with torch.cuda.graph(g, stream=self.stream):
+                fn()
+            self.graphs[name] = g
+        g.replay()
+
+
 def velocity_codes(n, axis, lid_axis=1, lid_value=1.0, top_axis=3):
     """(codes, boundary values) of velocity component `axis` (1..3)."""
     N = n + 2
@@ -45,9 +89,10 @@ def velocity_codes(n, axis, lid_axis=1, lid_value=1.0, top_axis=3):
 class Cavity:
     """GPU lid-driven cavity on libxflow_hip.so."""
 
-    def __init__(self, ctx, n=64, re=100.0, dt=1e-3, tol=1e-8, solver=None):
+    def __init__(self, ctx, n=64, re=100.0, dt=1e-3, tol=1e-8, solver=None, graphs=False):
         self.ctx, self.n, self.dt, self.tol = ctx, n, dt, tol
         self.solver = xf.XF_CG if solver is None else solver
+        self.graphs = StepGraphs(ctx, graphs)
         h = 1.0 / n
         self.h = h
         steps = np.full(n + 1, h)
@@ -67,41 +112,52 @@ class Cavity:
         for a in range(3):  # PreDefined ghosts hold 2 bc - inner = 2 bc at rest
             self.vel[a][torch.from_numpy(self.vcodes[a] == cases.PREDEFINED).cuda()] *= 2.0
         self.star = [z() for _ in range(3)]
-        self.tr = [z() for _ in range(3)]
-        # viscosity at the pressure nodes; the LOD step takes its two-node mean
-        # and the boundary right-hand side built as the reference builds them
+        # per component: transport velocities, the viscosity's two-node mean and
+        # the boundary right-hand side, built as the reference builds them from
+        # the viscosity at the pressure nodes
+        self.trs = [[z() for _ in range(3)] for _ in range(3)]
         self.vis = torch.full(shape, 1.0 / re, dtype=torch.float64, device="cuda")
-        self.tvis = z()
-        self.rp = z()
+        self.tviss = [z() for _ in range(3)]
+        self.rps = [z() for _ in range(3)]
         self.p = z()
         self.p0 = z()
         self.rhs = z()
         self.last = None
 
-    def step(self):
-        n = self.n
-        # velocity predictor: LOD transport per component (XProblem.cpp:5017-5446)
+    def _predict(self):
+        # velocity predictor: LOD transport per component (XProblem.cpp:5017-5446),
+        # the three components' LOD steps in one set of launches
         for a in range(3):
-            self.vgrids[a].transport_terms(a + 1, self.vel, self.tr, vis=self.vis, bc=self.bc[a], tvis=self.tvis,
-                                           rp=self.rp)
+            self.vgrids[a].transport_terms(a + 1, self.vel, self.trs[a], vis=self.vis, bc=self.bc[a],
+                                           tvis=self.tviss[a], rp=self.rps[a])
             self.star[a].copy_(self.vel[a])
-            self.vgrids[a].lod_transport(self.star[a], self.vel[a], self.tr[0], self.tr[1], self.tr[2], self.rp,
-                                         self.tvis, self.dt, 0)
-        # pressure equation (XProblem.cpp:5474-5591)
+        xf.lod_transport_batch(self.ctx, self.vgrids, self.star, self.vel, *zip(*self.trs), self.rps, self.tviss,
+                               self.dt, 0)
+        # pressure right-hand side (XProblem.cpp:5474-5513)
         self.gp.pressure_rhs(self.star[0], self.star[1], self.star[2], self.steps, self.steps, self.steps, self.dt,
                              self.rhs)
+
+    def _correct(self):
+        # velocity correction (XProblem.cpp:5598-5778)
+        for a in range(3):
+            self.vgrids[a].velocity_correct(a + 1, self.star[a], self.p, self.steps, self.dt, self.vel[a],
+                                            bc=self.bc[a])
+
+    def step(self):
+        return self.graphs.step(self._step)
+
+    def _step(self):
+        self.graphs.run("predict", self._predict)
         o = xf.solve_opts(self.solver, tol=self.tol, tol_mode=xf.XF_TOL_REL, project_mean=1)
         st = self.gp.solve(self.rhs, self.p0, self.p, o)
         if st.status != 0:
             raise xf.XfError(st.status, "pressure solve failed")
         self.p0.copy_(self.p)
-        # velocity correction (XProblem.cpp:5598-5778); its boundary-value
-        # checks are read back once, after the divergence monitor's wait
+        # the velocity correction's boundary-value checks are read back once,
+        # after the divergence monitor's wait
         self.ctx.set_deferred_errors(True)
         try:
-            for a in range(3):
-                self.vgrids[a].velocity_correct(a + 1, self.star[a], self.p, self.steps, self.dt, self.vel[a],
-                                                bc=self.bc[a])
+            self.graphs.run("correct", self._correct)
             div, _ = self.gp.divergence_max(self.vel[0], self.vel[1], self.vel[2], self.steps, self.steps,
                                             self.steps)
         finally:
@@ -162,8 +218,9 @@ class IBChannel:
     forcing F = (0 - U(X)) / dt at the vertices spread with the vertex
     volumes (area x h), pressure projection, LOD transport of the scalar."""
 
-    def __init__(self, ctx, ny=128, subdivisions=4, tol=1e-8, solver=None, precond=None):
+    def __init__(self, ctx, ny=128, subdivisions=4, tol=1e-8, solver=None, precond=None, graphs=False):
         self.ctx = ctx
+        self.graphs = StepGraphs(ctx, graphs)
         nx, nz = 2 * ny, ny
         self.E = (nx, ny, nz)
         h = 1.0 / ny
@@ -201,12 +258,12 @@ class IBChannel:
         u0[pre] = -1.0  # mirror of the inner u = 1 about the no-slip wall value 0
         self.vel[0] = torch.from_numpy(u0).cuda()
         self.star = [z() for _ in range(3)]
-        self.tr = [z() for _ in range(3)]
+        self.trs = [[z() for _ in range(3)] for _ in range(3)]  # per component (the last: the scalar's too)
         self.force = z()
         self.nu_f = torch.full(shape, self.nu, dtype=torch.float64, device="cuda")
         self.kap_f = torch.full(shape, self.kap, dtype=torch.float64, device="cuda")
-        self.tvis = z()
-        self.rp = z()
+        self.tviss = [z() for _ in range(3)]
+        self.rps = [z() for _ in range(3)]
         X = (np.arange(nx + 2) - 0.5) * h
         Y = (np.arange(ny + 2) - 0.5) * h
         Zc = (np.arange(nz + 2) - 0.5) * h
@@ -220,14 +277,14 @@ class IBChannel:
         self.U = torch.zeros(self.nodes.shape[0], dtype=torch.float64, device="cuda")
         self.last = None
 
-    def step(self):
+    def _predict(self):
         dt = self.dt
-        for a in range(3):  # velocity predictor
-            self.vgrids[a].transport_terms(a + 1, self.vel, self.tr, vis=self.nu_f, bc=self.bc[a], tvis=self.tvis,
-                                           rp=self.rp)
+        for a in range(3):  # velocity predictor; the components' LOD steps in one set of launches
+            self.vgrids[a].transport_terms(a + 1, self.vel, self.trs[a], vis=self.nu_f, bc=self.bc[a],
+                                           tvis=self.tviss[a], rp=self.rps[a])
             self.star[a].copy_(self.vel[a])
-            self.vgrids[a].lod_transport(self.star[a], self.vel[a], self.tr[0], self.tr[1], self.tr[2], self.rp,
-                                         self.tvis, dt, 0)
+        xf.lod_transport_batch(self.ctx, self.vgrids, self.star, self.vel, *zip(*self.trs), self.rps, self.tviss,
+                               dt, 0)
         for a in range(3):  # direct forcing at the sphere's vertices
             g = self.vgrids[a]
             g.ib_interp(a + 1, self.nodes, self.star[a], self.U)
@@ -235,6 +292,23 @@ class IBChannel:
             g.ib_spread(a + 1, self.nodes, F, self.force)
             g.lincomb(1.0, self.star[a], dt, self.force, self.star[a])
         self.gp.pressure_rhs(self.star[0], self.star[1], self.star[2], *self.steps, dt, self.rhs)
+
+    def _correct(self):
+        dt = self.dt
+        for a in range(3):
+            self.vgrids[a].velocity_correct(a + 1, self.star[a], self.p, self.steps[a], dt, self.vel[a], bc=self.bc[a])
+        # passive scalar, upwind LOD with cell-centred velocities
+        tr, rp = self.trs[2], self.rps[2]
+        self.gs.transport_terms(0, self.vel, tr, bc=self.zero, rp=rp)
+        self.c_new.copy_(self.c)
+        self.gs.lod_transport(self.c_new, self.c, tr[0], tr[1], tr[2], rp, self.kap_f, dt, 1)
+        self.c.copy_(self.c_new)
+
+    def step(self):
+        return self.graphs.step(self._step)
+
+    def _step(self):
+        self.graphs.run("predict", self._predict)
         o = xf.solve_opts(self.solver, tol=self.tol, tol_mode=xf.XF_TOL_REL, project_mean=1)
         o.verify = 1 if self.verify else 0
         if self.precond is not None:
@@ -247,14 +321,7 @@ class IBChannel:
         # after the divergence monitor's wait
         self.ctx.set_deferred_errors(True)
         try:
-            for a in range(3):
-                self.vgrids[a].velocity_correct(a + 1, self.star[a], self.p, self.steps[a], dt, self.vel[a],
-                                                bc=self.bc[a])
-            # passive scalar, upwind LOD with cell-centred velocities
-            self.gs.transport_terms(0, self.vel, self.tr, bc=self.zero, rp=self.rp)
-            self.c_new.copy_(self.c)
-            self.gs.lod_transport(self.c_new, self.c, self.tr[0], self.tr[1], self.tr[2], self.rp, self.kap_f, dt, 1)
-            self.c, self.c_new = self.c_new, self.c
+            self.graphs.run("correct", self._correct)
             div, _ = self.gp.divergence_max(self.vel[0], self.vel[1], self.vel[2], *self.steps)
         finally:
             self.ctx.set_deferred_errors(False)
