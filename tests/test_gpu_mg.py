@@ -50,9 +50,12 @@ def test_hierarchy_masks_match_numpy(ctx, shape, dirichlet):
 # (8, 10, 300): the residual restricted in the sweep pass with a shifted last
 # k-tile and a block whose last two rows lie past the grid; (16, 16, 64) and
 # (16, 16, 32): halved down to a grid of at most 512 cells, solved with its
-# pseudo-inverse (pure Neumann: singular; with a boundary-value plane: regular)
+# pseudo-inverse (pure Neumann: singular; with a boundary-value plane: regular),
+# the levels of at most 2048 cells in one workgroup (k_mg_bottom: from level 2,
+# level 1, and (8, 8, 16) from level 0)
 @pytest.mark.parametrize("shape,dirichlet", [((8, 8, 256), False), ((16, 12, 128), True), ((20, 16, 64), True),
-                                             ((8, 10, 300), True), ((16, 16, 64), False), ((16, 16, 32), True)])
+                                             ((8, 10, 300), True), ((16, 16, 64), False), ((16, 16, 32), True),
+                                             ((8, 8, 16), True)])
 def test_cycle_matches_restatement(ctx, shape, dirichlet):
     codes, h = case_of(shape, 12, dirichlet)
     g = xf.Grid(ctx, h[0], h[1], h[2], codes)

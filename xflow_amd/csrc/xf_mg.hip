@@ -38,6 +38,7 @@ constexpr int kSweeps = 2;        // pre- and post-smoothing sweeps per level
 constexpr int kCoarseSweeps = 8;  // coarsest-level sweeps without a network
 constexpr int kMgBlocks = 512;    // blocks of the reduction kernels
 constexpr int kCoarseCells = 512;   // cells (ring included) of a grid solved densely
+constexpr int kBottomCells = 2048;  // cells of the largest level the one-workgroup bottom of the cycle takes
 constexpr int kMaxLevels = 12;
 
 struct MgLevel {
@@ -49,6 +50,7 @@ struct MgHier {
   int nlev = 0;         // levels of the cycle without a network
   int net_lev = 0;      // levels of the cycle with the network (its coarsest has rows >= 64)
   int ndense = 0;       // unknowns of the deepest grid's dense solve (0: Jacobi sweeps there)
+  int bottom = -1;      // first level of the cycle without a network run by k_mg_bottom (-1: none)
   float *minv = nullptr;     // [ndense][ndense] its pseudo-inverse (symmetric)
   int32_t *uidx = nullptr;   // [ncell] cell -> unknown index, -1 elsewhere
   MgLevel lev[kMaxLevels];
@@ -435,6 +437,145 @@ __global__ __launch_bounds__(kCoarseCells) void k_mg_coarse_dense(int ncell, int
   zout[q] = (float)acc;
 }
 
+// The bottom of the cycle without a network in one workgroup: levels
+// first..last (each at most kBottomCells cells) down to the dense solve and
+// back up, every phase a loop over the level's interior cells separated by
+// barriers (the level fields stay in global memory, warm in the caches).
+// Each phase does what the multi-block path does at that level, in the same
+// floating-point order: the first two sweeps from z = 0, the residual, the
+// restriction, the dense solve, the prolongation and the two sweeps after.
+struct BotLevel {
+  Dims d;
+  const uint8_t *flags;
+  const float *cx, *cy, *cz;
+  float *r, *z, *z2, *t;
+};
+struct BotArgs {
+  BotLevel lv[kMaxLevels];
+  int first, last, nu, ndense;
+  const int32_t *uidx;
+  const float *minv;
+  const float *r_in;  // r of level first
+  float *z_out;       // z of level first
+};
+
+__device__ inline float bot_az(const BotLevel &L, uint8_t f, int64_t c, int i, int j, int k, const float *z) {
+  const int64_t sI = (int64_t)L.d.l * L.d.m, sJ = L.d.m;
+  const float cxm = L.cx[2 * i], cxp = L.cx[2 * i + 1], cym = L.cy[2 * j], cyp = L.cy[2 * j + 1];
+  const float czm = L.cz[2 * k], czp = L.cz[2 * k + 1];
+  const float zc = z[c];
+  float tx = ((f & FL_IP) ? cxp * (z[c + sI] - zc) : 0.f) - ((f & FL_IM) ? cxm * (zc - z[c - sI]) : 0.f);
+  float ty = ((f & FL_JP) ? cyp * (z[c + sJ] - zc) : 0.f) - ((f & FL_JM) ? cym * (zc - z[c - sJ]) : 0.f);
+  float tz = ((f & FL_KP) ? czp * (z[c + 1] - zc) : 0.f) - ((f & FL_KM) ? czm * (zc - z[c - 1]) : 0.f);
+  return -(tx + ty + tz);
+}
+
+__global__ __launch_bounds__(1024) void k_mg_bottom(BotArgs B) {
+  __shared__ float lut[kMaxLevels][64];
+  __shared__ float rs[kCoarseCells];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int q = tid; q < kMaxLevels * 64; q += nt) {
+    const int k = q >> 6, fb = q & 63;
+    if (k < B.first || k > B.last) continue;
+    const float a = B.lv[k].cx[2], b = B.lv[k].cy[2], e = B.lv[k].cz[2];
+    const float dg = diag_of((uint8_t)fb, a, a, b, b, e, e);
+    lut[k][fb] = dg > 0 ? kOmega / dg : 0.f;
+  }
+  __syncthreads();
+  // one pass over the interior cells of level k
+  auto interior = [&](int k, auto &&body) {
+    const Dims d = B.lv[k].d;
+    const int N = d.n - 2, Lx = d.l - 2, M = d.m - 2;
+    const int tot = N * Lx * M;
+    for (int q = tid; q < tot; q += nt) {
+      const int kk = 1 + q % M, jj = 1 + (q / M) % Lx, ii = 1 + q / (M * Lx);
+      body(((int64_t)ii * d.l + jj) * d.m + kk, ii, jj, kk);
+    }
+    __syncthreads();
+  };
+  for (int k = B.first; k < B.last; k++) {
+    const BotLevel &L = B.lv[k], &C = B.lv[k + 1];
+    const float *r = k == B.first ? B.r_in : L.r;
+    float *z = k == B.first ? B.z_out : L.z;
+    // the first sweep from z = 0: z2 = omega D^-1 r
+    interior(k, [&](int64_t c, int, int, int) {
+      const uint8_t f = L.flags[c];
+      L.z2[c] = (f & FL_ACTUAL) ? lut[k][f & 63] * r[c] : 0.f;
+    });
+    // the second: z = z2 + omega D^-1 (r - A z2); further sweeps alternate
+    for (int sw = 1; sw < B.nu; sw++) {
+      const float *zi = (sw & 1) ? L.z2 : z;
+      float *zo = (sw & 1) ? z : L.z2;
+      interior(k, [&](int64_t c, int i, int j, int kk) {
+        const uint8_t f = L.flags[c];
+        zo[c] = (f & FL_ACTUAL) ? zi[c] + lut[k][f & 63] * (r[c] - bot_az(L, f, c, i, j, kk, zi)) : 0.f;
+      });
+    }
+    // t = r - A z
+    interior(k, [&](int64_t c, int i, int j, int kk) {
+      const uint8_t f = L.flags[c];
+      L.t[c] = (f & FL_ACTUAL) ? r[c] - bot_az(L, f, c, i, j, kk, z) : 0.f;
+    });
+    // coarse r = mean of t over the unknown children
+    interior(k + 1, [&](int64_t cc, int I, int J, int K) {
+      float sum = 0;
+      int cnt = 0;
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+          for (int e = 0; e < 2; e++) {
+            const int64_t q = ((int64_t)(2 * I - 1 + a) * L.d.l + (2 * J - 1 + b)) * L.d.m + (2 * K - 1 + e);
+            if (L.flags[q] & FL_UNKNOWN) {
+              sum += L.t[q];
+              cnt++;
+            }
+          }
+      C.r[cc] = cnt ? sum / (float)cnt : 0.f;
+    });
+  }
+  {
+    // the deepest level: z = A^+ r
+    const BotLevel &D = B.lv[B.last];
+    const float *r = B.last == B.first ? B.r_in : D.r;
+    float *z = B.last == B.first ? B.z_out : D.z;
+    const int ncell = D.d.n * D.d.l * D.d.m;
+    for (int q = tid; q < ncell; q += nt) {
+      const int u = B.uidx[q];
+      if (u >= 0) rs[u] = r[q];
+    }
+    __syncthreads();
+    const int nu = B.ndense;
+    for (int q = tid; q < ncell; q += nt) {
+      const int u = B.uidx[q];
+      double acc = 0;
+      if (u >= 0)
+        for (int v = 0; v < nu; v++) acc += (double)B.minv[(size_t)v * nu + u] * (double)rs[v];
+      z[q] = (float)acc;
+    }
+    __syncthreads();
+  }
+  for (int k = B.last - 1; k >= B.first; k--) {
+    const BotLevel &L = B.lv[k], &C = B.lv[k + 1];
+    const float *r = k == B.first ? B.r_in : L.r;
+    float *z = k == B.first ? B.z_out : L.z;
+    const float *zc = C.z;
+    // z += the coarse correction of the parent at the unknowns
+    interior(k, [&](int64_t c, int i, int j, int kk) {
+      if (L.flags[c] & FL_UNKNOWN) z[c] += zc[((int64_t)((i + 1) >> 1) * C.d.l + ((j + 1) >> 1)) * C.d.m + ((kk + 1) >> 1)];
+    });
+    for (int sw = 0; sw < B.nu; sw++) {
+      const float *zi = (sw & 1) ? L.z2 : z;
+      float *zo = (sw & 1) ? z : L.z2;
+      interior(k, [&](int64_t c, int i, int j, int kk) {
+        const uint8_t f = L.flags[c];
+        zo[c] = (f & FL_ACTUAL) ? zi[c] + lut[k][f & 63] * (r[c] - bot_az(L, f, c, i, j, kk, zi)) : 0.f;
+      });
+    }
+    if (B.nu & 1) {  // an odd sweep count leaves the result in z2
+      interior(k, [&](int64_t c, int, int, int) { z[c] = L.z2[c]; });
+    }
+  }
+}
+
 // ------------------------------------------------------------------ host
 static dim3 cell_grid(const xf_grid g) { return dim3((g->m - 2 + 63) / 64, (g->l - 2 + 3) / 4, g->n - 2); }
 static Dims dims_of(const xf_grid g) { return Dims{g->n, g->l, g->m}; }
@@ -682,6 +823,10 @@ static MgHier *hierarchy(xf_ctx c, xf_grid g) {
   h->nlev = nlev;
   if (!h->net_lev) h->net_lev = nlev;
   if (h->lev[nlev - 1].g->ncell <= kCoarseCells) build_dense(c, h, fcodes);
+  if (h->ndense > 0) {
+    h->bottom = nlev - 1;
+    while (h->bottom > 0 && h->lev[h->bottom - 1].g->ncell <= kBottomCells) h->bottom--;
+  }
   for (int k = 0; k < nlev; k++) {
     MgLevel &L = h->lev[k];
     const int64_t nc = L.g->ncell;
@@ -771,6 +916,23 @@ static void coarse_solve(xf_ctx c, MgHier *h, struct xf_unet_s *net, float *last
 // keep theirs in lev[k].z)
 static void vcycle(xf_ctx c, MgHier *h, int k, const float *r, struct xf_unet_s *net, float *out) {
   float *last = k == 0 ? out : nullptr;
+  if (!net && k == h->bottom) {
+    BotArgs B{};
+    for (int q = k; q < h->nlev; q++) {
+      const MgLevel &L = h->lev[q];
+      B.lv[q] = BotLevel{dims_of(L.g), L.g->d_flags, L.g->d_cxf, L.g->d_cyf, L.g->d_czf, L.r, L.z, L.z2, L.t};
+    }
+    B.first = k;
+    B.last = h->nlev - 1;
+    B.nu = kSweeps;
+    B.ndense = h->ndense;
+    B.uidx = h->uidx;
+    B.minv = h->minv;
+    B.r_in = r;
+    B.z_out = k == 0 ? out : h->lev[k].z;
+    hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(1024), 0, c->stream, B);
+    return;
+  }
   if (k == last_level(h, net)) {
     if (k == 0) {
       // a single level: the coarse solve reads lev[0].r, which is the caller's r
