@@ -156,6 +156,8 @@ def main():
     ap.add_argument("--cpu-ref-iters", type=int, default=12)
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--skip-fullres", action="store_true")
+    ap.add_argument("--skip-bicgstab", action="store_true",
+                    help="leave out the two BiCGStab solves (reference statements and fused)")
     ap.add_argument("--io", action="store_true", help="also time the output files (flow VTK, .npy) of the solution")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--nn-inner-tol", type=float, default=1e-4)
@@ -275,6 +277,22 @@ def main():
         full = load_network(ctx, args.weights)
         if full is not None and not args.skip_fullres:
             solvers["nn_fullres_mp"] = timed(opts(xf.XF_FCG_NN, full), "MP flexible CG, full-resolution network")
+        if not args.skip_bicgstab:
+            # the reference's BiCGStab (max |r| <= 1e-5 absolute) on b scaled to
+            # max |b| = 1e3, one solve each: statement by statement and fused
+            bs = b * (1e3 / float(b.abs().max()))
+            for name, method in (("bicgstab_reference_statements", xf.XF_BICGSTAB),
+                                 ("bicgstab_fused", xf.XF_BICGSTAB_T)):
+                o = xf.solve_opts(method)
+                o.verify = 0
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                st = g.solve(bs, x0, x, o)
+                torch.cuda.synchronize()
+                ms = (time.perf_counter() - t0) * 1e3
+                solvers[name] = {"ms": round(ms, 2), "iterations": st.iters, "status": st.status,
+                                 "max_abs_residual": st.resid, "ms_per_iteration": round(ms / max(st.iters, 1), 4)}
+                log("%s: %.1f ms, %d iterations, max |r| %.2e" % (name, ms, st.iters, st.resid))
 
     # host-buffer boundary: the drop-in classes upload b and download x once per
     # solve; time that PCIe round trip (pinned host memory) to quote the

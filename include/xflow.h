@@ -114,7 +114,10 @@ int xf_absmax(xf_ctx ctx, xf_grid g, const double *x, double *val, int32_t *rn_i
 #define XF_FCG_NN 6    /* flexible CG with the learned preconditioner             */
 #define XF_FCG_MGNN 7  /* flexible CG, V-cycle preconditioner with the learned
                           network as its coarsest-grid solve (precond may be
-                          NULL: damped-Jacobi sweeps there instead)              */
+                          NULL: the full-depth classical cycle instead)          */
+#define XF_BICGSTAB_T 8 /* the reference's BiCGStab recurrence and stopping rule
+                          (max |r| <= tol, absolute; default 1e-5) on its
+                          operator rows in four fused passes per iteration       */
 
 #define XF_TOL_ABS 0   /* stop on ||r|| <= tol (the reference's rule)            */
 #define XF_TOL_REL 1   /* stop on ||r|| <= tol * ||b - A x0||                      */

@@ -9,11 +9,11 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (XF_BICG, XF_BICGSTAB, XF_CG, XF_CG3, XF_CG_MP, XF_FCG_MGNN, XF_FCG_NN, XF_MR, XF_OK, XF_TOL_ABS,
+from ._lib import (XF_BICG, XF_BICGSTAB, XF_BICGSTAB_T, XF_CG, XF_CG3, XF_CG_MP, XF_FCG_MGNN, XF_FCG_NN, XF_MR, XF_OK, XF_TOL_ABS,
                    XF_TOL_REL, SolveOpts, SolveStats, VtkFlow, XfError, lib)
 
 __all__ = ["Context", "Grid", "Pack", "XfError", "XF_CG3", "XF_BICG", "XF_MR", "XF_BICGSTAB", "XF_CG", "XF_CG_MP",
-           "XF_FCG_NN", "XF_FCG_MGNN", "XF_TOL_ABS", "XF_TOL_REL", "solve_opts", "write_vtk_flow",
+           "XF_FCG_NN", "XF_FCG_MGNN", "XF_BICGSTAB_T", "XF_TOL_ABS", "XF_TOL_REL", "solve_opts", "write_vtk_flow",
            "write_vtk_boundary", "npy_save"]
 
 

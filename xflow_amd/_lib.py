@@ -10,7 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxflow_hip.so")
 
 XF_OK, XF_EINVAL, XF_EMAXIT, XF_EBREAKDOWN, XF_ENAN, XF_EHIP, XF_ENOMEM, XF_ESTALL = 0, -1, -2, -3, -4, -5, -6, -7
-XF_CG3, XF_BICG, XF_MR, XF_BICGSTAB, XF_CG, XF_CG_MP, XF_FCG_NN, XF_FCG_MGNN = range(8)
+XF_CG3, XF_BICG, XF_MR, XF_BICGSTAB, XF_CG, XF_CG_MP, XF_FCG_NN, XF_FCG_MGNN, XF_BICGSTAB_T = range(9)
 XF_TOL_ABS, XF_TOL_REL = 0, 1
 
 STATUS_NAMES = {0: "ok", -1: "invalid", -2: "max-iterations", -3: "breakdown", -4: "nan", -5: "hip-error",

@@ -64,6 +64,7 @@ enum WsSlot {
   WS_LOD_ST0, WS_LOD_ST1, WS_LOD_ROWS, WS_LOD_SCRA, WS_LOD_SCRB,
   WS_IB_K0, WS_IB_K1, WS_IB_V0, WS_IB_V1, WS_IB_TMP,
   WS_THOMAS_A, WS_THOMAS_B,
+  WS_BT_VECS, WS_BT_PART,
   WS_COUNT
 };
 
@@ -250,6 +251,8 @@ void launch_apply_dot_gated(xf_ctx ctx, xf_grid g, const double *x, const double
 // throughput CG family (xf_cg.hip)
 void solve_hs_cg(xf_ctx ctx, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
                  xf_solve_stats &s);
+void solve_bicgstab_t(xf_ctx c, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
+                      xf_solve_stats &s);
 void solve_mp_cg(xf_ctx ctx, xf_grid g, const double *b, const double *x0, double *x, const xf_solve_opts &o,
                  xf_solve_stats &s);
 double true_residual(xf_ctx ctx, xf_grid g, const double *b, const double *x, const double *x0, double *work);

@@ -486,6 +486,10 @@ extern "C" void xf_solve_opts_default(int method, xf_solve_opts *o) {
     case XF_BICG: d.tol = 1e-8; break;      // XSLAE.cpp:166
     case XF_MR: d.tol = 1e-7; break;        // XSLAE.cpp:358
     case XF_BICGSTAB: d.tol = 1e-5; break;  // XSLAE.cpp:669
+    case XF_BICGSTAB_T:
+      d.tol = 1e-5;  // XSLAE.cpp:669
+      d.check_every = 16;
+      break;
     default:
       d.tol = 1e-8;
       d.tol_mode = XF_TOL_REL;
@@ -517,6 +521,7 @@ extern "C" int xf_solve(xf_ctx c, xf_grid g, const double *b, const double *x0, 
         break;
       }
       case XF_CG: solve_hs_cg(c, g, b, x0, x, o, s); break;
+      case XF_BICGSTAB_T: solve_bicgstab_t(c, g, b, x0, x, o, s); break;
       case XF_CG_MP: solve_mp_cg(c, g, b, x0, x, o, s); break;
       case XF_FCG_NN:
       case XF_FCG_MGNN: solve_fcg_nn(c, g, b, x0, x, o, s); break;
