@@ -1,14 +1,17 @@
 // Throughput BiCGStab (XF_BICGSTAB_T): the recurrence of
 // TSLAE::SolveByBiConjugateGradientsStab (XSLAE.cpp:667-767) with its
 // stopping rule (max |r| <= tol, absolute, tested before each iteration) on
-// the reference's operator rows, in four fused passes per iteration instead
+// the reference's operator rows, in five fused passes per iteration instead
 // of the parity solver's two dozen:
 //   KP  loop test, beta = (rho'/rho)(alpha/omega), p = r + beta (p - omega v)
 //   KV  v = A p                                   and the partials of (r^, v)
-//   KST alpha = rho' / (r^, v), s = r - alpha v, t = A s (s recomputed at the
-//       stencil points)                           and the partials of (t, s), (t, t)
+//   KS  alpha = rho' / (r^, v), s = r - alpha v
+//   KT  t = A s                                   and the partials of (t, s), (t, t)
 //   KX  omega = (t, s) / (t, t), x += alpha p + omega s, r = s - omega t
 //                                                 and the partials of (r^, r), max |r|
+// (Forming s at the stencil points inside KT instead cost 819 us per pass at
+// 256^3 against 232 + 60 us for the two passes: the exact rows read their
+// field several times per point.)
 // Each vector statement keeps the reference's rounding sequence (XSLAE.cpp:
 // 723-745).  Reductions are per-block partials that the next pass folds in a
 // fixed order in every block (no grid-wide atomics), so all blocks take the
@@ -112,7 +115,8 @@ __device__ inline void for_cells(const GridView &g, const F &body) {
   }
 }
 
-// r = b - A x, r^ = r, p = v = 0 (XSLAE.cpp:681-703); partials of (r^, r) and max |r|
+// r = b - A x, r^ = r, p = v = 0, s = t = 0 (XSLAE.cpp:681-703); partials of
+// (r^, r) and max |r|
 __global__ __launch_bounds__(kBlock) void k_bt_init(BtArgs A) {
   __shared__ double lds[kBlock];
   double rr = 0, mx = 0;
@@ -126,8 +130,10 @@ __global__ __launch_bounds__(kBlock) void k_bt_init(BtArgs A) {
     }
     A.r[c] = rv;
     A.rh[c] = rv;
-    A.p[c] = 0;
+    A.p[c] = 0;  // the vectors are 0 off the unknowns, where the rows read them too
     A.v[c] = 0;
+    A.s[c] = 0;
+    A.t[c] = 0;
   });
   put<false>(A, PS_RR, rr, lds);
   put<true>(A, PS_MAX, mx, lds);
@@ -182,38 +188,34 @@ __global__ __launch_bounds__(kBlock) void k_bt_v(BtArgs A) {
   if (blockIdx.x == 0 && threadIdx.x == 0) A.st->iter++;  // XSLAE.cpp:719
 }
 
-// alpha = rho' / (r^, v) (XSLAE.cpp:730), s = r - alpha v (732-733), t = A s
-// (735); partials of (t, s) and (t, t)
-struct SAt {
-  const double *r, *v;
-  const uint8_t *space;
-  double an;
-  __device__ double operator()(int64_t q) const {
-    if (!space[q]) return 0.0;
-    const double av = an * v[q];
-    return r[q] - av;
-  }
-};
-
-__global__ __launch_bounds__(kBlock) void k_bt_st(BtArgs A) {
+// alpha = rho' / (r^, v) (XSLAE.cpp:730) and s = r - alpha v (732-733)
+__global__ __launch_bounds__(kBlock) void k_bt_s(BtArgs A) {
   __shared__ double lds[kBlock];
   DevState *st = A.st;
   if (st->done) return;
   const double an = st->s[BT_PN1] / fold<false>(A, PS_RV, lds);
-  const SAt sat{A.r, A.v, A.g.space, an};
+  for_cells(A.g, [&](int64_t c, int, int, int) {
+    if (!A.g.space[c]) return;
+    const double av = an * A.v[c];
+    A.s[c] = A.r[c] - av;
+  });
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->s[BT_AN] = an;
+}
+
+// t = A s (XSLAE.cpp:735); partials of (t, s) and (t, t)
+__global__ __launch_bounds__(kBlock) void k_bt_t(BtArgs A) {
+  __shared__ double lds[kBlock];
+  if (A.st->done) return;
   double ts = 0, tt = 0;
   for_cells(A.g, [&](int64_t c, int i, int j, int k) {
     if (!A.g.space[c]) return;
-    const double sv = sat(c);
-    const double tv = row_value_t(A.g, sat, c, i, j, k);
-    A.s[c] = sv;
+    const double tv = row_value(A.g, A.s, c, i, j, k);
     A.t[c] = tv;
-    ts += tv * sv;
+    ts += tv * A.s[c];
     tt += tv * tv;
   });
   put<false>(A, PS_TS, ts, lds);
   put<false>(A, PS_TT, tt, lds);
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->s[BT_AN] = an;
 }
 
 // omega = (t, s) / (t, t) (XSLAE.cpp:736), x += alpha p + omega s (738-741),
@@ -288,7 +290,8 @@ void solve_bicgstab_t(xf_ctx c, xf_grid g, const double *b, const double *x0, do
     for (int q = 0; q < every; q++) {
       hipLaunchKernelGGL(k_bt_p, dim3(A.nb), dim3(kBlock), 0, c->stream, A);
       hipLaunchKernelGGL(k_bt_v, dim3(A.nb), dim3(kBlock), 0, c->stream, A);
-      hipLaunchKernelGGL(k_bt_st, dim3(A.nb), dim3(kBlock), 0, c->stream, A);
+      hipLaunchKernelGGL(k_bt_s, dim3(A.nb), dim3(kBlock), 0, c->stream, A);
+      hipLaunchKernelGGL(k_bt_t, dim3(A.nb), dim3(kBlock), 0, c->stream, A);
       hipLaunchKernelGGL(k_bt_x, dim3(A.nb), dim3(kBlock), 0, c->stream, A);
     }
     check_kernel("BiCGStab iteration");

@@ -5,7 +5,7 @@ residual and the velocity correction.  Reports ms per step (and of its
 pressure solve), pressure iterations and the largest divergence, every
 `--every` steps.
 
-    python -m xflow_amd.run_cavity [--n 64] [--steps 100] [--tol 1e-8] [--solver cg|cg_mp|cg3]
+    python -m xflow_amd.run_cavity [--n 64] [--steps 100] [--tol 1e-8] [--solver cg|cg_mp|cg3|cycle] [--graphs]
 
 `u_max` is the largest |u| over the x-velocity unknowns (ghost cells behind
 the lid hold 2 - u and are left out).
@@ -25,7 +25,7 @@ import xflow_amd as xf  # noqa: E402
 from xflow_amd import cases  # noqa: E402
 from xflow_amd.drivers import Cavity  # noqa: E402
 
-SOLVERS = {"cg": xf.XF_CG, "cg_mp": xf.XF_CG_MP, "cg3": xf.XF_CG3}
+SOLVERS = {"cg": xf.XF_CG, "cg_mp": xf.XF_CG_MP, "cg3": xf.XF_CG3, "cycle": xf.XF_FCG_MGNN}
 
 
 def main():
@@ -35,10 +35,12 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--every", type=int, default=20)
     ap.add_argument("--solver", choices=sorted(SOLVERS), default="cg")
+    ap.add_argument("--graphs", action="store_true",
+                    help="run the parts of the step around the pressure solve as HIP graphs")
     args = ap.parse_args()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     ctx = xf.Context(0)
-    cav = Cavity(ctx, n=args.n, tol=args.tol, solver=SOLVERS[args.solver])
+    cav = Cavity(ctx, n=args.n, tol=args.tol, solver=SOLVERS[args.solver], graphs=args.graphs)
     inner = torch.from_numpy(cav.vcodes[0] == cases.ACTUAL).cuda()
     hist, wall, pms, iters = [], 0.0, 0.0, 0
     for s in range(args.steps):
@@ -56,7 +58,7 @@ def main():
             hist.append(rec)
             log("step %d: pressure %d iterations, div max %.2e, true residual %.2e, max |u| %.3f" %
                 (s + 1, rec["pressure_iters"], rec["div_max"], rec["resid_true"], rec["u_max"]))
-    out = {"cells": [args.n] * 3, "steps": args.steps, "tol": args.tol, "solver": args.solver,
+    out = {"cells": [args.n] * 3, "steps": args.steps, "tol": args.tol, "solver": args.solver, "graphs": args.graphs,
            "ms_per_step": round(wall * 1e3 / args.steps, 3), "pressure_ms_per_step": round(pms / args.steps, 3),
            "pressure_iters_mean": iters / args.steps, "history": hist}
     print(json.dumps(out))
