@@ -10,8 +10,8 @@
 //   KX  omega = (t, s) / (t, t), x += alpha p + omega s, r = s - omega t
 //                                                 and the partials of (r^, r), max |r|
 // (Forming s at the stencil points inside KT instead cost 819 us per pass at
-// 256^3 against 232 + 60 us for the two passes: the exact rows read their
-// field several times per point.)
+// 256^3 against 94 + 216 us for KS and KT: the exact rows read their field
+// several times per point.)
 // Each vector statement keeps the reference's rounding sequence (XSLAE.cpp:
 // 723-745).  Reductions are per-block partials that the next pass folds in a
 // fixed order in every block (no grid-wide atomics), so all blocks take the

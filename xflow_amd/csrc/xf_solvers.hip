@@ -73,14 +73,26 @@ __global__ void k_absmax_dev(int64_t nc, const uint8_t *__restrict__ space, cons
     am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
   }
   __syncthreads();
-  if (!am_last || threadIdx.x != 0) return;
+  if (!am_last) return;
   __threadfence();
+  // the last block folds the partials: a strided share per thread, then the block
   double b = 0;
-  for (unsigned q = 0; q < gridDim.x; q++) {
+  for (unsigned q = threadIdx.x; q < gridDim.x; q += blockDim.x) {
     double o = __hip_atomic_load(&partials[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (o > b || o != o) b = o;
   }
-  *out = b;
+  __syncthreads();
+  lv[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      double o = lv[threadIdx.x + s];
+      if (o > lv[threadIdx.x] || o != o) lv[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  *out = lv[0];
   *counter = 0;
 }
 

@@ -111,17 +111,35 @@ __global__ void k_absmax(int64_t nc, const uint8_t *__restrict__ space, const in
     am_last = (prev == gridDim.x - 1);
   }
   __syncthreads();
-  if (!am_last || threadIdx.x != 0) return;
+  if (!am_last) return;
   __threadfence();
+  // the last block folds the partials: a strided share per thread, then the
+  // block (ties to the lowest Rn index, as in the serial scan)
   double b = 0;
   int32_t idx = 0x7fffffff;
-  for (unsigned q = 0; q < gridDim.x; q++) {
+  for (unsigned q = threadIdx.x; q < gridDim.x; q += blockDim.x) {
     double v = __hip_atomic_load(&pv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     int32_t iq = __hip_atomic_load(&pi[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (v > b || (v == b && iq < idx)) { b = v; idx = iq; }
   }
-  *out = b;
-  *out_idx = (b > 0) ? idx : 1;
+  __syncthreads();
+  lv[threadIdx.x] = b;
+  li[threadIdx.x] = idx;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      double v2 = lv[threadIdx.x + s];
+      int32_t i2 = li[threadIdx.x + s];
+      if (v2 > lv[threadIdx.x] || (v2 == lv[threadIdx.x] && i2 < li[threadIdx.x])) {
+        lv[threadIdx.x] = v2;
+        li[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  *out = lv[0];
+  *out_idx = (lv[0] > 0) ? li[0] : 1;
   *counter = 0;
 }
 
