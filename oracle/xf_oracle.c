@@ -24,6 +24,17 @@
  *       Appl. Math. 25, 1989; not in the reference) with s = A p recomputed,
  *       the throughput solver's recurrence; pinned by a dense numpy solve and
  *       by agreement with xo_cg_hs (the same method in exact arithmetic).
+ *   xo_transport_velocity, xo_pressure_rhs, xo_velocity_correct,
+ *   xo_divergence_max: the code around the pressure solve inside
+ *       TVPProblem::SolveBySplitting, which cannot be driven on its own; pinned
+ *       by closed forms in tests/test_oracle_math.py (linear fields give exact
+ *       face averages, divergences and gradients; a constant density of 2
+ *       gives the rho^2 factor of XProblem.cpp:5495; Defined and PreDefined
+ *       velocity points take the boundary value and the mirror).
+ *   xo_dot, xo_absmax, xo_apply_dot: bit-exact against the golden vectors of
+ *       the masked vector kernels (XSpace.cpp:2293-3487); xo_lincomb, xo_copy:
+ *       inside every solver whose iterates are bit-exact against golden
+ *       vectors, and their masking and zero-coefficient cases by closed form.
  *
  * Dot products are serial sums in i->j->k order, as in the reference
  * (XSpace.cpp:3219-3342).  Element-wise loops may run on several OpenMP

@@ -41,6 +41,7 @@ def lib():
             "xo_apply": (i, [gp, p, p, d, p]),
             "xo_apply_dot": (d, [gp, p, p]),
             "xo_lincomb": (None, [gp, d, p, d, p, p]),
+            "xo_copy": (None, [gp, p, p]),
             "xo_dot": (d, [gp, p, p]),
             "xo_absmax": (d, [gp, p, p, p]),
             "xo_cg3": (i, [gp, p, p, p, d, i, p, p]),

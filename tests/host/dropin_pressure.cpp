@@ -199,6 +199,15 @@ int main() {
     d63 = std::fmax(d63, std::fabs(r6[rn] - r3[rn]));
   }
 
+  // the fused BiCGStab (a GPU extension of TSLAE) on the step-2 system
+  TRnRelease3dSpace &p4 = *new TRnRelease3dSpace(modelP);
+  p4.Assign(p);  // from the first step's pressure, not the step-2 solution
+  TSLAE slae7(*APtr, p4, rightPart);
+  slae7.SolveByFastBiConjugateGradientsStab();
+  const int itFused = slae7.LastStats().iters;
+  p4 |= slae7.Result;
+  const double resFused = residual_max(*APtr, p4, rightPart, modelP);
+
   // errors the reference throws as const char*
   bool assignThrew = false, updateThrew = false;
   try {
@@ -214,11 +223,14 @@ int main() {
   EXPECT(assignThrew && updateThrew);
 
   // BiCGStab stops on max |r| <= 1e-5 (XSLAE.cpp:669, 712)
-  EXPECT(resPack <= 1e-4 && resGrid <= 1e-4 && resStep2 <= 1e-4);
+  EXPECT(resPack <= 1e-4 && resGrid <= 1e-4 && resStep2 <= 1e-4 && resFused <= 1e-4);
+  EXPECT(itFused > 0);
   EXPECT(diff <= 1e-5 * scale);  // both stop at max |r| <= 1e-5: their fields differ by about that
   EXPECT(d63 == 0);
   printf("{\"unknowns\": %d, \"it_pack\": %d, \"it_grid\": %d, \"res_pack\": %.3e, \"res_grid\": %.3e, "
-         "\"res_step2\": %.3e, \"pack_vs_grid\": %.3e, \"six_vs_three\": %.3e, \"failures\": %d}\n",
-         unknowns, itPack, itGrid, resPack, resGrid, resStep2, scale > 0 ? diff / scale : diff, d63, g_failures);
+         "\"res_step2\": %.3e, \"res_fused\": %.3e, \"it_fused\": %d, \"pack_vs_grid\": %.3e, \"six_vs_three\": %.3e, "
+         "\"failures\": %d}\n",
+         unknowns, itPack, itGrid, resPack, resGrid, resStep2, resFused, itFused, scale > 0 ? diff / scale : diff, d63,
+         g_failures);
   return g_failures ? 1 : 0;
 }

@@ -51,4 +51,5 @@ def test_dropin_pressure_solve_call_forms():
     res = json.loads(p.stdout.strip().splitlines()[-1])
     assert p.returncode == 0 and res["failures"] == 0, res
     assert res["res_pack"] <= 1e-4 and res["res_grid"] <= 1e-4 and res["res_step2"] <= 1e-4
+    assert res["res_fused"] <= 1e-4  # SolveByFastBiConjugateGradientsStab on the step-2 system
     assert res["pack_vs_grid"] <= 1e-5 and res["six_vs_three"] == 0

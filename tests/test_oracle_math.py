@@ -520,3 +520,27 @@ def test_mg_coarse_solve_solves_the_level(dirichlet):
         rhs = rhs - rhs.mean()
         assert abs(z[m].astype(np.float64).mean()) <= 1e-6 * np.abs(z[m]).max()
     assert np.linalg.norm(Az - rhs) <= 1e-5 * np.linalg.norm(rhs)
+
+
+def test_lincomb_and_copy_special_cases():
+    """xo_lincomb and xo_copy as the reference's masked vector statements
+    (XSpace.cpp:2293-2363, 2489-3010): only unknowns are written; a zero
+    coefficient drops its vector entirely (an Inf there does not reach the
+    result), both zero give 0; x |= s copies at unknowns only."""
+    codes = np.zeros((5, 6, 7), dtype=np.int32)
+    codes[1:-1, 1:-1, 1:-1] = 1
+    h = np.full(4, 0.25), np.full(5, 0.2), np.full(6, 0.1)
+    g = xo.Grid(*h, codes)
+    rng = np.random.default_rng(2)
+    s1, s2 = rng.standard_normal(codes.shape), rng.standard_normal(codes.shape)
+    s2_inf = s2.copy()
+    s2_inf[2, 2, 2] = np.inf
+    unk = codes == 1
+    for p1, p2, b, want in ((1.5, -0.5, s2, 1.5 * s1 + -0.5 * s2), (2.0, 0.0, s2_inf, 2.0 * s1),
+                            (0.0, 3.0, s2, 3.0 * s2), (0.0, 0.0, s2_inf, np.zeros_like(s1))):
+        res = np.full(codes.shape, 7.0)
+        xo.lib().xo_lincomb(g.ref, p1, s1.ctypes.data, p2, b.ctypes.data, res.ctypes.data)
+        assert np.array_equal(res[unk], want[unk]) and np.all(res[~unk] == 7.0), (p1, p2)
+    res = np.full(codes.shape, 7.0)
+    xo.lib().xo_copy(g.ref, s1.ctypes.data, res.ctypes.data)
+    assert np.array_equal(res[unk], s1[unk]) and np.all(res[~unk] == 7.0)

@@ -480,11 +480,14 @@ class TSLAE {
     solve(XF_CG_MP, rel_tol, project_mean);
   }
   // flexible CG in FP64 refinement with the V-cycle preconditioner; `net` (an
-  // xf_unet, e.g. with saved parameters loaded) is its coarsest-grid solve, or
-  // nullptr for damped-Jacobi sweeps there
+  // xf_unet, e.g. with saved parameters loaded) is the solve of the cycle cut
+  // at 64^3, or nullptr for the full-depth classical cycle (the faster one)
   void SolveByLearnedPreconditioner(xf_unet net, double rel_tol = 1e-8, bool project_mean = true) {
     solve(XF_FCG_MGNN, rel_tol, project_mean, net);
   }
+  // the reference's BiCGStab (recurrence, max-norm stop at its 1e-5 absolute
+  // tolerance, operator rows) in five fused passes per iteration
+  void SolveByFastBiConjugateGradientsStab() { solve(XF_BICGSTAB_T); }
   const xf_solve_stats &LastStats() const { return stats_; }
 
  private:
