@@ -21,10 +21,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# measured HBM traffic of one k_cgcg_step<double,double> launch at 256^3 (mean
-# of an even and an odd iteration; rocprofv3 counters scaled by a copy of
-# known size, as recorded in profiles/cg_kernels_traffic_pmc.md)
-KB_TRAFFIC_BYTES = 772.1e6
+# measured HBM traffic of one k_cgcg_fused<double,double,4,10,1> launch at
+# 256^3 (mean of an even and an odd iteration; rocprofv3 counters scaled by a
+# copy of known size, as recorded in profiles/cg_kernels_traffic_pmc.md)
+KF_TRAFFIC_BYTES = None
 
 
 def log(*a):
@@ -330,16 +330,21 @@ def main():
         log("output files: flow VTK %.1f MB in %.2f s, .npy %.1f MB in %.3f s" % (vtk_mb, vtk_s, npy_mb, npy_s))
 
     interior = (args.n) ** 3
+    # algorithmic bytes per interior cell of one iteration's kernel: r, p_old
+    # and the flag byte read, p and r written (33 B), plus x read and written
+    # on odd iterations (49 B; the sampled launches alternate, so the mean is
+    # 41 B).  The fused kernel forms the next iteration's dot products from
+    # the same bytes; where the rows are too long for it, a separate dots
+    # kernel reads r and the flags again (9 B).
+    kf = kstats.get("cg_fused_fp64", {"ms": float("nan"), "count": 0})
     ka = kstats.get("cg_dots_fp64", {"ms": float("nan"), "count": 0})
     kb = kstats.get("cg_step_fp64", {"ms": float("nan"), "count": 0})
-    # algorithmic bytes per interior cell: KA reads r and the flag byte (9 B);
-    # KB reads r, p and the flag byte and writes p and r (33 B), plus x read and
-    # written on odd iterations (49 B; the sampled launches alternate, so the
-    # mean is 41 B)
-    ka_bytes = 9.0 * interior
-    kb_bytes = 41.0 * interior
-    ka_gbs = ka_bytes / (ka["ms"] * 1e-3) / 1e9
-    kb_gbs = kb_bytes / (kb["ms"] * 1e-3) / 1e9
+    fused = kf["count"] > 0
+    kd = kf if fused else kb
+    kd_bytes = 41.0 * interior
+    kd_gbs = kd_bytes / (kd["ms"] * 1e-3) / 1e9
+    it_bytes = kd_bytes if fused else 50.0 * interior
+    it_us = kd["ms"] * 1e3 if fused else (ka["ms"] + kb["ms"]) * 1e3
     it_mean = plain["iterations"]
     mcups = fluid * it_mean / (wall_ms * 1e-3) / 1e6
 
@@ -367,27 +372,27 @@ def main():
         "data": "synthetic",
         "config": {"workload": "pressure Poisson %d^3, random obstacles seed %d, solve to 1e-8 relative residual" %
                                (args.n, args.seed),
-                   "value_solver": "plain FP64 CG (single-reduction, A p recomputed, two kernels per iteration)", "iterations": it_mean,
+                   "value_solver": "plain FP64 CG (single-reduction, A p recomputed, %s)" % (
+                       "one kernel per iteration" if fused else "two kernels per iteration"), "iterations": it_mean,
                    "fluid_cells": fluid, "mcell_updates_per_s": round(mcups, 1),
                    "solvers": solvers},
-        "roofline": {"bound": "hbm", "achieved": round(kb_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(kb_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": KB_TRAFFIC_BYTES if args.n == 256 else None,
+        "roofline": {"bound": "hbm", "achieved": round(kd_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(kd_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": KF_TRAFFIC_BYTES if (args.n == 256 and fused) else None,
                      "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by "
                                      "a 512 MiB copy (profiles/cg_kernels_traffic_pmc.md); 688 MB algorithmic, "
-                                     "the excess is the blocks' halo rows and planes read again",
-                     "kernel": "k_cgcg_step<double,double,2> (41 B per interior cell, mean of 33 and 49)",
-                     "kernel_us": round(kb["ms"] * 1e3, 2), "launches": kb["count"],
-                     "dots_kernel": "k_cgcg_dots<double,2> (9 B per interior cell)",
-                     "dots_gbs": round(ka_gbs, 1), "dots_us": round(ka["ms"] * 1e3, 2),
-                     "iteration_bytes_per_cell": 50,
-                     "iteration_gbs": round((ka_bytes + kb_bytes) / ((ka["ms"] + kb["ms"]) * 1e-3) / 1e9, 1),
+                                     "the excess is the halo rows and planes read again",
+                     "kernel": ("k_cgcg_fused<double,double,4,10,1>" if fused else "k_cgcg_step<double,double,2>") +
+                               " (41 B per interior cell, mean of 33 and 49)",
+                     "kernel_us": round(kd["ms"] * 1e3, 2), "launches": kd["count"],
+                     "iteration_bytes_per_cell": it_bytes / interior,
+                     "iteration_us": round(it_us, 2),
+                     "iteration_gbs": round(it_bytes / (it_us * 1e-6) / 1e9, 1),
                      "stream_ceiling_gbs": ceiling,
                      "stream_ceiling_note": "grid-stride 16-byte streams over 1 GiB buffers on the same GPU in the "
-                                            "same run: read-only reduction (the dots kernel's pattern), copy, and "
-                                            "two reads + two writes in place (the update kernel's pattern)",
-                     "kernel_frac_of_read2_write2_ceiling": round(kb_gbs / ceiling["read2_write2"], 3),
-                     "dots_frac_of_read_ceiling": round(ka_gbs / ceiling["read"], 3)},
+                                            "same run: read-only reduction, copy, and two reads + two writes in "
+                                            "place (the iteration kernel's pattern)",
+                     "kernel_frac_of_read2_write2_ceiling": round(kd_gbs / ceiling["read2_write2"], 3)},
         "cpu_baseline": cpu,
     }
     if port is not None:

@@ -463,6 +463,207 @@ __global__ __launch_bounds__(256) void k_cgcg_flush(int64_t nc, X *__restrict__ 
     x[c] = x[c] + a * (X)p[c];
 }
 
+// ------------------------------------------ plain CG: one kernel per iteration
+// Where one wave covers a whole row, KA's dot products move into the update
+// kernel.  Lane L holds cells OFF + L CPL .. OFF + L CPL + CPL - 1 of the row:
+// with OFF = 0 the whole row, ring included (m <= 64 CPL, a multiple of
+// CPL); with OFF = 1 its interior (m - 2 <= 64 CPL, a multiple of CPL), the
+// two ring cells beside it being zeros in p and r'.  KF applies
+// update `it` with KB's arithmetic (the same bits in p, r and x) and then
+// forms gamma = (r', r') and delta = (A r', r') of the new residual r' for
+// update it + 1, so KA runs once, for update 0.  A r' on a row needs r' on
+// the rows beside it, and r' there needs p one row further out: a block of
+// W waves computes p, A p and r' on rows j0 - 1 .. j0 + W - 2 and stores and
+// counts the inner TJ = W - 2 of them; the two outer waves also form p on
+// rows j0 - 2 and j0 + W - 1.  In i the march runs two planes ahead of its
+// dots (r' at plane s + 1 needs p at plane s + 2), starting two planes
+// before the tile (recomputing what the neighbouring tile stores, with the
+// same operations and so the same bits).  The k-neighbours are all in the
+// wave or on the ring (the ring cells are never Actual, so with OFF = 0
+// their own stencil values are never used).  p and r' go to LDS once per plane each,
+// double-buffered by plane parity, one barrier per plane.
+//   reads  r, p_old (W + 2 rows per TJ stored), flags (W rows), x (odd updates)
+//   writes p, r' (TJ rows), x (odd updates)              [41 B per cell, mean]
+struct FTiles {
+  int jt, it, ic, total;
+  int blocks;  // 8 * ceil(total / 8)
+};
+
+template <typename T, typename X, int CPL, int W, int OFF>
+__global__ __launch_bounds__(64 * W) void k_cgcg_fused(int n, int l, int m, FTiles tl, int npart,
+                                                        const double *__restrict__ part_in,
+                                                        double *__restrict__ part_out,
+                                                        const uint8_t *__restrict__ flags, const T *__restrict__ cx,
+                                                        const T *__restrict__ cy, const T *__restrict__ cz,
+                                                        X *__restrict__ x, const T *__restrict__ r,
+                                                        T *__restrict__ rout, const T *__restrict__ pin,
+                                                        T *__restrict__ pout, DevState *st, int it, int maxit) {
+  constexpr int TJ = W - 2;
+  using V = CgVec<T, CPL>;
+  using VX = CgVec<X, CPL>;
+  __shared__ double lds[W];
+  __shared__ V lp[2][W + 2][64];  // p(s + 1) on rows j0 - 2 + q
+  __shared__ V lr[2][W][64];      // r'(s + 1) on rows j0 - 1 + q
+  if (st->done) return;
+  double gam = 0, del = 0;
+  for (int q = threadIdx.x; q < npart; q += blockDim.x) {
+    gam += part_in[2 * q];
+    del += part_in[2 * q + 1];
+  }
+  gam = block_sum_all(gam, lds);
+  del = block_sum_all(del, lds);
+  double alpha, beta;
+  if (cg_decide(gam, del, st, it, maxit, alpha, beta)) return;
+  const T a = (T)alpha, bt = (T)beta;
+  const bool xup = it & 1;
+  const bool first = it == 0;
+  const X aold = xup ? (X)st->s[CG_ALPHA + (it & 1)] : X(0);
+  const int per = (tl.total + 7) / 8;
+  const int bx = gridDim.x >> 3;
+  const int xcd = blockIdx.x & 7;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t sI = (int64_t)l * m;
+  // the lane's cells; lanes past the row end repeat its last vector and own nothing
+  const bool kin = lane * CPL < m - 2 * OFF;
+  const int k0 = kin ? OFF + lane * CPL : m - OFF - CPL;
+  const bool kfirst = OFF && lane == 0, klast = OFF && lane == (m - 2) / CPL - 1;  // beside the ring
+  bool kown[CPL];  // interior cell of the row
+  bool kall = kin;
+  T czm[CPL], czp[CPL];
+#pragma unroll
+  for (int e = 0; e < CPL; e++) {
+    kown[e] = kin && k0 + e >= 1 && k0 + e <= m - 2;
+    kall = kall && kown[e];
+    czm[e] = cz[2 * (k0 + e)];
+    czp[e] = cz[2 * (k0 + e) + 1];
+  }
+  auto pnew = [&](int64_t at, V &rv, V &po) {
+    V q;
+    __builtin_memcpy(&rv, r + at, sizeof(V));
+    if (first) {
+#pragma unroll
+      for (int e = 0; e < CPL; e++) po.v[e] = T(0);
+    } else {
+      __builtin_memcpy(&po, pin + at, sizeof(V));
+    }
+#pragma unroll
+    for (int e = 0; e < CPL; e++) q.v[e] = rv.v[e] + bt * po.v[e];
+    return q;
+  };
+  auto pl = [&](int i) { return (int64_t)min(max(i, 0), n - 1) * sI; };
+  double g2 = 0, d2 = 0;
+  for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
+    const int t = xcd * per + tq;
+    if (t >= tl.total) break;
+    const int jtile = t % tl.jt, itile = t / tl.jt;
+    const int j0 = 1 + jtile * TJ;
+    const int jw = j0 - 1 + w;           // the wave's row (rows past the array repeat its last)
+    const int jl = min(jw, l - 1);
+    const bool rown = w >= 1 && w <= TJ && jw <= l - 2;
+    const bool outer = w == 0 || w == W - 1;
+    const int jo = w == 0 ? max(j0 - 2, 0) : min(j0 + TJ + 1, l - 1);
+    const int i0 = 1 + itile * tl.ic, i1 = min(i0 + tl.ic, n - 1);
+    const T cym = cy[2 * jl], cyp = cy[2 * jl + 1];
+    const int64_t rowc = (int64_t)jl * m + k0, rowo = (int64_t)jo * m + k0;
+    // march state of the own row: p at planes s, s + 1; r at s + 1; r' at
+    // s - 1, s; flags at s, s + 1
+    V pA, pB, rB, qA{}, qB{}, t1, t2;
+    uint8_t fA[CPL] = {}, fB[CPL];
+    pA = pnew(pl(i0 - 2) + rowc, t1, t2);
+    pB = pnew(pl(i0 - 1) + rowc, rB, t2);
+    __builtin_memcpy(fB, flags + pl(i0 - 1) + rowc, CPL);
+    for (int s = i0 - 2; s < i1; s++) {
+      const int par = (s - i0) & 1;
+      // 1. plane s + 2 of the own row (and its x, which needs no stencil)
+      V rC, poC;
+      const V pC = pnew(pl(s + 2) + rowc, rC, poC);
+      uint8_t fC[CPL];
+      __builtin_memcpy(fC, flags + pl(s + 2) + rowc, CPL);
+      if (xup && rown && s + 2 >= i0 && s + 2 < i1) {
+        const int64_t c = pl(s + 2) + rowc;
+        VX xx;
+        __builtin_memcpy(&xx, x + c, sizeof(VX));
+#pragma unroll
+        for (int e = 0; e < CPL; e++) xx.v[e] = (xx.v[e] + aold * (X)poC.v[e]) + (X)a * (X)pC.v[e];
+        if (kall) {
+          __builtin_memcpy(x + c, &xx, sizeof(VX));
+        } else {
+#pragma unroll
+          for (int e = 0; e < CPL; e++)
+            if (kown[e]) x[c + e] = xx.v[e];
+        }
+      }
+      V po{};
+      if (outer) po = pnew(pl(s + 1) + rowo, t1, t2);
+      // 2. p(s + 1) of every row to LDS
+      lp[par][w + 1][lane] = pB;
+      if (w == 0) lp[par][0][lane] = po;
+      if (w == W - 1) lp[par][W + 1][lane] = po;
+      __syncthreads();
+      // 3. A p and r' at plane s + 1
+      V qC;
+      {
+        const V pjm = lp[par][w][lane], pjp = lp[par][w + 2][lane];
+        T left = __shfl_up(pB.v[CPL - 1], 1, 64), right = __shfl_down(pB.v[0], 1, 64);
+        if (kfirst) left = T(0);
+        if (klast) right = T(0);
+        const int ip = min(max(s + 1, 0), n - 1);
+        const T cxm = cx[2 * ip], cxp = cx[2 * ip + 1];
+#pragma unroll
+        for (int e = 0; e < CPL; e++) {
+          const T sv = cg_stencil_cell<T, CPL>(fB[e], e, pB, pA, pC, pjm.v[e], pjp.v[e], left, right, cxm, cxp,
+                                               cym, cyp, czm[e], czp[e]);
+          qC.v[e] = rB.v[e] - a * sv;
+        }
+      }
+      // the ring cells of a stored row hold zeros in r and p_old, so p and
+      // r' there are zeros too and the whole row is stored
+      if (rown && kin && s + 1 >= i0 && s + 1 < i1) {
+        const int64_t c = pl(s + 1) + rowc;
+        __builtin_memcpy(pout + c, &pB, sizeof(V));
+        __builtin_memcpy(rout + c, &qC, sizeof(V));
+      }
+      // 4. r'(s + 1) to LDS for the next plane's dots
+      lr[par][w][lane] = qC;
+      // 5. (r', r') and (A r', r') at plane s, from r'(s) on the rows
+      // beside (published at the previous plane, before this plane's barrier)
+      if (s >= i0 && rown) {
+        const V qjm = lr[par ^ 1][w - 1][lane], qjp = lr[par ^ 1][w + 1][lane];
+        T left = __shfl_up(qB.v[CPL - 1], 1, 64), right = __shfl_down(qB.v[0], 1, 64);
+        if (kfirst) left = T(0);
+        if (klast) right = T(0);
+        const T cxm = cx[2 * s], cxp = cx[2 * s + 1];
+#pragma unroll
+        for (int e = 0; e < CPL; e++) {
+          const T v = cg_stencil_cell<T, CPL>(fA[e], e, qB, qA, qC, qjm.v[e], qjp.v[e], left, right, cxm, cxp,
+                                              cym, cyp, czm[e], czp[e]);
+          if ((fA[e] & FL_ACTUAL) && kown[e]) {
+            g2 += (double)qB.v[e] * (double)qB.v[e];
+            d2 += (double)v * (double)qB.v[e];
+          }
+        }
+      }
+      pA = pB;
+      pB = pC;
+      rB = rC;
+      qA = qB;
+      qB = qC;
+#pragma unroll
+      for (int e = 0; e < CPL; e++) {
+        fA[e] = fB[e];
+        fB[e] = fC[e];
+      }
+    }
+    __syncthreads();  // the next tile reuses both LDS buffers
+  }
+  g2 = block_sum_all(g2, lds);
+  d2 = block_sum_all(d2, lds);
+  if (threadIdx.x == 0) {
+    part_out[2 * blockIdx.x] = g2;
+    part_out[2 * blockIdx.x + 1] = d2;
+  }
+}
+
 // ------------------------------------- refinement residual and mean projection
 // Fence-free reductions: each kernel writes kRefPart doubles per block into
 // the context's partial buffer, and k_ref_finish (one block) adds them in
@@ -746,6 +947,118 @@ static void cgcg_run(xf_ctx c, xf_grid g, X *x, T *const R[2], T *const P[2], co
   check_kernel("cg flush");
 }
 
+// KF's shape: W rows per block (TJ = W - 2 of them stored), about one tile
+// per resident block.  Timing lab at 256^3 (258^3 array): FP64 at 4 cells
+// per lane needs 150-odd VGPRs, three waves per SIMD, so one block of 10
+// rows per CU (32 x 8 tiles, 180 us per iteration; 8 rows: 43 x 5 tiles,
+// 214 us); FP32 one block of 16 rows per CU (19 x 13 tiles, 96 us; 10 rows
+// two blocks per CU: 113 us; 8 rows: 102 us).
+template <typename T, int CPL>
+constexpr int fused_rows() {
+  return sizeof(T) == 8 && CPL == 4 ? 10 : 16;
+}
+constexpr int kFusedBlocks = 1024;  // KF blocks at most: two partial buffers of this many fit the KA buffer
+
+// The vector width (CPL) and offset (OFF) at which one wave covers a whole
+// row: 2 CPL + OFF, or 0 where none does.
+static int fused_layout(int m) {
+  for (int cpl = 1; cpl <= 4; cpl *= 2) {
+    if (m - 2 <= 64 * cpl && (m - 2) % cpl == 0) return 2 * cpl + 1;
+    if (m <= 64 * cpl && m % cpl == 0) return 2 * cpl;
+  }
+  return 0;
+}
+
+template <typename T, int CPL>
+static int dots_launch(xf_ctx c, xf_grid g, const T *cx, const T *cy, const T *cz, const T *r, double *part,
+                       int maxblocks) {
+  const Tiles ta = march_tiles(g, CPL, CgShape<T>::max_planes);
+  const int nb = std::min(ta.blocks, maxblocks);
+  hipLaunchKernelGGL((k_cgcg_dots<T, CPL>), dim3(nb), dim3(256), 0, c->stream, g->n, g->l, g->m, ta, g->d_flags, cx,
+                     cy, cz, r, c->state, part);
+  return nb;
+}
+
+// KA over r with the vector width the plain-CG kernels use for this row length.
+template <typename T>
+static int dots_once(xf_ctx c, xf_grid g, const T *cx, const T *cy, const T *cz, const T *r, double *part,
+                     int maxblocks) {
+  if constexpr (sizeof(T) == 4) {
+    if (g->m - 2 >= 64 * 4) return dots_launch<T, 4>(c, g, cx, cy, cz, r, part, maxblocks);
+  }
+  if (g->m - 2 >= 64 * 2) return dots_launch<T, 2>(c, g, cx, cy, cz, r, part, maxblocks);
+  return dots_launch<T, 1>(c, g, cx, cy, cz, r, part, maxblocks);
+}
+
+template <typename T, typename X, int CPL, int OFF>
+static void cgcg_run_fused(xf_ctx c, xf_grid g, X *x, T *const R[2], T *const P[2], const T *cx, const T *cy,
+                           const T *cz, double *part, int maxit, int every) {
+  constexpr int W = fused_rows<T, CPL>(), TJ = W - 2;
+  static const int resident = [] {
+    int dev = 0, cus = 0, per = 0;
+    XF_HIP(hipGetDevice(&dev));
+    XF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    XF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_cgcg_fused<T, X, CPL, W, OFF>, 64 * W, 0));
+    return std::max(1, cus * std::max(per, 1));
+  }();
+  FTiles tf;
+  tf.jt = (g->l - 2 + TJ - 1) / TJ;
+  const int want_it = std::max(1, resident / tf.jt);
+  tf.ic = (g->n - 2 + want_it - 1) / want_it;
+  tf.it = (g->n - 2 + tf.ic - 1) / tf.ic;
+  tf.total = tf.jt * tf.it;
+  tf.blocks = 8 * ((tf.total + 7) / 8);
+  const int nbF = std::min(tf.blocks, kFusedBlocks);
+  double *const pb[2] = {part, part + 2 * kFusedBlocks};
+  const bool f64 = sizeof(T) == 8;
+  const char *n1 = f64 ? "cg_dots_fp64" : "cg_dots_fp32";
+  const char *n2 = f64 ? "cg_fused_fp64" : "cg_fused_fp32";
+  constexpr int kProfEvery = 8;
+  std::vector<ProfMark> marks;
+  std::vector<int> sampled;
+  int e0 = c->prof ? prof_begin(c) : -1;
+  const int nbA = dots_once<T>(c, g, cx, cy, cz, R[0], pb[0], kFusedBlocks);
+  if (c->prof) {
+    prof_end(c, e0);
+    marks.push_back({n1, e0});
+  }
+  long long na = 1;
+  int it = 0;
+  for (;;) {
+    for (int q = 0; q < every; q++, it++) {
+      const bool pm = c->prof && it % kProfEvery < 2;
+      int e1 = pm ? prof_begin(c) : -1;
+      hipLaunchKernelGGL((k_cgcg_fused<T, X, CPL, W, OFF>), dim3(nbF), dim3(64 * W), 0, c->stream, g->n, g->l, g->m, tf,
+                         it == 0 ? nbA : nbF, pb[it & 1], pb[(it + 1) & 1], g->d_flags, cx, cy, cz, x, R[it & 1],
+                         R[(it + 1) & 1], P[it & 1], P[(it + 1) & 1], c->state, it, maxit);
+      if (pm) {
+        prof_end(c, e1);
+        marks.push_back({n2, e1});
+        sampled.push_back(it);
+      }
+    }
+    check_kernel("cg iterations");
+    XF_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+    XF_HIP(hipStreamSynchronize(c->stream));
+    if (c->prof) {
+      // the deciding launch only reduced: the updates before it are counted
+      const long long ran = c->h_state->iter;
+      long long nf = 0;
+      for (int v : sampled)
+        if (v < ran) nf++;
+      prof_collect(c, marks, {{n1, na}, {n2, nf}});
+      na = 0;
+      sampled.clear();
+    }
+    if (c->h_state->done) break;
+  }
+  const int applied = c->h_state->iter;
+  if (applied & 1)
+    hipLaunchKernelGGL((k_cgcg_flush<T, X>), dim3(stream_nb(g->ncell)), dim3(256), 0, c->stream, g->ncell, x,
+                       P[applied & 1], c->state);
+  check_kernel("cg flush");
+}
+
 // Runs single-reduction CG iterations on A e = r from p = r (r zero-padded,
 // in r0; x the FP64 or FP32 accumulator; r1, p0, p1 work vectors, zero on
 // the array's outer ring; part the dot kernel's partials).  State
@@ -757,6 +1070,16 @@ static void cgcg_iterations(xf_ctx c, xf_grid g, X *x, T *r0, T *r1, T *p0, T *p
                             const T *cz, double *part, int maxit, int every) {
   T *const R[2] = {r0, r1};
   T *const P[2] = {p0, p1};
+  // one kernel per iteration where a wave holds a whole row
+  switch (fused_layout(g->m)) {
+    case 2: return cgcg_run_fused<T, X, 1, 0>(c, g, x, R, P, cx, cy, cz, part, maxit, every);
+    case 3: return cgcg_run_fused<T, X, 1, 1>(c, g, x, R, P, cx, cy, cz, part, maxit, every);
+    case 4: return cgcg_run_fused<T, X, 2, 0>(c, g, x, R, P, cx, cy, cz, part, maxit, every);
+    case 5: return cgcg_run_fused<T, X, 2, 1>(c, g, x, R, P, cx, cy, cz, part, maxit, every);
+    case 8: return cgcg_run_fused<T, X, 4, 0>(c, g, x, R, P, cx, cy, cz, part, maxit, every);
+    case 9: return cgcg_run_fused<T, X, 4, 1>(c, g, x, R, P, cx, cy, cz, part, maxit, every);
+    default: break;
+  }
   // 16-byte vectors per lane where a row holds a whole wave of them
   if constexpr (sizeof(T) == 4) {
     if (g->m - 2 >= 64 * 4) return cgcg_run<T, X, 4>(c, g, x, R, P, cx, cy, cz, part, maxit, every);

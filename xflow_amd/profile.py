@@ -3,7 +3,7 @@ import ctypes as C
 
 from ._lib import lib
 
-KERNELS = ("cg_dots_fp64", "cg_step_fp64", "cg_dots_fp32", "cg_step_fp32",
+KERNELS = ("cg_dots_fp64", "cg_step_fp64", "cg_fused_fp64", "cg_dots_fp32", "cg_step_fp32", "cg_fused_fp32",
            "mp_residual_fp64", "mp_correct")
 
 
