@@ -39,6 +39,7 @@
 // order, partials added in block order.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <mutex>
 #include <string>
 
@@ -473,15 +474,21 @@ __global__ __launch_bounds__(256) void k_cgcg_flush(int64_t nc, X *__restrict__ 
 // forms gamma = (r', r') and delta = (A r', r') of the new residual r' for
 // update it + 1, so KA runs once, for update 0.  A r' on a row needs r' on
 // the rows beside it, and r' there needs p one row further out: a block of
-// W waves computes p, A p and r' on rows j0 - 1 .. j0 + W - 2 and stores and
-// counts the inner TJ = W - 2 of them; the two outer waves also form p on
-// rows j0 - 2 and j0 + W - 1.  In i the march runs two planes ahead of its
-// dots (r' at plane s + 1 needs p at plane s + 2), starting two planes
-// before the tile (recomputing what the neighbouring tile stores, with the
-// same operations and so the same bits).  The k-neighbours are all in the
-// wave or on the ring (the ring cells are never Actual, so with OFF = 0
-// their own stencil values are never used).  p and r' go to LDS once per plane each,
-// double-buffered by plane parity, one barrier per plane.
+// W + 2 waves has one row each, rows j0 - 2 .. j0 + W - 1.  All of them form
+// p; the inner W also form A p and r' (rows j0 - 1 .. j0 + W - 2), and the
+// inner TJ = W - 2 of those store p and r' and count the dots.  In i the
+// march runs two planes ahead of its dots (r' at plane s + 1 needs p at plane
+// s + 2), starting two planes before the tile (recomputing what the
+// neighbouring tile stores, with the same operations and so the same bits).
+// The k-neighbours are all in the wave or on the ring (the ring cells are
+// never Actual, so with OFF = 0 their own stencil values are never used).
+// p and r' go to LDS once per plane each, double-buffered by plane parity,
+// one barrier per plane.  The loads of plane s + 3 (r, p_old, the flag
+// bytes, x) are issued before the work on plane s + 1 and consumed one plane
+// later, so they are in flight across the barrier and the stencils; the flag
+// bytes stay packed in one integer until the stencil takes them apart (a use
+// right after the load would make the wave wait for every load before it).
+// The lane's k coefficients sit in LDS, not in registers.
 //   reads  r, p_old (W + 2 rows per TJ stored), flags (W rows), x (odd updates)
 //   writes p, r' (TJ rows), x (odd updates)              [41 B per cell, mean]
 struct FTiles {
@@ -489,8 +496,18 @@ struct FTiles {
   int blocks;  // 8 * ceil(total / 8)
 };
 
+// The flag bytes of a lane's CPL cells as one integer.
+template <int CPL>
+struct CgFlagWord {
+  using type = typename std::conditional<CPL == 4, uint32_t, typename std::conditional<CPL == 2, uint16_t, uint8_t>::type>::type;
+};
+template <typename W>
+__device__ inline uint8_t flag_byte(W w, int e) {
+  return (uint8_t)(w >> (8 * e));
+}
+
 template <typename T, typename X, int CPL, int W, int OFF>
-__global__ __launch_bounds__(64 * W) void k_cgcg_fused(int n, int l, int m, FTiles tl, int npart,
+__global__ __launch_bounds__(64 * (W + 2)) void k_cgcg_fused(int n, int l, int m, FTiles tl, int npart,
                                                         const double *__restrict__ part_in,
                                                         double *__restrict__ part_out,
                                                         const uint8_t *__restrict__ flags, const T *__restrict__ cx,
@@ -501,9 +518,10 @@ __global__ __launch_bounds__(64 * W) void k_cgcg_fused(int n, int l, int m, FTil
   constexpr int TJ = W - 2;
   using V = CgVec<T, CPL>;
   using VX = CgVec<X, CPL>;
-  __shared__ double lds[W];
-  __shared__ V lp[2][W + 2][64];  // p(s + 1) on rows j0 - 2 + q
-  __shared__ V lr[2][W][64];      // r'(s + 1) on rows j0 - 1 + q
+  __shared__ double lds[W + 2];
+  __shared__ V lp[2][W + 2][64];  // p(s + 1) on rows j0 - 2 + u
+  __shared__ V lr[2][W][64];      // r'(s + 1) on rows j0 - 1 + w
+  __shared__ V lcz[2][64];        // the lane's k coefficients (kept out of registers)
   if (st->done) return;
   double gam = 0, del = 0;
   for (int q = threadIdx.x; q < npart; q += blockDim.x) {
@@ -521,35 +539,30 @@ __global__ __launch_bounds__(64 * W) void k_cgcg_fused(int n, int l, int m, FTil
   const int per = (tl.total + 7) / 8;
   const int bx = gridDim.x >> 3;
   const int xcd = blockIdx.x & 7;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, u = threadIdx.x >> 6, w = u - 1;
+  const bool dotrow = u >= 1 && u <= W;  // waves 0 and W + 1 only form p on the halo rows
   const int64_t sI = (int64_t)l * m;
-  // the lane's cells; lanes past the row end repeat its last vector and own nothing
   const bool kin = lane * CPL < m - 2 * OFF;
   const int k0 = kin ? OFF + lane * CPL : m - OFF - CPL;
-  const bool kfirst = OFF && lane == 0, klast = OFF && lane == (m - 2) / CPL - 1;  // beside the ring
-  bool kown[CPL];  // interior cell of the row
+  const bool kfirst = OFF && lane == 0, klast = OFF && lane == (m - 2) / CPL - 1;
+  bool kown[CPL];
   bool kall = kin;
-  T czm[CPL], czp[CPL];
 #pragma unroll
   for (int e = 0; e < CPL; e++) {
     kown[e] = kin && k0 + e >= 1 && k0 + e <= m - 2;
     kall = kall && kown[e];
-    czm[e] = cz[2 * (k0 + e)];
-    czp[e] = cz[2 * (k0 + e) + 1];
   }
-  auto pnew = [&](int64_t at, V &rv, V &po) {
-    V q;
-    __builtin_memcpy(&rv, r + at, sizeof(V));
-    if (first) {
+  if (u == 0) {
+    V zm, zp;
 #pragma unroll
-      for (int e = 0; e < CPL; e++) po.v[e] = T(0);
-    } else {
-      __builtin_memcpy(&po, pin + at, sizeof(V));
+    for (int e = 0; e < CPL; e++) {
+      zm.v[e] = cz[2 * (k0 + e)];
+      zp.v[e] = cz[2 * (k0 + e) + 1];
     }
-#pragma unroll
-    for (int e = 0; e < CPL; e++) q.v[e] = rv.v[e] + bt * po.v[e];
-    return q;
-  };
+    lcz[0][lane] = zm;
+    lcz[1][lane] = zp;
+  }
+  __syncthreads();
   auto pl = [&](int i) { return (int64_t)min(max(i, 0), n - 1) * sI; };
   double g2 = 0, d2 = 0;
   for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
@@ -557,34 +570,61 @@ __global__ __launch_bounds__(64 * W) void k_cgcg_fused(int n, int l, int m, FTil
     if (t >= tl.total) break;
     const int jtile = t % tl.jt, itile = t / tl.jt;
     const int j0 = 1 + jtile * TJ;
-    const int jw = j0 - 1 + w;           // the wave's row (rows past the array repeat its last)
-    const int jl = min(jw, l - 1);
-    const bool rown = w >= 1 && w <= TJ && jw <= l - 2;
-    const bool outer = w == 0 || w == W - 1;
-    const int jo = w == 0 ? max(j0 - 2, 0) : min(j0 + TJ + 1, l - 1);
+    const int jrow = j0 - 2 + u;
+    const int jl = min(max(jrow, 0), l - 1);
+    const bool rown = w >= 1 && w <= TJ && jrow <= l - 2;
     const int i0 = 1 + itile * tl.ic, i1 = min(i0 + tl.ic, n - 1);
     const T cym = cy[2 * jl], cyp = cy[2 * jl + 1];
-    const int64_t rowc = (int64_t)jl * m + k0, rowo = (int64_t)jo * m + k0;
-    // march state of the own row: p at planes s, s + 1; r at s + 1; r' at
-    // s - 1, s; flags at s, s + 1
-    V pA, pB, rB, qA{}, qB{}, t1, t2;
-    uint8_t fA[CPL] = {}, fB[CPL];
-    pA = pnew(pl(i0 - 2) + rowc, t1, t2);
-    pB = pnew(pl(i0 - 1) + rowc, rB, t2);
-    __builtin_memcpy(fB, flags + pl(i0 - 1) + rowc, CPL);
+    const int64_t rowc = (int64_t)jl * m + k0;
+    // raw loads of one plane: r, p_old (zero on the first update), the
+    // flags (rows with dots) and x (stored rows on odd updates)
+    // the flags stay packed (one byte per cell) until the stencil takes them
+    // apart, so that no use right after their load makes it wait
+    using FW = typename CgFlagWord<CPL>::type;
+    V nr, np;
+    FW nf = 0;
+    VX nx;
+    auto fetch = [&](int i) {
+      const int64_t c = pl(i) + rowc;
+      __builtin_memcpy(&nr, r + c, sizeof(V));
+      if (first) {
+#pragma unroll
+        for (int e = 0; e < CPL; e++) np.v[e] = T(0);
+      } else {
+        __builtin_memcpy(&np, pin + c, sizeof(V));
+      }
+      if (dotrow) __builtin_memcpy(&nf, flags + c, CPL);
+      if (xup && rown && i >= i0 && i < i1) __builtin_memcpy(&nx, x + c, sizeof(VX));
+    };
+    V pA, pB, rB, qA{}, qB{};
+    FW fA = 0, fB = 0;
+    fetch(i0 - 2);
+#pragma unroll
+    for (int e = 0; e < CPL; e++) pA.v[e] = nr.v[e] + bt * np.v[e];
+    fetch(i0 - 1);
+#pragma unroll
+    for (int e = 0; e < CPL; e++) {
+      rB.v[e] = nr.v[e];
+      pB.v[e] = nr.v[e] + bt * np.v[e];
+    }
+    fB = nf;
+    fetch(i0);
     for (int s = i0 - 2; s < i1; s++) {
       const int par = (s - i0) & 1;
-      // 1. plane s + 2 of the own row (and its x, which needs no stencil)
-      V rC, poC;
-      const V pC = pnew(pl(s + 2) + rowc, rC, poC);
-      uint8_t fC[CPL];
-      __builtin_memcpy(fC, flags + pl(s + 2) + rowc, CPL);
+      // 1. plane s + 2 from the registers loaded one plane ago, and x there
+      // (two steps at once) before those registers take the next plane
+      V rC, pC;
+      const FW fC = nf;
+#pragma unroll
+      for (int e = 0; e < CPL; e++) {
+        rC.v[e] = nr.v[e];
+        pC.v[e] = nr.v[e] + bt * np.v[e];
+      }
       if (xup && rown && s + 2 >= i0 && s + 2 < i1) {
         const int64_t c = pl(s + 2) + rowc;
         VX xx;
-        __builtin_memcpy(&xx, x + c, sizeof(VX));
 #pragma unroll
-        for (int e = 0; e < CPL; e++) xx.v[e] = (xx.v[e] + aold * (X)poC.v[e]) + (X)a * (X)pC.v[e];
+        for (int e = 0; e < CPL; e++) xx.v[e] = (nx.v[e] + aold * (X)np.v[e]) + (X)a * (X)pC.v[e];
         if (kall) {
           __builtin_memcpy(x + c, &xx, sizeof(VX));
         } else {
@@ -593,53 +633,52 @@ __global__ __launch_bounds__(64 * W) void k_cgcg_fused(int n, int l, int m, FTil
             if (kown[e]) x[c + e] = xx.v[e];
         }
       }
-      V po{};
-      if (outer) po = pnew(pl(s + 1) + rowo, t1, t2);
-      // 2. p(s + 1) of every row to LDS
-      lp[par][w + 1][lane] = pB;
-      if (w == 0) lp[par][0][lane] = po;
-      if (w == W - 1) lp[par][W + 1][lane] = po;
+      // 2. loads of plane s + 3 stay in flight over this plane's work
+      if (s + 3 <= i1 + 1) fetch(s + 3);
+      // 4. p(s + 1) of every row to LDS
+      lp[par][u][lane] = pB;
       __syncthreads();
-      // 3. A p and r' at plane s + 1
-      V qC;
-      {
-        const V pjm = lp[par][w][lane], pjp = lp[par][w + 2][lane];
-        T left = __shfl_up(pB.v[CPL - 1], 1, 64), right = __shfl_down(pB.v[0], 1, 64);
-        if (kfirst) left = T(0);
-        if (klast) right = T(0);
-        const int ip = min(max(s + 1, 0), n - 1);
-        const T cxm = cx[2 * ip], cxp = cx[2 * ip + 1];
+      V qC{};
+      if (dotrow) {
+        // 5. A p and r' at plane s + 1
+        {
+          const V pjm = lp[par][w][lane], pjp = lp[par][w + 2][lane];
+          T left = __shfl_up(pB.v[CPL - 1], 1, 64), right = __shfl_down(pB.v[0], 1, 64);
+          if (kfirst) left = T(0);
+          if (klast) right = T(0);
+          const int ip = min(max(s + 1, 0), n - 1);
+          const T cxm = cx[2 * ip], cxp = cx[2 * ip + 1];
+          const V czm = lcz[0][lane], czp = lcz[1][lane];
 #pragma unroll
-        for (int e = 0; e < CPL; e++) {
-          const T sv = cg_stencil_cell<T, CPL>(fB[e], e, pB, pA, pC, pjm.v[e], pjp.v[e], left, right, cxm, cxp,
-                                               cym, cyp, czm[e], czp[e]);
-          qC.v[e] = rB.v[e] - a * sv;
+          for (int e = 0; e < CPL; e++) {
+            const T sv = cg_stencil_cell<T, CPL>(flag_byte(fB, e), e, pB, pA, pC, pjm.v[e], pjp.v[e], left, right, cxm, cxp,
+                                                 cym, cyp, czm.v[e], czp.v[e]);
+            qC.v[e] = rB.v[e] - a * sv;
+          }
         }
-      }
-      // the ring cells of a stored row hold zeros in r and p_old, so p and
-      // r' there are zeros too and the whole row is stored
-      if (rown && kin && s + 1 >= i0 && s + 1 < i1) {
-        const int64_t c = pl(s + 1) + rowc;
-        __builtin_memcpy(pout + c, &pB, sizeof(V));
-        __builtin_memcpy(rout + c, &qC, sizeof(V));
-      }
-      // 4. r'(s + 1) to LDS for the next plane's dots
-      lr[par][w][lane] = qC;
-      // 5. (r', r') and (A r', r') at plane s, from r'(s) on the rows
-      // beside (published at the previous plane, before this plane's barrier)
-      if (s >= i0 && rown) {
-        const V qjm = lr[par ^ 1][w - 1][lane], qjp = lr[par ^ 1][w + 1][lane];
-        T left = __shfl_up(qB.v[CPL - 1], 1, 64), right = __shfl_down(qB.v[0], 1, 64);
-        if (kfirst) left = T(0);
-        if (klast) right = T(0);
-        const T cxm = cx[2 * s], cxp = cx[2 * s + 1];
+        if (rown && kin && s + 1 >= i0 && s + 1 < i1) {
+          const int64_t c = pl(s + 1) + rowc;
+          __builtin_memcpy(pout + c, &pB, sizeof(V));
+          __builtin_memcpy(rout + c, &qC, sizeof(V));
+        }
+        // 6. r'(s + 1) to LDS; dots at plane s from r'(s) on the rows beside
+        lr[par][w][lane] = qC;
+        if (s >= i0 && rown) {
+          const V qjm = lr[par ^ 1][w - 1][lane], qjp = lr[par ^ 1][w + 1][lane];
+          T left = __shfl_up(qB.v[CPL - 1], 1, 64), right = __shfl_down(qB.v[0], 1, 64);
+          if (kfirst) left = T(0);
+          if (klast) right = T(0);
+          const T cxm = cx[2 * s], cxp = cx[2 * s + 1];
+          const V czm = lcz[0][lane], czp = lcz[1][lane];
 #pragma unroll
-        for (int e = 0; e < CPL; e++) {
-          const T v = cg_stencil_cell<T, CPL>(fA[e], e, qB, qA, qC, qjm.v[e], qjp.v[e], left, right, cxm, cxp,
-                                              cym, cyp, czm[e], czp[e]);
-          if ((fA[e] & FL_ACTUAL) && kown[e]) {
-            g2 += (double)qB.v[e] * (double)qB.v[e];
-            d2 += (double)v * (double)qB.v[e];
+          for (int e = 0; e < CPL; e++) {
+            const uint8_t fa = flag_byte(fA, e);
+            const T v = cg_stencil_cell<T, CPL>(fa, e, qB, qA, qC, qjm.v[e], qjp.v[e], left, right, cxm, cxp,
+                                                cym, cyp, czm.v[e], czp.v[e]);
+            if ((fa & FL_ACTUAL) && kown[e]) {
+              g2 += (double)qB.v[e] * (double)qB.v[e];
+              d2 += (double)v * (double)qB.v[e];
+            }
           }
         }
       }
@@ -648,11 +687,8 @@ __global__ __launch_bounds__(64 * W) void k_cgcg_fused(int n, int l, int m, FTil
       rB = rC;
       qA = qB;
       qB = qC;
-#pragma unroll
-      for (int e = 0; e < CPL; e++) {
-        fA[e] = fB[e];
-        fB[e] = fC[e];
-      }
+      fA = fB;
+      fB = fC;
     }
     __syncthreads();  // the next tile reuses both LDS buffers
   }
@@ -947,15 +983,15 @@ static void cgcg_run(xf_ctx c, xf_grid g, X *x, T *const R[2], T *const P[2], co
   check_kernel("cg flush");
 }
 
-// KF's shape: W rows per block (TJ = W - 2 of them stored), about one tile
-// per resident block.  Timing lab at 256^3 (258^3 array): FP64 at 4 cells
-// per lane needs 150-odd VGPRs, three waves per SIMD, so one block of 10
-// rows per CU (32 x 8 tiles, 180 us per iteration; 8 rows: 43 x 5 tiles,
-// 214 us); FP32 one block of 16 rows per CU (19 x 13 tiles, 96 us; 10 rows
-// two blocks per CU: 113 us; 8 rows: 102 us).
+// KF's shape: W + 2 waves per block (TJ = W - 2 rows stored), about one
+// tile per resident block.  Timing lab at 256^3 (258^3 array): FP64 at 4
+// cells per lane needs 163 VGPRs, three waves per SIMD, so one block of 12
+// waves per CU (32 x 8 tiles, 172 us per iteration; 12 rows: 191 us; 8 rows:
+// 214 us, both before the halo waves and the prefetch); FP32 14 rows, 16
+// waves per block (22 x 11 tiles, 88 us).  A block has at most 16 waves.
 template <typename T, int CPL>
 constexpr int fused_rows() {
-  return sizeof(T) == 8 && CPL == 4 ? 10 : 16;
+  return CPL == 4 && sizeof(T) == 8 ? 10 : 14;
 }
 constexpr int kFusedBlocks = 1024;  // KF blocks at most: two partial buffers of this many fit the KA buffer
 
@@ -998,7 +1034,7 @@ static void cgcg_run_fused(xf_ctx c, xf_grid g, X *x, T *const R[2], T *const P[
     int dev = 0, cus = 0, per = 0;
     XF_HIP(hipGetDevice(&dev));
     XF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    XF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_cgcg_fused<T, X, CPL, W, OFF>, 64 * W, 0));
+    XF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_cgcg_fused<T, X, CPL, W, OFF>, 64 * (W + 2), 0));
     return std::max(1, cus * std::max(per, 1));
   }();
   FTiles tf;
@@ -1028,7 +1064,7 @@ static void cgcg_run_fused(xf_ctx c, xf_grid g, X *x, T *const R[2], T *const P[
     for (int q = 0; q < every; q++, it++) {
       const bool pm = c->prof && it % kProfEvery < 2;
       int e1 = pm ? prof_begin(c) : -1;
-      hipLaunchKernelGGL((k_cgcg_fused<T, X, CPL, W, OFF>), dim3(nbF), dim3(64 * W), 0, c->stream, g->n, g->l, g->m, tf,
+      hipLaunchKernelGGL((k_cgcg_fused<T, X, CPL, W, OFF>), dim3(nbF), dim3(64 * (W + 2)), 0, c->stream, g->n, g->l, g->m, tf,
                          it == 0 ? nbA : nbF, pb[it & 1], pb[(it + 1) & 1], g->d_flags, cx, cy, cz, x, R[it & 1],
                          R[(it + 1) & 1], P[it & 1], P[(it + 1) & 1], c->state, it, maxit);
       if (pm) {
