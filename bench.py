@@ -24,7 +24,7 @@ HBM_PEAK_GBS = 8000.0
 # measured HBM traffic of one k_cgcg_fused<double,double,4,10,1> launch at
 # 256^3 (mean of an even and an odd iteration; rocprofv3 counters scaled by a
 # copy of known size, as recorded in profiles/cg_kernels_traffic_pmc.md)
-KF_TRAFFIC_BYTES = None
+KF_TRAFFIC_BYTES = 733.9e6
 
 
 def log(*a):
@@ -381,7 +381,8 @@ def main():
                      "traffic": KF_TRAFFIC_BYTES if (args.n == 256 and fused) else None,
                      "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by "
                                      "a 512 MiB copy (profiles/cg_kernels_traffic_pmc.md); 688 MB algorithmic, "
-                                     "the excess is the halo rows and planes read again",
+                                     "the excess is the halo rows and planes read again (mean of an even and "
+                                     "an odd update)",
                      "kernel": ("k_cgcg_fused<double,double,4,10,1>" if fused else "k_cgcg_step<double,double,2>") +
                                " (41 B per interior cell, mean of 33 and 49)",
                      "kernel_us": round(kd["ms"] * 1e3, 2), "launches": kd["count"],
@@ -389,10 +390,14 @@ def main():
                      "iteration_us": round(it_us, 2),
                      "iteration_gbs": round(it_bytes / (it_us * 1e-6) / 1e9, 1),
                      "stream_ceiling_gbs": ceiling,
-                     "stream_ceiling_note": "grid-stride 16-byte streams over 1 GiB buffers on the same GPU in the "
-                                            "same run: read-only reduction, copy, and two reads + two writes in "
-                                            "place (the iteration kernel's pattern)",
-                     "kernel_frac_of_read2_write2_ceiling": round(kd_gbs / ceiling["read2_write2"], 3)},
+                     "stream_ceiling_note": "the fastest of a grid-stride loop and flat launches (plain and "
+                                            "non-temporal) of 16-byte streams over 1 GiB buffers on the same GPU "
+                                            "in the same run: read-only reduction, copy, and two reads + two "
+                                            "writes in place (the iteration kernel's pattern)",
+                     "kernel_frac_of_read2_write2_ceiling": round(kd_gbs / ceiling["read2_write2"], 3),
+                     # the textbook Hestenes-Stiefel iteration (three fused kernels, 89 B per cell in
+                     # FP64: A p, two dots, three axpys) would take this long at 100 % of peak
+                     "textbook_hs_cg_us_at_peak": round(89.0 * interior / (HBM_PEAK_GBS * 1e9) * 1e6, 1)},
         "cpu_baseline": cpu,
     }
     if port is not None:

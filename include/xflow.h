@@ -361,8 +361,9 @@ int xf_npy_save(xf_ctx ctx, const char *path, const void *data, char kind, int w
 /* No reference counterpart (the reference only prints clock() deltas,
  * XProblem.cpp:5442-5445).  When enabled, the throughput solvers bracket
  * each kernel launch with HIP events on the context's stream; read returns
- * the average device time per launch of a named kernel ("cg_stencil_fp64",
- * "cg_update_fp64", "cg_stencil_fp32", "cg_update_fp32") since the last reset. */
+ * the average device time per launch of a named kernel ("cg_dots_fp64",
+ * "cg_step_fp64", "cg_fused_fp64", their "_fp32" forms, "mp_residual_fp64",
+ * "mp_correct") since the last reset. */
 int xf_profile_enable(xf_ctx ctx, int on);
 int xf_profile_reset(xf_ctx ctx);
 int xf_profile_read(xf_ctx ctx, const char *kernel, double *avg_ms, int64_t *launches);
@@ -370,7 +371,9 @@ int xf_profile_read(xf_ctx ctx, const char *kernel, double *avg_ms, int64_t *lau
  * the solver kernels' access patterns, measured with two buffers of `bytes`
  * each over `reps` launches after one warm-up launch: kind 0 a read-only
  * reduction, 1 a copy (one read, one write), 2 two reads and two writes in
- * place.  *gbs = bytes moved per second / 1e9.  Allocates its buffers. */
+ * place, each as a grid-stride loop and as a flat launch (plain and
+ * non-temporal accesses).  *gbs = bytes moved per second / 1e9 of the
+ * fastest form.  Allocates its buffers. */
 int xf_stream_ceiling(xf_ctx ctx, int kind, size_t bytes, int reps, double *gbs);
 
 #ifdef __cplusplus

@@ -10,8 +10,15 @@ fluid (the scaling the preconditioner applies).  Training draws random crops
 of the pairs and minimises the masked mean-squared error with Adam.
 
     python -m xflow_amd.train_precond --out weights.npy [--problems 40 --steps 1500]
+
+With --pairs N the harvest keeps about N pairs from ncell^3 problems, each as
+one random crop^3 window in BF16 (the mask as bytes), with the problems
+generated by --workers CPU processes while the GPU solves:
+
+    python -m xflow_amd.train_precond --out unet_poisson.npy --ncell 128 --pairs 20000 --steps 10000
 """
 import argparse
+import multiprocessing
 import os
 import sys
 import time
@@ -24,6 +31,8 @@ from xflow_amd import cases
 from xflow_amd.unet import LOSS_COSINE, LOSS_ENERGY, LOSS_MSE, UNet
 
 ITERATES = (0, 2, 5, 10, 20, 40, 80, 160, 320)
+# the iteration caps of the cropped harvest (18 pairs per problem)
+ITERATES_DENSE = (0, 1, 2, 3, 5, 7, 10, 14, 20, 28, 40, 56, 80, 113, 160, 226, 320, 452)
 
 
 def coarse_case(ncell, fine, seed):
@@ -95,11 +104,70 @@ def harvest(ctx, ncell, problems, seed0=100, coarsen_from=0, box=None, log=print
     return torch.stack(Rs), torch.stack(Es), torch.stack(Ms)
 
 
+def _obstacle_problem(arg):
+    """CPU worker: the random-obstacle problem of one seed."""
+    ncell, seed = arg
+    return seed, cases.poisson_case(ncell, cases.random_obstacles(seed), seed)
+
+
+def harvest_crops(ctx, ncell, pairs, crop, pool, seed0=100, log=print):
+    """About `pairs` pairs (r_k, e_k) from ncell^3 random-obstacle problems
+    (seeds seed0, seed0 + 1, ...; len(ITERATES_DENSE) CG iterates each), one
+    random crop^3 window of each kept: R, E in BF16, M as bytes, on the
+    device.  `pool` generates the problems on the CPU while the GPU solves."""
+    per = len(ITERATES_DENSE)
+    problems = (pairs + per - 1) // per
+    cap = problems * per
+    R = torch.empty((cap, crop, crop, crop), dtype=torch.bfloat16, device="cuda")
+    E = torch.empty_like(R)
+    M = torch.empty((cap, crop, crop, crop), dtype=torch.uint8, device="cuda")
+    gen = torch.Generator().manual_seed(seed0)
+    n, t0 = 0, time.time()
+    jobs = [(ncell, seed0 + q) for q in range(problems)]
+    for pidx, (seed, case) in enumerate(pool.imap(_obstacle_problem, jobs)):
+        g = xf.Grid(ctx, case["hx"], case["hy"], case["hz"], case["codes"])
+        b = torch.from_numpy(case["b"]).cuda()
+        x0 = torch.zeros_like(b)
+        xs = torch.zeros_like(b)
+        st = g.solve(b, x0, xs, xf.solve_opts(xf.XF_CG, tol=1e-12, tol_mode=xf.XF_TOL_REL, project_mean=1))
+        if st.status != 0:  # no reference solution: no pairs from this problem
+            g.close()
+            log("harvest problem %d/%d (seed %d): CG status %d after %d iterations, skipped" %
+                (pidx + 1, problems, seed, st.status, st.iters))
+            continue
+        fluid = torch.from_numpy((case["codes"] == 1).astype(np.float64)).cuda()
+        nf = fluid.sum()
+        for k in ITERATES_DENSE:
+            xk = torch.zeros_like(b)
+            if k > 0:
+                g.solve(b, x0, xk, xf.solve_opts(xf.XF_CG, tol=1e-30, tol_mode=xf.XF_TOL_REL, project_mean=1,
+                                                 maxit=k))
+            rk = torch.zeros_like(b)
+            g.apply(xk, rk, b, -1.0)  # A x - b
+            rk = -rk * fluid
+            ek = (xs - xk) * fluid
+            ek = ek - (ek.sum() / nf) * fluid
+            sc = torch.sqrt((rk * rk).sum() / nf)
+            if sc <= 0:
+                continue
+            o = [1 + int(torch.randint(0, ncell - crop + 1, (1,), generator=gen)) for _ in range(3)]
+            sl = tuple(slice(v, v + crop) for v in o)
+            R[n] = (rk[sl] / sc).to(torch.bfloat16)
+            E[n] = (ek[sl] / sc).to(torch.bfloat16)
+            M[n] = fluid[sl].to(torch.uint8)
+            n += 1
+        g.close()
+        if pidx % 25 == 0 or pidx == problems - 1:
+            log("harvest problem %d/%d (seed %d): CG %d iterations, %d pairs, %.1f s" %
+                (pidx + 1, problems, seed, st.iters, n, time.time() - t0))
+    return R[:n], E[:n], M[:n]
+
+
 def crops(R, E, M, B, S, gen):
     n, D, H, W = R.shape
-    if S is None or S <= 0:  # whole fields
-        idx = torch.randint(0, n, (B,), generator=gen)
-        return R[idx].contiguous(), M[idx].contiguous(), E[idx].contiguous()
+    if S is None or S <= 0 or S >= D:  # whole fields (or whole stored windows)
+        idx = torch.randint(0, n, (B,), generator=gen).to(R.device)
+        return R[idx].float().contiguous(), M[idx].float().contiguous(), E[idx].float().contiguous()
     idx = torch.randint(0, n, (B,), generator=gen)
     zs = torch.randint(0, D - S + 1, (B,), generator=gen)
     ys = torch.randint(0, H - S + 1, (B,), generator=gen)
@@ -181,14 +249,24 @@ def main():
     ap.add_argument("--eval-ncell", type=int, default=128)
     ap.add_argument("--coarsen-from", type=int, default=0,
                     help="train the coarsest-grid network of the V-cycle: fine^3 masks coarsened to ncell^3")
+    ap.add_argument("--pairs", type=int, default=0,
+                    help="cropped harvest of about this many pairs from ncell^3 problems (see the module notes)")
+    ap.add_argument("--workers", type=int, default=16, help="CPU processes generating problems (--pairs)")
     ap.add_argument("--box", default="",
                     help="train on the coarsest grid of a DxHxW-cell box (e.g. 256x128x128), whole fields")
     args = ap.parse_args()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    # the problem generators fork before this process touches the GPU
+    pool = multiprocessing.get_context("fork").Pool(args.workers) if args.pairs else None
     ctx = xf.Context(0)
     t0 = time.time()
     box = tuple(int(v) for v in args.box.split("x")) if args.box else None
-    R, E, M = harvest(ctx, args.ncell, args.problems, coarsen_from=args.coarsen_from, box=box, log=log)
+    if args.pairs:
+        R, E, M = harvest_crops(ctx, args.ncell, args.pairs, args.crop, pool, log=log)
+        pool.close()
+        pool.join()
+    else:
+        R, E, M = harvest(ctx, args.ncell, args.problems, coarsen_from=args.coarsen_from, box=box, log=log)
     if box:
         args.crop = 0
     log("harvested %d pairs in %.1f s" % (R.shape[0], time.time() - t0))
