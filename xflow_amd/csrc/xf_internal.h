@@ -180,6 +180,23 @@ inline int grid_blocks(int64_t n, int bs = kBlock) {
   return (int)(b > 0 ? b : 1);
 }
 
+// A vector store that streams past the caches (non-temporal) where the
+// vector is a whole number of 16-byte chunks, for fields that only a later
+// launch reads again.
+template <typename T>
+__device__ inline void store_stream(T *dst, const void *src, int bytes) {
+  if (bytes % 16 == 0) {
+    typedef float q4 __attribute__((ext_vector_type(4)));
+    for (int h = 0; h < bytes / 16; h++) {
+      q4 t;
+      __builtin_memcpy(&t, (const char *)src + 16 * h, 16);
+      __builtin_nontemporal_store(t, (q4 *)((char *)dst + 16 * h));
+    }
+  } else {
+    __builtin_memcpy(dst, src, bytes);
+  }
+}
+
 // Sum of v over the block; valid in thread 0.  blockDim.x must be a multiple of 64.
 __device__ inline double block_sum(double v, double *lds) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
