@@ -488,9 +488,10 @@ __global__ __launch_bounds__(256) void k_cgcg_flush(int64_t nc, X *__restrict__ 
 // later, so they are in flight across the barrier and the stencils; the flag
 // bytes stay packed in one integer until the stencil takes them apart (a use
 // right after the load would make the wave wait for every load before it).
-// The lane's k coefficients sit in LDS, not in registers.  p, r' and x are
-// stored non-temporally (FP32 88 -> 77 us at 256^3, FP64 unchanged; the
-// loads stay cached: halo rows are read by two blocks).
+// The lane's k coefficients sit in LDS, not in registers.  In FP32, p, r'
+// and x are stored non-temporally (88 -> 77 us at 256^3; in FP64 the same
+// time with 5 % more write traffic, so FP64 stores are plain; the loads stay
+// cached: halo rows are read by two blocks).
 //   reads  r, p_old (W + 2 rows per TJ stored), flags (W rows), x (odd updates)
 //   writes p, r' (TJ rows), x (odd updates)              [41 B per cell, mean]
 struct FTiles {
@@ -566,6 +567,11 @@ __global__ __launch_bounds__(64 * (W + 2)) void k_cgcg_fused(int n, int l, int m
   }
   __syncthreads();
   auto pl = [&](int i) { return (int64_t)min(max(i, 0), n - 1) * sI; };
+  // FP32 stores stream past the caches; FP64 ones are plain (same time, less write traffic)
+  auto put_row = [](auto *dst, const void *src, int bytes) {
+    if constexpr (sizeof(T) == 4) store_stream(dst, src, bytes);
+    else __builtin_memcpy(dst, src, bytes);
+  };
   double g2 = 0, d2 = 0;
   for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
     const int t = xcd * per + tq;
@@ -628,7 +634,7 @@ __global__ __launch_bounds__(64 * (W + 2)) void k_cgcg_fused(int n, int l, int m
 #pragma unroll
         for (int e = 0; e < CPL; e++) xx.v[e] = (nx.v[e] + aold * (X)np.v[e]) + (X)a * (X)pC.v[e];
         if (kall) {
-          store_stream(x + c, &xx, sizeof(VX));
+          put_row(x + c, &xx, sizeof(VX));
         } else {
 #pragma unroll
           for (int e = 0; e < CPL; e++)
@@ -660,8 +666,8 @@ __global__ __launch_bounds__(64 * (W + 2)) void k_cgcg_fused(int n, int l, int m
         }
         if (rown && kin && s + 1 >= i0 && s + 1 < i1) {
           const int64_t c = pl(s + 1) + rowc;
-          store_stream(pout + c, &pB, sizeof(V));
-          store_stream(rout + c, &qC, sizeof(V));
+          put_row(pout + c, &pB, sizeof(V));
+          put_row(rout + c, &qC, sizeof(V));
         }
         // 6. r'(s + 1) to LDS; dots at plane s from r'(s) on the rows beside
         lr[par][w][lane] = qC;
