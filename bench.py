@@ -35,23 +35,48 @@ def _size(case):
     return "x".join(str(d - 2) for d in case["codes"].shape)
 
 
+def host_cpu():
+    """The host the CPU baselines ran on: model name and logical CPUs."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count() or 1, "usable_cpus": usable}
+
+
 def cpu_port(case, gpu_iters, sample_iters):
     """The CPU oracle's FP64 CG (OpenMP stencil, serial dot products), timed
-    for a bounded number of iterations and scaled to the GPU's iteration count."""
+    for a bounded number of iterations on all usable cores (at most 16) and on
+    one, and scaled to the GPU's iteration count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import xf_oracle as xo
-    threads = min(16, os.cpu_count() or 1)
-    xo.set_threads(threads)
+    host = host_cpu()
+    threads = min(16, host["usable_cpus"])
     og = xo.Grid(case["hx"], case["hy"], case["hz"], case["codes"])
     x0 = np.zeros_like(case["b"])
-    t0 = time.perf_counter()
-    _, it, _, _ = xo.cg_cg(og, case["b"], x0, 1e-30, sample_iters, project_mean=True)
-    dt = time.perf_counter() - t0
-    per_iter_ms = dt * 1e3 / max(it, 1)
-    return {"value": per_iter_ms * gpu_iters, "unit": "ms/solve", "cores": threads, "kind": "port",
-            "sample": "%d iterations of the oracle FP64 single-reduction CG (xo_cg_cg, %d OpenMP threads for the stencil, serial "
-                      "dot products) at the same %s case, %.1f ms/iteration, scaled by the GPU solve's %d "
-                      "iterations" % (it, threads, _size(case), per_iter_ms, gpu_iters)}
+    per = {}
+    for nt in (threads, 1):
+        xo.set_threads(nt)
+        t0 = time.perf_counter()
+        _, it, _, _ = xo.cg_cg(og, case["b"], x0, 1e-30, sample_iters, project_mean=True)
+        per[nt] = (time.perf_counter() - t0) * 1e3 / max(it, 1)
+    xo.set_threads(threads)
+    out = {"value": per[threads] * gpu_iters, "unit": "ms/solve", "cores": threads, "kind": "port",
+           "one_thread_value": per[1] * gpu_iters,
+           "sample": "%d iterations of the oracle FP64 single-reduction CG (xo_cg_cg, OpenMP stencil, serial "
+                     "dot products) at the same %s case: %.1f ms/iteration on %d threads, %.1f ms on one, scaled "
+                     "by the GPU solve's %d iterations" % (it, _size(case), per[threads], threads, per[1], gpu_iters)}
+    out.update(host)
+    return out
 
 
 REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
@@ -87,7 +112,7 @@ def cpu_reference(case, gpu_iters, sample_iters):
         o = read_xfb(fo)
     it = int(o["iters"][0])
     per_iter_ms = float(o["seconds"][0]) * 1e3 / it
-    return {"value": per_iter_ms * gpu_iters, "unit": "ms/solve", "cores": 1, "kind": "reference",
+    return {**host_cpu(), "value": per_iter_ms * gpu_iters, "unit": "ms/solve", "cores": 1, "kind": "reference",
             "sample": "%d iterations of the reference's TSLAE::SolveByConjugateGradients (XSLAE.cpp:56-157, "
                       "compiled -O2 from its sources; serial, as the reference builds it) at the same %s case "
                       "after its first three operator sweeps, %.1f ms/iteration, scaled by the GPU solve's %d "
