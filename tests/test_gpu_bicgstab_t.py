@@ -12,7 +12,9 @@ relative takes 154 instead of 160 iterations (228-244 instead of 258 with
 the density) and moves the field by 4e-8 to 6e-7.  There the two GPU
 solvers, whose only difference is the order of the reductions' sums, must
 both stop with max |b - A x| <= 1e-5 (recomputed afterwards), agree on the
-field to 1e-5 relative and on the iteration count to within a third."""
+field to 1e-5 relative and on the iteration count to within a third.  The
+same bars hold against the oracle on a mask with PreDefined and PreNormal
+rows away from the array faces (the fused solver's list-driven pass)."""
 import numpy as np
 import pytest
 import torch
@@ -116,3 +118,68 @@ def test_sphere128_against_parity_solver(ctx):
     assert 3 * abs(sf.iters - sp.iters) <= max(sf.iters, sp.iters), (sf.iters, sp.iters)
     assert rel_l2(xt, xp) <= 1e-5
     assert sf.ms < sp.ms
+
+
+def interior_boundary_case(n=18, box=(7, 12), seed=5, kinds=(4, 6, 4)):
+    """A Dirichlet box (Defined ring) with a box obstacle inside whose face
+    cells (edges and corners excluded) are PreDefined (i and k faces) and
+    PreNormal (j faces) rows, normals pointing at the fluid: unknown rows that
+    are not Actual away from the array faces, which the fused solver's
+    marching passes leave to their list-driven pass."""
+    N = n + 2
+    codes = np.ones((N, N, N), np.int32)
+    codes[0], codes[-1] = 3, 3
+    codes[:, 0], codes[:, -1] = 3, 3
+    codes[:, :, 0], codes[:, :, -1] = 3, 3
+    a, b = box
+    codes[a:b, a:b, a:b] = 0
+    normals = np.zeros((N, N, N, 3))
+    has = np.zeros((N, N, N), np.int32)
+    for axis in range(3):
+        for lo in (True, False):
+            idx = [slice(a + 1, b - 1)] * 3
+            idx[axis] = a if lo else b - 1
+            idx = tuple(idx)
+            codes[idx] = kinds[axis]
+            normals[idx + (axis,)] = 1.0 if lo else -1.0
+            has[idx] = 1
+    h = np.full(N - 1, 1.0 / n)
+    rng = np.random.default_rng(seed)
+    space = np.isin(codes, [1, 4, 6])
+    b = np.where(space, rng.standard_normal(codes.shape), 0.0) * 10.0
+    return codes, normals, has, h, b
+
+
+@pytest.mark.parametrize("rho", [False, True])
+def test_interior_boundary_rows(ctx, rho):
+    """PreDefined and PreNormal rows inside the array (the fused solver's
+    list-driven pass beside the marching rows), with and without a variable
+    density: the fused solver, the statement-by-statement solver and the
+    oracle (the reference's arithmetic) each stop with max |b - A x| <= 1e-5;
+    the fused solver's field within 1e-5 relative of both others', its
+    iteration count within a third of theirs."""
+    import xf_oracle as xo
+    codes, normals, has, h, b = interior_boundary_case()
+    dens = 1.0 + 0.5 * np.random.default_rng(1).random(codes.shape) if rho else None
+    g = xf.Grid(ctx, h, h, h, codes, normals, has)
+    if rho:
+        g.set_density(dev(dens))
+    og = xo.Grid(h, h, h, codes, normals=normals, has_normal=has, rho=dens)
+    xr, itr, _, sto = xo.bicgstab(og, b, np.zeros_like(b), 1e-5, 100000)
+    assert sto == 0
+    out = {}
+    for m in (xf.XF_BICGSTAB, xf.XF_BICGSTAB_T):
+        x = torch.zeros_like(dev(b))
+        st = g.solve(dev(b), torch.zeros_like(dev(b)), x, xf.solve_opts(m))
+        assert st.status == 0, (m, st.status)
+        y = torch.zeros_like(dev(b))
+        g.apply(x, y, dev(b), -1.0)
+        out[m] = (st, x.cpu().numpy(), float(y.abs().max()))
+    (sp, xp, rp), (sf, xt, rt) = out[xf.XF_BICGSTAB], out[xf.XF_BICGSTAB_T]
+    print("oracle %d iterations, parity %d, fused %d; fused field vs oracle %.2e, vs parity %.2e; "
+          "max |b - A x| %.2e / %.2e" % (itr, sp.iters, sf.iters, rel_l2(xt, xr), rel_l2(xt, xp), rp, rt))
+    assert max(rp, rt) <= 1.05e-5
+    for other in (itr, sp.iters):
+        assert 3 * abs(sf.iters - other) <= max(sf.iters, other), (sf.iters, other)
+    assert rel_l2(xt, xr) <= 1e-5 and rel_l2(xt, xp) <= 1e-5
+    g.close()

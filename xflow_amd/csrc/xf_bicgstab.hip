@@ -11,7 +11,15 @@
 //                                                 and the partials of (r^, r), max |r|
 // (Forming s at the stencil points inside KT instead cost 819 us per pass at
 // 256^3 against 94 + 216 us for KS and KT: the exact rows read their field
-// several times per point.)
+// several times per point.)  KV and KT march: a block of four waves owns four
+// rows of 64 cells and walks them along i, the field and the density at
+// planes i - 1, i, i + 1 in registers, the k-neighbours by lane shuffles, the
+// Fictive tests from the flag byte's face bits; an Actual row is the
+// reference's expression in its operand order (the same bits as the flat
+// rows); the other unknown rows (PreDefined, PreNormal) go through the
+// generic row in a small list-driven pass beside them, which keeps the
+// generic row's registers out of the march.  Grids with unknowns on the
+// array faces (which the marching tiles do not visit) use the flat passes.
 // Each vector statement keeps the reference's rounding sequence (XSLAE.cpp:
 // 723-745).  Reductions are per-block partials that the next pass folds in a
 // fixed order in every block (no grid-wide atomics), so all blocks take the
@@ -27,7 +35,7 @@
 
 namespace {
 
-constexpr int kBtBlocks = 1024;  // blocks of every pass (partials per reduction)
+constexpr int kBtBlocks = 2048 + 64;  // blocks of every pass at most (partials per reduction)
 enum { BT_PN = 0, BT_AN, BT_WN, BT_PN1, BT_BN, BT_NORM, BT_EPS };
 enum { PS_RR = 0, PS_MAX, PS_RV, PS_TS, PS_TT };  // partial slots
 
@@ -74,9 +82,10 @@ __device__ double fold(const BtArgs &A, int k, double *lds) {
   return res;
 }
 
-// the block's sum (MAX: max) of per-thread values into partial slot k
+// the block's sum (MAX: max) of per-thread values into partial slot k, entry
+// first + blockIdx.x
 template <bool MAX>
-__device__ void put(const BtArgs &A, int k, double v, double *lds) {
+__device__ void put(const BtArgs &A, int k, double v, double *lds, int first = 0) {
   lds[threadIdx.x] = v;
   __syncthreads();
   for (int h = blockDim.x / 2; h > 0; h >>= 1) {
@@ -90,7 +99,7 @@ __device__ void put(const BtArgs &A, int k, double v, double *lds) {
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) slot(A, k)[blockIdx.x] = lds[0];
+  if (threadIdx.x == 0) slot(A, k)[first + blockIdx.x] = lds[0];
   __syncthreads();
 }
 
@@ -170,6 +179,185 @@ __global__ __launch_bounds__(kBlock) void k_bt_p(BtArgs A) {
     st->s[BT_NORM] = norm;
     st->s[BT_PN1] = pn1;
     st->s[BT_BN] = bn;
+  }
+}
+
+// ---------------------------------------------------------- marching rows
+constexpr int kBtRows = 4;  // rows (waves) per block of the marching passes
+
+struct BtTiles {
+  int kt, jt, it, ic, total;
+};
+
+// The Actual row of T3dLaplaceOperator / T3dVariableDivergentedOperator
+// (XOperator.cpp:985-1031, 1990-2025) from values already in registers: f and
+// rho at the cell and its six neighbours, the face bits of the flag byte
+// (set where the neighbour is not Fictive), the steps around the cell.
+template <bool RHO>
+__device__ inline double bt_actual_row(uint8_t fl, double fc, double fim, double fip, double fjm, double fjp,
+                                       double fkm, double fkp, double rc, double rim, double rip, double rjm,
+                                       double rjp, double rkm, double rkp, double hxm, double hxc, double hym,
+                                       double hyc, double hzm, double hzc) {
+  const double hx_2X = (hxm + hxc) / 2;
+  const double hx_2Y = (hym + hyc) / 2;
+  const double hx_2Z = (hzm + hzc) / 2;
+  if (!RHO) {
+    double f_i_1 = (fc - fim) / (hxm);
+    double f_i = (fip - fc) / (hxc);
+    double f_j_1 = (fc - fjm) / (hym);
+    double f_j = (fjp - fc) / (hyc);
+    double f_k_1 = (fc - fkm) / (hzm);
+    double f_k = (fkp - fc) / (hzc);
+    if (!(fl & FL_IM)) f_i_1 = 0;
+    if (!(fl & FL_IP)) f_i = 0;
+    if (!(fl & FL_JM)) f_j_1 = 0;
+    if (!(fl & FL_JP)) f_j = 0;
+    if (!(fl & FL_KM)) f_k_1 = 0;
+    if (!(fl & FL_KP)) f_k = 0;
+    return -((f_i - f_i_1) / hx_2X + (f_j - f_j_1) / hx_2Y + (f_k - f_k_1) / hx_2Z);
+  }
+  double f_i_1 = rim * (fc - fim) / (hxm);
+  double f_i = rip * (fip - fc) / (hxc);
+  double f_j_1 = rjm * (fc - fjm) / (hym);
+  double f_j = rjp * (fjp - fc) / (hyc);
+  double f_k_1 = rkm * (fc - fkm) / (hzm);
+  double f_k = rkp * (fkp - fc) / (hzc);
+  double d_i = (rip - rc) / (hxm);
+  double d_j = (rjp - rc) / (hym);
+  double d_k = (rkp - rc) / (hzm);
+  if (!(fl & FL_IM)) f_i_1 = 0;
+  if (!(fl & FL_IP)) { f_i = 0; d_i = 0; }
+  if (!(fl & FL_JM)) f_j_1 = 0;
+  if (!(fl & FL_JP)) { f_j = 0; d_j = 0; }
+  if (!(fl & FL_KM)) f_k_1 = 0;
+  if (!(fl & FL_KP)) { f_k = 0; d_k = 0; }
+  double lap = (f_i - f_i_1) / hx_2X + (f_j - f_j_1) / hx_2Y + (f_k - f_k_1) / hx_2Z;
+  double divd = d_i + d_j + d_k;
+  return -(rc * lap - divd * fc);
+}
+
+// MODE 0: v = A p with the partials of (r^, v) (XSLAE.cpp:729); MODE 1:
+// t = A s with the partials of (t, s) and (t, t) (XSLAE.cpp:735).  Blocks
+// are persistent over the tiles (a multiple of 8 of them), consecutive tiles
+// on one XCD.
+template <bool RHO, int MODE>
+__global__ __launch_bounds__(kBlock) void k_bt_rows(BtArgs A, BtTiles tl, const uint8_t *__restrict__ flags) {
+  __shared__ double lds[kBlock];
+  if (A.st->done) return;
+  const GridView &g = A.g;
+  const int n = g.n, l = g.l, m = g.m;
+  const double *__restrict__ f = MODE == 0 ? A.p : A.s;
+  double *__restrict__ out = MODE == 0 ? A.v : A.t;
+  const double *__restrict__ rho = g.rho;
+  const int64_t sI = (int64_t)l * m, sJ = m;
+  const int lane = threadIdx.x & 63;
+  const int per = (tl.total + 7) / 8, bx = gridDim.x >> 3, xcd = blockIdx.x & 7;
+  double acc0 = 0, acc1 = 0;
+  for (int tq = blockIdx.x >> 3; tq < per; tq += bx) {
+    const int t = xcd * per + tq;
+    if (t >= tl.total) break;
+    const int ktile = t % tl.kt, jtile = (t / tl.kt) % tl.jt, itile = t / (tl.kt * tl.jt);
+    // the last k-tile is shifted back to end at m - 2; its repeated cells are not owned
+    const int kcount = 1 + ktile * 64;
+    const int kstart = max(1, min(kcount, m - 1 - 64));
+    const int kraw = kstart + lane;
+    const int k = min(kraw, m - 2);
+    const bool kown = kraw >= kcount && kraw <= m - 2;
+    const int j = 1 + jtile * kBtRows + (threadIdx.x >> 6);
+    if (j > l - 2) continue;  // uniform per wave; no barrier below
+    const int i0 = 1 + itile * tl.ic, i1 = min(i0 + tl.ic, n - 1);
+    const double hym = g.hy[j - 1], hyc = g.hy[j], hzm = g.hz[k - 1], hzc = g.hz[k];
+    int64_t c = ((int64_t)i0 * l + j) * m + k;
+    double fm = f[c - sI], fc = f[c];
+    double rm = 0, rcur = 0;
+    if (RHO) {
+      rm = rho[c - sI];
+      rcur = rho[c];
+    }
+    const bool kload_r = lane == 63 || kraw >= m - 2;
+    for (int i = i0; i < i1; i++, c += sI) {
+      const double fp = f[c + sI], fjm = f[c - sJ], fjp = f[c + sJ];
+      double rp = 0, rjm = 0, rjp = 0;
+      if (RHO) {
+        rp = rho[c + sI];
+        rjm = rho[c - sJ];
+        rjp = rho[c + sJ];
+      }
+      const uint8_t fl = flags[c];
+      double fkm = __shfl_up(fc, 1, 64), fkp = __shfl_down(fc, 1, 64);
+      double rkm = 0, rkp = 0;
+      if (RHO) {
+        rkm = __shfl_up(rcur, 1, 64);
+        rkp = __shfl_down(rcur, 1, 64);
+      }
+      if (lane == 0) {
+        fkm = f[c - 1];
+        if (RHO) rkm = rho[c - 1];
+      }
+      if (kload_r) {
+        fkp = f[c + 1];
+        if (RHO) rkp = rho[c + 1];
+      }
+      if (kown && (fl & FL_UNKNOWN)) {
+        if (fl & FL_ACTUAL) {
+          const double v = bt_actual_row<RHO>(fl, fc, fm, fp, fjm, fjp, fkm, fkp, rcur, rm, rp, rjm, rjp, rkm, rkp,
+                                              g.hx[i - 1], g.hx[i], hym, hyc, hzm, hzc);
+          out[c] = v;
+          if (MODE == 0) {
+            acc0 += A.rh[c] * v;
+          } else {
+            acc0 += v * fc;
+            acc1 += v * v;
+          }
+        }
+      }
+      fm = fc;
+      fc = fp;
+      if (RHO) {
+        rm = rcur;
+        rcur = rp;
+      }
+    }
+  }
+  if (MODE == 0) {
+    put<false>(A, PS_RV, acc0, lds);
+    if (blockIdx.x == 0 && threadIdx.x == 0) A.st->iter++;  // XSLAE.cpp:719
+  } else {
+    put<false>(A, PS_TS, acc0, lds);
+    put<false>(A, PS_TT, acc1, lds);
+  }
+}
+
+// The unknown rows that are not Actual (PreDefined, PreNormal), from a list
+// of their cells, for the marching passes (which evaluate Actual rows only):
+// the generic row, and partials into the entries after the marching blocks'.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_bt_other(BtArgs A, const int64_t *__restrict__ cells, int64_t count,
+                                                     int first) {
+  __shared__ double lds[kBlock];
+  if (A.st->done) return;
+  const GridView &g = A.g;
+  const double *__restrict__ f = MODE == 0 ? A.p : A.s;
+  double *__restrict__ out = MODE == 0 ? A.v : A.t;
+  double acc0 = 0, acc1 = 0;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < count; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = cells[q];
+    int i, j, k;
+    decompose(c, g.l, g.m, i, j, k);
+    const double v = row_value(g, f, c, i, j, k);
+    out[c] = v;
+    if (MODE == 0) {
+      acc0 += A.rh[c] * v;
+    } else {
+      acc0 += v * f[c];
+      acc1 += v * v;
+    }
+  }
+  if (MODE == 0) {
+    put<false>(A, PS_RV, acc0, lds, first);
+  } else {
+    put<false>(A, PS_TS, acc0, lds, first);
+    put<false>(A, PS_TT, acc1, lds, first);
   }
 }
 
@@ -270,7 +458,55 @@ void solve_bicgstab_t(xf_ctx c, xf_grid g, const double *b, const double *x0, do
   A.t = w + 5 * nc;
   A.part = part;
   A.st = c->state;
-  A.nb = (int)std::max<int64_t>(1, std::min<int64_t>(kBtBlocks, (nc + kBlock - 1) / kBlock));
+  // the marching passes need every unknown inside the array faces
+  if (g->ring_unknowns < 0) {
+    const std::vector<uint8_t> &cd = host_codes(g);
+    const int gn = g->n, gl = g->l, gm = g->m;
+    auto unknown = [&](int ii, int jj, int kk) {
+      const uint8_t cv = cd[((int64_t)ii * gl + jj) * gm + kk];
+      return cv == PT_ACTUAL || cv == PT_PREDEFINED || cv == PT_NORMAL || cv == PT_PRENORMAL || cv == PT_PREEQUATION;
+    };
+    int cnt = 0;  // the six faces (edges counted more than once: only zero matters)
+    for (int a = 0; a < gl; a++)
+      for (int b = 0; b < gm; b++) cnt += unknown(0, a, b) + unknown(gn - 1, a, b);
+    for (int a = 0; a < gn; a++)
+      for (int b = 0; b < gm; b++) cnt += unknown(a, 0, b) + unknown(a, gl - 1, b);
+    for (int a = 0; a < gn; a++)
+      for (int b = 0; b < gl; b++) cnt += unknown(a, b, 0) + unknown(a, b, gm - 1);
+    g->ring_unknowns = cnt;
+  }
+  const bool march = g->ring_unknowns == 0 && g->n >= 3 && g->l >= 3 && g->m >= 3;
+  BtTiles tl{};
+  int nb_march = 0, nb_other = 0;
+  if (march && g->n_other_rows < 0) {
+    // the cells whose rows the marching passes leave to k_bt_other
+    const std::vector<uint8_t> &cd = host_codes(g);
+    std::vector<int64_t> cells;
+    for (int64_t q = 0; q < nc; q++)
+      if (cd[q] == PT_PREDEFINED || cd[q] == PT_PRENORMAL || cd[q] == PT_NORMAL || cd[q] == PT_PREEQUATION)
+        cells.push_back(q);
+    g->n_other_rows = (int64_t)cells.size();
+    if (!cells.empty()) {
+      XF_HIP(hipMalloc(&g->d_other_rows, sizeof(int64_t) * cells.size()));
+      XF_HIP(hipMemcpy(g->d_other_rows, cells.data(), sizeof(int64_t) * cells.size(), hipMemcpyHostToDevice));
+    }
+  }
+  if (march) {
+    tl.kt = (g->m - 2 + 63) / 64;
+    tl.jt = (g->l - 2 + kBtRows - 1) / kBtRows;
+    for (tl.ic = 32;; tl.ic /= 2) {
+      tl.it = (g->n - 2 + tl.ic - 1) / tl.ic;
+      tl.total = tl.kt * tl.jt * tl.it;
+      if (tl.total >= 2048 || tl.ic <= 2) break;
+    }
+    // a multiple of 8 marching blocks (the XCD mapping), then the blocks of
+    // the other rows; together at most one partial slot's worth
+    nb_other = g->n_other_rows > 0 ? (int)std::min<int64_t>(64, (g->n_other_rows + kBlock - 1) / kBlock) : 0;
+    nb_march = (int)std::min<int64_t>(2048, 8 * ((tl.total + 7) / 8));
+    A.nb = nb_march + nb_other;
+  } else {
+    A.nb = (int)std::max<int64_t>(1, std::min<int64_t>(kBtBlocks, (nc + kBlock - 1) / kBlock));
+  }
   A.maxit = o.maxit;
   DevState h{};
   h.s[BT_PN] = 1.0;
@@ -289,9 +525,31 @@ void solve_bicgstab_t(xf_ctx c, xf_grid g, const double *b, const double *x0, do
   for (;;) {
     for (int q = 0; q < every; q++) {
       hipLaunchKernelGGL(k_bt_p, dim3(A.nb), dim3(kBlock), 0, c->stream, A);
-      hipLaunchKernelGGL(k_bt_v, dim3(A.nb), dim3(kBlock), 0, c->stream, A);
+      if (!march) {
+        hipLaunchKernelGGL(k_bt_v, dim3(A.nb), dim3(kBlock), 0, c->stream, A);
+      } else {
+        if (A.g.rho) {
+          hipLaunchKernelGGL((k_bt_rows<true, 0>), dim3(nb_march), dim3(kBlock), 0, c->stream, A, tl, g->d_flags);
+        } else {
+          hipLaunchKernelGGL((k_bt_rows<false, 0>), dim3(nb_march), dim3(kBlock), 0, c->stream, A, tl, g->d_flags);
+        }
+        if (nb_other)
+          hipLaunchKernelGGL(k_bt_other<0>, dim3(nb_other), dim3(kBlock), 0, c->stream, A, g->d_other_rows,
+                             g->n_other_rows, nb_march);
+      }
       hipLaunchKernelGGL(k_bt_s, dim3(A.nb), dim3(kBlock), 0, c->stream, A);
-      hipLaunchKernelGGL(k_bt_t, dim3(A.nb), dim3(kBlock), 0, c->stream, A);
+      if (!march) {
+        hipLaunchKernelGGL(k_bt_t, dim3(A.nb), dim3(kBlock), 0, c->stream, A);
+      } else {
+        if (A.g.rho) {
+          hipLaunchKernelGGL((k_bt_rows<true, 1>), dim3(nb_march), dim3(kBlock), 0, c->stream, A, tl, g->d_flags);
+        } else {
+          hipLaunchKernelGGL((k_bt_rows<false, 1>), dim3(nb_march), dim3(kBlock), 0, c->stream, A, tl, g->d_flags);
+        }
+        if (nb_other)
+          hipLaunchKernelGGL(k_bt_other<1>, dim3(nb_other), dim3(kBlock), 0, c->stream, A, g->d_other_rows,
+                             g->n_other_rows, nb_march);
+      }
       hipLaunchKernelGGL(k_bt_x, dim3(A.nb), dim3(kBlock), 0, c->stream, A);
     }
     check_kernel("BiCGStab iteration");

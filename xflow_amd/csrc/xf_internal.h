@@ -129,6 +129,9 @@ struct xf_grid_s {
   int64_t bad_rows_lap = 0;           // rows the Laplace operator rejects
   int64_t bad_rows_div = 0;           // rows the divergented operator rejects
   int64_t special_rows = 0;           // unknown rows that are not Actual (codes 4, 5, 6, 8)
+  int ring_unknowns = -1;             // unknowns on the array faces (-1: not counted yet)
+  int64_t *d_other_rows = nullptr;    // cells of the unknown rows that are not Actual (fused BiCGStab)
+  int64_t n_other_rows = -1;          // their count (-1: not listed yet)
   // xf_transport_velocity per equation (0 concentration, 1..3 velocity
   // components): why the reference would throw on this grid's point types
   // (empty: it would not), and whether every Actual point has both
