@@ -23,6 +23,7 @@
 // output multiplied by s (so M(a r) = a M(r)), and masked to the unknowns.
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -186,33 +187,33 @@ __global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
   const int nsteps = a.Kpad / 32;
   const bf16 *wbase = a.wt + (int64_t)ct0 * 16 * a.Kpad + lane * 8;
   const int wstride = 16 * a.Kpad;
+  // Both rings are indexed at compile time by the unrolled step u, so no
+  // fragment is ever copied between registers: step u multiplies the weights
+  // aw[u] and the inputs bq[u], then refills aw[u] with step u + WP's weights
+  // (the MFMA has read them at issue) and bq[(u + 1) % WP] with step u + 1's
+  // inputs.  (Copying fragments through a "current" register set cost about
+  // twenty vector instructions per MFMA: the loop was issue-bound.)
   bf16x8 aw[WP][NT];
 #pragma unroll
   for (int u = 0; u < WP; u++)
 #pragma unroll
     for (int t = 0; t < NT; t++) aw[u][t] = *(const bf16x8 *)(wbase + u * 512 + t * wstride);
-  bf16x8 bcur[RPW];
-  load_b(bcur);
+  bf16x8 bq[WP][RPW];
+  load_b(bq[0]);
   for (int s0 = 0; s0 < nsteps; s0 += WP) {
 #pragma unroll
     for (int u = 0; u < WP; u++) {
       const int s = s0 + u;
-      bf16x8 af[NT];
-#pragma unroll
-      for (int t = 0; t < NT; t++) af[t] = aw[u][t];
-      const bf16 *wq = wbase + min(s + WP, nsteps - 1) * 512;
-#pragma unroll
-      for (int t = 0; t < NT; t++) aw[u][t] = *(const bf16x8 *)(wq + t * wstride);
       advance();
-      bf16x8 bnext[RPW];
-      load_b(bnext);
+      load_b(bq[(u + 1) % WP]);
 #pragma unroll
       for (int r = 0; r < RPW; r++)
 #pragma unroll
         for (int t = 0; t < NT; t++)
-          acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t], bcur[r], acc[r][t], 0, 0, 0);
+          acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw[u][t], bq[u][r], acc[r][t], 0, 0, 0);
+      const bf16 *wq = wbase + min(s + WP, nsteps - 1) * 512;
 #pragma unroll
-      for (int r = 0; r < RPW; r++) bcur[r] = bnext[r];
+      for (int t = 0; t < NT; t++) aw[u][t] = *(const bf16x8 *)(wq + t * wstride);
     }
   }
   // epilogue: lane holds D[co = 16t + 4 lg + e][cell = l16]
@@ -855,13 +856,27 @@ static void pack(xf_ctx c, xf_unet u) {
   u->packed = true;
 }
 
+// Every wave fetches the block's weight fragments for its own rows, so the
+// weight traffic per output cell falls with the rows per wave (TZ TY / 4):
+// the widest tile whose input (with its halo) fits the LDS a block may use.
 static void tile_of(int cin, int &TZ, int &TY) {
-  if (cin <= 32) { TZ = 4; TY = 4; }
-  else if (cin <= 64) { TZ = 2; TY = 4; }
-  else { TZ = 2; TY = 2; }
+  if (cin <= 64) { TZ = 4; TY = 4; }
+  else { TZ = 2; TY = 4; }
 }
 
 static size_t conv_lds(int TZ, int TY, int Cin) { return (size_t)(TZ + 2) * (TY + 2) * 18 * (Cin + 8) * sizeof(bf16); }
+
+// LDS a block may use on this device (160 KiB on gfx950)
+static size_t conv_lds_max() {
+  static const size_t v = [] {
+    int dev = 0, bytes = 65536;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      bytes = 65536;
+    return (size_t)std::max(bytes, 65536);
+  }();
+  return v;
+}
 
 // Launch shape: the widest tile (tile_of) and all Cout/16 channel tiles per
 // block while that gives at least kConvBlocks blocks; otherwise shorter tiles
@@ -881,7 +896,7 @@ static void conv_shape(int Cin, int Cout, int B, int D, int H, int W, int &TZ, i
     for (int q = 0; tiles[q][0]; q++) {
       const int tz = tiles[q][0], ty = tiles[q][1];
       if (q > 0 && tz * ty > tz0 * ty0) continue;
-      if (D % tz || H % ty || conv_lds(tz, ty, Cin) > 65536) continue;
+      if (D % tz || H % ty || conv_lds(tz, ty, Cin) > conv_lds_max()) continue;
       const int nb = (W / 16) * (H / ty) * (D / tz) * B * (ntmax / nt);
       if (nb > best) {
         best = nb;
@@ -918,7 +933,7 @@ static void conv(xf_ctx c, const bf16 *src0, int C0, int up0, const bf16 *src1, 
   conv_shape(Cin, Cout, B, D, H, W, a.TZ, a.TY, NT);
   a.nsplit = Cout / 16 / NT;
   const size_t lds = conv_lds(a.TZ, a.TY, Cin);
-  XF_REQUIRE(lds <= 65536 && a.TZ * a.TY >= 4 && a.TZ * a.TY <= 16 && (a.TZ * a.TY) % 4 == 0, XF_EINVAL,
+  XF_REQUIRE(lds <= conv_lds_max() && a.TZ * a.TY >= 4 && a.TZ * a.TY <= 16 && (a.TZ * a.TY) % 4 == 0, XF_EINVAL,
              "unet conv: no tile fits in LDS");
   const int nb = (W / 16) * (H / a.TY) * (D / a.TZ) * B * a.nsplit;
   a.m_cpc = fdiv_magic(Cin / 8);
@@ -926,6 +941,11 @@ static void conv(xf_ctx c, const bf16 *src0, int C0, int up0, const bf16 *src1, 
   const int rpw = a.TZ * a.TY / 4;
 #define XF_CONV_CASE(N, R) \
   if (NT == N && rpw == R) { \
+    static const bool big_lds_ok = [] { \
+      return hipFuncSetAttribute((const void *)k_conv_mfma<N, R>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                 (int)conv_lds_max()) == hipSuccess; \
+    }(); \
+    XF_REQUIRE(lds <= 65536 || big_lds_ok, XF_EHIP, "unet conv: the device refused the LDS size"); \
     hipLaunchKernelGGL((k_conv_mfma<N, R>), dim3(nb), dim3(256), lds, c->stream, a); \
     launched = true; \
   }
