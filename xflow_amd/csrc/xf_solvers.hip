@@ -23,7 +23,7 @@ __global__ void k_lincomb_dev(int64_t nc, const uint8_t *__restrict__ space, con
     else if (p1 != 0) v = p1 * s1[c];
     else if (p2 != 0) v = p2 * s2[c];
     else v = 0;
-    res[c] = v;
+    __builtin_nontemporal_store(v, res + c);  // read again only by a later launch
   }
 }
 
@@ -31,7 +31,7 @@ __global__ void k_copy_dev(int64_t nc, const uint8_t *__restrict__ space, const 
                            const DevState *st) {
   if (st->done) return;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
-    if (!space || space[c]) res[c] = s[c];
+    if (!space || space[c]) __builtin_nontemporal_store(s[c], res + c);
 }
 
 __global__ void k_dot_dev(int64_t nc, const uint8_t *__restrict__ space, const double *__restrict__ a,
