@@ -6,13 +6,16 @@
 // reference's i->j->k order, so each node's value is the reference's sum.
 // Only the cells with a non-zero weight are visited (r < 2h: 4 of the 7 per
 // axis, 5 where rounding puts a cell at 2h just inside; always consecutive);
-// the others add +-0 to the sum and leave it unchanged.  Spreading is a deterministic gather: every (node, non-zero
-// window cell) product is keyed by its cell, a stable radix sort over the
-// bits a cell index needs groups the keys while keeping node order inside
-// each cell, and one thread per cell adds its products in node order - the
-// order of the reference's serial scatter.
-#include <hipcub/hipcub.hpp>
-
+// the others add +-0 to the sum and leave it unchanged.  Spreading is a
+// deterministic gather: one pass forms every node's window (first index,
+// count and weights per axis) and the box all windows cover; then each block
+// owns a 1 x 4 x 64 tile of cells, writes zeros if the tile lies outside
+// that box, and otherwise walks the nodes in chunks of 256, keeps (in node
+// order) those whose window meets the tile, and lets every cell add the
+// products F * wx * wy * wz of the kept nodes that cover it, in node order -
+// the order of the reference's serial scatter, so each cell gets the same
+// sum.
+#include <climits>
 #include <cmath>
 
 #include "xf_internal.h"
@@ -88,41 +91,109 @@ __global__ void k_ib_interp(IbGeom G, int axis, int nn, const double *__restrict
   U[q] = acc;
 }
 
-// 125 (cell, value) slots per node: the products of its non-zero window
-// (at most 5 x 5 x 5) in i->j->k order; unused slots get the key 0xffffffff,
-// which sorts after every cell.
-__global__ void k_ib_products(IbGeom G, int axis, int nn, const double *__restrict__ xyz,
-                              const double *__restrict__ F, uint32_t *__restrict__ keys, double *__restrict__ vals) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  constexpr int kSlots = kIbPer * kIbPer * kIbPer;
-  if (t >= (int64_t)nn * kSlots) return;
-  const int q = (int)(t / kSlots), w = (int)(t % kSlots);
-  const int di = w / (kIbPer * kIbPer), dj = (w / kIbPer) % kIbPer, dk = w % kIbPer;
-  const AxisWeights ax = axis_weights(xyz[3 * q], 1, axis, G.h[0], G.E[0]);
-  const AxisWeights ay = axis_weights(xyz[3 * q + 1], 2, axis, G.h[1], G.E[1]);
-  const AxisWeights az = axis_weights(xyz[3 * q + 2], 3, axis, G.h[2], G.E[2]);
-  if (di >= ax.count || dj >= ay.count || dk >= az.count) {
-    keys[t] = 0xffffffffu;
-    vals[t] = 0;
-    return;
+// One node's window for spreading: per axis the first index with a
+// non-zero weight, the count and the weights, and the node's value.
+struct IbWin {
+  int first[3], count[3];
+  double w[3][kIbPer];
+  double F;
+};
+
+// Windows of the nodes, and per block the box [lo, hi] (per axis) that the
+// block's windows cover.
+__global__ __launch_bounds__(256) void k_ib_windows(IbGeom G, int axis, int nn, const double *__restrict__ xyz,
+                                                    const double *__restrict__ F, IbWin *__restrict__ win,
+                                                    int *__restrict__ part) {
+  __shared__ int red[6][256];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  if (q < nn) {
+    IbWin wq;
+    for (int d = 0; d < 3; d++) {
+      const AxisWeights a = axis_weights(xyz[3 * q + d], d + 1, axis, G.h[d], G.E[d]);
+      wq.first[d] = a.first;
+      wq.count[d] = a.count;
+      for (int v = 0; v < kIbPer; v++) wq.w[d][v] = a.w[v];
+    }
+    wq.F = F[q];
+    win[q] = wq;
+    if (wq.count[0] > 0 && wq.count[1] > 0 && wq.count[2] > 0)
+      for (int d = 0; d < 3; d++) {
+        lo[d] = wq.first[d];
+        hi[d] = wq.first[d] + wq.count[d] - 1;
+      }
   }
-  vals[t] = (F[q] * ax.w[di] * ay.w[dj] * az.w[dk]);  // XProblem.cpp:8596
-  keys[t] = (uint32_t)(((int64_t)(ax.first + di) * G.l + (ay.first + dj)) * G.m + (az.first + dk));
+  for (int d = 0; d < 3; d++) {
+    red[d][threadIdx.x] = lo[d];
+    red[3 + d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int d = 0; d < 3; d++) {
+        red[d][threadIdx.x] = min(red[d][threadIdx.x], red[d][threadIdx.x + h]);
+        red[3 + d][threadIdx.x] = max(red[3 + d][threadIdx.x], red[3 + d][threadIdx.x + h]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void k_ib_zero(int64_t n, double *__restrict__ f) {
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) f[c] = 0;
-}
-
-__global__ void k_ib_segsum(int64_t np, const uint32_t *__restrict__ keys, const double *__restrict__ vals,
-                            double *__restrict__ force) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= np) return;
-  const uint32_t k = keys[t];
-  if (k == 0xffffffffu || (t > 0 && keys[t - 1] == k)) return;
+// force on every cell of the grid (zero where no window reaches); one block
+// per 1 x 4 x 64 tile (i, j, k), one thread per cell.
+constexpr int kGj = 4, kGk = 64;
+__global__ __launch_bounds__(256) void k_ib_gather(int n, int l, int m, int nn, int nparts,
+                                                   const IbWin *__restrict__ win, const int *__restrict__ part,
+                                                   double *__restrict__ force) {
+  __shared__ IbWin kept[256];
+  __shared__ int box[6], wcount[4];
+  const int mt = (m + kGk - 1) / kGk, lt = (l + kGj - 1) / kGj;
+  const int64_t tl = blockIdx.x;
+  const int k0 = (int)(tl % mt) * kGk, j0 = (int)((tl / mt) % lt) * kGj, i = (int)(tl / ((int64_t)mt * lt));
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int j = j0 + (tid >> 6), k = k0 + (tid & 63);
+  if (tid < 6) {  // the box of all windows, folded from the per-block boxes
+    int v = tid < 3 ? INT_MAX : INT_MIN;
+    for (int b = 0; b < nparts; b++) v = tid < 3 ? min(v, part[b * 6 + tid]) : max(v, part[b * 6 + tid]);
+    box[tid] = v;
+  }
+  __syncthreads();
+  const int tlo[3] = {i, j0, k0}, thi[3] = {i, min(j0 + kGj, l) - 1, min(k0 + kGk, m) - 1};
+  bool inside = true;
+  for (int d = 0; d < 3; d++) inside = inside && box[d] <= thi[d] && box[3 + d] >= tlo[d];
   double acc = 0;  // force |= 0.0 (XProblem.cpp:8552)
-  for (int64_t u = t; u < np && keys[u] == k; u++) acc += vals[u];
-  force[k] = acc;
+  if (inside) {
+    for (int base = 0; base < nn; base += 256) {
+      const int q = base + tid;
+      bool hit = false;
+      IbWin wq;
+      if (q < nn) {
+        wq = win[q];
+        hit = true;
+        for (int d = 0; d < 3; d++)
+          hit = hit && wq.count[d] > 0 && wq.first[d] <= thi[d] && wq.first[d] + wq.count[d] - 1 >= tlo[d];
+      }
+      const unsigned long long mask = __ballot(hit);
+      if (lane == 0) wcount[wave] = __popcll(mask);
+      __syncthreads();
+      int pos = __popcll(mask & ((1ull << lane) - 1)), cnt = 0;
+      for (int w = 0; w < 4; w++) {
+        if (w < wave) pos += wcount[w];
+        cnt += wcount[w];
+      }
+      if (hit) kept[pos] = wq;
+      __syncthreads();
+      for (int e = 0; e < cnt; e++) {  // XProblem.cpp:8583-8599, in node order
+        const IbWin &w = kept[e];
+        const unsigned di = (unsigned)(i - w.first[0]), dj = (unsigned)(j - w.first[1]),
+                       dk = (unsigned)(k - w.first[2]);
+        if (di < (unsigned)w.count[0] && dj < (unsigned)w.count[1] && dk < (unsigned)w.count[2])
+          acc += (w.F * w.w[0][di] * w.w[1][dj] * w.w[2][dk]);  // XProblem.cpp:8596
+      }
+      __syncthreads();
+    }
+  }
+  if (j < l && k < m) force[((int64_t)i * l + j) * m + k] = acc;
 }
 
 __global__ void k_ib_move(int nn, double *__restrict__ xyz, double *__restrict__ prev, const double *__restrict__ U,
@@ -152,7 +223,7 @@ static void ib_check(xf_grid g, int axis, int nn) {
   XF_REQUIRE(nn >= 0, XF_EINVAL, "negative node count");
   XF_REQUIRE(g->n >= 2 * RANGE + 2 && g->l >= 2 * RANGE + 2 && g->m >= 2 * RANGE + 2, XF_EINVAL,
              "immersed boundary: the grid must have at least 8 nodes per axis (7-point window)");
-  XF_REQUIRE(g->ncell < 0xffffffffll, XF_EINVAL, "immersed boundary: grid too large for 32-bit cell keys");
+  XF_REQUIRE(g->ncell < 0xffffffffll, XF_EINVAL, "immersed boundary: grid too large (2^32 - 1 cells or more)");
 }
 
 extern "C" int xf_ib_interp(xf_ctx c, xf_grid g, int axis, int nn, const double *xyz, const double *vel, double *U) {
@@ -170,26 +241,17 @@ extern "C" int xf_ib_spread(xf_ctx c, xf_grid g, int axis, int nn, const double 
   return guarded(c, [&] {
     XF_REQUIRE(g && force && (nn == 0 || (xyz && F)), XF_EINVAL, "xf_ib_spread: null argument");
     ib_check(g, axis, nn);
-    // a fill kernel rather than a memset: callers may capture this call in a graph
-    hipLaunchKernelGGL(k_ib_zero, dim3((unsigned)std::min<int64_t>((g->ncell + 255) / 256, 8192)), dim3(256), 0,
-                       c->stream, g->ncell, force);
-    if (nn == 0) return;
-    const int64_t np = (int64_t)nn * kIbPer * kIbPer * kIbPer;
-    int bits = 1;  // cell keys < 2^bits - 1, so that the unused slots' key sorts after them
-    while (((int64_t)1 << bits) <= g->ncell) bits++;
-    uint32_t *k0 = (uint32_t *)ws_get(c, WS_IB_K0, sizeof(uint32_t) * np);
-    uint32_t *k1 = (uint32_t *)ws_get(c, WS_IB_K1, sizeof(uint32_t) * np);
-    double *v0 = (double *)ws_get(c, WS_IB_V0, sizeof(double) * np);
-    double *v1 = (double *)ws_get(c, WS_IB_V1, sizeof(double) * np);
-    hipLaunchKernelGGL(k_ib_products, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, geom(g), axis, nn,
-                       xyz, F, k0, v0);
-    check_kernel("k_ib_products");
-    size_t tb = 0;
-    XF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, (int)np, 0, bits, c->stream));
-    void *tmp = ws_get(c, WS_IB_TMP, tb);
-    XF_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, (int)np, 0, bits, c->stream));
-    hipLaunchKernelGGL(k_ib_segsum, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, np, k1, v1, force);
-    check_kernel("k_ib_segsum");
+    const int nparts = (nn + 255) / 256;
+    IbWin *win = (IbWin *)ws_get(c, WS_IB_WIN, sizeof(IbWin) * (size_t)std::max(nn, 1));
+    int *part = (int *)ws_get(c, WS_IB_PART, sizeof(int) * 6 * (size_t)std::max(nparts, 1));
+    if (nn > 0) {
+      hipLaunchKernelGGL(k_ib_windows, dim3(nparts), dim3(256), 0, c->stream, geom(g), axis, nn, xyz, F, win, part);
+      check_kernel("k_ib_windows");
+    }
+    const int64_t tiles = (int64_t)g->n * ((g->l + kGj - 1) / kGj) * ((g->m + kGk - 1) / kGk);
+    hipLaunchKernelGGL(k_ib_gather, dim3((unsigned)tiles), dim3(256), 0, c->stream, g->n, g->l, g->m, nn, nparts,
+                       win, part, force);
+    check_kernel("k_ib_gather");
   });
 }
 

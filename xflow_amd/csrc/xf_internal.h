@@ -61,8 +61,8 @@ struct DevState {
 // uses one runs on the context's stream, so a later call's reuse is ordered
 // after an earlier call's kernels.
 enum WsSlot {
-  WS_LOD_ST0, WS_LOD_ST1, WS_LOD_ROWS, WS_LOD_SCRA, WS_LOD_SCRB,
-  WS_IB_K0, WS_IB_K1, WS_IB_V0, WS_IB_V1, WS_IB_TMP,
+  WS_LOD_ST0, WS_LOD_ST1, WS_LOD_ROWS, WS_LOD_SCRA,
+  WS_IB_WIN, WS_IB_PART,
   WS_THOMAS_A, WS_THOMAS_B,
   WS_BT_VECS, WS_BT_PART,
   WS_COUNT

@@ -36,7 +36,7 @@ struct LodArgs {
   double *stage[3];          // f_1/3, f_2/3, f (output)
   double tau;
   int scheme;
-  double *scrA, *scrB;       // [maxlen][nlines]
+  double *scrA;              // (A_m, B_m) pairs, [maxlen][nlines][2]
 };
 
 // Up to kLodBatch independent LOD problems advanced by the same launches
@@ -185,6 +185,16 @@ __global__ __launch_bounds__(256) void k_lod_rows(LodSet S) {
 // Normal / PreNormal ends (XProblem.cpp:3064-3155), forward elimination and
 // back substitution (XSLAE.cpp:428-441), written into stage[dg-1]
 // (XProblem.cpp:3439-3449).  The host has checked every line's sectors.
+//
+// Rows arrive in two register blocks of P rows that alternate: while one
+// block is eliminated the other block's loads are in flight (no register
+// copies between the blocks, which would wait for every load outstanding).
+// A block whose rows are all Actual inside an open sector takes the short
+// path; any other block (a sector opens or closes in it, or the line ends)
+// walks its rows again from memory through the general per-row code, which
+// is inlined once per block.  The elimination's (A_m, B_m) pairs go to
+// scratch line-interleaved, one 16-byte store per row, and the back
+// substitution reads them in two alternating register blocks of Q pairs.
 template <int AXIS>
 __global__ __launch_bounds__(64) void k_lod_solve(LodSet S, int dg) {
   const LodArgs &A = S.a[blockIdx.y];
@@ -198,19 +208,20 @@ __global__ __launch_bounds__(64) void k_lod_solve(LodSet S, int dg) {
   line_geometry(A, AXIS, t, base, sa, len, nl);
   const double *h = A.h[AXIS];
   double *fa = A.stage[dg - 1];
-  double *__restrict__ sA = A.scrA;
-  double *__restrict__ sB = A.scrB;
+  double2 *__restrict__ sAB = reinterpret_cast<double2 *>(A.scrA);
+  // Row q of the line and its point type.  Past the line's end the last
+  // point is read again: it is never Actual (the host has checked that every
+  // sector closes), so a block that reaches past the end takes the general
+  // walk, which stops at the end.  The loads carry no condition, so the
+  // waits for one block do not extend to the other block's loads.
   auto load = [&](int q, Row4 &r, int &cd) {
-    if (q < len) {
-      if (AXIS == 2) {
-        cd = codes_kt[(int64_t)q * nl + t];
-        r = rows[(int64_t)q * nl + t];
-      } else {
-        cd = A.codes[base + (int64_t)q * sa];
-        r = rows[base + (int64_t)q * sa];
-      }
+    const int64_t qq = min(q, len - 1);
+    if (AXIS == 2) {
+      cd = codes_kt[qq * nl + t];
+      r = rows[qq * nl + t];
     } else {
-      cd = PT_FICTIVE;
+      cd = A.codes[base + qq * sa];
+      r = rows[base + qq * sa];
     }
   };
   bool in = false;
@@ -220,11 +231,55 @@ __global__ __launch_bounds__(64) void k_lod_solve(LodSet S, int dg) {
     const double den = ra * Am1 + rb;
     const double Am = (-rc) / den;
     const double Bm = (rf - ra * Bm1) / den;
-    sA[(int64_t)s * nl + t] = Am;
-    sB[(int64_t)s * nl + t] = Bm;
+    sAB[(int64_t)s * nl + t] = double2{Am, Bm};
     Am1 = Am;
     Bm1 = Bm;
     s++;
+  };
+  // back substitution of the sector that closes at point `end` of type mend
+  auto close = [&](int end, int mend) {
+    const int fr = (mend != PT_DEFINED && mend != PT_EQUATION);
+    const int sd = end - beg + fl + fr;  // dim - 1
+    const int64_t cb = base + (int64_t)beg * sa, ce = base + (int64_t)end * sa;
+    if (fr) {
+      if (mend == PT_PREDEFINED) fwd(0.5, 0.5, 1, A.rp[ce]);
+      else fwd(-1 / h[end - 1], 1 / h[end - 1], 1, A.rp[ce]);
+    }
+    double u = fr ? 0.0 : A.rp[ce];
+    if (!fr) fa[base + (int64_t)(beg + sd - fl) * sa] = u;
+    constexpr int Q = 8;
+    double2 b0[Q], b1[Q];
+    auto ldblk = [&](int top, double2 *x) {  // pairs top, top-1, ... (below 1: row 0, unused)
+#pragma unroll
+      for (int v = 0; v < Q; v++) x[v] = sAB[(int64_t)max(top - v, 0) * nl + t];
+    };
+    // Every row of a block stores: values that belong to no cell of the
+    // sector (rows below 1, the boundary rows) go to an unused scratch row of
+    // this line, so the number of stores in flight is the same on every path.
+    double *const spare = reinterpret_cast<double *>(sAB + (int64_t)(len + 1) * nl + t);
+    auto useblk = [&](int top, const double2 *x) {
+#pragma unroll
+      for (int v = 0; v < Q; v++) {
+        const int ss = top - v;
+        const double un = x[v].x * u + x[v].y;
+        u = ss > 0 ? un : u;
+        const bool put = ss > 0 && ss >= fl && ss <= sd - fr;
+        *(put ? fa + base + (int64_t)(beg + ss - fl) * sa : spare) = u;
+      }
+    };
+    int s2 = sd - 1;
+    ldblk(s2, b0);
+    while (s2 > 0) {
+      ldblk(s2 - Q, b1);
+      useblk(s2, b0);
+      s2 -= Q;
+      if (s2 <= 0) break;
+      ldblk(s2 - Q, b0);
+      useblk(s2, b1);
+      s2 -= Q;
+    }
+    if (!fl) fa[cb] = u0;
+    in = false;
   };
   auto visit = [&](int q, int cq, const Row4 &rw) {
     if (!in) {
@@ -247,74 +302,45 @@ __global__ __launch_bounds__(64) void k_lod_solve(LodSet S, int dg) {
       }
     }
     pc = cq;
-    if (cq == PT_ACTUAL) {
-      fwd(rw.a, rw.b, rw.c, rw.f);
-      return;
-    }
-    const int end = q, mend = cq;
-    const int fr = (mend != PT_DEFINED && mend != PT_EQUATION);
-    const int sd = end - beg + fl + fr;  // dim - 1
-    const int64_t cb = base + (int64_t)beg * sa, ce = base + (int64_t)end * sa;
-    if (fr) {
-      if (mend == PT_PREDEFINED) fwd(0.5, 0.5, 1, A.rp[ce]);
-      else fwd(-1 / h[end - 1], 1 / h[end - 1], 1, A.rp[ce]);
-    }
-    double u = fr ? 0.0 : A.rp[ce];
-    if (!fr) fa[base + (int64_t)(beg + sd - fl) * sa] = u;
-    // blocks of Q rows from the top, the next block's A, B loaded before
-    // the current one is used
-    constexpr int Q = 8;
-    double ca[Q], cbk[Q], na[Q], nbk[Q];
-    auto ldblk = [&](int top, double *xa, double *xb) {
-#pragma unroll
-      for (int v = 0; v < Q; v++) {
-        const int ss = top - v;
-        if (ss > 0) {
-          xa[v] = sA[(int64_t)ss * nl + t];
-          xb[v] = sB[(int64_t)ss * nl + t];
-        }
-      }
-    };
-    int s2 = sd - 1;
-    ldblk(s2, ca, cbk);
-    while (s2 > 0) {
-      ldblk(s2 - Q, na, nbk);
-#pragma unroll
-      for (int v = 0; v < Q; v++) {
-        const int ss = s2 - v;
-        if (ss > 0) {
-          u = ca[v] * u + cbk[v];
-          if (ss >= fl && ss <= sd - fr) fa[base + (int64_t)(beg + ss - fl) * sa] = u;
-        }
-      }
-#pragma unroll
-      for (int v = 0; v < Q; v++) {
-        ca[v] = na[v];
-        cbk[v] = nbk[v];
-      }
-      s2 -= Q;
-    }
-    if (!fl) fa[cb] = u0;
-    in = false;
+    if (cq == PT_ACTUAL) fwd(rw.a, rw.b, rw.c, rw.f);
+    else close(q, cq);
   };
   constexpr int P = 8;
-  Row4 nxt[P];
-  int ncd[P];
+  // rows q0 .. q0+P-1 held in (r, cd): the short path, or the general walk
+  auto block = [&](int q0, const Row4 *r, const int *cd) {
+    bool quick = in;
 #pragma unroll
-  for (int v = 0; v < P; v++) load(v, nxt[v], ncd[v]);
-  for (int q0 = 0; q0 < len; q0 += P) {
-    Row4 cur[P];
-    int cc[P];
+    for (int v = 0; v < P; v++) quick = quick && cd[v] == PT_ACTUAL;
+    if (quick) {
 #pragma unroll
-    for (int v = 0; v < P; v++) {
-      cur[v] = nxt[v];
-      cc[v] = ncd[v];
+      for (int v = 0; v < P; v++) fwd(r[v].a, r[v].b, r[v].c, r[v].f);
+      pc = PT_ACTUAL;
+    } else {
+      const int qe = min(q0 + P, len);
+      for (int q = q0; q < qe; q++) {
+        Row4 rw;
+        int cq;
+        load(q, rw, cq);
+        visit(q, cq, rw);
+      }
     }
+  };
+  Row4 r0[P], r1[P];
+  int c0[P], c1[P];
 #pragma unroll
-    for (int v = 0; v < P; v++) load(q0 + P + v, nxt[v], ncd[v]);
+  for (int v = 0; v < P; v++) load(v, r0[v], c0[v]);
+  // (the scheduling barriers keep a block's loads ahead of the other
+  // block's elimination instead of letting them sink into it)
+  for (int q0 = 0; q0 < len; q0 += 2 * P) {
 #pragma unroll
-    for (int v = 0; v < P; v++)
-      if (q0 + v < len) visit(q0 + v, cc[v], cur[v]);
+    for (int v = 0; v < P; v++) load(q0 + P + v, r1[v], c1[v]);
+    __builtin_amdgcn_sched_barrier(0);
+    block(q0, r0, c0);
+    if (q0 + P >= len) break;
+#pragma unroll
+    for (int v = 0; v < P; v++) load(q0 + 2 * P + v, r0[v], c0[v]);
+    __builtin_amdgcn_sched_barrier(0);
+    block(q0 + P, r1, c1);
   }
 }
 
@@ -458,8 +484,7 @@ static void run_lod(xf_ctx c, int count, const xf_grid *grids, double *const *f,
   double *st0 = (double *)ws_get(c, WS_LOD_ST0, sizeof(double) * ncs);
   double *st1 = (double *)ws_get(c, WS_LOD_ST1, sizeof(double) * ncs);
   Row4 *rows = (Row4 *)ws_get(c, WS_LOD_ROWS, sizeof(Row4) * ncs);
-  double *scrA = (double *)ws_get(c, WS_LOD_SCRA, sizeof(double) * scs);
-  double *scrB = (double *)ws_get(c, WS_LOD_SCRB, sizeof(double) * scs);
+  double *scrA = (double *)ws_get(c, WS_LOD_SCRA, 2 * sizeof(double) * scs);  // (A_m, B_m) pairs
   LodSet S{};
   S.count = count;
   int64_t oc = 0, os = 0;
@@ -474,7 +499,7 @@ static void run_lod(xf_ctx c, int count, const xf_grid *grids, double *const *f,
     A.stage[0] = st0 + oc; A.stage[1] = st1 + oc; A.stage[2] = f[p];
     A.tau = tau;
     A.scheme = scheme;
-    A.scrA = scrA + os; A.scrB = scrB + os;
+    A.scrA = scrA + 2 * os;
     S.rows[p] = rows + oc;
     S.codes_kt[p] = g->d_codes_kt;
     const int E[3] = {g->n, g->l, g->m};
