@@ -2,8 +2,9 @@
 // (TImmersedBoundaryProblem::Interpolate / SpreadForce / Index / Coord /
 // OptimalDirichlet, XProblem.cpp:8550-8791, 8931-9046).
 //
-// Interpolation: one thread per Lagrangian node adds its window in the
-// reference's i->j->k order, so each node's value is the reference's sum.
+// Interpolation: one wave per Lagrangian node; its lanes form the window's
+// products, one lane adds them in the reference's i->j->k order, so each
+// node's value is the reference's sum.
 // Only the cells with a non-zero weight are visited (r < 2h: 4 of the 7 per
 // axis, 5 where rounding puts a cell at 2h just inside; always consecutive);
 // the others add +-0 to the sum and leave it unchanged.  Spreading is a
@@ -72,22 +73,34 @@ __device__ inline AxisWeights axis_weights(double X, int type, int axis, double 
   return a;
 }
 
-__global__ void k_ib_interp(IbGeom G, int axis, int nn, const double *__restrict__ xyz, const double *__restrict__ vel,
-                            double *__restrict__ U) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+// One block (one wave) per node: three lanes form the axis windows, every
+// lane forms the products of its window cells, one lane adds them in the
+// window's i->j->k order.  Slots outside the window hold +0, which leaves the
+// sum unchanged (it starts at +0 and so is never -0).
+__global__ __launch_bounds__(64) void k_ib_interp(IbGeom G, int axis, int nn, const double *__restrict__ xyz,
+                                                  const double *__restrict__ vel, double *__restrict__ U) {
+  constexpr int kSlots = kIbPer * kIbPer * kIbPer;
+  __shared__ AxisWeights aw[3];
+  __shared__ double prod[kSlots];
+  const int q = blockIdx.x, t = threadIdx.x;
   if (q >= nn) return;
-  const AxisWeights ax = axis_weights(xyz[3 * q], 1, axis, G.h[0], G.E[0]);
-  const AxisWeights ay = axis_weights(xyz[3 * q + 1], 2, axis, G.h[1], G.E[1]);
-  const AxisWeights az = axis_weights(xyz[3 * q + 2], 3, axis, G.h[2], G.E[2]);
+  if (t < 3) aw[t] = axis_weights(xyz[3 * q + t], t + 1, axis, G.h[t], G.E[t]);
+  __syncthreads();
   const double ha = G.h[axis - 1];
   const double cb = ha * ha * ha;  // CB(separator->Separation[0]), XProblem.cpp:8782
+  for (int w = t; w < kSlots; w += 64) {
+    const int a = w / (kIbPer * kIbPer), b = (w / kIbPer) % kIbPer, e = w % kIbPer;
+    double pr = 0;
+    if (a < aw[0].count && b < aw[1].count && e < aw[2].count) {
+      const int64_t c = ((int64_t)(aw[0].first + a) * G.l + (aw[1].first + b)) * G.m + (aw[2].first + e);
+      pr = (vel[c] * aw[0].w[a] * aw[1].w[b] * aw[2].w[e]) * cb;  // XProblem.cpp:8742-8783
+    }
+    prod[w] = pr;
+  }
+  __syncthreads();
+  if (t != 0) return;
   double acc = 0;
-  for (int a = 0; a < ax.count; a++)
-    for (int b = 0; b < ay.count; b++)
-      for (int e = 0; e < az.count; e++) {
-        const int64_t c = ((int64_t)(ax.first + a) * G.l + (ay.first + b)) * G.m + (az.first + e);
-        acc += (vel[c] * ax.w[a] * ay.w[b] * az.w[e]) * cb;  // XProblem.cpp:8742-8783
-      }
+  for (int w = 0; w < kSlots; w++) acc += prod[w];
   U[q] = acc;
 }
 
@@ -103,7 +116,7 @@ struct IbWin {
 // block's windows cover.
 __global__ __launch_bounds__(256) void k_ib_windows(IbGeom G, int axis, int nn, const double *__restrict__ xyz,
                                                     const double *__restrict__ F, IbWin *__restrict__ win,
-                                                    int *__restrict__ part) {
+                                                    int *__restrict__ nbox, int *__restrict__ part) {
   __shared__ int red[6][256];
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
@@ -122,6 +135,10 @@ __global__ __launch_bounds__(256) void k_ib_windows(IbGeom G, int axis, int nn, 
         lo[d] = wq.first[d];
         hi[d] = wq.first[d] + wq.count[d] - 1;
       }
+    for (int d = 0; d < 3; d++) {  // an empty window gets an empty box
+      nbox[6 * q + d] = lo[d];
+      nbox[6 * q + 3 + d] = hi[d];
+    }
   }
   for (int d = 0; d < 3; d++) {
     red[d][threadIdx.x] = lo[d];
@@ -140,38 +157,49 @@ __global__ __launch_bounds__(256) void k_ib_windows(IbGeom G, int axis, int nn, 
 }
 
 // force on every cell of the grid (zero where no window reaches); one block
-// per 1 x 4 x 64 tile (i, j, k), one thread per cell.
+// per 1 x 4 x 64 tile (i, j, k), one thread per cell.  Node chunks (the
+// blocks of k_ib_windows) whose box misses the tile are skipped whole; in
+// the others each node's box is tested before its window is read.
 constexpr int kGj = 4, kGk = 64;
 __global__ __launch_bounds__(256) void k_ib_gather(int n, int l, int m, int nn, int nparts,
-                                                   const IbWin *__restrict__ win, const int *__restrict__ part,
-                                                   double *__restrict__ force) {
+                                                   const IbWin *__restrict__ win, const int *__restrict__ nbox,
+                                                   const int *__restrict__ part, double *__restrict__ force) {
+  constexpr int kChunksLds = 64;  // chunk boxes kept in LDS (the rest read from memory)
   __shared__ IbWin kept[256];
-  __shared__ int box[6], wcount[4];
+  __shared__ int box[6], wcount[4], cbox[6 * kChunksLds];
   const int mt = (m + kGk - 1) / kGk, lt = (l + kGj - 1) / kGj;
   const int64_t tl = blockIdx.x;
   const int k0 = (int)(tl % mt) * kGk, j0 = (int)((tl / mt) % lt) * kGj, i = (int)(tl / ((int64_t)mt * lt));
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int j = j0 + (tid >> 6), k = k0 + (tid & 63);
-  if (tid < 6) {  // the box of all windows, folded from the per-block boxes
-    int v = tid < 3 ? INT_MAX : INT_MIN;
-    for (int b = 0; b < nparts; b++) v = tid < 3 ? min(v, part[b * 6 + tid]) : max(v, part[b * 6 + tid]);
-    box[tid] = v;
+  const int tlo[3] = {i, j0, k0}, thi[3] = {i, min(j0 + kGj, l) - 1, min(k0 + kGk, m) - 1};
+  auto meets = [&](const int *bx) {  // box (lo[3], hi[3]) meets the tile
+    bool r = true;
+    for (int d = 0; d < 3; d++) r = r && bx[d] <= thi[d] && bx[3 + d] >= tlo[d];
+    return r;
+  };
+  // the box of all windows, folded from the chunks' boxes
+  if (tid < 6) box[tid] = tid < 3 ? INT_MAX : INT_MIN;
+  __syncthreads();
+  for (int t = tid; t < nparts * 6; t += 256) {
+    const int d = t % 6, v = part[t];
+    if (t < 6 * kChunksLds) cbox[t] = v;
+    if (d < 3) atomicMin(&box[d], v);
+    else atomicMax(&box[d], v);
   }
   __syncthreads();
-  const int tlo[3] = {i, j0, k0}, thi[3] = {i, min(j0 + kGj, l) - 1, min(k0 + kGk, m) - 1};
-  bool inside = true;
-  for (int d = 0; d < 3; d++) inside = inside && box[d] <= thi[d] && box[3 + d] >= tlo[d];
   double acc = 0;  // force |= 0.0 (XProblem.cpp:8552)
-  if (inside) {
-    for (int base = 0; base < nn; base += 256) {
-      const int q = base + tid;
+  if (meets(box)) {
+    for (int ch = 0; ch < nparts; ch++) {
+      int cb[6];
+      for (int d = 0; d < 6; d++) cb[d] = ch < kChunksLds ? cbox[6 * ch + d] : part[6 * ch + d];
+      if (!meets(cb)) continue;  // the same for every thread of the block
+      const int q = ch * 256 + tid;
       bool hit = false;
-      IbWin wq;
       if (q < nn) {
-        wq = win[q];
-        hit = true;
-        for (int d = 0; d < 3; d++)
-          hit = hit && wq.count[d] > 0 && wq.first[d] <= thi[d] && wq.first[d] + wq.count[d] - 1 >= tlo[d];
+        int nb[6];
+        for (int d = 0; d < 6; d++) nb[d] = nbox[6 * q + d];
+        hit = meets(nb);
       }
       const unsigned long long mask = __ballot(hit);
       if (lane == 0) wcount[wave] = __popcll(mask);
@@ -181,7 +209,7 @@ __global__ __launch_bounds__(256) void k_ib_gather(int n, int l, int m, int nn, 
         if (w < wave) pos += wcount[w];
         cnt += wcount[w];
       }
-      if (hit) kept[pos] = wq;
+      if (hit) kept[pos] = win[q];
       __syncthreads();
       for (int e = 0; e < cnt; e++) {  // XProblem.cpp:8583-8599, in node order
         const IbWin &w = kept[e];
@@ -231,7 +259,7 @@ extern "C" int xf_ib_interp(xf_ctx c, xf_grid g, int axis, int nn, const double 
     XF_REQUIRE(g && (nn == 0 || (xyz && vel && U)), XF_EINVAL, "xf_ib_interp: null argument");
     ib_check(g, axis, nn);
     if (nn == 0) return;
-    hipLaunchKernelGGL(k_ib_interp, dim3((nn + 63) / 64), dim3(64), 0, c->stream, geom(g), axis, nn, xyz, vel, U);
+    hipLaunchKernelGGL(k_ib_interp, dim3(nn), dim3(64), 0, c->stream, geom(g), axis, nn, xyz, vel, U);
     check_kernel("k_ib_interp");
   });
 }
@@ -243,14 +271,16 @@ extern "C" int xf_ib_spread(xf_ctx c, xf_grid g, int axis, int nn, const double 
     ib_check(g, axis, nn);
     const int nparts = (nn + 255) / 256;
     IbWin *win = (IbWin *)ws_get(c, WS_IB_WIN, sizeof(IbWin) * (size_t)std::max(nn, 1));
+    int *nbox = (int *)ws_get(c, WS_IB_BOX, sizeof(int) * 6 * (size_t)std::max(nn, 1));
     int *part = (int *)ws_get(c, WS_IB_PART, sizeof(int) * 6 * (size_t)std::max(nparts, 1));
     if (nn > 0) {
-      hipLaunchKernelGGL(k_ib_windows, dim3(nparts), dim3(256), 0, c->stream, geom(g), axis, nn, xyz, F, win, part);
+      hipLaunchKernelGGL(k_ib_windows, dim3(nparts), dim3(256), 0, c->stream, geom(g), axis, nn, xyz, F, win, nbox,
+                         part);
       check_kernel("k_ib_windows");
     }
     const int64_t tiles = (int64_t)g->n * ((g->l + kGj - 1) / kGj) * ((g->m + kGk - 1) / kGk);
     hipLaunchKernelGGL(k_ib_gather, dim3((unsigned)tiles), dim3(256), 0, c->stream, g->n, g->l, g->m, nn, nparts,
-                       win, part, force);
+                       win, nbox, part, force);
     check_kernel("k_ib_gather");
   });
 }

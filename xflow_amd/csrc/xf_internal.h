@@ -62,7 +62,7 @@ struct DevState {
 // after an earlier call's kernels.
 enum WsSlot {
   WS_LOD_ST0, WS_LOD_ST1, WS_LOD_ROWS, WS_LOD_SCRA,
-  WS_IB_WIN, WS_IB_PART,
+  WS_IB_WIN, WS_IB_BOX, WS_IB_PART,
   WS_THOMAS_A, WS_THOMAS_B,
   WS_BT_VECS, WS_BT_PART,
   WS_COUNT

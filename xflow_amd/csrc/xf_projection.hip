@@ -92,14 +92,42 @@ __global__ void k_vel_correct_predef(int n, int l, int m, const uint8_t *__restr
   }
 }
 
-// max |div u| over Actual pressure cells with the first (i->j->k) maximising cell
+// (value, cell) pairs: the larger value wins, ties go to the lower cell
+// index, and -1 (no cell) loses every tie
+__device__ inline void max_pair(double &b, int64_t &bi, double v2, int64_t i2) {
+  if (v2 > b || (v2 == b && i2 >= 0 && (bi < 0 || i2 < bi))) {
+    b = v2;
+    bi = i2;
+  }
+}
+
+__device__ inline void block_max_pair(double &b, int64_t &bi, double *lv, int64_t *li) {
+  lv[threadIdx.x] = b;
+  li[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      double v = lv[threadIdx.x];
+      int64_t i = li[threadIdx.x];
+      max_pair(v, i, lv[threadIdx.x + s], li[threadIdx.x + s]);
+      lv[threadIdx.x] = v;
+      li[threadIdx.x] = i;
+    }
+    __syncthreads();
+  }
+  b = lv[0];
+  bi = li[0];
+}
+
+// max |div u| over Actual pressure cells with the first (i->j->k) maximising
+// cell: per-block partials here, folded by k_divmax_finish (no device-scope
+// fences or atomics between the blocks)
 __global__ void k_divmax(int n, int l, int m, const uint8_t *__restrict__ codes, const double *__restrict__ u,
                          const double *__restrict__ v, const double *__restrict__ w, const double *__restrict__ hxU,
-                         const double *__restrict__ hyV, const double *__restrict__ hzW, double *pv, int64_t *pi,
-                         unsigned *counter, double *out, int64_t *out_idx) {
+                         const double *__restrict__ hyV, const double *__restrict__ hzW, double *__restrict__ pv,
+                         int64_t *__restrict__ pi) {
   __shared__ double lv[kBlock];
   __shared__ int64_t li[kBlock];
-  __shared__ bool am_last;
   const int64_t nc = (int64_t)n * l * m, sI = (int64_t)l * m, sJ = m;
   double best = 0;
   int64_t bi = -1;
@@ -112,60 +140,24 @@ __global__ void k_divmax(int n, int l, int m, const uint8_t *__restrict__ codes,
     const double a = fabs(div);
     if (a > best) { best = a; bi = c; }
   }
-  lv[threadIdx.x] = best;
-  li[threadIdx.x] = bi;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      const double v2 = lv[threadIdx.x + s];
-      const int64_t i2 = li[threadIdx.x + s];
-      if (v2 > lv[threadIdx.x] || (v2 == lv[threadIdx.x] && i2 >= 0 && (li[threadIdx.x] < 0 || i2 < li[threadIdx.x]))) {
-        lv[threadIdx.x] = v2;
-        li[threadIdx.x] = i2;
-      }
-    }
-    __syncthreads();
-  }
+  block_max_pair(best, bi, lv, li);
   if (threadIdx.x == 0) {
-    pv[blockIdx.x] = lv[0];
-    pi[blockIdx.x] = li[0];
-    __threadfence();
-    am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    pv[blockIdx.x] = best;
+    pi[blockIdx.x] = bi;
   }
-  __syncthreads();
-  if (!am_last) return;
-  __threadfence();
-  // the last block: every thread folds a strided share of the partials, then
-  // the block folds the threads' results (ties to the lowest cell index)
+}
+
+__global__ void k_divmax_finish(int np, const double *__restrict__ pv, const int64_t *__restrict__ pi, double *out,
+                                int64_t *out_idx) {
+  __shared__ double lv[kBlock];
+  __shared__ int64_t li[kBlock];
   double b = 0;
   int64_t idx = -1;
-  for (unsigned q = threadIdx.x; q < gridDim.x; q += blockDim.x) {
-    const double v2 = __hip_atomic_load(&pv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int64_t i2 = __hip_atomic_load(&pi[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (v2 > b || (v2 == b && i2 >= 0 && (idx < 0 || i2 < idx))) {
-      b = v2;
-      idx = i2;
-    }
-  }
-  __syncthreads();
-  lv[threadIdx.x] = b;
-  li[threadIdx.x] = idx;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      const double v2 = lv[threadIdx.x + s];
-      const int64_t i2 = li[threadIdx.x + s];
-      if (v2 > lv[threadIdx.x] || (v2 == lv[threadIdx.x] && i2 >= 0 && (li[threadIdx.x] < 0 || i2 < li[threadIdx.x]))) {
-        lv[threadIdx.x] = v2;
-        li[threadIdx.x] = i2;
-      }
-    }
-    __syncthreads();
-  }
+  for (int q = threadIdx.x; q < np; q += blockDim.x) max_pair(b, idx, pv[q], pi[q]);
+  block_max_pair(b, idx, lv, li);
   if (threadIdx.x != 0) return;
-  *out = lv[0];
-  *out_idx = lv[0] > 0 ? li[0] : -1;
-  *counter = 0;
+  *out = b;
+  *out_idx = b > 0 ? idx : -1;
 }
 
 namespace {
@@ -277,7 +269,8 @@ extern "C" int xf_divergence_max(xf_ctx c, xf_grid gp, const double *u, const do
     int64_t *pi = (int64_t *)(c->partials + kMaxPartials);
     int64_t *d_idx = (int64_t *)(c->partials + 2 * kMaxPartials);
     hipLaunchKernelGGL(k_divmax, dim3(nb), dim3(kBlock), 0, c->stream, gp->n, gp->l, gp->m, gp->d_codes, u, v, w,
-                       sx, sy, sz, pv, pi, c->counters, c->d_scalar, d_idx);
+                       sx, sy, sz, pv, pi);
+    hipLaunchKernelGGL(k_divmax_finish, dim3(1), dim3(kBlock), 0, c->stream, nb, pv, pi, c->d_scalar, d_idx);
     check_kernel("k_divmax");
     int64_t idx;
     XF_HIP(hipMemcpyAsync(max_abs, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
