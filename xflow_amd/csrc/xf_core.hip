@@ -289,6 +289,7 @@ void free_grid(xf_grid g) {
   hipFree(g->d_codes); hipFree(g->d_flags); hipFree(g->d_space); hipFree(g->d_rn);
   hipFree(g->d_normals); hipFree(g->d_has_normal); hipFree(g->d_rho);
   hipFree(g->d_codes_kt);
+  hipFree(g->d_lod_border);
   hipFree(g->d_other_rows);
   for (int k = 0; k < 4; k++) hipFree(g->steps_d[k]);
   free_cg_work(g->work);

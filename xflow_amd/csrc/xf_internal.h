@@ -147,6 +147,8 @@ struct xf_grid_s {
   bool lod_checked = false;
   int lod_err = 0;
   uint8_t *d_codes_kt = nullptr;
+  int32_t *d_lod_border = nullptr;  // cells neither Actual nor Fictive (the LOD's border passes)
+  int64_t n_lod_border = 0;
   // projection glue: point-type errors per call form (computed on first use,
   // -1 = not yet), and device copies of the face steps last passed
   int glue_rhs_err = -1;

@@ -12,9 +12,12 @@
 //                line solve reads its rows coalesced;
 //   k_lod_solve  one thread per line: the sector walk, the boundary rows and
 //                the Thomas elimination (XSLAE.cpp:416-445) in the
-//                reference's order, rows prefetched eight cells ahead, the
-//                elimination's A[m], B[m] line-interleaved in scratch;
-//   k_lod_border the two border-only passes, one thread per border cell (a
+//                reference's order, rows in two alternating register blocks
+//                of eight, the elimination's (A[m], B[m]) pairs
+//                line-interleaved in scratch;
+//   k_lod_border the two border-only passes, one thread per cell of a
+//                per-grid list of the cells that are neither Actual nor
+//                Fictive (a
 //                sector end reads only its Actual neighbour, which no pass
 //                writes; where a cell ends one sector and starts the next, or
 //                borders sectors of both passes, the later rule wins, as in
@@ -46,6 +49,8 @@ struct LodSet {
   LodArgs a[kLodBatch];
   struct Row4 *rows[kLodBatch];
   const uint8_t *codes_kt[kLodBatch];
+  const int32_t *border[kLodBatch];  // cells neither Actual nor Fictive
+  int64_t nborder[kLodBatch];
   int count;
 };
 
@@ -345,17 +350,19 @@ __global__ __launch_bounds__(64) void k_lod_solve(LodSet S, int dg) {
 }
 
 // The border-only passes of direction group dg along ax1, then ax2
-// (XProblem.cpp:2965-3033), one thread per cell.
+// (XProblem.cpp:2965-3033), one thread per cell of the grid's list of cells
+// that are neither Actual nor Fictive (the only ones a pass can write).
 __global__ __launch_bounds__(256) void k_lod_border(LodSet set, int dg, int ax1, int ax2) {
   const LodArgs &A = set.a[blockIdx.y];
+  const int32_t *__restrict__ list = set.border[blockIdx.y];
+  const int64_t cnt = set.nborder[blockIdx.y];
   const int n = A.n, l = A.l, m = A.m;
-  const int64_t nc = (int64_t)n * l * m;
   const int64_t S[3] = {(int64_t)l * m, (int64_t)m, 1};
   const int E[3] = {n, l, m};
   double *fa = A.stage[dg - 1];
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = list[t];
     const int cc = A.codes[c];
-    if (cc == PT_ACTUAL || cc == PT_FICTIVE) continue;
     const int idx3[3] = {(int)(c / S[0]), (int)((c / m) % l), (int)(c % m)};
     bool any = false;
     double v = 0;
@@ -433,6 +440,15 @@ static int lod_check(xf_grid g) {
         if (in) err |= 8;
       }
   }
+  if (!err && !g->d_lod_border) {
+    std::vector<int32_t> bl;
+    for (int64_t q = 0; q < g->ncell; q++)
+      if (cd[(size_t)q] != PT_ACTUAL && cd[(size_t)q] != PT_FICTIVE) bl.push_back((int32_t)q);
+    g->n_lod_border = (int64_t)bl.size();
+    XF_HIP(hipMalloc(&g->d_lod_border, sizeof(int32_t) * std::max<size_t>(bl.size(), 1)));
+    if (!bl.empty())
+      XF_HIP(hipMemcpy(g->d_lod_border, bl.data(), sizeof(int32_t) * bl.size(), hipMemcpyHostToDevice));
+  }
   if (!err && !g->d_codes_kt) {
     // point types of lines along k in line-interleaved order
     const int64_t nl = (int64_t)g->n * g->l;
@@ -487,6 +503,7 @@ static void run_lod(xf_ctx c, int count, const xf_grid *grids, double *const *f,
   double *scrA = (double *)ws_get(c, WS_LOD_SCRA, 2 * sizeof(double) * scs);  // (A_m, B_m) pairs
   LodSet S{};
   S.count = count;
+  int64_t nbmax = 0;
   int64_t oc = 0, os = 0;
   for (int p = 0; p < count; p++) {
     const xf_grid g = grids[p];
@@ -502,6 +519,9 @@ static void run_lod(xf_ctx c, int count, const xf_grid *grids, double *const *f,
     A.scrA = scrA + 2 * os;
     S.rows[p] = rows + oc;
     S.codes_kt[p] = g->d_codes_kt;
+    S.border[p] = g->d_lod_border;
+    S.nborder[p] = g->n_lod_border;
+    nbmax = std::max(nbmax, g->n_lod_border);
     const int E[3] = {g->n, g->l, g->m};
     int64_t maxlines = 0;
     int maxlen = 0;
@@ -514,6 +534,7 @@ static void run_lod(xf_ctx c, int count, const xf_grid *grids, double *const *f,
   }
   const unsigned cb = (unsigned)std::min<int64_t>((ncmax + 255) / 256, 16384);
   const dim3 cells(cb, count);
+  const dim3 borders((unsigned)std::max<int64_t>(1, std::min<int64_t>((nbmax + 255) / 256, 4096)), count);
   hipLaunchKernelGGL(k_lod_stages, cells, dim3(256), 0, c->stream, S);
   for (int dg = 1; dg < 4; dg++) {  // XProblem.cpp:2847-2864
     const int axis = dg - 1;
@@ -521,16 +542,16 @@ static void run_lod(xf_ctx c, int count, const xf_grid *grids, double *const *f,
     if (axis == 0) {
       hipLaunchKernelGGL(k_lod_rows<0>, cells, dim3(256), 0, c->stream, S);
       hipLaunchKernelGGL(k_lod_solve<0>, lines, dim3(64), 0, c->stream, S, dg);
-      hipLaunchKernelGGL(k_lod_border, cells, dim3(256), 0, c->stream, S, dg, 1, 2);
+      hipLaunchKernelGGL(k_lod_border, borders, dim3(256), 0, c->stream, S, dg, 1, 2);
     } else if (axis == 1) {
       hipLaunchKernelGGL(k_lod_rows<1>, cells, dim3(256), 0, c->stream, S);
       hipLaunchKernelGGL(k_lod_solve<1>, lines, dim3(64), 0, c->stream, S, dg);
-      hipLaunchKernelGGL(k_lod_border, cells, dim3(256), 0, c->stream, S, dg, 0, 2);
+      hipLaunchKernelGGL(k_lod_border, borders, dim3(256), 0, c->stream, S, dg, 0, 2);
     } else {
       hipLaunchKernelGGL(k_lod_rows<2>, dim3((mmax + 15) / 16, (lmax + 15) / 16, nmax * count), dim3(256), 0,
                          c->stream, S);
       hipLaunchKernelGGL(k_lod_solve<2>, lines, dim3(64), 0, c->stream, S, dg);
-      hipLaunchKernelGGL(k_lod_border, cells, dim3(256), 0, c->stream, S, dg, 0, 1);
+      hipLaunchKernelGGL(k_lod_border, borders, dim3(256), 0, c->stream, S, dg, 0, 1);
     }
     check_kernel("LOD sweep");
   }
