@@ -290,6 +290,7 @@ void free_grid(xf_grid g) {
   hipFree(g->d_normals); hipFree(g->d_has_normal); hipFree(g->d_rho);
   hipFree(g->d_codes_kt);
   hipFree(g->d_lod_border);
+  hipFree(g->d_predef);
   hipFree(g->d_other_rows);
   for (int k = 0; k < 4; k++) hipFree(g->steps_d[k]);
   free_cg_work(g->work);

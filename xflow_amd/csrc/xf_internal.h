@@ -149,6 +149,8 @@ struct xf_grid_s {
   uint8_t *d_codes_kt = nullptr;
   int32_t *d_lod_border = nullptr;  // cells neither Actual nor Fictive (the LOD's border passes)
   int64_t n_lod_border = 0;
+  int32_t *d_predef = nullptr;  // PreDefined cells (velocity correction)
+  int64_t n_predef = -1;
   // projection glue: point-type errors per call form (computed on first use,
   // -1 = not yet), and device copies of the face steps last passed
   int glue_rhs_err = -1;

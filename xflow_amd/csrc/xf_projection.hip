@@ -63,13 +63,15 @@ __global__ void k_vel_correct(int n, int l, int m, int axis, const uint8_t *__re
   }
 }
 
-// pass 2: PreDefined points mirror their single Actual neighbour (XProblem.cpp:5698-5758)
+// pass 2: PreDefined points mirror their single Actual neighbour
+// (XProblem.cpp:5698-5758); one thread per cell of the grid's list of
+// PreDefined cells
 __global__ void k_vel_correct_predef(int n, int l, int m, const uint8_t *__restrict__ codes,
+                                     const int32_t *__restrict__ list, int64_t cnt,
                                      const double *__restrict__ us, const double *__restrict__ bc,
                                      double *__restrict__ out, int *err) {
-  const int64_t nc = (int64_t)n * l * m;
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
-    if (codes[c] != PT_PREDEFINED) continue;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = list[t];
     const int k = (int)(c % m), j = (int)((c / m) % l), i = (int)(c / ((int64_t)l * m));
     int64_t nb = -1;
     int count = 0;
@@ -249,8 +251,19 @@ extern "C" int xf_velocity_correct(xf_ctx c, xf_grid gv, int axis, const double 
     const double *sp = device_steps(c, gv, 3, hP, E[axis - 1] - 1);
     hipLaunchKernelGGL(k_vel_correct, dim3(nblocks(gv->ncell)), dim3(kBlock), 0, c->stream, gv->n, gv->l, gv->m,
                        axis - 1, gv->d_codes, u_star, p, rho, sp, bc, tau, u_out, c->d_err);
-    hipLaunchKernelGGL(k_vel_correct_predef, dim3(nblocks(gv->ncell)), dim3(kBlock), 0, c->stream, gv->n, gv->l,
-                       gv->m, gv->d_codes, u_star, bc, u_out, c->d_err);
+    if (gv->n_predef < 0) {  // the PreDefined cells, listed once per grid
+      const std::vector<uint8_t> &cd = host_codes(gv);
+      std::vector<int32_t> pl;
+      for (int64_t q = 0; q < gv->ncell; q++)
+        if (cd[(size_t)q] == PT_PREDEFINED) pl.push_back((int32_t)q);
+      XF_HIP(hipMalloc(&gv->d_predef, sizeof(int32_t) * std::max<size_t>(pl.size(), 1)));
+      if (!pl.empty())
+        XF_HIP(hipMemcpy(gv->d_predef, pl.data(), sizeof(int32_t) * pl.size(), hipMemcpyHostToDevice));
+      gv->n_predef = (int64_t)pl.size();
+    }
+    if (gv->n_predef > 0)
+      hipLaunchKernelGGL(k_vel_correct_predef, dim3(nblocks(gv->n_predef)), dim3(kBlock), 0, c->stream, gv->n, gv->l,
+                         gv->m, gv->d_codes, gv->d_predef, gv->n_predef, u_star, bc, u_out, c->d_err);
     check_kernel("k_vel_correct");
     raise_device_errors(c);
   });
