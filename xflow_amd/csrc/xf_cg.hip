@@ -1068,6 +1068,12 @@ static void cgcg_run_fused(xf_ctx c, xf_grid g, X *x, T *const R[2], T *const P[
   }
   long long na = 1;
   int it = 0;
+  // Without profiling the host checks each batch's state one batch late:
+  // the next batch is already queued when it waits, so the GPU does not idle
+  // for the round trip (launches after convergence return at once).
+  const bool ahead = !c->prof;
+  DevState *const hs[2] = {c->h_state, c->h_state2};
+  int batch = 0, pending = -1, final_slot = 0;
   for (;;) {
     for (int q = 0; q < every; q++, it++) {
       const bool pm = c->prof && it % kProfEvery < 2;
@@ -1082,6 +1088,21 @@ static void cgcg_run_fused(xf_ctx c, xf_grid g, X *x, T *const R[2], T *const P[
       }
     }
     check_kernel("cg iterations");
+    if (ahead) {
+      const int slot = batch & 1;
+      XF_HIP(hipMemcpyAsync(hs[slot], c->state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+      XF_HIP(hipEventRecord(c->evp[slot], c->stream));
+      batch++;
+      if (pending >= 0) {
+        XF_HIP(hipEventSynchronize(c->evp[pending]));
+        if (hs[pending]->done) {
+          final_slot = pending;
+          break;
+        }
+      }
+      pending = slot;
+      continue;
+    }
     XF_HIP(hipMemcpyAsync(c->h_state, c->state, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
     XF_HIP(hipStreamSynchronize(c->stream));
     if (c->prof) {
@@ -1095,6 +1116,10 @@ static void cgcg_run_fused(xf_ctx c, xf_grid g, X *x, T *const R[2], T *const P[
       sampled.clear();
     }
     if (c->h_state->done) break;
+  }
+  if (ahead) {  // the batch queued after the deciding one has returned; its copy is the same state
+    XF_HIP(hipStreamSynchronize(c->stream));
+    if (final_slot != 0) *c->h_state = *hs[final_slot];
   }
   const int applied = c->h_state->iter;
   if (applied & 1)

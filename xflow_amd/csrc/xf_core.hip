@@ -36,6 +36,9 @@ extern "C" int xf_init(int device, xf_ctx *out) {
     XF_HIP(hipMalloc(&c->state, sizeof(DevState)));
     XF_HIP(hipMemset(c->state, 0, sizeof(DevState)));
     XF_HIP(hipHostMalloc((void **)&c->h_state, sizeof(DevState), hipHostMallocDefault));
+    XF_HIP(hipHostMalloc((void **)&c->h_state2, sizeof(DevState), hipHostMallocDefault));
+    XF_HIP(hipEventCreateWithFlags(&c->evp[0], hipEventDisableTiming));
+    XF_HIP(hipEventCreateWithFlags(&c->evp[1], hipEventDisableTiming));
     XF_HIP(hipEventCreate(&c->ev0));
     XF_HIP(hipEventCreate(&c->ev1));
     XF_HIP(hipMalloc(&c->d_err, sizeof(int)));
@@ -60,6 +63,9 @@ extern "C" int xf_shutdown(xf_ctx c) {
   hipFree(c->d_iscalar);
   hipFree(c->state);
   hipHostFree(c->h_state);
+  hipHostFree(c->h_state2);
+  if (c->evp[0]) hipEventDestroy(c->evp[0]);
+  if (c->evp[1]) hipEventDestroy(c->evp[1]);
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
   hipFree(c->d_err);

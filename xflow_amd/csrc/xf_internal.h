@@ -92,7 +92,9 @@ struct xf_ctx_s {
   int32_t *d_iscalar = nullptr;   // [kSlots]
   DevState *state = nullptr;      // solver state
   DevState *h_state = nullptr;    // pinned mirror
+  DevState *h_state2 = nullptr;   // a second pinned mirror (the one-batch-late check)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evp[2] = {nullptr, nullptr};  // the state copies of the one-batch-late check
   // per-kernel event timing of the throughput solvers (xf_profile_enable)
   bool prof = false;
   std::vector<hipEvent_t> prof_events;
