@@ -12,7 +12,7 @@ FP32 fields, as on the device; the GPU result agrees to FP32 rounding
 import numpy as np
 
 FICTIVE, ACTUAL = 0, 1
-OMEGA, SWEEPS, COARSE_SWEEPS = np.float32(0.8), 2, 8
+OMEGA, SWEEPS, COARSE_SWEEPS = np.float32(0.857142857), 4, 8
 COARSE_CELLS = 512
 
 

@@ -33,8 +33,8 @@
 
 #include "xf_internal.h"
 
-constexpr float kOmega = 0.8f;
-constexpr int kSweeps = 2;        // pre- and post-smoothing sweeps per level
+constexpr float kOmega = 0.857142857f;  // 6/7
+constexpr int kSweeps = 4;        // pre- and post-smoothing sweeps per level
 constexpr int kCoarseSweeps = 8;  // coarsest-level sweeps without a network
 constexpr int kMgBlocks = 512;    // blocks of the reduction kernels
 constexpr int kCoarseCells = 512;   // cells (ring included) of a grid solved densely
