@@ -6,7 +6,9 @@
 // followed by two border-only passes along the other directions.  Here each
 // direction is three launches:
 //   k_lod_rows   the tridiagonal row (a, b, c, F) of every Actual cell, one
-//                thread per cell (a row reads only the previous level and the
+//                thread per cell (the first direction's pass also starts the
+//                two stages as copies of the previous level; a row reads only
+//                the previous level and the
 //                stages of earlier directions); rows of lines along k are
 //                written line-interleaved through an LDS transpose, so every
 //                line solve reads its rows coalesced;
@@ -154,6 +156,11 @@ __global__ __launch_bounds__(256) void k_lod_rows(LodSet S) {
   if (AXIS < 2) {
     const int64_t nc = (int64_t)n * l * m;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+      if (AXIS == 0) {  // f_1/3 and f_2/3 start as copies of the previous level (XProblem.cpp:2796-2797)
+        const double v = A.f_prev[c];
+        A.stage[0][c] = v;
+        A.stage[1][c] = v;
+      }
       if (A.codes[c] != PT_ACTUAL) continue;
       const int idx3[3] = {(int)(c / ((int64_t)l * m)), (int)((c / m) % l), (int)(c % m)};
       const Row r = cell_row(A, AXIS, AXIS + 1, c, idx3);
@@ -388,17 +395,6 @@ __global__ __launch_bounds__(256) void k_lod_border(LodSet set, int dg, int ax1,
   }
 }
 
-// f_1/3 and f_2/3 start as copies of the previous level (XProblem.cpp:2796-2797)
-__global__ __launch_bounds__(256) void k_lod_stages(LodSet S) {
-  const LodArgs &A = S.a[blockIdx.y];
-  const int64_t n = (int64_t)A.n * A.l * A.m;
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
-    const double v = A.f_prev[c];
-    A.stage[0][c] = v;
-    A.stage[1][c] = v;
-  }
-}
-
 // The reference's per-line sector rules (XProblem.cpp:2928-2962) over every
 // line of every axis: 1 = an Actual point first on its line ("Initial border
 // not found"), 2 = a Fictive sector border, 4 = a border type the LOD has no
@@ -535,7 +531,6 @@ static void run_lod(xf_ctx c, int count, const xf_grid *grids, double *const *f,
   const unsigned cb = (unsigned)std::min<int64_t>((ncmax + 255) / 256, 16384);
   const dim3 cells(cb, count);
   const dim3 borders((unsigned)std::max<int64_t>(1, std::min<int64_t>((nbmax + 255) / 256, 4096)), count);
-  hipLaunchKernelGGL(k_lod_stages, cells, dim3(256), 0, c->stream, S);
   for (int dg = 1; dg < 4; dg++) {  // XProblem.cpp:2847-2864
     const int axis = dg - 1;
     const dim3 lines((unsigned)((nlmax[axis] + 63) / 64), count);
