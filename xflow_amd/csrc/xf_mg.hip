@@ -13,7 +13,7 @@
 // otherwise; coarse steps are the doubled fine steps, so each coarse grid
 // carries the same masked 7-point operator rediscretised (its own xf_grid).
 // One cycle, all in FP32 with FP64 dot products:
-//   level k < coarsest: nu damped-Jacobi sweeps from z = 0 (omega = 0.8),
+//   level k < coarsest: nu damped-Jacobi sweeps from z = 0 (omega = 6/7),
 //     t = r - A z, r_{k+1} = mean of t over the unknown children,
 //     V-cycle on level k+1, z += z_{k+1}(parent) on the unknowns,
 //     nu damped-Jacobi sweeps;
@@ -442,8 +442,8 @@ __global__ __launch_bounds__(kCoarseCells) void k_mg_coarse_dense(int ncell, int
 // back up, every phase a loop over the level's interior cells separated by
 // barriers (the level fields stay in global memory, warm in the caches).
 // Each phase does what the multi-block path does at that level, in the same
-// floating-point order: the first two sweeps from z = 0, the residual, the
-// restriction, the dense solve, the prolongation and the two sweeps after.
+// floating-point order: the sweeps from z = 0, the residual, the
+// restriction, the dense solve, the prolongation and the sweeps after.
 struct BotLevel {
   Dims d;
   const uint8_t *flags;
